@@ -1,0 +1,168 @@
+/*
+ * crm_hip.h -- C ABI of the MI355X (gfx950) cloud-resolving model library.
+ *
+ * This library replaces one Fortran entry point of the reference:
+ *
+ *   subroutine crm(lchnk, icol, ncrms, dt_gl, plev,
+ *                  crm_input, crm_state, crm_rad, crm_ecpp_output, crm_output)
+ *     components/cam/src/physics/crm/crm_module.F90:45-56
+ *
+ * called by crm_physics_tend (components/cam/src/physics/cam/crm_physics.F90:1145-1156).
+ * The Fortran types it exchanges keep their layouts; here each becomes a "view": a struct
+ * of plain pointers into the caller's arrays plus the caller's leading dimension.
+ *
+ * Layout convention (identical to the Fortran arrays, column-major, pcols leading):
+ *   (pcols)                   element icrm              at p[icrm]
+ *   (pcols, plev)             element (icrm,k)          at p[icrm + ld*k]
+ *   (pcols, plev+1)           element (icrm,k)          at p[icrm + ld*k]
+ *   (pcols, nx, ny)           element (icrm,i,j)        at p[icrm + ld*(i + nx*j)]
+ *   (pcols, nx, ny, nz)       element (icrm,i,j,k)      at p[icrm + ld*(i + nx*(j + ny*k))]
+ *   (pcols, nxr, nyr, nz)     element (icrm,ir,jr,k)    at p[icrm + ld*(ir + nxr*(jr + nyr*k))]
+ * All indices above are 0-based; icrm < ncrms <= ld. GCM level k=0 is the model top,
+ * k=plev-1 the lowest level (CAM ordering); CRM level 0 is the lowest CRM level.
+ *
+ * Only instances 0..ncrms-1 are read or written (crm_module.F90:268-281 slices 1:ncrms).
+ * The caller owns all memory.
+ */
+#ifndef CRM_HIP_H
+#define CRM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference aborts the whole model (endrun / task_abort / stop); this
+ * library returns a code instead and the Fortran shim turns a nonzero code into endrun. */
+enum {
+  CRM_OK = 0,
+  CRM_ERR_ARG = 1,        /* bad dims or null pointer                                       */
+  CRM_ERR_CFL = 2,        /* ncycle > 16 (kurant.F90:64-75)                                 */
+  CRM_ERR_NXRAD = 3,      /* nx not divisible by nx_rad (crm_module.F90:686-692)            */
+  CRM_ERR_DAMPING = 4,    /* tau_min < 2*dt in the sponge (damping.F90:39-42)               */
+  CRM_ERR_NAN = 5,        /* non-finite prognostic field detected after the call            */
+  CRM_ERR_DEVICE = 6,     /* HIP runtime error                                              */
+  CRM_ERR_NOMEM = 7,      /* device allocation failed                                       */
+  CRM_ERR_UNSUPPORTED = 8 /* configuration not provided by this build                       */
+};
+
+/* Arithmetic of the bulk stencils. The pressure solve (DFT + Thomas sweep), the hydrostatic
+ * reference profiles and the column budget sums stay FP64 in both modes. */
+enum { CRM_PREC_FP64 = 0, CRM_PREC_MIXED = 1 };
+
+typedef struct crm_dims {
+  int ncrms;        /* instances in this call                         (crm_module.F90:103)  */
+  int ld;           /* leading dimension of every caller array (pcols, crm_module.F90:60)   */
+  int plev;         /* GCM levels (pver)                              (crm_module.F90:104)  */
+  int nx, ny, nz;   /* crm_nx, crm_ny, crm_nz  (cam/crmdims.F90:17-19); ny==1 means 2D      */
+  int nx_rad, ny_rad; /* crm_nx_rad, crm_ny_rad (cam/crmdims.F90:21-22)                     */
+  double dx, dt;    /* crm_dx [m], crm_dt [s]  (cam/crmdims.F90:24-26)                      */
+} crm_dims;
+
+/* crm_input_type (crm/crm_input_module.F90:10-47). */
+typedef struct crm_input_view {
+  const double *zmid;    /* (ld,plev)   height of mid levels [m]          */
+  const double *zint;    /* (ld,plev+1) height of interfaces [m]          */
+  const double *tl;      /* (ld,plev)   temperature [K]                   */
+  const double *ql;      /* (ld,plev)   water vapour [kg/kg]              */
+  const double *qccl;    /* (ld,plev)   cloud liquid [kg/kg]              */
+  const double *qiil;    /* (ld,plev)   cloud ice [kg/kg]                 */
+  const double *ps;      /* (ld)        surface pressure [Pa]             */
+  const double *pmid;    /* (ld,plev)   mid-level pressure [Pa]           */
+  const double *pint;    /* (ld,plev+1) interface pressure [Pa]           */
+  const double *pdel;    /* (ld,plev)   layer thickness [Pa]              */
+  const double *phis;    /* (ld)        surface geopotential              */
+  const double *ul;      /* (ld,plev)   u [m/s]                           */
+  const double *vl;      /* (ld,plev)   v [m/s]                           */
+  const double *ocnfrac; /* (ld)                                          */
+  const double *tau00;   /* (ld)        surface stress [N/m2]             */
+  const double *wndls;   /* (ld)        surface wind [m/s]                */
+  const double *bflxls;  /* (ld)        surface buoyancy flux [K m/s]     */
+  const double *fluxu00, *fluxv00, *fluxt00, *fluxq00; /* (ld) unused by crm() */
+} crm_input_view;
+
+/* crm_state_type (crm/crm_state_module.F90:12-56), sam1mom members. (ld,nx,ny,nz) each. */
+typedef struct crm_state_view {
+  double *u_wind, *v_wind, *w_wind, *temperature, *qt, *qp, *qn;
+} crm_state_view;
+
+/* crm_rad_type (crm/crm_rad_module.F90:17-34). (ld,nx_rad,ny_rad,nz) each.
+ * qrad is read; the other five are overwritten with time averages. */
+typedef struct crm_rad_view {
+  const double *qrad;
+  double *temperature, *qv, *qc, *qi, *cld;
+} crm_rad_view;
+
+/* crm_output_type (crm/crm_output_module.F90:7-133): the members crm() writes for the
+ * 1-moment microphysics. cltot/clhgh/clmed/cllow are accumulated onto the values the
+ * caller passes in (crm_module.F90:1057-1069,1474-1477), as in the reference. */
+typedef struct crm_output_view {
+  /* (ld,nx,ny,nz) */
+  double *qcl, *qci, *qpl, *qpi, *tk, *tkh;
+  /* (ld,nx,ny) */
+  double *prec_crm;
+  /* (ld) */
+  double *cltot, *clhgh, *clmed, *cllow;
+  double *precc, *precl, *precsc, *precsl;
+  double *jt_crm, *mx_crm, *prectend, *precstend, *taux, *tauy, *z0m, *timing_factor;
+  /* (ld,plev) */
+  double *cldtop, *qc_mean, *qi_mean, *qs_mean, *qg_mean, *qr_mean;
+  double *sltend, *qltend, *qcltend, *qiltend;
+  double *cld, *gicewp, *gliqwp, *mctot, *mcup, *mcdn, *mcuup, *mcudn;
+  double *mu_crm, *md_crm, *du_crm, *eu_crm, *ed_crm;
+  double *flux_qt, *fluxsgs_qt, *tkez, *tkesgsz, *tkz, *flux_u, *flux_v, *flux_qp;
+  double *precflux, *qt_ls, *qt_trans, *qp_trans, *qp_fall, *qp_src, *qp_evp, *t_ls;
+} crm_output_view;
+
+/* Per-call GCM-side context that the reference reads through CAM modules
+ * (crm_module.F90:287-291,679): get_rlat_p/get_rlon_p [radians], get_gcol_p, get_nstep. */
+typedef struct crm_column_info {
+  int lchnk;
+  const int *gcol;      /* (ncrms) global column ids, seed = gcol*1000 (crm_module.F90:679) */
+  const double *lat;    /* (ncrms) latitude  [radians]                                      */
+  const double *lon;    /* (ncrms) longitude [radians]                                      */
+  int igstep;           /* GCM step number; setperturb runs when igstep <= 1                */
+} crm_column_info;
+
+typedef struct crm_hip_config {
+  int device;           /* HIP device ordinal                                               */
+  int precision;        /* CRM_PREC_FP64 or CRM_PREC_MIXED                                  */
+  int max_ncrms;        /* instances to size device buffers for (0: grow on demand)         */
+} crm_hip_config;
+
+typedef struct crm_hip_stats {
+  int nstop;            /* substeps of the last call                                        */
+  int ncycle_max;       /* largest kurant subcycle count seen in the last call              */
+  int nprec_max;        /* largest precipitation sub-step count seen in the last call       */
+  double ms_device;     /* device time of the last call's substep loop [ms] (HIP events)    */
+} crm_hip_stats;
+
+typedef struct crm_hip_ctx crm_hip_ctx;
+
+/* Create a context: one HIP stream plus persistent device buffers. Calls on one context are
+ * serialised by the caller; separate contexts are independent. */
+int crm_hip_init(crm_hip_ctx **ctx, const crm_hip_config *cfg);
+
+/* One GCM physics step of the CRM on host (caller) memory -- the drop-in for crm()
+ * (crm_module.F90:45-1718). Copies the views to the device, advances nstop=dt_gl/dt substeps,
+ * writes crm_state, crm_rad and crm_output back. Stateless across calls (crm_module.F90:727,
+ * vars.F90:387-389): device buffers persist, values do not. */
+int crm_hip_run(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col,
+                double dt_gl, const crm_input_view *in, crm_state_view *state,
+                crm_rad_view *rad, crm_output_view *out);
+
+/* Same as crm_hip_run but every view pointer is a device pointer (caller-resident in HBM),
+ * and col->gcol/lat/lon stay host pointers. No host<->device copies of the views. */
+int crm_hip_run_device(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col,
+                       double dt_gl, const crm_input_view *in, crm_state_view *state,
+                       crm_rad_view *rad, crm_output_view *out);
+
+int crm_hip_get_stats(const crm_hip_ctx *ctx, crm_hip_stats *stats);
+int crm_hip_finalize(crm_hip_ctx *ctx);
+const char *crm_hip_errstr(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRM_HIP_H */

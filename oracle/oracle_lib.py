@@ -1,0 +1,73 @@
+"""Loaders for the CPU oracles.  TEST INFRASTRUCTURE ONLY.
+
+Two CPU implementations of crm() (crm/crm_module.F90:45-1718) serve as the checker:
+  * the C restatement in oracle/crm_oracle.c  -> oracle/_build/libcrm_oracle.so
+    (sizes are run-time arguments; one library for every size class);
+  * the reference phase routines compiled from the reference tree plus the Fortran
+    harness oracle/ref/crm_ref_harness.F90 -> oracle/_ref/libcrmref_<class>.so
+    (sizes are compile-time constants in the reference, so one library per class).
+Both take the views of include/crm_hip.h.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py import this module.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "acme-ecp_amd"))
+from crm_views import CrmDims, CrmColumnInfo  # noqa: E402
+
+ORACLE_SO = os.path.join(HERE, "_build", "libcrm_oracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+
+def ref_class_name(nx, ny, nz, dx, dt, nx_rad=None, ny_rad=None):
+    name = f"{nx}x{ny}x{nz}_dx{int(dx)}_dt{int(dt)}"
+    if (nx_rad or nx) != nx or (ny_rad or ny) != ny:
+        name += f"_r{nx_rad}x{ny_rad}"
+    return name
+
+
+def ref_so_path(case):
+    return os.path.join(REF_DIR, "libcrmref_" + ref_class_name(
+        case.nx, case.ny, case.nz, case.dx, case.dt, case.nx_rad, case.ny_rad) + ".so")
+
+
+class _Runner:
+    def __init__(self, path, entry, getter):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+        self.fn = getattr(self.lib, entry)
+        self.fn.restype = C.c_int
+        self.get = getattr(self.lib, getter)
+        self.get.restype = C.c_int
+        self.get.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
+
+    def run(self, case, dt_gl, phase_stop=0):
+        """Run one crm() call in place on case's arrays; returns the status code."""
+        dims = case.dims()
+        col = case.column_info()
+        vi, vs, vr, vo = case.views()
+        return self.fn(C.byref(dims), C.byref(col), C.c_double(dt_gl), C.byref(vi),
+                       C.byref(vs), C.byref(vr), C.byref(vo), C.c_int(phase_stop))
+
+    def field(self, name, size):
+        buf = np.zeros(size)
+        n = self.get(name.encode(), buf.ctypes.data_as(C.POINTER(C.c_double)))
+        if n < 0:
+            raise KeyError(name)
+        return buf[:n]
+
+
+def reference(case):
+    """Reference CRM (compiled reference phases + harness) for case's size class."""
+    return _Runner(ref_so_path(case), "ref_crm", "ref_get")
+
+
+def oracle():
+    """C restatement of the CRM."""
+    return _Runner(ORACLE_SO, "oracle_crm", "oracle_get")
