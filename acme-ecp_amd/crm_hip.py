@@ -1,0 +1,189 @@
+"""Host binding of libcrm_hip.so (include/crm_hip.h) for Python callers.
+
+The drop-in target is crm() (components/cam/src/physics/crm/crm_module.F90:45-56): one call
+advances every CRM instance of a GCM chunk by dt_gl.  `CrmHip.run(case, dt_gl)` does that on
+host arrays (a `crm_views.CrmCase`), `CrmHip.run_device(...)` on arrays already resident in
+HBM.  Status codes of the library become `CrmError`, with the same meaning as the reference's
+aborts (kurant.F90:64-75, crm_module.F90:686-692, damping.F90:39-42).
+
+There is no CPU fallback: without the built library every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from crm_views import (CrmColumnInfo, CrmDims, CrmInputView, CrmOutputView, CrmRadView,
+                       CrmStateView, views_from_addresses)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcrm_hip.so")
+
+CRM_OK, CRM_ERR_ARG, CRM_ERR_CFL, CRM_ERR_NXRAD, CRM_ERR_DAMPING = 0, 1, 2, 3, 4
+CRM_ERR_NAN, CRM_ERR_DEVICE, CRM_ERR_NOMEM, CRM_ERR_UNSUPPORTED = 5, 6, 7, 8
+PRECISION = {"fp64": 0, "mixed": 1}
+
+# Every function include/crm_hip.h declares.
+ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_get_stats",
+               "crm_hip_finalize", "crm_hip_errstr"]
+# include/crm_hip_debug.h
+DEBUG_SYMBOLS = ["crm_hip_debug_run", "crm_hip_debug_get", "crm_hip_fft991_operators"]
+
+
+class CrmHipConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("max_ncrms", C.c_int)]
+
+
+class CrmHipStats(C.Structure):
+    _fields_ = [("nstop", C.c_int), ("ncycle_max", C.c_int), ("nprec_max", C.c_int),
+                ("ms_device", C.c_double)]
+
+
+class CrmError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__(f"{where}: {errstr(code)} (status {code})")
+
+
+_lib = None
+
+
+def load():
+    """The HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP CRM library missing: {LIB_PATH} (build it first)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    lib.crm_hip_init.argtypes = [C.POINTER(P), C.POINTER(CrmHipConfig)]
+    views = [P, C.POINTER(CrmDims), C.POINTER(CrmColumnInfo), C.c_double,
+             C.POINTER(CrmInputView), C.POINTER(CrmStateView), C.POINTER(CrmRadView),
+             C.POINTER(CrmOutputView)]
+    lib.crm_hip_run.argtypes = views
+    lib.crm_hip_run_device.argtypes = views
+    lib.crm_hip_debug_run.argtypes = views + [C.c_int]
+    lib.crm_hip_debug_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
+    lib.crm_hip_get_stats.argtypes = [P, C.POINTER(CrmHipStats)]
+    lib.crm_hip_finalize.argtypes = [P]
+    lib.crm_hip_errstr.argtypes = [C.c_int]
+    lib.crm_hip_errstr.restype = C.c_char_p
+    lib.crm_hip_fft991_operators.argtypes = [C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double)]
+    _lib = lib
+    return lib
+
+
+def errstr(code):
+    return load().crm_hip_errstr(code).decode()
+
+
+def fft991_operators(n):
+    """Dense forward (Ft[i][s]) and inverse (Gt[s][i]) matrices of the reference's x-transform
+    (crm/fft.F90:5-153) in the library's coefficient-slot order."""
+    Ft, Gt = np.zeros((n, n)), np.zeros((n, n))
+    dp = C.POINTER(C.c_double)
+    rc = load().crm_hip_fft991_operators(n, Ft.ctypes.data_as(dp), Gt.ctypes.data_as(dp))
+    if rc:
+        raise CrmError(rc, "crm_hip_fft991_operators")
+    return Ft, Gt
+
+
+class CrmHip:
+    """One library context: a HIP stream and persistent device buffers on one GPU."""
+
+    def __init__(self, device=0, precision="fp64", max_ncrms=0):
+        self.lib = load()
+        self.ctx = C.c_void_p()
+        cfg = CrmHipConfig(device, PRECISION[precision], max_ncrms)
+        rc = self.lib.crm_hip_init(C.byref(self.ctx), C.byref(cfg))
+        if rc:
+            raise CrmError(rc, "crm_hip_init")
+
+    def close(self):
+        if self.ctx:
+            self.lib.crm_hip_finalize(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, fn, dims, col, dt_gl, views, *extra):
+        vi, vs, vr, vo = views
+        rc = fn(self.ctx, C.byref(dims), C.byref(col), C.c_double(dt_gl), C.byref(vi),
+                C.byref(vs), C.byref(vr), C.byref(vo), *extra)
+        return rc
+
+    def run(self, case, dt_gl, check=True):
+        """crm() on a CrmCase's host arrays, in place.  Returns the status code."""
+        rc = self._call(self.lib.crm_hip_run, case.dims(), case.column_info(), dt_gl,
+                        case.views())
+        if rc and check:
+            raise CrmError(rc, "crm_hip_run")
+        return rc
+
+    def run_device(self, dcase, dt_gl, check=True):
+        """crm() on a DeviceCase (arrays resident on the GPU), in place."""
+        rc = self._call(self.lib.crm_hip_run_device, dcase.case.dims(),
+                        dcase.case.column_info(), dt_gl, dcase.views())
+        if rc and check:
+            raise CrmError(rc, "crm_hip_run_device")
+        return rc
+
+    def debug_run(self, case, dt_gl, phase_stop):
+        return self._call(self.lib.crm_hip_debug_run, case.dims(), case.column_info(), dt_gl,
+                          case.views(), C.c_int(phase_stop))
+
+    def debug_get(self, name, size):
+        buf = np.zeros(size)
+        n = self.lib.crm_hip_debug_get(self.ctx, name.encode(),
+                                       buf.ctypes.data_as(C.POINTER(C.c_double)))
+        if n < 0:
+            raise KeyError(name)
+        return buf[:n]
+
+    def stats(self):
+        s = CrmHipStats()
+        rc = self.lib.crm_hip_get_stats(self.ctx, C.byref(s))
+        if rc:
+            raise CrmError(rc, "crm_hip_get_stats")
+        return s
+
+
+class DeviceCase:
+    """Device-resident copies (torch tensors in HBM) of a CrmCase's arrays.
+
+    Each array keeps the caller's column-major layout: a Fortran (ld, a, b, c) array becomes a
+    contiguous torch tensor of shape (c, b, a, ld)."""
+
+    def __init__(self, case, device="cuda:0"):
+        import torch
+        self.case = case
+        self.torch = torch
+        self.arrays = {}
+        for group in ("inp", "state", "rad", "out"):
+            self.arrays[group] = {k: torch.from_numpy(np.ascontiguousarray(v.T)).to(device)
+                                  for k, v in getattr(case, group).items()}
+
+    def views(self):
+        addr = {g: {k: t.data_ptr() for k, t in d.items()} for g, d in self.arrays.items()}
+        return views_from_addresses(addr["inp"], addr["state"], addr["rad"], addr["out"])
+
+    def download(self, case=None):
+        """Copy state, rad and output arrays back into case (default: the source case)."""
+        case = case or self.case
+        for group in ("state", "rad", "out"):
+            dst = getattr(case, group)
+            for k, t in self.arrays[group].items():
+                dst[k][...] = t.cpu().numpy().T
+        return case

@@ -1,0 +1,1762 @@
+// crm_kernels.hpp -- the CRM substep as HIP kernels for gfx950 (FP64 form).
+//
+// Each kernel is one phase group of the reference substep (crm/crm_module.F90:743-888) done
+// as a single sweep over levels: lane = column, 64/NX instances per wave64, neighbours in x
+// by lane rotation, neighbours in z by a register window.  Level index k is 0-based here
+// (reference level k+1).
+#pragma once
+#include "../../include/crm_hip.h"
+#include "crm_common.hpp"
+
+namespace crm {
+
+// ------------------------------------------------------------------------------------------
+// Reference profiles from crm_input: crm_module.F90:320-375; grid factors sgs.F90:219-228;
+// sponge profile damping.F90:44-61; roughness crm_module.F90:583-590 (z0_est
+// crmsurface.F90:104-131).  One thread per instance.
+// ------------------------------------------------------------------------------------------
+__device__ double z0_est(double z, double bflx, double wnd, double ustar, double c1) {
+  const double vonk = R4(0.4), am = R4(4.8), bm = R4(19.3), eps = R4(1.e-10);
+  double rlmo = -bflx * vonk / (ustar * ustar * ustar + eps);
+  double zeta = fmin(1.0, z * rlmo), psi1;
+  if (zeta >= 0.) {
+    psi1 = -am * zeta;
+  } else {
+    double x = sqrt(sqrt(1.0 - bm * zeta));
+    psi1 = 2. * log(1.0 + x) + log(1.0 + x * x) - 2. * atan(x) + c1;
+  }
+  double lnz = fmax(0.0, vonk * wnd / (ustar + eps) + psi1);
+  return z * exp(-lnz);
+}
+
+__global__ void k_setup(Dev d, crm_input_view in) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.ncrms) return;
+  const int ld = d.ld, plev = d.plev, nzm = d.nzm;
+  auto IN = [&](const double* p, int l) { return p[(size_t)c + (size_t)ld * l]; };
+  double* z = prof(d, P_Z, c);
+  double* zi = prof(d, P_ZI, c);
+  double* pres = prof(d, P_PRES, c);
+  double* presi = prof(d, P_PRESI, c);
+  double* adz = prof(d, P_ADZ, c);
+  double* adzw = prof(d, P_ADZW, c);
+  double* rho = prof(d, P_RHO, c);
+  double* rhow = prof(d, P_RHOW, c);
+  double* bet = prof(d, P_BET, c);
+  double* gamaz = prof(d, P_GAMAZ, c);
+  double* grdf = prof(d, P_GRDFX, c);
+  double* tau = prof(d, P_TAU, c);
+  double zs = IN(in.zint, plev);
+  for (int k = 0; k < nzm; k++) {
+    z[k] = IN(in.zmid, plev - 1 - k) - zs;
+    zi[k] = IN(in.zint, plev - k) - zs;
+    pres[k] = IN(in.pmid, plev - 1 - k) / 100.;
+    presi[k] = IN(in.pint, plev - k) / 100.;
+    bet[k] = GGR / IN(in.tl, plev - 1 - k);
+    gamaz[k] = GGR / CP * z[k];
+  }
+  zi[nzm] = IN(in.zint, plev - nzm) - zs;
+  presi[nzm] = IN(in.pint, plev - nzm) / 100.;
+  double dz = 0.5 * (z[0] + z[1]);
+  inst(d, I_DZ, c) = dz;
+  for (int k = 1; k < nzm; k++) adzw[k] = (z[k] - z[k - 1]) / dz;
+  adzw[0] = 1.;
+  adzw[nzm] = adzw[nzm - 1];
+  for (int k = 0; k < nzm; k++) adz[k] = (zi[k + 1] - zi[k]) / dz;
+  for (int k = 0; k < nzm; k++) rho[k] = IN(in.pdel, plev - 1 - k) / GGR / (adz[k] * dz);
+  for (int k = 1; k < nzm; k++) rhow[k] = (IN(in.pmid, plev - k) - IN(in.pmid, plev - 1 - k)) / GGR / (adzw[k] * dz);
+  rhow[0] = 2. * rhow[1] - rhow[2];
+  rhow[nzm] = 2. * rhow[nzm - 1] - rhow[nzm - 2];
+  for (int k = 0; k < nzm; k++) {
+    double a = adz[k] * dz;
+    grdf[k] = fmin(16.0, d.dx * d.dx / (a * a));
+  }
+  // sponge: n_damp is the deepest level within 0.4 of the domain height from the top
+  int n_damp = 0;
+  for (int k = nzm - 1; k >= 0; k--)
+    if (z[nzm - 1] - z[k] < R4(0.4) * z[nzm - 1]) n_damp = nzm - k;
+  inst(d, I_NDAMP, c) = (double)n_damp;
+  int kbot = nzm - 1 - n_damp;  // 0-based level of the reference's z(nzm-n_damp)
+  if (kbot < 0) kbot = 0;
+  for (int k = 0; k < nzm; k++) {
+    tau[k] = 0.0;
+    if (k + 1 >= nzm - n_damp) {
+      double tt = 60. * pow(450. / 60., (z[nzm - 1] - z[k]) / (z[nzm - 1] - z[kbot]));
+      tau[k] = 1. / tt;
+    }
+  }
+  double ustar = sqrt(in.tau00[c] / rho[0]);
+  double z0 = z0_est(z[0], in.bflxls[c], in.wndls[c], ustar, d.z0c1);
+  inst(d, I_Z0, c) = fmax(R4(0.00001), fmin(1.0, z0));
+}
+
+// micro_diagnose (MICRO_SAM1MOM/microphysics.F90:331-357) for one point.
+__device__ __forceinline__ void micro_diag(double qt, double qn, double qp, double tabs, double a_bg,
+                                           double a_pr, double& qv, double& qcl, double& qci,
+                                           double& qpl, double& qpi) {
+  qv = qt - qn;
+  double omn = fmax(0.0, fmin(1.0, (tabs - TBGMIN) * a_bg));
+  qcl = qn * omn;
+  qci = qn * (1. - omn);
+  double omp = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * a_pr));
+  qpl = qp * omp;
+  qpi = qp * (1. - omp);
+}
+
+__device__ __forceinline__ double A_BG() { return 1. / (TBGMAX - TBGMIN); }
+__device__ __forceinline__ double A_PR() { return 1. / (TPRMAX - TPRMIN); }
+__device__ __forceinline__ double A_GR() { return 1. / (TGRMAX - TGRMIN); }
+
+// ------------------------------------------------------------------------------------------
+// Initial fields, means and large-scale tendencies: crm_module.F90:379-594, micro_init
+// (microphysics.F90:160-245), sgs_init (sgs.F90:187-213).  Segment per instance.
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_init(Dev d, crm_input_view in, crm_state_view st, crm_output_view out, double dt_gl) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, nz = d.nz, ld = d.ld, plev = d.plev, i = s.i;
+  const double umax = 0.5 * d.dx / d.dt, factor_xy = 1. / (double)NX, idt_gl = 1. / dt_gl;
+  auto G4 = [&](const double* p, int cc, int ii, int k) {
+    return p[(size_t)cc + (size_t)ld * ((size_t)ii + (size_t)NX * k)];
+  };
+  // first-substep wind clamp decided by instance 0 (crm_module.F90:399-417)
+  bool clamp = G4(st.u_wind, 0, 0, 0) == G4(st.u_wind, 0, 1, 0) && G4(st.u_wind, 0, 2, 1) == G4(st.u_wind, 0, 3, 1);
+  const double a_bg = A_BG(), a_pr = A_PR();
+  double* gamaz = prof(d, P_GAMAZ, c);
+  double colprec = 0.0, colprecs = 0.0;
+  int kmin = nzm, kmax = -1;
+  for (int k = 0; k < nzm; k++) {
+    size_t o = ((size_t)c * nzm + k) * NX + i;
+    size_t ow = ((size_t)c * nz + k) * NX + i;
+    double u = G4(st.u_wind, c, i, k);
+    if (clamp) u = fmin(umax, fmax(-umax, u));
+    double w = G4(st.w_wind, c, i, k);
+    double tabs = G4(st.temperature, c, i, k);
+    double qt = G4(st.qt, c, i, k), qp = G4(st.qp, c, i, k), qn = G4(st.qn, c, i, k);
+    double qv, qcl, qci, qpl, qpi;
+    micro_diag(qt, qn, qp, tabs, a_bg, a_pr, qv, qcl, qci, qpl, qpi);
+    double t = tabs + gamaz[k] - FAC_COND * qcl - FAC_SUB * qci - FAC_COND * qpl - FAC_SUB * qpi;
+    int l = plev - 1 - k;
+    double pd = in.pdel[(size_t)c + (size_t)ld * l];
+    colprec += s.sum((qpl + qpi) * pd);
+    colprecs += s.sum(qpi * pd);
+    double u0 = s.sum(u) * factor_xy;
+    double t0 = s.sum(t) * factor_xy;
+    double t00 = s.sum(t + FAC_COND * qpl + FAC_SUB * qpi) * factor_xy;
+    double tabs0 = s.sum(tabs) * factor_xy;
+    double q0 = s.sum(qv + qcl + qci) * factor_xy;
+    double qv0 = s.sum(qv) * factor_xy;
+    double qn0 = s.sum(qcl + qci) * factor_xy;
+    double qp0 = s.sum(qpl + qpi) * factor_xy;
+    if (qcl + qci > 0. && tabs < R4(273.15)) {
+      kmin = ::min(kmin, k);
+      kmax = ::max(kmax, k);
+    }
+    if (live) {
+      d.u[o] = u;
+      d.w[ow] = w;
+      d.t[o] = t;
+      d.qt[o] = qt;
+      d.qp[o] = qp;
+      d.qn[o] = qn;
+      d.tabs[o] = tabs;
+      d.qv[o] = qv;
+      d.qcl[o] = qcl;
+      d.qci[o] = qci;
+      d.qpl[o] = qpl;
+      d.qpi[o] = qpi;
+      d.tk[o] = 0.0;
+      d.tkh[o] = 0.0;
+      d.tke[o] = 0.0;
+      for (int sl = 0; sl < 3; sl++) {
+        d.dudt[sl][o] = 0.0;
+        d.dwdt[sl][ow] = 0.0;
+      }
+      if (i == 0) {
+        auto IN = [&](const double* p) { return p[(size_t)c + (size_t)ld * l]; };
+        double uln = fmin(umax, fmax(-umax, IN(in.ul)));
+        prof(d, P_U0, c)[k] = u0;
+        prof(d, P_T0, c)[k] = t0;
+        prof(d, P_TABS0, c)[k] = tabs0;
+        prof(d, P_Q0, c)[k] = q0;
+        prof(d, P_QV0, c)[k] = qv0;
+        prof(d, P_QN0, c)[k] = qn0;
+        prof(d, P_QP0, c)[k] = qp0;
+        prof(d, P_TTEND, c)[k] = (IN(in.tl) + gamaz[k] - FAC_COND * (IN(in.qccl) + IN(in.qiil)) -
+                                  FAC_FUS * IN(in.qiil) - t00) * idt_gl;
+        prof(d, P_QTEND, c)[k] = (IN(in.ql) + IN(in.qccl) + IN(in.qiil) - q0) * idt_gl;
+        prof(d, P_UTEND, c)[k] = (uln - u0) * idt_gl;
+      }
+    }
+  }
+  kmin = s.imin(kmin);
+  kmax = s.imax(kmax);
+  if (!live) return;
+  size_t ow = ((size_t)c * nz + nzm) * NX + i;
+  d.w[ow] = 0.0;
+  for (int sl = 0; sl < 3; sl++) d.dwdt[sl][ow] = 0.0;
+  d.precsfc[(size_t)c * NX + i] = 0.0;
+  d.precssfc[(size_t)c * NX + i] = 0.0;
+  if (i == 0) {
+    inst(d, I_KMIN, c) = kmin;
+    inst(d, I_KMAX, c) = kmax;
+    inst(d, I_COLPREC, c) = colprec;
+    inst(d, I_COLPRECS, c) = colprecs;
+    inst(d, I_CLTOT, c) = out.cltot[c];
+    inst(d, I_CLHGH, c) = out.clhgh[c];
+    inst(d, I_CLMED, c) = out.clmed[c];
+    inst(d, I_CLLOW, c) = out.cllow[c];
+  }
+  for (int p = P_UWLE; p < NPROF; p++)
+    for (int k = i; k < nz; k += NX) prof(d, p, c)[k] = 0.0;
+  for (int q = 0; q < 5; q++)
+    for (int k = i; k < nzm * d.nx_rad; k += NX) d.rad[((size_t)q * d.ncrms + c) * nzm * d.nx_rad + k] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Initial noise on the first GCM step: setperturb.F90:7-86 with the Mersenne twister of
+// cam/RNG_MT.F90:60-139 (whose upper mask is -1147483648).  One thread per instance; the
+// 624-word state lives in thread-private memory.  Only runs when igstep <= 1.
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void k_setperturb(Dev d, const int* gcol) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.ncrms) return;
+  uint32_t mt[624];
+  mt[0] = (uint32_t)(gcol[c] * 1000);
+  for (int m = 1; m < 624; m++) mt[m] = 69069u * mt[m - 1];
+  int mti = 624;
+  const uint32_t UMASK = (uint32_t)-1147483648LL, LMASK = 2147483647u, MATA = (uint32_t)-1727483681LL;
+  auto gen = [&]() -> double {
+    uint32_t y;
+    if (mti >= 624) {
+      int kk;
+      for (kk = 0; kk < 624 - 397; kk++) {
+        y = (mt[kk] & UMASK) | (mt[kk + 1] & LMASK);
+        mt[kk] = mt[kk + 397] ^ (y >> 1) ^ ((y & 1u) ? MATA : 0u);
+      }
+      for (; kk < 623; kk++) {
+        y = (mt[kk] & UMASK) | (mt[kk + 1] & LMASK);
+        mt[kk] = mt[kk + (397 - 624)] ^ (y >> 1) ^ ((y & 1u) ? MATA : 0u);
+      }
+      y = (mt[623] & UMASK) | (mt[0] & LMASK);
+      mt[623] = mt[396] ^ (y >> 1) ^ ((y & 1u) ? MATA : 0u);
+      mti = 0;
+    }
+    y = mt[mti++];
+    y ^= y >> 11;
+    y ^= (y << 7) & (uint32_t)-1658038656LL;
+    y ^= (y << 15) & (uint32_t)-272236544LL;
+    y ^= y >> 18;
+    int32_t ys = (int32_t)y;
+    if (ys < 0) return ((double)ys + 4294967296.0) / 4294967296.0;
+    return ((double)ys + 0.5) / 4294967296.0;
+  };
+  const double factor_xy = 1. / (double)NX;
+  for (int k = 0; k < 5 && k < d.nzm; k++) {
+    double scaling = (double)(5 - k) / 5.0;
+    double t02 = 0.0;
+    double* t = d.t + ((size_t)c * d.nzm + k) * NX;
+    for (int i = 0; i < NX; i++) {
+      double r = 1. - 2. * gen();
+      t[i] = t[i] + 1 * r * scaling;
+      t02 = t02 + t[i] * factor_xy;
+    }
+    double t0 = prof(d, P_T0, c)[k];
+    for (int i = 0; i < NX; i++) t[i] = t[i] * t0 / t02;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// CFL check: kurant.F90:9-83 and kurant_sgs (sgs.F90:329-372).  Batch-global maximum via an
+// atomic max on the (non-negative) double's bits.
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_kurant(Dev d) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, nz = d.nz, i = s.i;
+  const double dz = inst(d, I_DZ, c);
+  const double* adzw = prof(d, P_ADZW, c);
+  const double* grdf = prof(d, P_GRDFX, c);
+  const double dxf = sqrt((1. / d.dx) * (1. / d.dx) + 0.0);
+  double cfl = 0.0, wm_prev = 0.0, uhm_prev = 0.0;
+  for (int k = 0; k <= nzm; k++) {
+    double wm = 0.0, uhm = 0.0;
+    if (k < nzm) {
+      size_t o = ((size_t)c * nzm + k) * NX + i;
+      wm = s.max(fabs(d.w[((size_t)c * nz + k) * NX + i]));
+      double uu = d.u[o];
+      uhm = s.max(sqrt(uu * uu + 0.0));
+      double tkhmax = s.max(d.tkh[o]);
+      double dzz = dz * adzw[k];
+      double sgs = fmax(fmax(0.5 * tkhmax * 1.0 * d.dt / (dzz * dzz), 0.5 * tkhmax * grdf[k] * d.dt / (d.dx * d.dx)), 0.0);
+      cfl = fmax(cfl, sgs);
+    }
+    if (k >= 1) {
+      int kk = k - 1;
+      cfl = fmax(cfl, fmax(uhm_prev * d.dt * dxf, fmax(wm_prev, wm) * d.dt / (dz * adzw[kk])));
+    }
+    wm_prev = wm;
+    uhm_prev = uhm;
+  }
+  if (live && i == 0) atomic_max_nonneg(&d.gmax[0], cfl);
+}
+
+// ------------------------------------------------------------------------------------------
+// zero + buoyancy + forcing + radiative heating + sponge + cloud-ice fall, one sweep:
+// zero.F90:8-25, buoyancy.F90:7-38, forcing.F90:7-79, crm_module.F90:766-777,
+// damping.F90:9-117, ice_fall.F90:6-173.  Writes dudt(na), dwdt(na) from scratch.
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_pre(Dev d, int na, const double* qrad) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, nz = d.nz, i = s.i;
+  const double dtn = d.dtn, dz = inst(d, I_DZ, c);
+  const double* adz = prof(d, P_ADZ, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* bet = prof(d, P_BET, c);
+  const double* tabs0 = prof(d, P_TABS0, c);
+  const double* qv0 = prof(d, P_QV0, c);
+  const double* qn0 = prof(d, P_QN0, c);
+  const double* qp0 = prof(d, P_QP0, c);
+  const double* ttend = prof(d, P_TTEND, c);
+  const double* qtend = prof(d, P_QTEND, c);
+  const double* utend = prof(d, P_UTEND, c);
+  const double* tau = prof(d, P_TAU, c);
+  const int n_damp = (int)inst(d, I_NDAMP, c);
+  const int kmin = (int)inst(d, I_KMIN, c), kmax = (int)inst(d, I_KMAX, c);
+  const int nxr = d.nx_rad, ir = i / (NX / nxr);
+  double* dudt = d.dudt[na];
+  double* dwdt = d.dwdt[na];
+  auto F = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  // cloud-ice flux fz(k) (ice_fall.F90:94-128); zero outside [kmin-1, kmax]
+  auto qci_at = [&](int k) { return F(d.qci, k); };
+  auto icefz = [&](int k) -> double {
+    if (k < ::max(0, kmin - 1) || k > kmax || k > nzm - 1) return 0.0;
+    int kc = ::min(nzm - 1, k + 1), kb = ::max(0, k - 1);
+    double coef = dtn / (0.5 * (adz[kb] + adz[k]) * dz);
+    double qiu = rho[kc] * qci_at(kc), qic = rho[k] * qci_at(k), qid = rho[kb] * qci_at(kb);
+    double vt = fmin(R4(0.4), R4(8.66) * pow(fmax(0.0, qic) + R4(1.e-10), R4(0.24)));
+    double phi;
+    if (fabs(qic - qid) < R4(1.0e-25)) {
+      phi = 0.;
+    } else {
+      double th = (qiu - qic) / (qic - qid);
+      phi = fmax(0.0, fmin(fmin(0.5 * (1. + th), 2.0), 2. * th));
+    }
+    return -vt * (qic - 0.5 * (1. - coef * vt) * phi * (qic - qid));
+  };
+  double fz_k = icefz(0);
+  // level k-1 thermodynamics for the buoyancy average
+  double pb_tabs = 0, pb_qv = 0, pb_qcl = 0, pb_qci = 0, pb_qpl = 0, pb_qpi = 0;
+  for (int k = 0; k < nzm; k++) {
+    size_t o = ((size_t)c * nzm + k) * NX + i;
+    double tabs = d.tabs[o], qv = d.qv[o], qcl = d.qcl[o], qci = d.qci[o], qpl = d.qpl[o], qpi = d.qpi[o];
+    double u = d.u[o], w = d.w[((size_t)c * nz + k) * NX + i];
+    double t = d.t[o], qt = d.qt[o];
+    // buoyancy on w level k from scalar levels k-1 and k
+    double dw = 0.0;
+    if (k >= 1) {
+      int kb = k - 1;
+      double betu = adz[kb] / (adz[k] + adz[kb]);
+      double betd = adz[k] / (adz[k] + adz[kb]);
+      double up = tabs0[k] * (EPSV * (qv - qv0[k]) - (qcl + qci - qn0[k] + qpl + qpi - qp0[k])) +
+                  (tabs - tabs0[k]) * (1. + EPSV * qv0[k] - qn0[k] - qp0[k]);
+      double dn = tabs0[kb] * (EPSV * (pb_qv - qv0[kb]) - (pb_qcl + pb_qci - qn0[kb] + pb_qpl + pb_qpi - qp0[kb])) +
+                  (pb_tabs - tabs0[kb]) * (1. + EPSV * qv0[kb] - qn0[kb] - qp0[kb]);
+      dw = 0.0 + bet[k] * betu * up + bet[kb] * betd * dn;
+    }
+    pb_tabs = tabs;
+    pb_qv = qv;
+    pb_qcl = qcl;
+    pb_qci = qci;
+    pb_qpl = qpl;
+    pb_qpi = qpi;
+    // large-scale forcing with level-wise refill of negative vapour
+    t = t + ttend[k] * dtn;
+    qt = qt + qtend[k] * dtn;
+    bool neg = qt < 0.;
+    double nneg = s.sum(neg ? 1.0 : 0.0);
+    double qneg = s.sum(neg ? qt : 0.0);
+    double qpoz = s.sum(neg ? 0.0 : qt);
+    double du = 0.0 + utend[k];
+    if (nneg > 0.5 && qpoz + qneg > 0.) qt = fmax(0.0, qt * (1. + qneg / qpoz));
+    // radiative heating on column groups
+    t = t + qrad[(size_t)c + (size_t)d.ld * ((size_t)ir + (size_t)nxr * k)] * dtn;
+    // sponge layer
+    if (k + 1 >= nzm - n_damp) {
+      double tk = tau[k];
+      double u0l = s.sum(u / (double)NX), t0l = s.sum(t / (double)NX);
+      du = du - (u - u0l) * tk;
+      dw = dw - w * tk;
+      t = t - dtn * (t - t0l) * tk;
+      qt = qt - dtn * (qv - qv0[k]) * tk;
+    }
+    // cloud-ice sedimentation
+    double fz_kp = icefz(k + 1);
+    if (k >= ::max(0, kmin - 2) && k <= kmax) {
+      double coef = dtn / (dz * adz[k] * rho[k]);
+      double dqi = coef * (fz_k - fz_kp);
+      qt = qt + dqi;
+      t = t - (FAC_COND + FAC_FUS) * dqi;
+    }
+    if (k == 0 && live) {
+      double dqi = -(dtn / dz) * fz_k;
+      d.precsfc[(size_t)c * NX + i] += dqi;
+      d.precssfc[(size_t)c * NX + i] += dqi;
+    }
+    fz_k = fz_kp;
+    if (live) {
+      d.t[o] = t;
+      d.qt[o] = qt;
+      dudt[o] = du;
+      dwdt[((size_t)c * nz + k) * NX + i] = dw;
+    }
+  }
+  if (live) dwdt[((size_t)c * nz + nzm) * NX + i] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Smagorinsky eddy diffusivity: shear_prod2D.F90:7-96 and tke_full.F90:11-317 (dosmagor).
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_sgs(Dev d) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, nz = d.nz, i = s.i;
+  const double dz = inst(d, I_DZ, c), dx = d.dx;
+  const double* adz = prof(d, P_ADZ, c);
+  const double* adzw = prof(d, P_ADZW, c);
+  const double* bet = prof(d, P_BET, c);
+  const double* presi = prof(d, P_PRESI, c);
+  const double* u0 = prof(d, P_U0, c);
+  const double* grdf = prof(d, P_GRDFX, c);
+  const double Cs = R4(0.15), Ck = R4(0.1), Pr = 1.;
+  const double Ce = Ck * Ck * Ck / (Cs * Cs * Cs * Cs);
+  const double rdx0 = 1. / dx;
+  auto F = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto W = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
+  // window: u at k-1,k,k+1 and w at k,k+1
+  double um = 0, u = F(d.u, 0), up = F(d.u, 1);
+  double w0 = W(0), w1 = W(1);
+  double bsv_prev = 0.0;
+  // thermodynamics at k (carried) and k+1
+  double th_tabs = F(d.tabs, 0), th_qcl = F(d.qcl, 0), th_qci = F(d.qci, 0), th_qv = F(d.qv, 0);
+  double th_qpl = F(d.qpl, 0), th_qpi = F(d.qpi, 0), th_t = F(d.t, 0), th_tkh = F(d.tkh, 0);
+  for (int k = 0; k < nzm - 1; k++) {
+    int kc = k + 1, kb = k - 1;
+    double up_next = (kc + 1 < nzm) ? F(d.u, kc + 1) : 0.0;
+    double w2 = W(kc + 1);
+    // shear production def2 at level k
+    double rdz = 1. / (dz * adz[k]);
+    double rdx = rdx0 * sqrt(dx * rdz);
+    double u_ip = s.rot(u, 1);
+    double du = (u_ip - u) * rdx;
+    double dwz = (w1 - w0) * rdz;
+    double def2 = 2. * (du * du + dwz * dwz);
+    double w0m = s.rot(w0, -1), w0p = s.rot(w0, 1), w1m = s.rot(w1, -1), w1p = s.rot(w1, 1);
+    double up_ip = s.rot(up, 1);
+    {
+      double rdzw_up = 1. / (dz * adzw[kc]);
+      double rdx_up = rdx0 * sqrt(dx * rdzw_up);
+      double a1 = (up_ip - u0[kc] - u_ip + u0[k]) * rdzw_up + (w1p - w1) * rdx_up;
+      double a2 = (up - u0[kc] - u + u0[k]) * rdzw_up + (w1 - w1m) * rdx_up;
+      if (k >= 1) {
+        double rdzw_dn = 1. / (dz * adzw[k]);
+        double rdx_dn = rdx0 * sqrt(dx * rdzw_dn);
+        double um_ip = s.rot(um, 1);
+        double a3 = (u_ip - u0[k] - um_ip + u0[kb]) * rdzw_dn + (w0p - w0) * rdx_dn;
+        double a4 = (u - u0[k] - um + u0[kb]) * rdzw_dn + (w0 - w0m) * rdx_dn;
+        def2 = def2 + 0.5 * (a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4);
+      } else {
+        def2 = def2 + 0.5 * (a1 * a1 + a2 * a2);
+      }
+    }
+    // SGS buoyancy flux at the interface between k and k+1 (tke_full.F90:120-237)
+    double n_tabs = F(d.tabs, kc), n_qcl = F(d.qcl, kc), n_qci = F(d.qci, kc), n_qv = F(d.qv, kc);
+    double n_qpl = F(d.qpl, kc), n_qpi = F(d.qpi, kc), n_t = F(d.t, kc), n_tkh = F(d.tkh, kc);
+    double bsv;
+    {
+      double betdz = 0.5 * (bet[kc] + bet[k]) / dz / adzw[k + 1];
+      double tabs_if = 0.5 * (n_tabs + FAC_COND * n_qcl + FAC_SUB * n_qci + th_tabs + FAC_COND * th_qcl + FAC_SUB * th_qci);
+      double qtot_if = 0.5 * (n_qv + n_qcl + n_qci + th_qv + th_qcl + th_qci);
+      double qp_if = 0.5 * (n_qpl + n_qpi + th_qpl + th_qpi);
+      double bbb = 1. + EPSV * qtot_if - qp_if;
+      double dtt = n_t - th_t;
+      double dqt = n_qv + n_qcl + n_qci - th_qv - th_qcl - th_qci;
+      bsv = betdz * (bbb * dtt + EPSV * tabs_if * dqt + (bbb * FAC_COND - tabs_if) * (n_qpl - th_qpl) +
+                     (bbb * FAC_SUB - tabs_if) * (n_qpi - th_qpi));
+      double qctot = n_qcl + n_qci + th_qcl + th_qci;
+      if (qctot > 0.) {
+        double omn = (n_qcl + th_qcl) / (qctot + R4(1.e-20));
+        double pp = presi[k + 1];
+        double qsat_check = omn * qsatw(tabs_if, pp) + (1. - omn) * qsati(tabs_if, pp);
+        if (qtot_if > qsat_check) {
+          double lstarn = FAC_COND + (1. - omn) * FAC_FUS;
+          double tif = 0.5 * (n_tabs + th_tabs);
+          double dqsat = omn * dtqsatw(tif, pp) + (1. - omn) * dtqsati(tif, pp);
+          double qsatt = omn * qsatw(tif, pp) + (1. - omn) * qsati(tif, pp);
+          double b2 = 1. + EPSV * qsatt + qsatt - qtot_if - qp_if + R4(1.61) * tif * dqsat;
+          b2 = b2 / (1. + lstarn * dqsat);
+          double tk_ = th_tabs;
+          bsv = betdz * (b2 * dtt + (b2 * lstarn - (1. + lstarn * dqsat) * tif) * dqt +
+                         (b2 * FAC_COND - (1. + FAC_COND * dqsat) * tk_) * (n_qpl - th_qpl) +
+                         (b2 * FAC_SUB - (1. + FAC_SUB * dqsat) * tk_) * (n_qpi - th_qpi));
+        }
+      }
+    }
+    (void)th_tkh;
+    (void)n_tkh;
+    // eddy viscosity at level k (tke_full.F90:246-308)
+    {
+      double grd = dz * adz[k];
+      double Ce1 = Ce / R4(0.7) * R4(0.19);
+      double Ce2 = Ce / R4(0.7) * R4(0.51);
+      double cx = dx * dx / d.dt / grdf[k];
+      double dzm = dz * fmin(adzw[k], adzw[k + 1]);
+      double cz = dzm * dzm / d.dt / 1.0;
+      double tkmax = R4(0.09) / (1. / cx + 1. / cx + 1. / cz);
+      double buoy = 0.5 * (bsv_prev + bsv);
+      size_t o = ((size_t)c * nzm + k) * NX + i;
+      double smix;
+      if (buoy <= 0.)
+        smix = grd;
+      else
+        smix = fmin(grd, fmax(R4(0.1) * grd, sqrt(R4(0.76) * d.tk[o] / Ck / sqrt(buoy + R4(1.e-10)))));
+      double ratio = smix / grd;
+      double Cee = Ce1 + Ce2 * ratio;
+      double tk = sqrt(Ck * Ck * Ck / Cee * fmax(0.0, def2 - Pr * buoy)) * (smix * smix);
+      double tt = tk / (Ck * smix);
+      tk = fmin(tk, tkmax);
+      if (live) {
+        d.tke[o] = tt * tt;
+        d.tk[o] = tk;
+        d.tkh[o] = Pr * tk;
+      }
+    }
+    bsv_prev = bsv;
+    th_tabs = n_tabs;
+    th_qcl = n_qcl;
+    th_qci = n_qci;
+    th_qv = n_qv;
+    th_qpl = n_qpl;
+    th_qpi = n_qpi;
+    th_t = n_t;
+    th_tkh = n_tkh;
+    um = u;
+    u = up;
+    up = up_next;
+    w0 = w1;
+    w1 = w2;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Momentum advection and SGS momentum diffusion: advect2_mom_xy.F90:60-84,
+// advect2_mom_z.F90:7-118, diffuse_mom2D.F90:7-161.  Each level's four contributions are
+// added in the reference's order.
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_mom(Dev d, int na) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, nz = d.nz, i = s.i;
+  const double dz = inst(d, I_DZ, c), dx = d.dx;
+  const double* adz = prof(d, P_ADZ, c);
+  const double* adzw = prof(d, P_ADZW, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  const double* grdf = prof(d, P_GRDFX, c);
+  double* dudt = d.dudt[na];
+  double* dwdt = d.dwdt[na];
+  const double dx25 = 0.25 / dx, dz25 = 1. / (4. * dz);
+  const double rdx2 = 1. / dx / dx, rdx25 = 0.25 * rdx2;
+  const double rdz = 1. / dz, dzx = dz / dx;
+  auto F = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto W = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
+  double u_k = F(d.u, 0), u_n = F(d.u, 1);
+  double w_k = W(0), w_n = W(1);
+  double tk_k = F(d.tk, 0), tk_n = F(d.tk, 1);
+  double fuz_k = 0.0;    // fuz at level k (0 at the surface)
+  double fwz_prev = 0.0; // fwz(k-1)
+  double fuv_k = 0.0;    // vertical SGS u flux at the bottom of level k
+  double fwv_k = 0.0;    // vertical SGS w flux at w level k
+  double pend_adv = 0.0, pend_dif = 0.0;  // horizontal contributions for w level k
+  double uwsb_sum = 0.0;
+  for (int k = 0; k < nzm; k++) {
+    int kn = k + 1;
+    bool has_n = kn < nzm;
+    int kcu = has_n ? kn : k;
+    double u_nn = (kn + 1 < nzm) ? F(d.u, kn + 1) : 0.0;
+    double w_nn = (kn + 1 < nz) ? W(kn + 1) : 0.0;
+    double tk_nn = (kn + 1 < nzm) ? F(d.tk, kn + 1) : 0.0;
+    double u_m = s.rot(u_k, -1), u_p = s.rot(u_k, 1);
+    double ucu = has_n ? u_n : u_k, ucu_p = s.rot(ucu, 1);
+    double wn_m = s.rot(w_n, -1), wn_p = s.rot(w_n, 1);
+    double tkcu = has_n ? tk_n : tk_k;
+    double tk_p = s.rot(tk_k, 1), tkcu_p = s.rot(tkcu, 1);
+    double tk_m = s.rot(tk_k, -1), tkn_m = s.rot(tk_n, -1);
+    // ---- dudt(k)
+    size_t o = ((size_t)c * nzm + k) * NX + i;
+    double du = dudt[o];
+    double fu1 = dx25 * (u_k + u_m) * (u_m + u_k);
+    double fu2 = dx25 * (u_p + u_k) * (u_k + u_p);
+    du = du - (fu2 - fu1);
+    double fuz_n = 0.0;
+    if (has_n) {
+      double rhoi = dz25 * rhow[kn];
+      double www = rhoi * (w_n + wn_m);
+      fuz_n = www * (u_n + u_k);
+    }
+    double rhoi_k = 1. / (rho[k] * adz[k]);
+    du = du - (fuz_n - fuz_k) * rhoi_k;
+    double fuh = -2. * (rdx2 * grdf[k] * tk_k) * (u_p - u_k);
+    double fuh_m = s.rot(fuh, -1);
+    du = du - (fuh - fuh_m);
+    // vertical SGS fluxes at the top of level k (w level k+1)
+    double fuv_n = 0.0, fwv_n;
+    if (has_n) {
+      double rdz2 = rdz * rdz * 1.0;
+      double rdz25 = 0.25 * rdz2;
+      double iadz = 1. / adz[k], iadzw = 1. / adzw[kn];
+      double tkz = rdz2 * tk_k;
+      fwv_n = -2. * tkz * (w_n - w_k) * rho[k] * iadz;
+      tkz = rdz25 * (tk_k + tk_m + tk_n + tkn_m);
+      fuv_n = -tkz * ((u_n - u_k) * iadzw + (w_n - wn_m) * dzx) * rhow[kn];
+      uwsb_sum = s.sum(fuv_n);
+      if (live && i == 0) prof(d, P_UWSB, c)[kn] = uwsb_sum;
+    } else {
+      double rdz2 = rdz * rdz * 1.0;
+      double tkz = rdz2 * 1.0 * tk_k;
+      fwv_n = -2. * tkz * (w_n - w_k) / adz[k] * rho[k];
+    }
+    du = du - (fuv_n - fuv_k) * rhoi_k;
+    if (live) dudt[o] = du;
+    double fuz_sum = s.sum(fuz_n);
+    if (has_n && live && i == 0) prof(d, P_UWLE, c)[kn] = fuz_sum;
+    // ---- dwdt(k) for w levels 1..nzm-1
+    double fwz_k = dz25 * (w_n * rhow[kn] + w_k * rhow[k]) * (w_n + w_k);
+    if (k >= 1) {
+      size_t ow = ((size_t)c * nz + k) * NX + i;
+      double dw = dwdt[ow];
+      double rhoiw = 1. / (rhow[k] * adzw[k]);
+      dw = dw - pend_adv;
+      dw = dw - (fwz_k - fwz_prev) * rhoiw;
+      dw = dw - pend_dif;
+      dw = dw - (fwv_n - fwv_k) * rhoiw;
+      if (live) dwdt[ow] = dw;
+    }
+    // ---- horizontal contributions to w level k+1 (applied next iteration)
+    {
+      double irho = 1. / (rhow[kn] * adzw[kn]);
+      double fw1 = dx25 * (u_k * rho[k] * adz[k] + ucu * rho[kcu] * adz[kcu]) * (wn_m + w_n);
+      double fw2 = dx25 * (u_p * rho[k] * adz[k] + ucu_p * rho[kcu] * adz[kcu]) * (w_n + wn_p);
+      pend_adv = irho * (fw2 - fw1);
+      double dxz = dx / (dz * adzw[kn]);
+      double tkx = rdx25 * grdf[k] * (tk_k + tk_p + tkcu + tkcu_p);
+      double fwh = -tkx * (wn_p - w_n + (ucu_p - u_p) * dxz);
+      double fwh_m = s.rot(fwh, -1);
+      pend_dif = fwh - fwh_m;
+    }
+    fuz_k = fuz_n;
+    fwz_prev = fwz_k;
+    fuv_k = fuv_n;
+    fwv_k = fwv_n;
+    u_k = u_n;
+    u_n = u_nn;
+    w_k = w_n;
+    w_n = w_nn;
+    tk_k = tk_n;
+    tk_n = tk_nn;
+  }
+  // lid (w level nzm): horizontal contributions only
+  if (live) {
+    size_t ow = ((size_t)c * nz + nzm) * NX + i;
+    double dw = dwdt[ow];
+    dw = dw - pend_adv;
+    dw = dw - pend_dif;
+    dwdt[ow] = dw;
+    if (i == 0) {
+      prof(d, P_UWLE, c)[0] = 0.0;
+      prof(d, P_UWSB, c)[0] = 0.0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Pressure: press_rhs.F90:8-106, the fft991 x-transform as dense operators (see
+// fft991_operator.hpp), the vertical Thomas sweep of pressure.F90:160-237, the inverse
+// transform, press_grad.F90:8-76, then adams.F90:7-40.  4 waves share the 64/NX instances of
+// one wave; levels are dealt round-robin to the waves; the slab lives in LDS; the Thomas
+// sweep runs on wave 0 with its factor in registers (NZM compile-time).
+// ------------------------------------------------------------------------------------------
+template <int NX, int NZM>
+__global__ void __launch_bounds__(256) k_press(Dev d, int na, int nb, int nc) {
+  extern __shared__ double slab[];  // [64/NX instances][NZM][NX] == [NZM][64] per lane column
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = lane & (NX - 1);
+  const int cg = blockIdx.x * (64 / NX) + lane / NX;
+  const bool live = cg < d.ncrms;
+  const int c = live ? cg : d.ncrms - 1;
+  Seg<NX> s;
+  const int nz = NZM + 1;
+  const double dz = inst(d, I_DZ, c);
+  const double* adz = prof(d, P_ADZ, c);
+  const double* adzw = prof(d, P_ADZW, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  auto P = [&](int k, int col) -> double& { return slab[(size_t)k * 64 + (lane - i) + col]; };
+  const double rdx = 1. / d.dx, btat = d.bt / d.at, ctat = d.ct / d.at;
+  const double dta = 1. / d.dt3na / d.at;
+  const double *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
+  const double *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
+  // 1. right-hand side
+  for (int k = wave; k < NZM; k += 4) {
+    size_t o = ((size_t)c * NZM + k) * NX + i;
+    size_t ow = ((size_t)c * nz + k) * NX + i;
+    size_t own = ow + NX;
+    double rdz = 1. / (adz[k] * dz);
+    double rup = rhow[k + 1] / rho[k] * rdz;
+    double rdn = rhow[k] / rho[k] * rdz;
+    double uu = d.u[o], uap = ua[o], ubp = ub[o], ucp = uc[o];
+    double pv = (rdx * (s.rot(uu, 1) - uu) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
+                (rdx * (s.rot(uap, 1) - uap) + (wa[own] * rup - wa[ow] * rdn)) +
+                btat * (rdx * (s.rot(ubp, 1) - ubp) + (wb[own] * rup - wb[ow] * rdn)) +
+                ctat * (rdx * (s.rot(ucp, 1) - ucp) + (wc[own] * rup - wc[ow] * rdn));
+    P(k, i) = pv * rho[k];
+  }
+  __syncthreads();
+  // 2. forward transform: slot m of level k (every lane reads the whole row before any
+  // lane overwrites its entry, hence the barrier between the sum and the store)
+  for (int k0 = 0; k0 < NZM; k0 += 4) {
+    const int k = k0 + wave;
+    double acc = 0.0;
+    if (k < NZM) {
+#pragma unroll 8
+      for (int q = 0; q < NX; q++) acc += d.Ft[q * NX + i] * P(k, q);
+    }
+    __syncthreads();
+    if (k < NZM) P(k, i) = acc;
+  }
+  __syncthreads();
+  // 3. Thomas sweep per slot (wave 0)
+  if (wave == 0) {
+    const double eig = d.eign[i];
+    double alfa[NZM];
+    double dz2 = dz * dz;
+    auto acoef = [&](int k) { return rhow[k] / (adz[k] * adzw[k] * dz * dz); };
+    auto ccoef = [&](int k) { return rhow[k + 1] / (adz[k] * adzw[k + 1] * dz * dz); };
+    (void)dz2;
+    double b;
+    if (i == 0)
+      b = 1.0 / (eig * rho[0] - acoef(0) - ccoef(0));
+    else
+      b = 1.0 / (eig * rho[0] - ccoef(0));
+    alfa[0] = -ccoef(0) * b;
+    double beta = P(0, i) * b;
+    P(0, i) = beta;
+#pragma unroll
+    for (int k = 1; k < NZM - 1; k++) {
+      double ak = acoef(k), ck = ccoef(k);
+      double e = 1.0 / (eig * rho[k] - ak - ck + ak * alfa[k - 1]);
+      alfa[k] = -ck * e;
+      beta = (P(k, i) - ak * beta) * e;
+      P(k, i) = beta;
+    }
+    {
+      int k = NZM - 1;
+      double ak = acoef(k);
+      double top = (P(k, i) - ak * beta) / (eig * rho[k] - ak + ak * alfa[k - 1]);
+      P(k, i) = top;
+#pragma unroll
+      for (int kk = NZM - 2; kk >= 0; kk--) {
+        top = alfa[kk] * top + P(kk, i);
+        P(kk, i) = top;
+      }
+    }
+  }
+  __syncthreads();
+  // 4. inverse transform (same read-all-then-store pattern)
+  for (int k0 = 0; k0 < NZM; k0 += 4) {
+    const int k = k0 + wave;
+    double acc = 0.0;
+    if (k < NZM) {
+#pragma unroll 8
+      for (int q = 0; q < NX; q++) acc += d.Gt[q * NX + i] * P(k, q);
+    }
+    __syncthreads();
+    if (k < NZM) P(k, i) = acc;
+  }
+  __syncthreads();
+  // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities
+  const double dtdx = d.dtn / d.dx, dtdz = d.dtn / dz, dt3 = d.dt3na;
+  double* uan = d.dudt[na];
+  double* wan = d.dwdt[na];
+  double* unc = d.dudt[nc];
+  double* wnc = d.dwdt[nc];
+  for (int k = wave; k < NZM; k += 4) {
+    size_t o = ((size_t)c * NZM + k) * NX + i;
+    size_t ow = ((size_t)c * nz + k) * NX + i;
+    double pk = P(k, i);
+    double pim = s.rot(pk, -1);
+    double pkb = P(k > 0 ? k - 1 : 0, i);
+    double rdz = 1. / (dz * adzw[k]);
+    double du_a = ua[o] - (pk - pim) * rdx;
+    double dw_a = wa[ow] - (pk - pkb) * rdz;
+    double uu = d.u[o], ww = d.w[ow];
+    double un = uu + dt3 * (d.at * du_a + d.bt * ub[o] + d.ct * uc[o]);
+    double wn = ww + dt3 * (d.at * dw_a + d.bt * wb[ow] + d.ct * wc[ow]);
+    if (live) {
+      uan[o] = du_a;
+      wan[ow] = dw_a;
+      unc[o] = un;
+      wnc[ow] = wn;
+      d.u[o] = 0.5 * (uu + un) * (rho[k] * dtdx);
+      d.w[ow] = 0.5 * (ww + wn) * (rhow[k] * dtdz);
+    }
+  }
+  if (wave == 0 && live) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// MPDATA scalar advection: ADV_MPDATA/advect_scalar2D.F90:8-244 with the statistics of
+// advect_scalar.F90:9-76.  Software-pipelined over levels: stage s processes level K-s.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double andiff(double x1, double x2, double a, double b) {
+  return (fabs(a) - a * a * b) * 0.5 * (x2 - x1);
+}
+__device__ __forceinline__ double across(double x1, double a1, double a2) { return 0.03125 * a1 * a2 * x1; }
+__device__ __forceinline__ double ppos(double y) { return fmax(0.0, y); }
+__device__ __forceinline__ double pneg(double y) { return -fmin(0.0, y); }
+
+// terminal velocity of precipitation, microphysics.F90:453-486
+__device__ __forceinline__ double term_vel(const TermVel& tv, double q, double rho, double tabs) {
+  double v = 0.;
+  if (q > QP_THRESHOLD) {
+    double omp = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * A_PR()));
+    if (omp == 1.) {
+      v = tv.vrain * pow(rho * q, tv.crain);
+    } else if (omp == 0.) {
+      double omg = fmax(0.0, fmin(1.0, (tabs - TGRMIN) * A_GR()));
+      double qgg = omg * q, qss = q - qgg;
+      v = (omg * tv.vgrau * pow(rho * qgg, tv.cgrau) + (1. - omg) * tv.vsnow * pow(rho * qss, tv.csnow));
+    } else {
+      double omg = fmax(0.0, fmin(1.0, (tabs - TGRMIN) * A_GR()));
+      double qrr = omp * q, qss = q - qrr, qgg = omg * qss;
+      qss = qss - qgg;
+      v = (omp * tv.vrain * pow(rho * qrr, tv.crain) +
+           (1. - omp) * (omg * tv.vgrau * pow(rho * qgg, tv.cgrau) + (1. - omg) * tv.vsnow * pow(rho * qss, tv.csnow)));
+    }
+  }
+  return v;
+}
+
+
+// One scalar of MPDATA, software-pipelined over levels.  At iteration K stage s works on
+// level K-s; ring variables are named by how far behind the front they are at the top of the
+// iteration.  Statistics: fadv (advect_scalar.F90:62-72) and the vertical flux of
+// advect_scalar2D.F90:113-118,204-206.  With want_cfl, also the precipitation Courant number
+// of precip_fall (microphysics.F90:596-611) on the advected field.
+template <int NX>
+__device__ void mpdata_one(const Dev& d, const Seg<NX>& s, int c, bool live, double* f, double* fadv,
+                           double* flux, bool want_cfl) {
+  const int nzm = d.nzm, nz = d.nz, i = s.i;
+  const double* adz = prof(d, P_ADZ, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  const double eps = R4(1.e-10);
+  auto F = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto Wv = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
+  const double dz = inst(d, I_DZ, c);
+  double pcfl = 0.0;
+  double f_0 = 0, f_1 = 0, f_2 = 0;
+  double U_0 = 0, U_1 = 0, U_2 = 0, U_3 = 0;
+  double W_0 = 0, W_1 = 0, W_2 = 0;
+  double uu1_0 = 0, uu1_1 = 0, ww1_0 = 0, ww1_1 = 0;
+  double f1_a = 0, f1_b = 0, f1_c = 0, f1_d = 0;
+  double mx0_a = 0, mn0_a = 0;
+  double uu2_a = 0, uu2_b = 0, ww2_a = 0, ww2_b = 0;
+  double mx1_a = 0, mn1_a = 0;
+  double mx_a = 0, mx_b = 0, mn_a = 0, mn_b = 0;
+  double uu3_a = 0, ww3_a = 0;
+  for (int K = 0; K <= nzm + 4; K++) {
+    // stage 0: donor-cell fluxes at level K
+    if (K < nzm) {
+      f_0 = F(f, K);
+      U_0 = F(d.u, K);
+      W_0 = Wv(K);
+      double fkb = (K >= 1) ? f_1 : f_0;
+      uu1_0 = fmax(0.0, U_0) * s.rot(f_0, -1) + fmin(0.0, U_0) * f_0;
+      ww1_0 = fmax(0.0, W_0) * fkb + fmin(0.0, W_0) * f_0;
+    } else {
+      f_0 = U_0 = W_0 = uu1_0 = ww1_0 = 0.0;
+    }
+    // stage 1: donor-cell update and first bounds at level K-1
+    const int L1 = K - 1;
+    double f1_new = 0, mx0_new = 0, mn0_new = 0;
+    if (L1 >= 0 && L1 < nzm) {
+      double fkb = (L1 >= 1) ? f_2 : f_1, fkc = (L1 + 1 < nzm) ? f_0 : f_1;
+      f1_new = f_1 - (s.rot(uu1_1, 1) - uu1_1 + (ww1_0 - ww1_1) * (1. / adz[L1])) * (1. / rho[L1]);
+      double fm = s.rot(f_1, -1), fp = s.rot(f_1, 1);
+      mx0_new = fmax(fmax(fmax(fmax(fm, fp), fkb), fkc), f_1);
+      mn0_new = fmin(fmin(fmin(fmin(fm, fp), fkb), fkc), f_1);
+      double s1 = s.sum(ww1_1);
+      if (flux && live && i == 0) flux[L1] = s1;
+    }
+    // stage 2: antidiffusive velocities and second bounds at level K-2
+    const int L2 = K - 2;
+    double uu2_new = 0, ww2_new = 0, mx1_new = 0, mn1_new = 0;
+    if (L2 >= 0 && L2 < nzm) {
+      int kc = (L2 + 1 < nzm) ? L2 + 1 : L2, kb = (L2 >= 1) ? L2 - 1 : 0;
+      double f1L = f1_a;
+      double f1c = (kc == L2) ? f1_a : f1_new;
+      double f1b = (kb == L2) ? f1_a : f1_b;
+      double Uc = U_2, Ub = (kb == L2) ? U_2 : U_3;
+      double Wc = W_2, Wkc = (kc == L2) ? W_2 : W_1;
+      double irho = 1. / rho[L2], irhow = 1. / (rhow[L2] * adz[L2]);
+      double dd = (double)(2.0f / (float)(kc - kb)) / adz[L2];
+      double f1L_m = s.rot(f1L, -1), f1L_p = s.rot(f1L, 1);
+      double f1c_m = s.rot(f1c, -1), f1b_m = s.rot(f1b, -1), f1b_p = s.rot(f1b, 1);
+      double Wc_m = s.rot(Wc, -1), Wkc_m = s.rot(Wkc, -1);
+      double Uc_p = s.rot(Uc, 1), Ub_p = s.rot(Ub, 1);
+      uu2_new = andiff(f1L_m, f1L, Uc, irho) -
+                across(dd * (f1c_m + f1c - f1b_m - f1b), Uc, Wc_m + Wkc_m + Wc + Wkc) * irho;
+      ww2_new = andiff(f1b, f1L, Wc, irhow) -
+                across(f1b_p + f1L_p - f1b_m - f1L_m, Wc, Ub + Uc + Uc_p + Ub_p) * irho;
+      if (L2 == 0) ww2_new = 0.;
+      mx1_new = fmax(fmax(fmax(fmax(fmax(f1L_m, f1L_p), f1b), f1c), f1L), mx0_a);
+      mn1_new = fmin(fmin(fmin(fmin(fmin(f1L_m, f1L_p), f1b), f1c), f1L), mn0_a);
+    }
+    // stage 3: normalised limiters at level K-3
+    const int L3 = K - 3;
+    double mx_new = 0, mn_new = 0;
+    if (L3 >= 0 && L3 < nzm) {
+      double uu2p = s.rot(uu2_a, 1);
+      double ww2c = (L3 + 1 < nzm) ? ww2_new : ww2_a;
+      double iadz = 1. / adz[L3];
+      mx_new = rho[L3] * (mx1_a - f1_b) / (pneg(uu2p) + ppos(uu2_a) + iadz * (pneg(ww2c) + ppos(ww2_a)) + eps);
+      mn_new = rho[L3] * (f1_b - mn1_a) / (ppos(uu2p) + pneg(uu2_a) + iadz * (ppos(ww2c) + pneg(ww2_a)) + eps);
+    }
+    // stage 4: limited antidiffusive fluxes at level K-4
+    const int L4 = K - 4;
+    double uu3_new = 0, ww3_new = 0;
+    if (L4 >= 0 && L4 < nzm) {
+      double mxm = s.rot(mx_a, -1), mnm = s.rot(mn_a, -1);
+      double mxb = (L4 >= 1) ? mx_b : mx_a, mnb = (L4 >= 1) ? mn_b : mn_a;
+      uu3_new = ppos(uu2_b) * fmin(fmin(1.0, mx_a), mnm) - pneg(uu2_b) * fmin(fmin(1.0, mxm), mn_a);
+      ww3_new = ppos(ww2_b) * fmin(fmin(1.0, mx_a), mnb) - pneg(ww2_b) * fmin(fmin(1.0, mxb), mn_a);
+      double s3 = s.sum(ww3_new);
+      if (flux && live && i == 0) flux[L4] = flux[L4] + s3;
+    }
+    // stage 5: corrected update at level K-5
+    const int L5 = K - 5;
+    if (L5 >= 0 && L5 < nzm) {
+      double ww3up = (L5 + 1 < nzm) ? ww3_new : 0.0;
+      double fn = fmax(0.0, f1_d - (s.rot(uu3_a, 1) - uu3_a + (ww3up - ww3_a) * (1. / adz[L5])) * (1. / rho[L5]));
+      size_t o = ((size_t)c * nzm + L5) * NX + i;
+      double forig = f[o];
+      double adv = s.sum(fn - forig);
+      if (live) {
+        f[o] = fn;
+        if (fadv && i == 0) fadv[L5] = adv;
+      }
+      if (want_cfl) {
+        double wmax = dz * adz[L5 >= 1 ? L5 - 1 : 0] / d.dtn;
+        double wp = sqrt(R4(1.29) / rho[L5]) * term_vel(d.tv, fn, rho[L5], d.tabs[o]);
+        pcfl = fmax(pcfl, wp * (1. / wmax));
+      }
+    }
+    f_2 = f_1;
+    f_1 = f_0;
+    U_3 = U_2;
+    U_2 = U_1;
+    U_1 = U_0;
+    W_2 = W_1;
+    W_1 = W_0;
+    uu1_1 = uu1_0;
+    ww1_1 = ww1_0;
+    f1_d = f1_c;
+    f1_c = f1_b;
+    f1_b = f1_a;
+    f1_a = f1_new;
+    mx0_a = mx0_new;
+    mn0_a = mn0_new;
+    uu2_b = uu2_a;
+    uu2_a = uu2_new;
+    ww2_b = ww2_a;
+    ww2_a = ww2_new;
+    mx1_a = mx1_new;
+    mn1_a = mn1_new;
+    mx_b = mx_a;
+    mx_a = mx_new;
+    mn_b = mn_a;
+    mn_a = mn_new;
+    uu3_a = uu3_new;
+    ww3_a = ww3_new;
+  }
+  if (want_cfl) {
+    pcfl = s.max(pcfl);
+    if (live && i == 0) atomic_max_nonneg(&d.gmax[1], pcfl);
+  }
+}
+
+// advect_all_scalars.F90:7-94: t, then the two microphysics fields.
+template <int NX>
+__global__ void __launch_bounds__(256) k_mpdata(Dev d) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  mpdata_one<NX>(d, s, c, live, d.t, nullptr, nullptr, false);
+  mpdata_one<NX>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
+  mpdata_one<NX>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
+}
+
+// ------------------------------------------------------------------------------------------
+// Precipitation sedimentation: precip_fall (microphysics.F90:529-833) with the batch-global
+// sub-step count nprec read from the maximum Courant number gathered in k_mpdata.
+// Per column, one pipelined sweep per sub-step.
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_precip(Dev d) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, i = s.i;
+  const double dz = inst(d, I_DZ, c), dtn = d.dtn;
+  const double* adz = prof(d, P_ADZ, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  double* qpfall = prof(d, P_QPFALL, c);
+  double* precflux = prof(d, P_PRECFLUX, c);
+  const double eps = R4(1.e-10);
+  const double a_pr = A_PR();
+  double prec_cfl = __longlong_as_double((long long)d.gmax[1]);
+  int nprec = 1;
+  if (prec_cfl > R4(0.9)) nprec = (int)ceil(prec_cfl / R4(0.9));
+  const double rnprec = (double)nprec;
+  auto Q = [&](double* p, int k) -> double& { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto lfac_of = [&](int k) {
+    double om = fmax(0.0, fmin(1.0, (Q(d.tabs, k) - TPRMIN) * a_pr));
+    return FAC_COND + (1 - om) * FAC_FUS;
+  };
+  const double omega0 = fmax(0.0, fmin(1.0, (Q(d.tabs, 0) - TPRMIN) * a_pr));
+  for (int iprec = 1; iprec <= nprec; iprec++) {
+    double tq_0 = 0, tq_1 = 0, tq_2 = 0;
+    double wp_0 = 0, wp_1 = 0, wp_2 = 0;
+    double fz0_0 = 0, fz0_1 = 0, fz0_2 = 0;
+    double tq1_a = 0, tq1_b = 0;
+    double mx0_a = 0, mn0_a = 0;
+    double www_a = 0;
+    double mx_a = 0, mn_a = 0;
+    double fz_a = 0;
+    for (int K = 0; K <= nzm + 2; K++) {
+      // a: level K, upwind flux
+      if (K < nzm) {
+        tq_0 = Q(d.qp, K);
+        double v = sqrt(R4(1.29) / rho[K]) * term_vel(d.tv, tq_0, rho[K], Q(d.tabs, K));
+        wp_0 = -v * rhow[K] * dtn / dz / rnprec;
+        fz0_0 = tq_0 * wp_0;
+      } else {
+        tq_0 = wp_0 = fz0_0 = 0.0;
+      }
+      // b: level K-1, upwind update, first bounds, antidiffusive flux
+      const int B = K - 1;
+      double tq1_new = 0, mx0_new = 0, mn0_new = 0, www_new = 0;
+      if (B >= 0 && B < nzm) {
+        double irhoadz = 1. / (rho[B] * adz[B]);
+        tq1_new = tq_1 - (fz0_0 - fz0_1) * irhoadz;
+        double tkb = (B >= 1) ? tq_2 : tq_1, tkc = (B + 1 < nzm) ? tq_0 : tq_1;
+        mx0_new = fmax(fmax(tkb, tkc), tq_1);
+        mn0_new = fmin(fmin(tkb, tkc), tq_1);
+        double t1b = (B >= 1) ? tq1_a : tq1_new, wpb = (B >= 1) ? wp_2 : wp_1;
+        www_new = 0.5 * (1. + wp_1 * irhoadz) * (t1b * wpb - tq1_new * wp_1);
+      }
+      // d: level K-2, limiter and corrected flux
+      const int D = K - 2;
+      double mx_new = 0, mn_new = 0, fz_new = 0;
+      if (D >= 0 && D < nzm) {
+        double t1b = (D >= 1) ? tq1_b : tq1_a, t1c = (D + 1 < nzm) ? tq1_new : tq1_a;
+        double mx1 = fmax(fmax(fmax(t1b, t1c), tq1_a), mx0_a);
+        double mn1 = fmin(fmin(fmin(t1b, t1c), tq1_a), mn0_a);
+        double wwc = (D + 1 < nzm) ? www_new : www_a;
+        double ra = rho[D] * adz[D];
+        mx_new = ra * (mx1 - tq1_a) / (pneg(wwc) + ppos(www_a) + eps);
+        mn_new = ra * (tq1_a - mn1) / (ppos(wwc) + pneg(www_a) + eps);
+        double mxb = (D >= 1) ? mx_a : mx_new, mnb = (D >= 1) ? mn_a : mn_new;
+        fz_new = fz0_2 + ppos(www_a) * fmin(fmin(1.0, mx_new), mnb) - pneg(www_a) * fmin(fmin(1.0, mxb), mn_new);
+      }
+      // f: level K-3, update of qp and t, statistics
+      const int Fl = K - 3;
+      if (Fl >= 0 && Fl < nzm) {
+        double irhoadz = 1. / (rho[Fl] * adz[Fl]);
+        double fzF = fz_a, fzF1 = (Fl + 1 < nzm) ? fz_new : 0.0;
+        double lf = lfac_of(Fl), lf1 = (Fl + 1 < nzm) ? lfac_of(Fl + 1) : 0.0;
+        double dq = -(fzF1 - fzF) * irhoadz;
+        double lat_heat = -(lf1 * fzF1 - lf * fzF) * irhoadz;
+        double sfall = s.sum(dq * 1.);
+        double sflux = s.sum(fzF * 1.);
+        if (live) {
+          Q(d.qp, Fl) = Q(d.qp, Fl) - (fzF1 - fzF) * irhoadz;
+          Q(d.t, Fl) = Q(d.t, Fl) - lat_heat;
+          if (i == 0) {
+            qpfall[Fl] = qpfall[Fl] + sfall;
+            precflux[Fl] = precflux[Fl] - sflux;
+          }
+          if (Fl == 0) {
+            d.precsfc[(size_t)c * NX + i] -= fzF * 1.;
+            d.precssfc[(size_t)c * NX + i] -= fzF * (1. - omega0) * 1.;
+          }
+        }
+      }
+      tq_2 = tq_1;
+      tq_1 = tq_0;
+      wp_2 = wp_1;
+      wp_1 = wp_0;
+      fz0_2 = fz0_1;
+      fz0_1 = fz0_0;
+      tq1_b = tq1_a;
+      tq1_a = tq1_new;
+      mx0_a = mx0_new;
+      mn0_a = mn0_new;
+      www_a = www_new;
+      mx_a = mx_new;
+      mn_a = mn_new;
+      fz_a = fz_new;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// SGS diffusion of t, qt, qp: diffuse_scalar.F90:8-72 and diffuse_scalar2D.F90:5-141 (zero
+// surface and top fluxes in the CRM).
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__device__ void diffuse_one(const Dev& d, const Seg<NX>& s, int c, bool live, double* f, double* fdiff,
+                            double* flux) {
+  const int nzm = d.nzm, i = s.i;
+  const double dz = inst(d, I_DZ, c), dtn = d.dtn;
+  const double* adz = prof(d, P_ADZ, c);
+  const double* adzw = prof(d, P_ADZW, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  const double* grdf = prof(d, P_GRDFX, c);
+  const double rdx2 = 1. / (d.dx * d.dx), rdz2 = 1. / (dz * dz);
+  auto Q = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  double f_k = Q(f, 0), f_n = Q(f, 1), tkh_k = Q(d.tkh, 0), tkh_n = Q(d.tkh, 1);
+  double flx_prev = 0.0 * (1. / dz) * rhow[0];
+  for (int k = 0; k < nzm; k++) {
+    double f_nn = (k + 2 < nzm) ? Q(f, k + 2) : 0.0;
+    double tkh_nn = (k + 2 < nzm) ? Q(d.tkh, k + 2) : 0.0;
+    double rdx5 = 0.5 * rdx2 * grdf[k];
+    double tkx = rdx5 * (tkh_k + s.rot(tkh_k, 1));
+    double flxh = -tkx * (s.rot(f_k, 1) - f_k);
+    double dfh = 0. - (flxh - s.rot(flxh, -1));
+    double flxv;
+    if (k <= nzm - 2) {
+      double rhoi = rhow[k + 1] / adzw[k + 1];
+      double rdz5 = 0.5 * rdz2 * 1.0;
+      double tkz = rdz5 * (tkh_k + tkh_n);
+      flxv = -tkz * (f_n - f_k) * rhoi;
+      double sv = s.sum(flxv);
+      if (flux && live && i == 0) flux[k + 1] = sv;
+    } else {
+      flxv = 0.0 * (1. / dz) * (1. / adzw[nzm]) * rhow[nzm];
+    }
+    double rhoi = 1. / (adz[k] * rho[k]);
+    double dfdt = dtn * (dfh - (flxv - flx_prev) * rhoi);
+    double fnew = f_k + dfdt;
+    double sd = s.sum(fnew - f_k);
+    if (live) {
+      f[((size_t)c * nzm + k) * NX + i] = fnew;
+      if (fdiff && i == 0) fdiff[k] = sd;
+    }
+    flx_prev = flxv;
+    f_k = f_n;
+    f_n = f_nn;
+    tkh_k = tkh_n;
+    tkh_n = tkh_nn;
+  }
+  if (flux && live && i == 0) flux[0] = 0.0;
+}
+
+template <int NX>
+__global__ void __launch_bounds__(256) k_diffuse(Dev d) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  diffuse_one<NX>(d, s, c, live, d.t, nullptr, nullptr);
+  diffuse_one<NX>(d, s, c, live, d.qt, prof(d, P_MKDIFF1, c), prof(d, P_MKWSB1, c));
+  diffuse_one<NX>(d, s, c, live, d.qp, prof(d, P_MKDIFF2, c), prof(d, P_MKWSB2, c));
+}
+
+// ------------------------------------------------------------------------------------------
+// Bulk microphysics coefficients per level: precip_init.F90:47-91 (one thread per level).
+// ------------------------------------------------------------------------------------------
+__global__ void k_precip_init(Dev d) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= d.ncrms * d.nzm) return;
+  int c = idx / d.nzm, k = idx - c * d.nzm;
+  double rho = prof(d, P_RHO, c)[k], t0 = prof(d, P_TABS0, c)[k], pres = prof(d, P_PRES, c)[k];
+  auto C = [&](int q) -> double& { return d.coef[((size_t)q * d.ncrms + c) * d.nz + k]; };
+  const Gam& g = d.gam;
+  double pratio = sqrt(R4(1.29) / rho);
+  double rrr1 = 393. / (t0 + 120.) * pow(t0 / 273., 1.5);
+  double rrr2 = pow(t0 / 273., R4(1.94)) * (1000. / pres);
+  double estw = 100. * esatw(t0);
+  double esti = 100. * esati(t0);
+  double coef1 = 0.25 * PI_R4 * NZEROS * A_SNOW * g.gams1 * pratio / pow(PI_R4 * RHOS * NZEROS / rho, (3 + B_SNOW) / 4.);
+  double coef2 = exp(R4(0.025) * (t0 - R4(273.15)));
+  C(C_ACCRSI) = coef1 * coef2 * ESICOEF;
+  C(C_ACCRSC) = coef1 * 1.0;
+  C(C_COEFICE) = coef2;
+  coef1 = (LSUB / (t0 * RV) - 1.) * LSUB / (THERCO * rrr1 * t0);
+  coef2 = RV * t0 / (DIFFELQ * rrr2 * esti);
+  C(C_EVAPS1) = R4(0.65) * 4. * NZEROS / sqrt(PI_R4 * RHOS * NZEROS) / (coef1 + coef2) / sqrt(rho);
+  C(C_EVAPS2) = R4(0.49) * 4. * NZEROS * g.gams2 * sqrt(A_SNOW / (MUELQ * rrr1)) / pow(PI_R4 * RHOS * NZEROS, (5 + B_SNOW) / 8.) /
+                (coef1 + coef2) * pow(rho, (1 + B_SNOW) / 8.) * sqrt(pratio);
+  coef1 = 0.25 * PI_R4 * NZEROG * A_GRAU * g.gamg1 * pratio / pow(PI_R4 * RHOG * NZEROG / rho, (3 + B_GRAU) / 4.);
+  coef2 = exp(R4(0.025) * (t0 - R4(273.15)));
+  C(C_ACCRGI) = coef1 * coef2 * EGICOEF;
+  C(C_ACCRGC) = coef1 * 1.0;
+  coef1 = (LSUB / (t0 * RV) - 1.) * LSUB / (THERCO * rrr1 * t0);
+  coef2 = RV * t0 / (DIFFELQ * rrr2 * esti);
+  C(C_EVAPG1) = R4(0.65) * 4. * NZEROG / sqrt(PI_R4 * RHOG * NZEROG) / (coef1 + coef2) / sqrt(rho);
+  C(C_EVAPG2) = R4(0.49) * 4. * NZEROG * g.gamg2 * sqrt(A_GRAU / (MUELQ * rrr1)) / pow(PI_R4 * RHOG * NZEROG, (5 + B_GRAU) / 8.) /
+                (coef1 + coef2) * pow(rho, (1 + B_GRAU) / 8.) * sqrt(pratio);
+  C(C_ACCRRC) = 0.25 * PI_R4 * NZEROR * A_RAIN * g.gamr1 * pratio / pow(PI_R4 * RHOR * NZEROR / rho, (3 + B_RAIN) / 4.) * 1.0;
+  coef1 = (LCOND / (t0 * RV) - 1.) * LCOND / (THERCO * rrr1 * t0);
+  coef2 = RV * t0 / (DIFFELQ * rrr2 * estw);
+  C(C_EVAPR1) = R4(0.78) * 2. * PI_R4 * NZEROR / sqrt(PI_R4 * RHOR * NZEROR) / (coef1 + coef2) / sqrt(rho);
+  C(C_EVAPR2) = R4(0.31) * 2. * PI_R4 * NZEROR * g.gamr2 * R4(0.89) * sqrt(A_RAIN / (MUELQ * rrr1)) /
+                pow(PI_R4 * RHOR * NZEROR, (5 + B_RAIN) / 8.) / (coef1 + coef2) * pow(rho, (1 + B_RAIN) / 8.) * sqrt(pratio);
+}
+
+// ------------------------------------------------------------------------------------------
+// Saturation adjustment (cloud.F90:7-114), precipitation processes (precip_proc.F90:7-151),
+// micro_diagnose (microphysics.F90:331-357) and diagnose (diagnose.F90:8-101), pointwise
+// per level followed by the level means.  Also finds the cloud-ice level range for the next
+// ice fall (ice_fall.F90:34-46).
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_micro(Dev d) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, i = s.i;
+  const double dtn = d.dtn, factor = 1. / (double)NX;
+  const double* gamaz = prof(d, P_GAMAZ, c);
+  const double* presp = prof(d, P_PRES, c);
+  const double an = 1. / (TBGMAX - TBGMIN), bn = TBGMIN * an;
+  const double ap = 1. / (TPRMAX - TPRMIN), bp = TPRMIN * ap;
+  const double fac1 = FAC_COND + (1 + bp) * FAC_FUS, fac2 = FAC_FUS * ap;
+  const double a_bg = A_BG(), a_pr = A_PR(), a_gr = A_GR();
+  const double powr1 = (3 + B_RAIN) / 4., powr2 = (5 + B_RAIN) / 8.;
+  const double pows1 = (3 + B_SNOW) / 4., pows2 = (5 + B_SNOW) / 8.;
+  const double powg1 = (3 + B_GRAU) / 4., powg2 = (5 + B_GRAU) / 8.;
+  int kmin = nzm, kmax = -1;
+  for (int k = 0; k < nzm; k++) {
+    size_t o = ((size_t)c * nzm + k) * NX + i;
+    auto CO = [&](int q) { return d.coef[((size_t)q * d.ncrms + c) * d.nz + k]; };
+    const double pres = presp[k];
+    double t = d.t[o];
+    double q = fmax(0.0, d.qt[o]);
+    double qp = d.qp[o];
+    // cloud(): saturation adjustment with Newton iterations
+    double tabs = t - gamaz[k];
+    double tabs1 = (tabs + fac1 * qp) / (1. + fac2 * qp);
+    double qsatt, om;
+    if (tabs1 >= TBGMAX) {
+      tabs1 = tabs + FAC_COND * qp;
+      qsatt = qsatw(tabs1, pres);
+    } else if (tabs1 <= TBGMIN) {
+      tabs1 = tabs + FAC_SUB * qp;
+      qsatt = qsati(tabs1, pres);
+    } else {
+      om = an * tabs1 - bn;
+      qsatt = om * qsatw(tabs1, pres) + (1. - om) * qsati(tabs1, pres);
+    }
+    double qn;
+    if (q > qsatt) {
+      int niter = 0;
+      double dtabs = 100., dqsat = 0.;
+      while (fabs(dtabs) > R4(0.01) && niter < 10) {
+        double lstarn, dlstarn, lstarp, dlstarp;
+        if (tabs1 >= TBGMAX) {
+          lstarn = FAC_COND;
+          dlstarn = 0.;
+          qsatt = qsatw(tabs1, pres);
+          dqsat = dtqsatw(tabs1, pres);
+        } else if (tabs1 <= TBGMIN) {
+          lstarn = FAC_SUB;
+          dlstarn = 0.;
+          qsatt = qsati(tabs1, pres);
+          dqsat = dtqsati(tabs1, pres);
+        } else {
+          om = an * tabs1 - bn;
+          lstarn = FAC_COND + (1. - om) * FAC_FUS;
+          dlstarn = an * FAC_FUS;
+          qsatt = om * qsatw(tabs1, pres) + (1. - om) * qsati(tabs1, pres);
+          dqsat = om * dtqsatw(tabs1, pres) + (1. - om) * dtqsati(tabs1, pres);
+        }
+        if (tabs1 >= TPRMAX) {
+          lstarp = FAC_COND;
+          dlstarp = 0.;
+        } else if (tabs1 <= TPRMIN) {
+          lstarp = FAC_SUB;
+          dlstarp = 0.;
+        } else {
+          double omp = ap * tabs1 - bp;
+          lstarp = FAC_COND + (1. - omp) * FAC_FUS;
+          dlstarp = ap * FAC_FUS;
+        }
+        double fff = tabs - tabs1 + lstarn * (q - qsatt) + lstarp * qp;
+        double dfff = dlstarn * (q - qsatt) + dlstarp * qp - lstarn * dqsat - 1.;
+        dtabs = -fff / dfff;
+        niter++;
+        tabs1 = tabs1 + dtabs;
+      }
+      qsatt = qsatt + dqsat * dtabs;
+      qn = fmax(0.0, q - qsatt);
+    } else {
+      qn = 0.;
+    }
+    tabs = tabs1;
+    qp = fmax(0.0, qp);
+    // precip_proc()
+    double src = 0.0, evp = 0.0;
+    if (qn + qp > 0.) {
+      double omn = fmax(0.0, fmin(1.0, (tabs - TBGMIN) * a_bg));
+      double omp = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * a_pr));
+      double omg = fmax(0.0, fmin(1.0, (tabs - TGRMIN) * a_gr));
+      if (qn > 0.) {
+        double qcc = qn * omn, qii = qn * (1. - omn);
+        double autor = (qcc > QCW0) ? ALPHAELQ : 0.;
+        double autos = (qii > QCI0) ? BETAELQ * CO(C_COEFICE) : 0.;
+        double accrr = 0.;
+        if (omp > R4(0.001)) accrr = CO(C_ACCRRC) * pow(qp * omp, powr1);
+        double accrcs = 0., accris = 0.;
+        if (omp < R4(0.999) && omg < R4(0.999)) {
+          double tmp = pow(qp * (1. - omp) * (1. - omg), pows1);
+          accrcs = CO(C_ACCRSC) * tmp;
+          accris = CO(C_ACCRSI) * tmp;
+        }
+        double accrcg = 0., accrig = 0.;
+        if (omp < R4(0.999) && omg > R4(0.001)) {
+          double tmp = pow(qp * (1. - omp) * omg, powg1);
+          accrcg = CO(C_ACCRGC) * tmp;
+          accrig = CO(C_ACCRGI) * tmp;
+        }
+        qcc = (qcc + dtn * autor * QCW0) / (1. + dtn * (accrr + accrcs + accrcg + autor));
+        qii = (qii + dtn * autos * QCI0) / (1. + dtn * (accris + accrig + autos));
+        double dq = dtn * (accrr * qcc + autor * (qcc - QCW0) + (accris + accrig) * qii + (accrcs + accrcg) * qcc + autos * (qii - QCI0));
+        dq = fmin(dq, qn);
+        qp = qp + dq;
+        q = q - dq;
+        qn = qn - dq;
+        src = dq;
+      } else if (qp > QP_THRESHOLD && qn == 0.) {
+        double qs = 0.;
+        if (omn > R4(0.001)) qs = qs + omn * qsatw(tabs, pres);
+        if (omn < R4(0.999)) qs = qs + (1. - omn) * qsati(tabs, pres);
+        double dq = 0.;
+        if (omp > R4(0.001)) {
+          double qrr = qp * omp;
+          dq = dq + CO(C_EVAPR1) * sqrt(qrr) + CO(C_EVAPR2) * pow(qrr, powr2);
+        }
+        if (omp < R4(0.999) && omg < R4(0.999)) {
+          double qss = qp * (1. - omp) * (1. - omg);
+          dq = dq + CO(C_EVAPS1) * sqrt(qss) + CO(C_EVAPS2) * pow(qss, pows2);
+        }
+        if (omp < R4(0.999) && omg > R4(0.001)) {
+          double qgg = qp * (1. - omp) * omg;
+          dq = dq + CO(C_EVAPG1) * sqrt(qgg) + CO(C_EVAPG2) * pow(qgg, powg2);
+        }
+        dq = dq * dtn * (q / qs - 1.);
+        dq = fmax(-0.5 * qp, dq);
+        qp = qp + dq;
+        q = q - dq;
+        evp = dq;
+      } else {
+        q = q + qp;
+        evp = -qp;
+        qp = 0.;
+      }
+    }
+    {
+      double dq = qp;
+      qp = fmax(0.0, qp);
+      q = q + (dq - qp);
+    }
+    double ssrc = s.sum(src), sevp = s.sum(evp);
+    // micro_diagnose, then diagnose
+    double qv, qcl, qci, qpl, qpi;
+    micro_diag(q, qn, qp, tabs, a_bg, a_pr, qv, qcl, qci, qpl, qpi);
+    tabs = t - gamaz[k] + FAC_COND * (qcl + qpl) + FAC_SUB * (qci + qpi);
+    double u = d.u[o];
+    double u0 = s.sum(u) * factor;
+    double t0 = s.sum(t) * factor;
+    double tabs0 = s.sum(tabs) * factor;
+    double q0 = s.sum(qv + qcl + qci) * factor;
+    double qn0 = s.sum(qcl + qci) * factor;
+    double qp0 = s.sum(qpl + qpi) * factor;
+    if (qcl + qci > 0. && tabs < R4(273.15)) {
+      kmin = ::min(kmin, k);
+      kmax = ::max(kmax, k);
+    }
+    if (live) {
+      d.qt[o] = q;
+      d.qp[o] = qp;
+      d.qn[o] = qn;
+      d.tabs[o] = tabs;
+      d.qv[o] = qv;
+      d.qcl[o] = qcl;
+      d.qci[o] = qci;
+      d.qpl[o] = qpl;
+      d.qpi[o] = qpi;
+      if (i == 0) {
+        prof(d, P_QPSRC, c)[k] = ssrc;
+        prof(d, P_QPEVP, c)[k] = sevp;
+        prof(d, P_U0, c)[k] = u0;
+        prof(d, P_T0, c)[k] = t0;
+        prof(d, P_TABS0, c)[k] = tabs0;
+        prof(d, P_Q0, c)[k] = q0;
+        prof(d, P_QN0, c)[k] = qn0;
+        prof(d, P_QP0, c)[k] = qp0;
+        prof(d, P_QV0, c)[k] = q0 - qn0;
+      }
+    }
+  }
+  kmin = s.imin(kmin);
+  kmax = s.imax(kmax);
+  if (live && i == 0) {
+    inst(d, I_KMIN, c) = kmin;
+    inst(d, I_KMAX, c) = kmax;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Per-GCM-substep statistics: crm_module.F90:896-1073.  Accumulated on the device in
+// profiles indexed by CRM level (w level for the mass fluxes) and per-instance sums.
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_stats(Dev d) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, nz = d.nz, i = s.i;
+  const double dz = inst(d, I_DZ, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  const double* adz = prof(d, P_ADZ, c);
+  const double* pres = prof(d, P_PRES, c);
+  const double thr = R4(0.001), wmin = 2.;
+  const int nxr = d.nx_rad, g = NX / nxr;
+  auto Q = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto Wv = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
+  double cwp = 0, cwph = 0, cwpm = 0, cwpl = 0, clt = 0, cmt = 0, cht = 0, ctt = 0;
+  bool flag_top = true;
+  // level k+1 values (above) for the downdraft test, level k-1 read ahead for the updraft test
+  double w_up = Wv(nzm);  // lid
+  double qc_up = 0, qi_up = 0, qpl_up = 0, qpi_up = 0, tabs_up = 0;
+  for (int k = nzm - 1; k >= 0; k--) {
+    double qcl = Q(d.qcl, k), qci = Q(d.qci, k), qpl = Q(d.qpl, k), qpi = Q(d.qpi, k), tabs = Q(d.tabs, k);
+    double wk = Wv(k);
+    // column water paths from the top (the reference's level nz-k)
+    double tmp1 = rho[k] * adz[k] * dz * (qcl + qci);
+    cwp += tmp1;
+    ctt = fmax(1.0, ctt);
+    double top_hit = 0.0;
+    if (cwp > thr && flag_top) {
+      top_hit = 1.0;
+      flag_top = false;
+    }
+    if (pres[k] >= 700.) {
+      cwpl += tmp1;
+      clt = fmax(1.0, clt);
+    } else if (pres[k] < 400.) {
+      cwph += tmp1;
+      cht = fmax(1.0, cht);
+    } else {
+      cwpm += tmp1;
+      cmt = fmax(1.0, cmt);
+    }
+    // level statistics (the reference's level k)
+    double mass = rho[k] * adz[k] * dz;
+    double ws = w_up + wk;
+    double rw = rho[k] * 0.5 * ws;
+    double cld = 0, mcup = 0, mcdn = 0, mcuup = 0, mcudn = 0;
+    if (mass * (qcl + qci) > thr) {
+      cld = 1.0;
+      if (ws > 2 * wmin) {
+        mcup = rw * 1.0;
+        mcuup = rw * (1.0 - 1.0);
+      }
+      if (ws < -2 * wmin) {
+        mcdn = rw * 1.0;
+        mcudn = rw * (1. - 1.0);
+      }
+    } else {
+      if (ws > 2 * wmin) mcuup = rw;
+      if (ws < -2 * wmin) mcudn = rw;
+    }
+    // mass fluxes through w level k (crm_module.F90:1017-1049)
+    double mui = 0, mdi = 0;
+    if (wk > 0.) {
+      int kx = k >= 1 ? k - 1 : 0;
+      double qcx = k >= 1 ? Q(d.qcl, kx) : qcl, qix = k >= 1 ? Q(d.qci, kx) : qci;
+      double tx = k >= 1 ? Q(d.tabs, kx) : tabs;
+      double qs = qsatw(tx, pres[kx]);
+      if (qcx + qix > fmin(R4(1.e-5), 0.01 * qs)) mui = rhow[k] * wk;
+    } else if (wk < 0.) {
+      int kx = k + 1 < nzm ? k + 1 : nzm - 1;
+      double qcx = k + 1 < nzm ? qc_up : qcl, qix = k + 1 < nzm ? qi_up : qci;
+      double qplx = k + 1 < nzm ? qpl_up : qpl, qpix = k + 1 < nzm ? qpi_up : qpi;
+      double tx = k + 1 < nzm ? tabs_up : tabs;
+      double qs = qsatw(tx, pres[kx]);
+      if (qcx + qix > fmin(R4(1.e-5), 0.01 * qs))
+        mdi = rhow[k] * wk;
+      else if (qplx + qpix > R4(1.0e-4))
+        mdi = rhow[k] * wk;
+    }
+    double s_top = s.sum(top_hit), s_cld = s.sum(cld), s_mcup = s.sum(mcup), s_mcdn = s.sum(mcdn);
+    double s_mcuup = s.sum(mcuup), s_mcudn = s.sum(mcudn), s_liq = s.sum(qcl), s_ice = s.sum(qci);
+    double s_mui = s.sum(mui), s_mdi = s.sum(mdi);
+    // radiation-group sums (crm_module.F90:979-1015)
+    double r_t = s.gsum(tabs, g), r_qv = s.gsum(fmax(0.0, Q(d.qv, k)), g), r_qc = s.gsum(qcl, g);
+    double r_qi = s.gsum(qci, g), r_cld = s.gsum((qcl + qci > 0) ? 1.0 : 0.0, g);
+    if (live) {
+      if (i == 0) {
+        prof(d, P_CLDTOP, c)[nzm - 1 - k] += s_top;  // the reference's index mix (crm_module.F90:918-925)
+        prof(d, P_CLD, c)[k] += s_cld;
+        prof(d, P_MCUP, c)[k] += s_mcup;
+        prof(d, P_MCDN, c)[k] += s_mcdn;
+        prof(d, P_MCUUP, c)[k] += s_mcuup;
+        prof(d, P_MCUDN, c)[k] += s_mcudn;
+        prof(d, P_GLIQ, c)[k] += s_liq;
+        prof(d, P_GICE, c)[k] += s_ice;
+        prof(d, P_MUI, c)[k] += s_mui;
+        prof(d, P_MDI, c)[k] += s_mdi;
+      }
+      if ((i & (g - 1)) == 0) {
+        int ir = i / g;
+        size_t base = (size_t)c * nzm * nxr + (size_t)k * nxr + ir;
+        size_t blk = (size_t)d.ncrms * nzm * nxr;
+        d.rad[0 * blk + base] += r_t;
+        d.rad[1 * blk + base] += r_qv;
+        d.rad[2 * blk + base] += r_qc;
+        d.rad[3 * blk + base] += r_qi;
+        d.rad[4 * blk + base] += r_cld;
+      }
+    }
+    w_up = wk;
+    qc_up = qcl;
+    qi_up = qci;
+    qpl_up = qpl;
+    qpi_up = qpi;
+    tabs_up = tabs;
+  }
+  double a = s.sum(cwp > thr ? ctt : 0.0), b = s.sum(cwph > thr ? cht : 0.0);
+  double m = s.sum(cwpm > thr ? cmt : 0.0), l = s.sum(cwpl > thr ? clt : 0.0);
+  if (live && i == 0) {
+    inst(d, I_CLTOT, c) += a;
+    inst(d, I_CLHGH, c) += b;
+    inst(d, I_CLMED, c) += m;
+    inst(d, I_CLLOW, c) += l;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// End of call: crm_module.F90:1078-1659.  State write-back and every crm_output member.
+// ------------------------------------------------------------------------------------------
+template <int NX>
+__global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_state_view st, crm_rad_view rad, crm_output_view out,
+                        double dt_gl, int nstop) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, nz = d.nz, ld = d.ld, plev = d.plev, i = s.i;
+  const double factor_xy = 1. / (double)NX, factor_xyt = factor_xy / (double)nstop;
+  const double icrm_run_time = 1. / dt_gl, dz = inst(d, I_DZ, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  const double* u0 = prof(d, P_U0, c);
+  const int ptop0 = plev - nzm;  // 0-based GCM index of the top CRM level
+  auto IN = [&](const double* p, int l) { return p[(size_t)c + (size_t)ld * l]; };
+  auto O2 = [&](double* p, int l) -> double& { return p[(size_t)c + (size_t)ld * l]; };
+  auto G4 = [&](double* p, int k) -> double& { return p[(size_t)c + (size_t)ld * ((size_t)i + (size_t)NX * k)]; };
+  auto Q = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto Wv = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
+  const double a_gr = A_GR();
+  double colprec = 0, colprecs = 0;
+  for (int k = 0; k < nzm; k++) {
+    int l = plev - 1 - k;
+    double tabs = Q(d.tabs, k), qv = Q(d.qv, k), qcl = Q(d.qcl, k), qci = Q(d.qci, k);
+    double qpl = Q(d.qpl, k), qpi = Q(d.qpi, k), u = Q(d.u, k), wk = Wv(k), wk1 = Wv(k + 1);
+    double pd = IN(in.pdel, l);
+    colprec += s.sum((qpl + qpi) * pd);
+    colprecs += s.sum(qpi * pd);
+    double tln = s.sum(tabs) * factor_xy, qln = s.sum(qv) * factor_xy;
+    double qccln = s.sum(qcl) * factor_xy, qiiln = s.sum(qci) * factor_xy;
+    double omg = fmax(0.0, fmin(1.0, (tabs - TGRMIN) * a_gr));
+    double qc_mean = s.sum(qcl) * factor_xy, qi_mean = s.sum(qci) * factor_xy, qr_mean = s.sum(qpl) * factor_xy;
+    double qg_mean = s.sum(qpi * omg) * factor_xy, qs_mean = s.sum(qpi * (1. - omg)) * factor_xy;
+    double du = u - u0[k];
+    double u2z = s.sum(du * du), w2z = s.sum(0.5 * (wk1 * wk1 + wk * wk));
+    double tke = s.sum(Q(d.tke, k)), tkz = s.sum(Q(d.tk, k));
+    if (live) {
+      G4(st.u_wind, k) = u;
+      G4(st.v_wind, k) = 0.0;
+      G4(st.w_wind, k) = wk;
+      G4(st.temperature, k) = tabs;
+      G4(st.qt, k) = Q(d.qt, k);
+      G4(st.qp, k) = Q(d.qp, k);
+      G4(st.qn, k) = Q(d.qn, k);
+      G4(out.tk, k) = Q(d.tk, k);
+      G4(out.tkh, k) = Q(d.tkh, k);
+      G4(out.qcl, k) = qcl;
+      G4(out.qci, k) = qci;
+      G4(out.qpl, k) = qpl;
+      G4(out.qpi, k) = qpi;
+      if (i == 0) {
+        bool top2 = (l == ptop0 || l == ptop0 + 1);
+        O2(out.sltend, l) = top2 ? 0.0 : CP * (tln - IN(in.tl, l)) * icrm_run_time;
+        O2(out.qltend, l) = top2 ? 0.0 : (qln - IN(in.ql, l)) * icrm_run_time;
+        O2(out.qcltend, l) = top2 ? 0.0 : (qccln - IN(in.qccl, l)) * icrm_run_time;
+        O2(out.qiltend, l) = top2 ? 0.0 : (qiiln - IN(in.qiil, l)) * icrm_run_time;
+        O2(out.qc_mean, l) = qc_mean;
+        O2(out.qi_mean, l) = qi_mean;
+        O2(out.qr_mean, l) = qr_mean;
+        O2(out.qg_mean, l) = qg_mean;
+        O2(out.qs_mean, l) = qs_mean;
+        O2(out.cld, l) = fmin(1.0, prof(d, P_CLD, c)[k] * factor_xyt);
+        O2(out.cldtop, l) = fmin(1.0, prof(d, P_CLDTOP, c)[k] * factor_xyt);
+        O2(out.gicewp, l) = prof(d, P_GICE, c)[k] * pd * 1000. / GGR * factor_xyt;
+        O2(out.gliqwp, l) = prof(d, P_GLIQ, c)[k] * pd * 1000. / GGR * factor_xyt;
+        double mcup = prof(d, P_MCUP, c)[k] * factor_xyt, mcdn = prof(d, P_MCDN, c)[k] * factor_xyt;
+        double mcuup = prof(d, P_MCUUP, c)[k] * factor_xyt, mcudn = prof(d, P_MCUDN, c)[k] * factor_xyt;
+        O2(out.mcup, l) = mcup;
+        O2(out.mcdn, l) = mcdn;
+        O2(out.mcuup, l) = mcuup;
+        O2(out.mcudn, l) = mcudn;
+        O2(out.mctot, l) = mcup + mcdn + mcuup + mcudn;
+        double t1 = dz / rhow[k], t2 = t1 / d.dtn;
+        double mkwsb1 = prof(d, P_MKWSB1, c)[k] * t1 * rhow[k] * factor_xy / nstop;
+        double mkwsb2 = prof(d, P_MKWSB2, c)[k] * t1 * rhow[k] * factor_xy / nstop;
+        double mkwle1 = prof(d, P_MKWLE1, c)[k] * t2 * rhow[k] * factor_xy / nstop;
+        double mkwle2 = prof(d, P_MKWLE2, c)[k] * t2 * rhow[k] * factor_xy / nstop;
+        double mkadv1 = prof(d, P_MKADV1, c)[k] * factor_xy * icrm_run_time;
+        double mkadv2 = prof(d, P_MKADV2, c)[k] * factor_xy * icrm_run_time;
+        double mkdiff1 = prof(d, P_MKDIFF1, c)[k] * factor_xy * icrm_run_time;
+        double mkdiff2 = prof(d, P_MKDIFF2, c)[k] * factor_xy * icrm_run_time;
+        O2(out.flux_u, l) = (prof(d, P_UWLE, c)[k] + prof(d, P_UWSB, c)[k]) * t1 * factor_xy / nstop;
+        O2(out.flux_v, l) = 0.0 * t1 * factor_xy / nstop;
+        O2(out.flux_qt, l) = mkwle1 + mkwsb1;
+        O2(out.fluxsgs_qt, l) = mkwsb1;
+        O2(out.flux_qp, l) = mkwle2 + mkwsb2;
+        O2(out.qt_trans, l) = mkadv1 + mkdiff1;
+        O2(out.qp_trans, l) = mkadv2 + mkdiff2;
+        double tkesgsz = rho[k] * tke * factor_xy;
+        O2(out.tkesgsz, l) = tkesgsz;
+        O2(out.tkez, l) = rho[k] * 0.5 * (u2z + 0.0 + w2z) * factor_xy + tkesgsz;
+        O2(out.tkz, l) = tkz * factor_xy;
+        O2(out.precflux, l) = prof(d, P_PRECFLUX, c)[k] * factor_xy * dz / d.dt / nstop / 1000.;
+        O2(out.qp_fall, l) = prof(d, P_QPFALL, c)[k] * factor_xy * icrm_run_time;
+        O2(out.qp_evp, l) = prof(d, P_QPEVP, c)[k] * factor_xy * icrm_run_time;
+        O2(out.qp_src, l) = prof(d, P_QPSRC, c)[k] * factor_xy * icrm_run_time;
+        O2(out.qt_ls, l) = prof(d, P_QTEND, c)[k];
+        O2(out.t_ls, l) = prof(d, P_TTEND, c)[k];
+      }
+    }
+  }
+  // GCM levels above the CRM
+  if (live && i == 0) {
+    for (int l = 0; l < ptop0; l++) {
+      O2(out.sltend, l) = 0.0;
+      O2(out.qltend, l) = 0.0;
+      O2(out.qcltend, l) = 0.0;
+      O2(out.qiltend, l) = 0.0;
+      double* zero2[] = {out.qc_mean, out.qi_mean, out.qr_mean, out.qg_mean, out.qs_mean, out.cld, out.cldtop,
+                         out.gicewp, out.gliqwp, out.mcup, out.mcdn, out.mcuup, out.mcudn, out.mctot,
+                         out.flux_u, out.flux_v, out.flux_qt, out.fluxsgs_qt, out.flux_qp, out.qt_trans,
+                         out.qp_trans, out.tkesgsz, out.tkez, out.tkz, out.precflux, out.qp_fall, out.qp_evp,
+                         out.qp_src, out.qt_ls, out.t_ls};
+      for (double* p : zero2) O2(p, l) = 0.0;
+    }
+    // mass fluxes: interface l (0..plev) is w level plev-l
+    double jt = plev * 1.0, mx = 1.0;
+    const double* mui = prof(d, P_MUI, c);
+    const double* mdi = prof(d, P_MDI, c);
+    auto MU = [&](int lw) { int kw = plev - lw; return (kw >= 0 && kw <= nzm - 1) ? mui[kw] : 0.0; };
+    auto MD = [&](int lw) { int kw = plev - lw; return (kw >= 0 && kw <= nzm - 1) ? mdi[kw] : 0.0; };
+    for (int l = 0; l < plev; l++) {
+      double pd = IN(in.pdel, l);
+      double mu_a = MU(l), mu_b = MU(l + 1), md_a = MD(l), md_b = MD(l + 1);
+      double mu = 0.5 * (mu_a + mu_b) * GGR / 100.;
+      double md = 0.5 * (md_a + md_b) * GGR / 100.;
+      O2(out.mu_crm, l) = mu;
+      O2(out.md_crm, l) = md;
+      O2(out.eu_crm, l) = 0.;
+      O2(out.du_crm, l) = 0.;
+      O2(out.ed_crm, l) = 0.;
+      if (mu_a - mu_b > 0)
+        O2(out.eu_crm, l) = (mu_a - mu_b) * GGR / pd;
+      else
+        O2(out.du_crm, l) = -1.0 * (mu_a - mu_b) * GGR / pd;
+      if (md_b - md_a < 0) O2(out.ed_crm, l) = (md_a - md_b) * GGR / pd;
+      if (fabs(mu) > R4(1.0e-15) || fabs(md) > R4(1.0e-15)) {
+        jt = fmin((double)(l + 1), jt);
+        mx = fmax((double)(l + 1), mx);
+      }
+    }
+    out.jt_crm[c] = jt;
+    out.mx_crm[c] = mx;
+    out.prectend[c] = (colprec - inst(d, I_COLPREC, c)) / GGR * factor_xy * icrm_run_time;
+    out.precstend[c] = (colprecs - inst(d, I_COLPRECS, c)) / GGR * factor_xy * icrm_run_time;
+    out.cltot[c] = inst(d, I_CLTOT, c) * factor_xyt;
+    out.clhgh[c] = inst(d, I_CLHGH, c) * factor_xyt;
+    out.clmed[c] = inst(d, I_CLMED, c) * factor_xyt;
+    out.cllow[c] = inst(d, I_CLLOW, c) * factor_xyt;
+    out.z0m[c] = inst(d, I_Z0, c);
+    out.taux[c] = 0.0 / (double)nstop;
+    out.tauy[c] = 0.0 / (double)nstop;
+    out.timing_factor[c] = (double)nstop / nstop;
+  }
+  // surface precipitation (crm_module.F90:1417-1481)
+  double ps = d.precsfc[(size_t)c * NX + i] * dz / d.dt / (double)nstop;
+  double pss = d.precssfc[(size_t)c * NX + i] * dz / d.dt / (double)nstop;
+  bool conv = ps > (double)(10.f / 86400.f);
+  double precc = s.sum(conv ? ps : 0.0), precsc = s.sum(conv ? pss : 0.0);
+  double precl = s.sum(conv ? 0.0 : ps), precsl = s.sum(conv ? 0.0 : pss);
+  if (live) {
+    out.prec_crm[(size_t)c + (size_t)ld * i] = ps / 1000.;
+    if (i == 0) {
+      out.precc[c] = precc * factor_xy / 1000.;
+      out.precl[c] = precl * factor_xy / 1000.;
+      out.precsc[c] = precsc * factor_xy / 1000.;
+      out.precsl[c] = precsl * factor_xy / 1000.;
+    }
+  }
+  // time-averaged radiation groups
+  if (live) {
+    const int nxr = d.nx_rad;
+    const double f = ((double)nxr / (double)NX) * 1.0 / (double)nstop;
+    size_t blk = (size_t)d.ncrms * nzm * nxr;
+    double* outs[5] = {rad.temperature, rad.qv, rad.qc, rad.qi, rad.cld};
+    for (int idx = i; idx < nzm * nxr; idx += NX) {
+      int k = idx / nxr, ir = idx - k * nxr;
+      size_t src = (size_t)c * nzm * nxr + idx;
+      size_t dst = (size_t)c + (size_t)ld * ((size_t)ir + (size_t)nxr * k);
+      for (int q = 0; q < 5; q++) outs[q][dst] = d.rad[q * blk + src] * f;
+    }
+  }
+}
+
+}  // namespace crm

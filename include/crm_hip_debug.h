@@ -1,0 +1,45 @@
+/*
+ * crm_hip_debug.h -- inspection entry points of libcrm_hip.so used by the parity tests.
+ *
+ * They have no counterpart in the reference's crm() interface; they expose intermediate
+ * states of the same computation so that a mismatch can be traced to one phase of the
+ * substep loop of crm_module.F90:727-900.
+ */
+#ifndef CRM_HIP_DEBUG_H
+#define CRM_HIP_DEBUG_H
+
+#include "crm_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* crm_hip_run, stopping inside the last substep after a phase group:
+ *   7  zeroing, buoyancy, forcing, radiative heating, damping, ice sedimentation,
+ *      boundaries and surface fluxes          (crm_module.F90:751-805)
+ *   9  sgs_proc                               (crm_module.F90:809-813)
+ *   11 advect_mom + sgs_mom                   (crm_module.F90:817-821)
+ *   13 pressure + adams                       (crm_module.F90:829-838)
+ *   15 scalar advection and sedimentation     (crm_module.F90:842)
+ *   17 uvw                                    (crm_module.F90:846-850)
+ *   18 sgs_scalars                            (crm_module.F90:854)
+ *   20 micro_proc + diagnose                  (crm_module.F90:865-880)
+ * 0 runs the whole call. Views are host pointers as for crm_hip_run. */
+int crm_hip_debug_run(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col,
+                      double dt_gl, const crm_input_view *in, crm_state_view *state,
+                      crm_rad_view *rad, crm_output_view *out, int phase_stop);
+
+/* Copy an internal field of the last debug run to buf: grid fields as [c][k][i] (u, w, t,
+ * tabs, qv, qcl, qci, qpl, qpi, qt, qp, qn, tk, tkh, tke, dudt, dwdt, precsfc, ...), level
+ * profiles as [c][k]. Returns the number of values written, -1 for an unknown name. */
+int crm_hip_debug_get(crm_hip_ctx *ctx, const char *name, double *buf);
+
+/* Dense operators of the reference's periodic x-transform (crm/fft.F90:5-153 as called by
+ * crm/pressure.F90:117-140): forward Ft[i][s] and inverse Gt[s][i], n x n row-major, slot
+ * s = 0 mean, 1 Nyquist cosine, 2m / 2m+1 cosine / sine of mode m. Host-only. */
+int crm_hip_fft991_operators(int n, double *Ft, double *Gt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRM_HIP_DEBUG_H */

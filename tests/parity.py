@@ -1,0 +1,51 @@
+"""Shared helpers of the parity tests: field comparison and the standard test cases."""
+import numpy as np
+
+from crm_inputs import add_convection, make_case
+
+# Fields of crm_state / crm_rad / crm_output compared after a call.
+STATE = ["u_wind", "w_wind", "temperature", "qt", "qp", "qn"]
+RAD = ["temperature", "qv", "qc", "qi", "cld"]
+OUT = ["qcl", "qci", "qpl", "qpi", "tk", "tkh", "prec_crm", "cltot", "clhgh", "clmed", "cllow",
+       "precc", "precl", "precsc", "precsl", "jt_crm", "mx_crm", "prectend", "precstend", "z0m",
+       "cldtop", "qc_mean", "qi_mean", "qs_mean", "qg_mean", "qr_mean", "sltend", "qltend",
+       "qcltend", "qiltend", "cld", "gicewp", "gliqwp", "mctot", "mcup", "mcdn", "mcuup",
+       "mcudn", "mu_crm", "md_crm", "du_crm", "eu_crm", "ed_crm", "flux_qt", "fluxsgs_qt",
+       "tkez", "tkesgsz", "tkz", "flux_u", "flux_v", "flux_qp", "precflux", "qt_ls",
+       "qt_trans", "qp_trans", "qp_fall", "qp_src", "qp_evp", "t_ls"]
+
+# Internal fields readable from both the oracle and the HIP library's debug entry.
+GRID = ["u", "w", "t", "tabs", "qv", "qcl", "qci", "qpl", "qpi", "qt", "qp", "qn", "tk", "tkh",
+        "tke", "dudt_na", "dudt_nb", "dudt_nc", "dwdt_na", "dwdt_nb", "dwdt_nc", "precsfc",
+        "precssfc"]
+PROF = ["t0", "q0", "qv0", "tabs0", "u0", "qn0", "qp0", "rho", "rhow", "adz", "adzw", "pres",
+        "presi", "z", "zi", "bet", "gamaz", "ttend", "qtend", "utend", "dz"]
+
+
+def relerr(a, b):
+    """Normwise relative difference max|a-b| / max|b| (absolute when b is all zero)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.shape != b.shape:
+        return np.inf
+    d = np.abs(a - b).max() if a.size else 0.0
+    s = np.abs(b).max() if b.size else 0.0
+    return d / s if s > 0 else d
+
+
+def compare_case(got, want, groups=("state", "rad", "out")):
+    """{group.field: relerr} between two CrmCase objects over instances 0..ncrms-1."""
+    n = want.ncrms
+    errs = {}
+    names = {"state": STATE, "rad": RAD, "out": OUT}
+    for g in groups:
+        gd, wd = getattr(got, g), getattr(want, g)
+        for f in names[g]:
+            errs[f"{g}.{f}"] = relerr(gd[f][:n], wd[f][:n])
+    return errs
+
+
+def convective_case(ncrms=4, nx=32, nz=28, dx=4000.0, dt=20.0, seed=1, **kw):
+    """A case whose first substep exercises every process: a raining updraft, ice aloft,
+    velocity noise, per-instance randomized soundings."""
+    return add_convection(make_case(ncrms, nx, nz, dx, dt, randomize=True, **kw), seed=seed)
