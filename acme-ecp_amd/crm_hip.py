@@ -27,7 +27,11 @@ PRECISION = {"fp64": 0, "mixed": 1}
 ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_get_stats",
                "crm_hip_finalize", "crm_hip_errstr"]
 # include/crm_hip_debug.h
-DEBUG_SYMBOLS = ["crm_hip_debug_run", "crm_hip_debug_get", "crm_hip_fft991_operators"]
+DEBUG_SYMBOLS = ["crm_hip_debug_run", "crm_hip_debug_get", "crm_hip_fft991_operators",
+                 "crm_hip_profile_kernels", "crm_hip_kernel_time", "crm_hip_kernel_name"]
+# Kernel classes of one call (CRM_K_* of include/crm_hip_debug.h), in launch order.
+KERNELS = ["setup", "init", "setperturb", "kurant", "precip_init", "pre", "sgs", "mom", "press",
+           "mpdata", "precip", "diffuse", "micro", "stats", "final"]
 
 
 class CrmHipConfig(C.Structure):
@@ -71,6 +75,11 @@ def load():
     lib.crm_hip_errstr.restype = C.c_char_p
     lib.crm_hip_fft991_operators.argtypes = [C.c_int, C.POINTER(C.c_double),
                                              C.POINTER(C.c_double)]
+    lib.crm_hip_profile_kernels.argtypes = [P, C.c_uint]
+    lib.crm_hip_kernel_time.argtypes = [P, C.c_int, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_longlong)]
+    lib.crm_hip_kernel_name.argtypes = [C.c_int]
+    lib.crm_hip_kernel_name.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -151,6 +160,26 @@ class CrmHip:
         if n < 0:
             raise KeyError(name)
         return buf[:n]
+
+    def profile_kernels(self, names):
+        """Time every launch of the named kernel classes with HIP events on the library's
+        stream from now on (resets the totals); an empty list turns timing off."""
+        mask = 0
+        for n in names:
+            mask |= 1 << KERNELS.index(n)
+        rc = self.lib.crm_hip_profile_kernels(self.ctx, mask)
+        if rc:
+            raise CrmError(rc, "crm_hip_profile_kernels")
+
+    def kernel_times(self):
+        """{kernel class: (total ms, launches)} since the last profile_kernels()."""
+        out = {}
+        for kid, n in enumerate(KERNELS):
+            ms, cnt = C.c_double(), C.c_longlong()
+            self.lib.crm_hip_kernel_time(self.ctx, kid, C.byref(ms), C.byref(cnt))
+            if cnt.value:
+                out[n] = (ms.value, cnt.value)
+        return out
 
     def stats(self):
         s = CrmHipStats()

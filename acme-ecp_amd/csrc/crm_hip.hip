@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/crm_hip.h"
+#include "../../include/crm_hip_debug.h"
 #include "crm_kernels.hpp"
 #include "fft991_operator.hpp"
 
@@ -49,7 +50,50 @@ struct crm_hip_ctx {
   int na = 0, nb = 1, nc = 2;
   crm_hip_stats stats{};
   bool last_ok = false;
+  // per-kernel-class timing with HIP events on `stream` (crm_hip_profile_kernels)
+  unsigned prof_mask = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_kid;  // kernel class of each recorded pair
+  size_t ev_used = 0;       // pairs recorded in the current call
+  double kt_ms[CRM_K_COUNT] = {};
+  long long kt_n[CRM_K_COUNT] = {};
 };
+
+// Start timing one launch of kernel class kid; returns the pair index or -1.
+static long prof_begin(crm_hip_ctx* ctx, int kid) {
+  if (!((ctx->prof_mask >> kid) & 1u)) return -1;
+  size_t pair = ctx->ev_used;
+  while (ctx->ev_pool.size() < 2 * (pair + 1)) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    ctx->ev_pool.push_back(e);
+  }
+  if (ctx->ev_kid.size() < pair + 1) ctx->ev_kid.resize(pair + 1);
+  ctx->ev_kid[pair] = kid;
+  ctx->ev_used = pair + 1;
+  (void)hipEventRecord(ctx->ev_pool[2 * pair], ctx->stream);
+  return (long)pair;
+}
+static void prof_end(crm_hip_ctx* ctx, long pair) {
+  if (pair >= 0) (void)hipEventRecord(ctx->ev_pool[2 * pair + 1], ctx->stream);
+}
+// After the stream has drained: fold the recorded pairs into the per-class totals.
+static void prof_harvest(crm_hip_ctx* ctx) {
+  for (size_t q = 0; q < ctx->ev_used; q++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev_pool[2 * q], ctx->ev_pool[2 * q + 1]) == hipSuccess) {
+      ctx->kt_ms[ctx->ev_kid[q]] += ms;
+      ctx->kt_n[ctx->ev_kid[q]] += 1;
+    }
+  }
+  ctx->ev_used = 0;
+}
+#define TIMED(kid, stmt)                   \
+  do {                                     \
+    long pr_ = prof_begin(ctx, kid);       \
+    stmt;                                  \
+    prof_end(ctx, pr_);                    \
+  } while (0)
 
 #define HIPCHECK(x)                                                                 \
   do {                                                                              \
@@ -254,8 +298,8 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
   ctx->last_ok = false;
   ctx->stats = crm_hip_stats{};
 
-  k_setup<<<L.grid_inst(64), dim3(64), 0, s>>>(d, *in);
-  NX_DISPATCH(nx, (k_init<NXV><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *out, dt_gl)));
+  TIMED(CRM_K_SETUP, (k_setup<<<L.grid_inst(64), dim3(64), 0, s>>>(d, *in)));
+  TIMED(CRM_K_INIT, NX_DISPATCH(nx, (k_init<NXV><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *out, dt_gl))));
   if (col && col->igstep <= 1) {
     if (ctx->gcol_cap < ncrms) {
       if (ctx->gcol_dev) HIPCHECK(hipFree(ctx->gcol_dev));
@@ -263,7 +307,7 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
       ctx->gcol_cap = ncrms;
     }
     HIPCHECK(hipMemcpyAsync(ctx->gcol_dev, col->gcol, sizeof(int) * ncrms, hipMemcpyHostToDevice, s));
-    NX_DISPATCH(nx, (k_setperturb<NXV><<<L.grid_inst(64), dim3(64), 0, s>>>(d, ctx->gcol_dev)));
+    TIMED(CRM_K_SETPERTURB, NX_DISPATCH(nx, (k_setperturb<NXV><<<L.grid_inst(64), dim3(64), 0, s>>>(d, ctx->gcol_dev))));
   }
   HIPCHECK(hipGetLastError());
 
@@ -280,7 +324,7 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
     d.nstep = nstep;
     unsigned long long bits = 0;
     HIPCHECK(hipMemsetAsync(ctx->gmax, 0, sizeof(unsigned long long), s));
-    NX_DISPATCH(nx, (k_kurant<NXV><<<L.grid_seg(), b256, 0, s>>>(d)));
+    TIMED(CRM_K_KURANT, NX_DISPATCH(nx, (k_kurant<NXV><<<L.grid_seg(), b256, 0, s>>>(d))));
     HIPCHECK(hipMemcpyAsync(&bits, ctx->gmax, sizeof(bits), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     double cfl;
@@ -296,36 +340,38 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
       d.dtfactor = d.dtn / dt;
       abcoefs(nstep, dt3, na, nb, nc, d.at, d.bt, d.ct);
       d.dt3na = dt3[na];
-      if (icyc == 1) k_precip_init<<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d);
+      if (icyc == 1)
+        TIMED(CRM_K_PRECIP_INIT,
+              (k_precip_init<<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d)));
       const double* qrad = rad->qrad;
-      NX_DISPATCH(nx, (k_pre<NXV><<<L.grid_seg(), b256, 0, s>>>(d, na, qrad)));
+      TIMED(CRM_K_PRE, NX_DISPATCH(nx, (k_pre<NXV><<<L.grid_seg(), b256, 0, s>>>(d, na, qrad))));
       if (last && phase_stop == 7) goto stopped;
-      NX_DISPATCH(nx, (k_sgs<NXV><<<L.grid_seg(), b256, 0, s>>>(d)));
+      TIMED(CRM_K_SGS, NX_DISPATCH(nx, (k_sgs<NXV><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 9) goto stopped;
-      NX_DISPATCH(nx, (k_mom<NXV><<<L.grid_seg(), b256, 0, s>>>(d, na)));
+      TIMED(CRM_K_MOM, NX_DISPATCH(nx, (k_mom<NXV><<<L.grid_seg(), b256, 0, s>>>(d, na))));
       if (last && phase_stop == 11) goto stopped;
-      NX_DISPATCH(nx, (launch_press<NXV>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
+      TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
       if (last && phase_stop == 13) goto stopped;
       HIPCHECK(hipMemsetAsync(ctx->gmax + 1, 0, sizeof(unsigned long long), s));
-      NX_DISPATCH(nx, (k_mpdata<NXV><<<L.grid_seg(), b256, 0, s>>>(d)));
-      NX_DISPATCH(nx, (k_precip<NXV><<<L.grid_seg(), b256, 0, s>>>(d)));
+      TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV><<<L.grid_seg(), b256, 0, s>>>(d))));
+      TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 15) goto stopped;
       std::swap(d.u, d.dudt[nc]);
       std::swap(d.w, d.dwdt[nc]);
       if (last && phase_stop == 17) goto stopped;
-      NX_DISPATCH(nx, (k_diffuse<NXV><<<L.grid_seg(), b256, 0, s>>>(d)));
+      TIMED(CRM_K_DIFFUSE, NX_DISPATCH(nx, (k_diffuse<NXV><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 18) goto stopped;
-      NX_DISPATCH(nx, (k_micro<NXV><<<L.grid_seg(), b256, 0, s>>>(d)));
+      TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_micro<NXV><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 20) goto stopped;
       int nn = na;
       na = nc;
       nc = nb;
       nb = nn;
     }
-    NX_DISPATCH(nx, (k_stats<NXV><<<L.grid_seg(), b256, 0, s>>>(d)));
+    TIMED(CRM_K_STATS, NX_DISPATCH(nx, (k_stats<NXV><<<L.grid_seg(), b256, 0, s>>>(d))));
     HIPCHECK(hipGetLastError());
   }
-  NX_DISPATCH(nx, (k_final<NXV><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop)));
+  TIMED(CRM_K_FINAL, NX_DISPATCH(nx, (k_final<NXV><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop))));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(ctx->ev1, s));
   HIPCHECK(hipStreamSynchronize(s));
@@ -334,6 +380,7 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
     HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.ms_device = ms;
   }
+  prof_harvest(ctx);
   ctx->na = na;
   ctx->nb = nb;
   ctx->nc = nc;
@@ -342,6 +389,7 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
 stopped:
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(s));
+  prof_harvest(ctx);
   ctx->na = na;
   ctx->nb = nb;
   ctx->nc = nc;
@@ -531,6 +579,7 @@ int crm_hip_finalize(crm_hip_ctx* ctx) {
   if (ctx->stage) (void)hipFree(ctx->stage);
   if (ctx->gmax) (void)hipFree(ctx->gmax);
   if (ctx->gcol_dev) (void)hipFree(ctx->gcol_dev);
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -626,6 +675,31 @@ int crm_hip_debug_get(crm_hip_ctx* ctx, const char* name, double* buf) {
     return ncrms;
   }
   return -1;
+}
+
+static const char* const kernel_names[CRM_K_COUNT] = {
+    "setup", "init", "setperturb", "kurant", "precip_init", "pre", "sgs", "mom",
+    "press", "mpdata", "precip", "diffuse", "micro", "stats", "final"};
+
+int crm_hip_profile_kernels(crm_hip_ctx* ctx, unsigned mask) {
+  if (!ctx) return CRM_ERR_ARG;
+  ctx->prof_mask = mask;
+  for (int k = 0; k < CRM_K_COUNT; k++) {
+    ctx->kt_ms[k] = 0.0;
+    ctx->kt_n[k] = 0;
+  }
+  return CRM_OK;
+}
+
+int crm_hip_kernel_time(const crm_hip_ctx* ctx, int kid, double* ms_total, long long* launches) {
+  if (!ctx || kid < 0 || kid >= CRM_K_COUNT || !ms_total || !launches) return CRM_ERR_ARG;
+  *ms_total = ctx->kt_ms[kid];
+  *launches = ctx->kt_n[kid];
+  return CRM_OK;
+}
+
+const char* crm_hip_kernel_name(int kid) {
+  return (kid >= 0 && kid < CRM_K_COUNT) ? kernel_names[kid] : "unknown";
 }
 
 // Host-side transform operators for one nx (tests compare them with the reference transform).

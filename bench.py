@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Throughput of the HIP cloud-resolving model on one MI355X.
+
+Workload: 4096 2D CRM instances of 64 columns x 58 levels (dx 1 km, dt 5 s, model top 40 km,
+stretched dz 50 m .. 1 km), one 1200 s GCM step per call = 240 substeps; per-instance seeded
+tropical soundings (SST ~ U[290,304] K, surface RH ~ U[0.6,0.9], shear ~ U[0,10] m/s per
+10 km) with +/-0.1 K noise in the lowest 5 levels.  One "step" is one crm() call
+(crm/crm_module.F90:45-1718) over the whole batch with every array already resident in HBM
+(crm_hip_run_device); consecutive steps continue from the state the previous one left, as
+consecutive GCM steps do.
+
+Printed: one JSON line with the throughput in CRM column-substeps per second, the roofline of
+the dominant kernel (its average launch time measured with HIP events on the library's own
+stream over the timed steps), and the reference CRM timed on this host's CPU on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "acme-ecp_amd"))
+
+import numpy as np  # noqa: E402
+
+from crm_inputs import make_case  # noqa: E402
+
+WORKLOAD = dict(ncrms=4096, nx=64, nz=58, dx=1000.0, dt=5.0, dt_gl=1200.0)
+HBM_PEAK_GBS = 8000.0
+# Compulsory HBM words per grid cell of one launch of each fused kernel: every field the
+# kernel reads counted once, every field it writes counted once (2D, FP64 words).
+KERNEL_WORDS = {
+    "kurant": 3,     # u, w, tkh
+    "pre": 14,       # R tabs qv qcl qci qpl qpi u w t qt; W t qt dudt dwdt
+    "sgs": 13,       # R u w t tabs qv qcl qci qpl qpi tk; W tk tkh tke
+    "mom": 7,        # R u w tk dudt dwdt; W dudt dwdt
+    "press": 14,     # R u w dudt(3) dwdt(3); W dudt(na,nc) dwdt(na,nc) u w
+    "mpdata": 8,     # R/W t qt qp, R u w
+    "precip": 5,     # per sedimentation sweep: R qp tabs t, W qp t
+    "diffuse": 7,    # R/W t qt qp, R tkh
+    "micro": 13,     # R t qt qp u; W qt qp qn tabs qv qcl qci qpl qpi
+}
+# Whole-step compulsory state traffic per column-substep: prognostics u w t qt qp read and
+# written once plus Adams-Bashforth history (read 2, write 1 per velocity component) = 16
+# words per level.
+STATE_WORDS_PER_LEVEL = 16
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ncrms_sample):
+    """The reference CRM (compiled from the reference sources, oracle/_ref) on one core over a
+    bounded sample of the workload; falls back to the C restatement if it is not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_lib
+    w = WORKLOAD
+    case = make_case(ncrms_sample, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True)
+    kind = "reference"
+    try:
+        runner = oracle_lib.reference(case)
+    except (FileNotFoundError, OSError):
+        runner, kind = oracle_lib.oracle(), "port"
+    t0 = time.perf_counter()
+    rc = runner.run(case, w["dt_gl"])
+    sec = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"CPU baseline returned status {rc}")
+    nstop = int(w["dt_gl"] / w["dt"])
+    colsteps = ncrms_sample * w["nx"] * nstop
+    return {"value": colsteps / sec, "unit": "column-steps/s", "cores": 1, "kind": kind,
+            "sample": f"{ncrms_sample} of the 4096 instances, full 1200 s GCM step "
+                      f"({nstop} substeps), one process, {sec:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ncrms", type=int, default=WORKLOAD["ncrms"])
+    ap.add_argument("--cpu-ncrms", type=int, default=32,
+                    help="instances in the CPU baseline sample (0 skips it)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(instances are independent: a multi-GPU run is N copies of this one)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from crm_hip import KERNELS, CrmHip, DeviceCase
+
+    w = WORKLOAD
+    ncrms = args.ncrms
+    nstop = int(w["dt_gl"] / w["dt"])
+    log(f"building inputs: {ncrms} x {w['nx']}x{w['nz']}")
+    case = make_case(ncrms, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True)
+    dcase = DeviceCase(case, "cuda:0")
+    torch.cuda.synchronize()
+    hip = CrmHip(0, "fp64")
+
+    # warmup: the first call also sizes the device pool; time every kernel class to find the
+    # dominant one
+    hip.profile_kernels(KERNELS)
+    for n in range(args.warmup):
+        hip.run_device(dcase, w["dt_gl"])
+        log(f"warmup step {n + 1}/{args.warmup} done")
+    warm = hip.kernel_times()
+    dominant = max((k for k in warm if k in KERNEL_WORDS), key=lambda k: warm[k][0])
+    log("warmup kernel time [ms]: " + ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in warm.items()))
+
+    # timed region: only the dominant kernel carries events
+    hip.profile_kernels([dominant])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ncyc = []
+    for n in range(args.steps):
+        hip.run_device(dcase, w["dt_gl"])
+        ncyc.append(hip.stats().ncycle_max)
+        log(f"step {n + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    ms_step = 1e3 * sec / args.steps
+    colsteps = ncrms * w["nx"] * nstop
+    value = colsteps * args.steps / sec
+
+    kms, klaunch = hip.kernel_times()[dominant]
+    cells = ncrms * w["nx"] * w["nz"]
+    bytes_launch = KERNEL_WORDS[dominant] * 8 * cells
+    avg_ms = kms / klaunch
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    whole = STATE_WORDS_PER_LEVEL * 8 * w["nz"] * value / 1e9
+
+    cpu = None
+    if args.cpu_ncrms > 0:
+        log(f"CPU baseline on {args.cpu_ncrms} instances")
+        cpu = cpu_baseline(args.cpu_ncrms)
+
+    line = {
+        "metric": "CRM column-steps/s on 1 MI355X, FP64 vs mixed-prec at stated error; HBM %peak",
+        "value": value,
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded tropical soundings per instance",
+        "config": {"workload": f"{ncrms} 2D CRMs of 64x58, dx 1 km, dt 5 s, one 1200 s GCM "
+                               f"step ({nstop} substeps) per call, inputs resident in HBM",
+                   "ncrms": ncrms, "nx": w["nx"], "nz": w["nz"], "substeps": nstop,
+                   "precision": "fp64", "max_ncycle": max(ncyc)},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_ms, "launches": klaunch,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "whole_step_hbm": {"achieved": whole, "frac": whole / HBM_PEAK_GBS, "unit": "GB/s",
+                           "bytes_per_column_substep": STATE_WORDS_PER_LEVEL * 8 * w["nz"]},
+        "kernel_ms_per_step_warmup": {k: v[0] / max(1, args.warmup) for k, v in warm.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    hip.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,5 +1,6 @@
 /*
- * crm_hip_debug.h -- inspection entry points of libcrm_hip.so used by the parity tests.
+ * crm_hip_debug.h -- inspection and timing entry points of libcrm_hip.so (parity tests and
+ * the benchmark).
  *
  * They have no counterpart in the reference's crm() interface; they expose intermediate
  * states of the same computation so that a mismatch can be traced to one phase of the
@@ -38,6 +39,35 @@ int crm_hip_debug_get(crm_hip_ctx *ctx, const char *name, double *buf);
  * crm/pressure.F90:117-140): forward Ft[i][s] and inverse Gt[s][i], n x n row-major, slot
  * s = 0 mean, 1 Nyquist cosine, 2m / 2m+1 cosine / sine of mode m. Host-only. */
 int crm_hip_fft991_operators(int n, double *Ft, double *Gt);
+
+/* Kernel classes of one call, in launch order. */
+enum {
+  CRM_K_SETUP = 0,   /* reference profiles            crm_module.F90:320-375           */
+  CRM_K_INIT,        /* state in, means, tendencies   crm_module.F90:379-594           */
+  CRM_K_SETPERTURB,  /* first-step noise              setperturb.F90:7-86              */
+  CRM_K_KURANT,      /* CFL maximum                   kurant.F90:9-83                  */
+  CRM_K_PRECIP_INIT, /* microphysics coefficients     precip_init.F90:8-97             */
+  CRM_K_PRE,         /* buoyancy..ice_fall            crm_module.F90:751-805           */
+  CRM_K_SGS,         /* sgs_proc                      tke_full.F90:11-317              */
+  CRM_K_MOM,         /* advect_mom + sgs_mom          crm_module.F90:817-821           */
+  CRM_K_PRESS,       /* pressure + adams              crm_module.F90:829-834           */
+  CRM_K_MPDATA,      /* advect_all_scalars            advect_all_scalars.F90:7-94      */
+  CRM_K_PRECIP,      /* precip_fall                   microphysics.F90:529-833         */
+  CRM_K_DIFFUSE,     /* sgs_scalars                   sgs.F90:389-451                  */
+  CRM_K_MICRO,       /* micro_proc + diagnose         crm_module.F90:865-880           */
+  CRM_K_STATS,       /* per-substep statistics        crm_module.F90:896-1073          */
+  CRM_K_FINAL,       /* outputs                       crm_module.F90:1078-1659         */
+  CRM_K_COUNT
+};
+
+/* Time every launch of the kernel classes in `mask` (bit CRM_K_x) with a HIP event pair on
+ * the context's stream, and reset the totals. 0 turns timing off. */
+int crm_hip_profile_kernels(crm_hip_ctx *ctx, unsigned mask);
+
+/* Total device time [ms] and launch count of kernel class kid since the last reset. */
+int crm_hip_kernel_time(const crm_hip_ctx *ctx, int kid, double *ms_total, long long *launches);
+
+const char *crm_hip_kernel_name(int kid);
 
 #ifdef __cplusplus
 }

@@ -49,3 +49,21 @@ def convective_case(ncrms=4, nx=32, nz=28, dx=4000.0, dt=20.0, seed=1, **kw):
     """A case whose first substep exercises every process: a raining updraft, ice aloft,
     velocity noise, per-instance randomized soundings."""
     return add_convection(make_case(ncrms, nx, nz, dx, dt, randomize=True, **kw), seed=seed)
+
+
+def load_golden(path):
+    """(case, dt_gl, expected) from a fixture written by tests/golden/make_golden.py; expected
+    maps "state.x" / "rad.x" / "out.x" to the reference's arrays after the call."""
+    from crm_views import CrmCase
+    z = np.load(path)
+    m = {k[5:]: z[k].item() for k in z.files if k.startswith("meta.")}
+    case = CrmCase(m["ncrms"], m["plev"], m["nx"], m["ny"], m["nz"], m["dx"], m["dt"],
+                   nx_rad=m["nx_rad"], ny_rad=m["ny_rad"], ld=m["ld"])
+    case.igstep = m["igstep"]
+    case.gcol = z["gcol"].astype(np.int32)
+    for k in z.files:
+        if k.startswith("in:"):
+            group, name = k[3:].split(".", 1)
+            getattr(case, group)[name][...] = z[k]
+    expected = {k[4:]: z[k] for k in z.files if k.startswith("ref:")}
+    return case, m["dt_gl"], expected

@@ -1,0 +1,178 @@
+"""Parity of the HIP CRM with the oracle and the reference fixtures on an MI355X.
+
+Bar: after one crm_dt substep every field agrees with the oracle to 1e-10 normwise relative
+(FP64).  Longer runs are held to 1e-8; over a full GCM step the domain means are also tested
+against the spread of an ensemble of oracle runs with perturbed initial temperature."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from crm_hip import CrmHip
+from crm_inputs import make_case
+from parity import GRID, PROF, compare_case, convective_case, load_golden, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL_1 = 1e-10   # one substep
+TOL_N = 1e-8    # several substeps up to a full GCM step
+
+SHAPES = [  # (ncrms, nx, nz, dx, dt): every compiled column count, both level counts,
+    (4, 32, 28, 4000.0, 20.0),   # partially filled waves (ncrms not a multiple of 64/nx)
+    (3, 64, 58, 1000.0, 5.0),
+    (5, 16, 28, 4000.0, 20.0),
+    (9, 8, 28, 4000.0, 20.0),
+]
+PHASES = [7, 9, 11, 13, 15, 17, 18, 20]
+
+
+@pytest.fixture(scope="module")
+def hip():
+    h = CrmHip(0)
+    yield h
+    h.close()
+
+
+def _worst(errs):
+    return sorted(errs.items(), key=lambda kv: -kv[1])[:4]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}x{s[2]}")
+@pytest.mark.parametrize("phase", PHASES)
+def test_phase_parity(hip, shape, phase):
+    n, nx, nz, dx, dt = shape
+    case = convective_case(n, nx=nx, nz=nz, dx=dx, dt=dt, seed=phase)
+    o = oracle_lib.oracle()
+    assert o.run(case.copy(), dt, phase) == 0
+    assert hip.debug_run(case.copy(), dt, phase) == 0
+    size = n * (nz + 1) * nx
+    errs = {}
+    for name in GRID + PROF:
+        # after uvw the library swaps buffers where the reference copies: dudt(nc) and
+        # dwdt(nc) are dead values then
+        if phase >= 17 and name in ("dudt_nc", "dwdt_nc"):
+            continue
+        errs[name] = relerr(hip.debug_get(name, size), o.field(name, size))
+    print(shape, phase, _worst(errs))
+    assert max(errs.values()) <= TOL_1, _worst(errs)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}x{s[2]}")
+@pytest.mark.parametrize("nsub", [1, 10])
+def test_full_call_parity(hip, shape, nsub):
+    n, nx, nz, dx, dt = shape
+    case = convective_case(n, nx=nx, nz=nz, dx=dx, dt=dt, seed=nsub)
+    want, got = case.copy(), case.copy()
+    assert oracle_lib.oracle().run(want, dt * nsub) == 0
+    hip.run(got, dt * nsub)
+    errs = compare_case(got, want)
+    print(shape, nsub, _worst(errs))
+    assert max(errs.values()) <= (TOL_1 if nsub == 1 else TOL_N), _worst(errs)
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_reference_fixture_parity(hip, path):
+    """Directly against the reference CRM's outputs (tests/golden)."""
+    case, dt_gl, expected = load_golden(path)
+    hip.run(case, dt_gl)
+    tol = TOL_1 if abs(dt_gl - case.dt) < 1e-9 else TOL_N
+    errs = {}
+    for key, want in expected.items():
+        group, name = key.split(".", 1)
+        errs[key] = relerr(getattr(case, group)[name][: case.ncrms], want[: case.ncrms])
+    print(os.path.basename(path), _worst(errs))
+    assert max(errs.values()) <= tol, _worst(errs)
+
+
+def test_866_instances_one_substep(hip):
+    """866 instances of the 32x28 CRM with per-instance soundings, one substep."""
+    case = make_case(866, 32, 28, 4000.0, 20.0, randomize=True)
+    want, got = case.copy(), case.copy()
+    assert oracle_lib.oracle().run(want, 20.0) == 0
+    hip.run(got, 20.0)
+    errs = compare_case(got, want)
+    print(_worst(errs))
+    assert max(errs.values()) <= TOL_1, _worst(errs)
+
+
+def test_first_gcm_step_perturbation(hip):
+    """igstep <= 1 seeds the Mersenne twister with gcol*1000 per instance and perturbs the
+    lowest 5 levels (setperturb.F90:7-86) on the device."""
+    case = make_case(6, 32, 28, 4000.0, 20.0, randomize=True, seed_base=21)
+    case.igstep = 1
+    case.gcol = np.array([1, 2, 999, 4096, 77777, 12], dtype=np.int32)
+    want, got = case.copy(), case.copy()
+    assert oracle_lib.oracle().run(want, 20.0) == 0
+    hip.run(got, 20.0)
+    errs = compare_case(got, want)
+    assert max(errs.values()) <= TOL_1, _worst(errs)
+
+
+def test_batch_global_subcycling(hip):
+    """One fast instance forces subcycling of the whole batch (kurant.F90:53-64 takes the
+    maximum over all instances), exactly as the reference does."""
+    case = convective_case(5, seed=9)
+    case.state["w_wind"][2, 7, 0, 3] = 45.0  # w*dt/dz well above 0.7
+    want, got = case.copy(), case.copy()
+    assert oracle_lib.oracle().run(want, 40.0) == 0
+    hip.run(got, 40.0)
+    assert hip.stats().ncycle_max >= 2
+    errs = compare_case(got, want)
+    assert max(errs.values()) <= TOL_N, _worst(errs)
+
+
+def test_radiation_groups(hip):
+    """crm_nx_rad < crm_nx: radiation inputs and outputs on column groups
+    (crm_module.F90:766-777, 979-1015, 1086-1107)."""
+    case = convective_case(3, seed=13, nx_rad=8)
+    case.rad["qrad"][...] = np.random.default_rng(1).uniform(-2e-5, 2e-5, case.rad["qrad"].shape)
+    want, got = case.copy(), case.copy()
+    assert oracle_lib.oracle().run(want, 60.0) == 0
+    hip.run(got, 60.0)
+    errs = compare_case(got, want)
+    assert max(errs.values()) <= TOL_N, _worst(errs)
+
+
+def _domain_means(c):
+    return {"heating": c.out["sltend"][: c.ncrms] / 1004.64,
+            "moistening": c.out["qltend"][: c.ncrms],
+            "precip": (c.out["precc"] + c.out["precl"])[: c.ncrms]}
+
+
+def test_gcm_step_inside_ensemble_spread(hip):
+    """Over a full (chaotic, convective) 1200 s GCM step the domain-mean heating, moistening
+    and surface precipitation of the GPU run are statistically one more member of a 16-member
+    oracle ensemble whose initial temperature is perturbed by a relative 1e-14 (the size of
+    E3SM's pertlim; an absolute 1e-14 K is below half an ulp of 300 K and would change
+    nothing).  Criterion: the RMS over points of (gpu - ensemble mean) / ensemble std is at
+    most 3 (an exchangeable member gives about 1), and points with no spread agree."""
+    case = convective_case(16, seed=5)
+    got = case.copy()
+    hip.run(got, 1200.0)
+    members = []
+    rng = np.random.default_rng(7)
+    for m in range(16):
+        c = case.copy()
+        t = c.state["temperature"]
+        t[...] = t * (1.0 + 1e-14 * rng.uniform(-1.0, 1.0, size=t.shape))
+        assert oracle_lib.oracle().run(c, 1200.0) == 0
+        members.append(_domain_means(c))
+    g = _domain_means(got)
+    for key in g:
+        ens = np.stack([m[key] for m in members])
+        mean = ens.mean(axis=0)
+        sd = ens.std(axis=0, ddof=1)
+        dev = np.abs(g[key] - mean)
+        live = sd > 0
+        z = dev[live] / sd[live]
+        rms_z = float(np.sqrt(np.mean(z * z))) if z.size else 0.0
+        print(key, "ensemble rel spread %.1e  rms z %.2f  max z %.1f" % (
+            float(np.median(sd[live] / np.maximum(np.abs(mean[live]), 1e-300))) if z.size else 0.0,
+            rms_z, float(z.max()) if z.size else 0.0))
+        assert rms_z <= 3.0, key
+        assert np.all(dev[~live] <= 1e-13 * max(np.abs(mean).max(), 1e-300)), key
