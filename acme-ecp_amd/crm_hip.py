@@ -59,6 +59,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"HIP CRM library missing: {LIB_PATH} (build it first)")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 and loads it under a
+    # file name that does not match the soname, so if this library pulled in the system copy
+    # first, torch would start a second runtime that finds no GPU.  Loading torch first lets
+    # the library bind to torch's runtime by soname.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     P = C.c_void_p
     lib.crm_hip_init.argtypes = [C.POINTER(P), C.POINTER(CrmHipConfig)]

@@ -183,6 +183,12 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
   atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
+// Accumulate into a statistics location that exactly one lane updates per launch: a
+// returnless hardware FP64 add, so the sweep does not wait a memory round trip per level.
+// Launches are stream-ordered, so the sequence of additions (and the result) is the same as
+// a plain read-modify-write.
+__device__ __forceinline__ void accum(double* p, double v) { unsafeAtomicAdd(p, v); }
+
 // ---- saturation functions, crm/sat.F90:12-139 (REAL(4) coefficients) ----------------------
 __device__ __forceinline__ double poly8(const float* a, double dt) {
   return a[0] + dt * (a[1] + dt * (a[2] + dt * (a[3] + dt * (a[4] + dt * (a[5] + dt * (a[6] + dt * (a[7] + a[8] * dt)))))));

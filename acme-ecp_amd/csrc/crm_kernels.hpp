@@ -877,7 +877,12 @@ __device__ void mpdata_one(const Dev& d, const Seg<NX>& s, int c, bool live, dou
   auto Wv = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
   const double dz = inst(d, I_DZ, c);
   double pcfl = 0.0;
-  double f_0 = 0, f_1 = 0, f_2 = 0;
+  double f_0 = 0, f_1 = 0, f_2 = 0, f_3 = 0, f_4 = 0, f_5 = 0;
+  // inputs of the next two levels, loaded two iterations before use
+  double nf1 = F(f, 0), nu1 = F(d.u, 0), nw1 = Wv(0);
+  double nf2 = F(f, 1), nu2 = F(d.u, 1), nw2 = Wv(1);
+  // tabs for the precipitation CFL of stage 5, loaded two iterations before use
+  double tq_a = 0, tq_b = 0;
   double U_0 = 0, U_1 = 0, U_2 = 0, U_3 = 0;
   double W_0 = 0, W_1 = 0, W_2 = 0;
   double uu1_0 = 0, uu1_1 = 0, ww1_0 = 0, ww1_1 = 0;
@@ -888,11 +893,22 @@ __device__ void mpdata_one(const Dev& d, const Seg<NX>& s, int c, bool live, dou
   double mx_a = 0, mx_b = 0, mn_a = 0, mn_b = 0;
   double uu3_a = 0, ww3_a = 0;
   for (int K = 0; K <= nzm + 4; K++) {
+    const double cf = nf1, cu = nu1, cw = nw1;
+    nf1 = nf2;
+    nu1 = nu2;
+    nw1 = nw2;
+    if (K + 2 < nzm) {
+      nf2 = F(f, K + 2);
+      nu2 = F(d.u, K + 2);
+      nw2 = Wv(K + 2);
+    }
+    double tq_new = 0.0;
+    if (want_cfl && K >= 3 && K - 3 < nzm) tq_new = F(d.tabs, K - 3);
     // stage 0: donor-cell fluxes at level K
     if (K < nzm) {
-      f_0 = F(f, K);
-      U_0 = F(d.u, K);
-      W_0 = Wv(K);
+      f_0 = cf;
+      U_0 = cu;
+      W_0 = cw;
       double fkb = (K >= 1) ? f_1 : f_0;
       uu1_0 = fmax(0.0, U_0) * s.rot(f_0, -1) + fmin(0.0, U_0) * f_0;
       ww1_0 = fmax(0.0, W_0) * fkb + fmin(0.0, W_0) * f_0;
@@ -962,20 +978,24 @@ __device__ void mpdata_one(const Dev& d, const Seg<NX>& s, int c, bool live, dou
       double ww3up = (L5 + 1 < nzm) ? ww3_new : 0.0;
       double fn = fmax(0.0, f1_d - (s.rot(uu3_a, 1) - uu3_a + (ww3up - ww3_a) * (1. / adz[L5])) * (1. / rho[L5]));
       size_t o = ((size_t)c * nzm + L5) * NX + i;
-      double forig = f[o];
-      double adv = s.sum(fn - forig);
+      double adv = s.sum(fn - f_5);
       if (live) {
         f[o] = fn;
         if (fadv && i == 0) fadv[L5] = adv;
       }
       if (want_cfl) {
         double wmax = dz * adz[L5 >= 1 ? L5 - 1 : 0] / d.dtn;
-        double wp = sqrt(R4(1.29) / rho[L5]) * term_vel(d.tv, fn, rho[L5], d.tabs[o]);
+        double wp = sqrt(R4(1.29) / rho[L5]) * term_vel(d.tv, fn, rho[L5], tq_b);
         pcfl = fmax(pcfl, wp * (1. / wmax));
       }
     }
+    f_5 = f_4;
+    f_4 = f_3;
+    f_3 = f_2;
     f_2 = f_1;
     f_1 = f_0;
+    tq_b = tq_a;
+    tq_a = tq_new;
     U_3 = U_2;
     U_2 = U_1;
     U_1 = U_0;
@@ -1043,13 +1063,17 @@ __global__ void __launch_bounds__(256) k_precip(Dev d) {
   if (prec_cfl > R4(0.9)) nprec = (int)ceil(prec_cfl / R4(0.9));
   const double rnprec = (double)nprec;
   auto Q = [&](double* p, int k) -> double& { return p[((size_t)c * nzm + k) * NX + i]; };
-  auto lfac_of = [&](int k) {
-    double om = fmax(0.0, fmin(1.0, (Q(d.tabs, k) - TPRMIN) * a_pr));
+  auto lfac_of = [&](double tabs) {
+    double om = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * a_pr));
     return FAC_COND + (1 - om) * FAC_FUS;
   };
   const double omega0 = fmax(0.0, fmin(1.0, (Q(d.tabs, 0) - TPRMIN) * a_pr));
   for (int iprec = 1; iprec <= nprec; iprec++) {
-    double tq_0 = 0, tq_1 = 0, tq_2 = 0;
+    double tq_0 = 0, tq_1 = 0, tq_2 = 0, tq_3 = 0;
+    double ta_0 = 0, ta_1 = 0, ta_2 = 0, ta_3 = 0;  // tabs ring
+    double tt_a = 0, tt_b = 0;                       // t of level K-2, K-3
+    // qp and tabs of the next two levels, loaded two iterations before use
+    double nq1 = Q(d.qp, 0), na1 = Q(d.tabs, 0), nq2 = Q(d.qp, 1), na2 = Q(d.tabs, 1);
     double wp_0 = 0, wp_1 = 0, wp_2 = 0;
     double fz0_0 = 0, fz0_1 = 0, fz0_2 = 0;
     double tq1_a = 0, tq1_b = 0;
@@ -1058,10 +1082,19 @@ __global__ void __launch_bounds__(256) k_precip(Dev d) {
     double mx_a = 0, mn_a = 0;
     double fz_a = 0;
     for (int K = 0; K <= nzm + 2; K++) {
+      const double cq = nq1, ca = na1;
+      nq1 = nq2;
+      na1 = na2;
+      if (K + 2 < nzm) {
+        nq2 = Q(d.qp, K + 2);
+        na2 = Q(d.tabs, K + 2);
+      }
+      const double tt_new = (K >= 1 && K - 1 < nzm) ? Q(d.t, K - 1) : 0.0;
       // a: level K, upwind flux
       if (K < nzm) {
-        tq_0 = Q(d.qp, K);
-        double v = sqrt(R4(1.29) / rho[K]) * term_vel(d.tv, tq_0, rho[K], Q(d.tabs, K));
+        tq_0 = cq;
+        ta_0 = ca;
+        double v = sqrt(R4(1.29) / rho[K]) * term_vel(d.tv, tq_0, rho[K], ta_0);
         wp_0 = -v * rhow[K] * dtn / dz / rnprec;
         fz0_0 = tq_0 * wp_0;
       } else {
@@ -1098,17 +1131,17 @@ __global__ void __launch_bounds__(256) k_precip(Dev d) {
       if (Fl >= 0 && Fl < nzm) {
         double irhoadz = 1. / (rho[Fl] * adz[Fl]);
         double fzF = fz_a, fzF1 = (Fl + 1 < nzm) ? fz_new : 0.0;
-        double lf = lfac_of(Fl), lf1 = (Fl + 1 < nzm) ? lfac_of(Fl + 1) : 0.0;
+        double lf = lfac_of(ta_3), lf1 = (Fl + 1 < nzm) ? lfac_of(ta_2) : 0.0;
         double dq = -(fzF1 - fzF) * irhoadz;
         double lat_heat = -(lf1 * fzF1 - lf * fzF) * irhoadz;
         double sfall = s.sum(dq * 1.);
         double sflux = s.sum(fzF * 1.);
         if (live) {
-          Q(d.qp, Fl) = Q(d.qp, Fl) - (fzF1 - fzF) * irhoadz;
-          Q(d.t, Fl) = Q(d.t, Fl) - lat_heat;
+          Q(d.qp, Fl) = tq_3 - (fzF1 - fzF) * irhoadz;
+          Q(d.t, Fl) = tt_b - lat_heat;
           if (i == 0) {
-            qpfall[Fl] = qpfall[Fl] + sfall;
-            precflux[Fl] = precflux[Fl] - sflux;
+            accum(&qpfall[Fl], sfall);
+            accum(&precflux[Fl], -sflux);
           }
           if (Fl == 0) {
             d.precsfc[(size_t)c * NX + i] -= fzF * 1.;
@@ -1116,8 +1149,14 @@ __global__ void __launch_bounds__(256) k_precip(Dev d) {
           }
         }
       }
+      tq_3 = tq_2;
       tq_2 = tq_1;
       tq_1 = tq_0;
+      ta_3 = ta_2;
+      ta_2 = ta_1;
+      ta_1 = ta_0;
+      tt_b = tt_a;
+      tt_a = tt_new;
       wp_2 = wp_1;
       wp_1 = wp_0;
       fz0_2 = fz0_1;
@@ -1534,26 +1573,26 @@ __global__ void __launch_bounds__(256) k_stats(Dev d) {
     double r_qi = s.gsum(qci, g), r_cld = s.gsum((qcl + qci > 0) ? 1.0 : 0.0, g);
     if (live) {
       if (i == 0) {
-        prof(d, P_CLDTOP, c)[nzm - 1 - k] += s_top;  // the reference's index mix (crm_module.F90:918-925)
-        prof(d, P_CLD, c)[k] += s_cld;
-        prof(d, P_MCUP, c)[k] += s_mcup;
-        prof(d, P_MCDN, c)[k] += s_mcdn;
-        prof(d, P_MCUUP, c)[k] += s_mcuup;
-        prof(d, P_MCUDN, c)[k] += s_mcudn;
-        prof(d, P_GLIQ, c)[k] += s_liq;
-        prof(d, P_GICE, c)[k] += s_ice;
-        prof(d, P_MUI, c)[k] += s_mui;
-        prof(d, P_MDI, c)[k] += s_mdi;
+        accum(&prof(d, P_CLDTOP, c)[nzm - 1 - k], s_top);  // the reference's index mix (crm_module.F90:918-925)
+        accum(&prof(d, P_CLD, c)[k], s_cld);
+        accum(&prof(d, P_MCUP, c)[k], s_mcup);
+        accum(&prof(d, P_MCDN, c)[k], s_mcdn);
+        accum(&prof(d, P_MCUUP, c)[k], s_mcuup);
+        accum(&prof(d, P_MCUDN, c)[k], s_mcudn);
+        accum(&prof(d, P_GLIQ, c)[k], s_liq);
+        accum(&prof(d, P_GICE, c)[k], s_ice);
+        accum(&prof(d, P_MUI, c)[k], s_mui);
+        accum(&prof(d, P_MDI, c)[k], s_mdi);
       }
       if ((i & (g - 1)) == 0) {
         int ir = i / g;
         size_t base = (size_t)c * nzm * nxr + (size_t)k * nxr + ir;
         size_t blk = (size_t)d.ncrms * nzm * nxr;
-        d.rad[0 * blk + base] += r_t;
-        d.rad[1 * blk + base] += r_qv;
-        d.rad[2 * blk + base] += r_qc;
-        d.rad[3 * blk + base] += r_qi;
-        d.rad[4 * blk + base] += r_cld;
+        accum(&d.rad[0 * blk + base], r_t);
+        accum(&d.rad[1 * blk + base], r_qv);
+        accum(&d.rad[2 * blk + base], r_qc);
+        accum(&d.rad[3 * blk + base], r_qi);
+        accum(&d.rad[4 * blk + base], r_cld);
       }
     }
     w_up = wk;
@@ -1566,10 +1605,10 @@ __global__ void __launch_bounds__(256) k_stats(Dev d) {
   double a = s.sum(cwp > thr ? ctt : 0.0), b = s.sum(cwph > thr ? cht : 0.0);
   double m = s.sum(cwpm > thr ? cmt : 0.0), l = s.sum(cwpl > thr ? clt : 0.0);
   if (live && i == 0) {
-    inst(d, I_CLTOT, c) += a;
-    inst(d, I_CLHGH, c) += b;
-    inst(d, I_CLMED, c) += m;
-    inst(d, I_CLLOW, c) += l;
+    accum(&inst(d, I_CLTOT, c), a);
+    accum(&inst(d, I_CLHGH, c), b);
+    accum(&inst(d, I_CLMED, c), m);
+    accum(&inst(d, I_CLLOW, c), l);
   }
 }
 

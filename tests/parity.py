@@ -33,15 +33,38 @@ def relerr(a, b):
     return d / s if s > 0 else d
 
 
-def compare_case(got, want, groups=("state", "rad", "out")):
-    """{group.field: relerr} between two CrmCase objects over instances 0..ncrms-1."""
+# Outputs that are differences of two states divided by dt_gl (crm_module.F90:1109-1251 and
+# 572-579): their error is measured against the scale of the states differenced, since
+# cancellation makes the difference itself arbitrarily small.  Values: (state field whose
+# magnitude sets the scale, factor).
+CP = 1.00464e3
+G = 9.80616
+DIFFERENCE_OUTPUTS = {
+    "sltend": ("temperature", CP), "t_ls": ("temperature", 1.0),
+    "qltend": ("qt", 1.0), "qcltend": ("qt", 1.0), "qiltend": ("qt", 1.0),
+    "qt_ls": ("qt", 1.0),
+    "prectend": ("qp", 1.0e5 / G), "precstend": ("qp", 1.0e5 / G),
+}
+
+
+def compare_case(got, want, groups=("state", "rad", "out"), dt_gl=None):
+    """{group.field: relerr} between two CrmCase objects over instances 0..ncrms-1.  With
+    dt_gl, the outputs listed in DIFFERENCE_OUTPUTS are normalised by the scale of the states
+    they difference (e.g. cp * max|T| / dt_gl for sltend)."""
     n = want.ncrms
     errs = {}
     names = {"state": STATE, "rad": RAD, "out": OUT}
     for g in groups:
         gd, wd = getattr(got, g), getattr(want, g)
         for f in names[g]:
-            errs[f"{g}.{f}"] = relerr(gd[f][:n], wd[f][:n])
+            a, b = gd[f][:n], wd[f][:n]
+            if g == "out" and dt_gl and f in DIFFERENCE_OUTPUTS:
+                field, factor = DIFFERENCE_OUTPUTS[f]
+                scale = factor * np.abs(want.state[field][:n]).max() / dt_gl
+                scale = max(scale, np.abs(b).max())
+                errs[f"{g}.{f}"] = np.abs(a - b).max() / scale if scale > 0 else 0.0
+            else:
+                errs[f"{g}.{f}"] = relerr(a, b)
     return errs
 
 

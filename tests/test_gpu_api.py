@@ -67,7 +67,7 @@ def test_context_grows_and_shrinks(hip):
         want, got = case.copy(), case.copy()
         assert oracle_lib.oracle().run(want, dt) == 0
         hip.run(got, dt)
-        assert max(compare_case(got, want).values()) <= 1e-10
+        assert max(compare_case(got, want, dt_gl=dt).values()) <= 1e-10
 
 
 def test_cfl_violation_is_reported(hip):
@@ -105,4 +105,4 @@ def test_context_usable_after_errors(hip):
     want, got = case.copy(), case.copy()
     assert oracle_lib.oracle().run(want, 20.0) == 0
     hip.run(got, 20.0)
-    assert max(compare_case(got, want).values()) <= 1e-10
+    assert max(compare_case(got, want, dt_gl=20.0).values()) <= 1e-10
