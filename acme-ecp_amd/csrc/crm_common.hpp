@@ -62,6 +62,10 @@ constexpr double FAC_SUB = LSUB / CP;
 #define QP_THRESHOLD R4(1.e-8)
 
 constexpr int NCOEF = 12;  // precip_init coefficients per level
+// Largest number of precipitation sub-steps of one substep (the reference has no limit; a
+// count this large only arises from a non-physical state, which is reported as an error).
+constexpr int NPREC_MAX = 512;
+constexpr unsigned long long ERR_FLAG_PRECIP = 1, ERR_FLAG_NAN = 2;
 enum {
   C_ACCRSC = 0, C_ACCRSI, C_ACCRRC, C_COEFICE, C_ACCRGC, C_ACCRGI,
   C_EVAPS1, C_EVAPS2, C_EVAPR1, C_EVAPR2, C_EVAPG1, C_EVAPG2
@@ -112,7 +116,7 @@ struct Dev {
   double *rad;                // 5 blocks of [ncrms][nz][nx_rad]: t, qv, qc, qi, cld
   const double *Ft, *Gt;      // transform operators [NX][NX]
   const double *eign;         // [NX] horizontal eigenvalues per slot
-  unsigned long long *gmax;   // [0] kurant cfl bits, [1] precipitation cfl bits
+  unsigned long long *gmax;   // [0] kurant cfl bits, [1] precipitation cfl bits, [2] error flag
   TermVel tv;
   Gam gam;
   double z0c1;                // 3.14159/2 - 3 ln 2 in REAL(4) (crmsurface.F90:125)

@@ -205,6 +205,13 @@ static int get_transform(crm_hip_ctx* ctx, int nx, double dx, Transform& out) {
 // ---- kernel dispatch over the compile-time column count ------------------------------------
 template <int NX>
 static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, const Dev& d, int na, int nb, int nc) {
+  if (NX == 64) {  // matrix-core transforms; one instance per block
+    if (nzm == 28)
+      k_press64<28><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+    else
+      k_press64<58><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+    return;
+  }
   if (nzm == 28)
     k_press<NX, 28><<<g, b, sh, s>>>(d, na, nb, nc);
   else
@@ -310,7 +317,9 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
     TIMED(CRM_K_SETPERTURB, NX_DISPATCH(nx, (k_setperturb<NXV><<<L.grid_inst(64), dim3(64), 0, s>>>(d, ctx->gcol_dev))));
   }
   HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemsetAsync(ctx->gmax + 2, 0, sizeof(unsigned long long), s));
 
+  if (!(dt_gl >= dm->dt) || !(dt_gl / dm->dt < 1.0e6)) return CRM_ERR_ARG;
   const int nstop = (int)(dt_gl / dm->dt);
   const double dt = dt_gl / nstop;
   d.dt = dt;
@@ -319,6 +328,7 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
   int na = 0, nb = 1, nc = 2;
   const dim3 gpress((unsigned)((ncrms + (64 / nx) - 1) / (64 / nx)));
   const size_t press_lds = (size_t)nzm * 64 * sizeof(double);
+  unsigned long long err_flag = 0;
   HIPCHECK(hipEventRecord(ctx->ev0, s));
   for (int nstep = 1; nstep <= nstop; nstep++) {
     d.nstep = nstep;
@@ -329,9 +339,13 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
     HIPCHECK(hipStreamSynchronize(s));
     double cfl;
     std::memcpy(&cfl, &bits, sizeof(cfl));
-    int ncycle = std::max(1, (int)std::ceil(cfl / (double)0.7f));
+    // ncycle = max(1, ceiling(cfl / 0.7)) > 16 is an abort in the reference (kurant.F90:53-75);
+    // the test is made before the conversion so that huge or non-finite values cannot overflow
+    const double q = cfl / (double)0.7f;
+    if (q != q) return CRM_ERR_NAN;
+    if (!(q <= 16.0)) return CRM_ERR_CFL;
+    int ncycle = std::max(1, (int)std::ceil(q));
     ctx->stats.ncycle_max = std::max(ctx->stats.ncycle_max, ncycle);
-    if (ncycle > 16) return CRM_ERR_CFL;
     const bool last = (nstep == nstop);
     for (int icyc = 1; icyc <= ncycle; icyc++) {
       d.icycle = icyc;
@@ -374,6 +388,7 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
   TIMED(CRM_K_FINAL, NX_DISPATCH(nx, (k_final<NXV><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop))));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(ctx->ev1, s));
+  HIPCHECK(hipMemcpyAsync(&err_flag, ctx->gmax + 2, sizeof(err_flag), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
   {
     float ms = 0.f;
@@ -384,6 +399,7 @@ static int run_core(crm_hip_ctx* ctx, const crm_dims* dm, const crm_column_info*
   ctx->na = na;
   ctx->nb = nb;
   ctx->nc = nc;
+  if (err_flag) return err_flag == ERR_FLAG_NAN ? CRM_ERR_NAN : CRM_ERR_CFL;
   ctx->last_ok = true;
   return CRM_OK;
 stopped:
@@ -537,7 +553,7 @@ int crm_hip_init(crm_hip_ctx** out, const crm_hip_config* cfg) {
   }
   if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-      hipMalloc(&ctx->gmax, 2 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&ctx->gmax, 3 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
     return CRM_ERR_DEVICE;
   }
@@ -591,7 +607,7 @@ const char* crm_hip_errstr(int code) {
   switch (code) {
     case CRM_OK: return "ok";
     case CRM_ERR_ARG: return "invalid argument (dims or null view pointer)";
-    case CRM_ERR_CFL: return "kurant: the number of subcycles exceeded 16";
+    case CRM_ERR_CFL: return "Courant number too large: more than 16 subcycles (kurant) or a runaway precipitation sub-step count";
     case CRM_ERR_NXRAD: return "crm_nx_rad and crm_ny_rad need to divide nx and ny";
     case CRM_ERR_DAMPING: return "damping: tau_min is too small for dt";
     case CRM_ERR_NAN: return "non-finite prognostic field";

@@ -827,6 +827,155 @@ __global__ void __launch_bounds__(256) k_press(Dev d, int na, int nb, int nc) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Pressure for 64-column CRMs with the x-transforms on the matrix cores.  Same algorithm and
+// operation order per level as k_press; the two transforms are the GEMMs
+//   C[k][s] = sum_i P[k][i] Ft[i][s]   and   P'[k][i] = sum_s C[k][s] Gt[s][i]
+// (levels padded to 64 rows) done with v_mfma_f64_16x16x4f64: wave w owns output columns
+// 16w..16w+15 and keeps the matching 16 k-step fragments of Ft (then Gt) in registers; the
+// A fragments stream from the LDS slab.  One instance per block of 4 waves.
+//
+// MFMA f64 16x16x4 operand layout (lane l of the wave):
+//   A[16x4]: row l%16, col l/16      B[4x16]: row l/16, col l%16
+//   D[16x16]: element j of the result vector is row 4*j + l/16, col l%16
+// ------------------------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int PLD = 66;  // slab row stride in doubles: conflict-free A-fragment reads
+
+template <int NZM>
+__global__ void __launch_bounds__(256) k_press64(Dev d, int na, int nb, int nc) {
+  constexpr int NX = 64, NK = 64;
+  extern __shared__ double lds[];
+  double* P = lds;              // [NK][PLD] slab: rhs, coefficients, pressure
+  double* ALF = lds + NK * PLD; // [NZM][64] Thomas factors
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x;  // grid is exactly ncrms blocks
+  const int i = lane;
+  Seg<NX> s;
+  const int nz = NZM + 1;
+  const double dz = inst(d, I_DZ, c);
+  const double* adz = prof(d, P_ADZ, c);
+  const double* adzw = prof(d, P_ADZW, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  const double rdx = 1. / d.dx, btat = d.bt / d.at, ctat = d.ct / d.at;
+  const double dta = 1. / d.dt3na / d.at;
+  const double *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
+  const double *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
+  const int r16 = lane & 15, q4 = lane >> 4;
+  // 1. right-hand side (rows >= NZM of the slab are zero)
+  for (int k = wave; k < NK; k += 4) {
+    double pv = 0.0;
+    if (k < NZM) {
+      size_t o = ((size_t)c * NZM + k) * NX + i;
+      size_t ow = ((size_t)c * nz + k) * NX + i;
+      size_t own = ow + NX;
+      double rdz = 1. / (adz[k] * dz);
+      double rup = rhow[k + 1] / rho[k] * rdz;
+      double rdn = rhow[k] / rho[k] * rdz;
+      double uu = d.u[o], uap = ua[o], ubp = ub[o], ucp = uc[o];
+      pv = (rdx * (s.rot(uu, 1) - uu) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
+           (rdx * (s.rot(uap, 1) - uap) + (wa[own] * rup - wa[ow] * rdn)) +
+           btat * (rdx * (s.rot(ubp, 1) - ubp) + (wb[own] * rup - wb[ow] * rdn)) +
+           ctat * (rdx * (s.rot(ucp, 1) - ucp) + (wc[own] * rup - wc[ow] * rdn));
+      pv = pv * rho[k];
+    }
+    P[k * PLD + i] = pv;
+  }
+  // one 64x64x64 transform of the slab in place: P <- P * M (M row-major 64x64)
+  auto transform = [&](const double* M) {
+    double bf[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) bf[j] = M[(4 * j + q4) * NX + 16 * wave + r16];
+    v4d acc[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[r] = v4d{0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        double a = P[(16 * r + r16) * PLD + 4 * j + q4];
+        acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[j], acc[r], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) P[(16 * r + 4 * jj + q4) * PLD + 16 * wave + r16] = acc[r][jj];
+    __syncthreads();
+  };
+  // 2. forward transform
+  transform(d.Ft);
+  // 3. Thomas sweep per slot (wave 0, lane = slot), factors in LDS
+  if (wave == 0) {
+    const double eig = d.eign[i];
+    auto acoef = [&](int k) { return rhow[k] / (adz[k] * adzw[k] * dz * dz); };
+    auto ccoef = [&](int k) { return rhow[k + 1] / (adz[k] * adzw[k + 1] * dz * dz); };
+    double b;
+    if (i == 0)
+      b = 1.0 / (eig * rho[0] - acoef(0) - ccoef(0));
+    else
+      b = 1.0 / (eig * rho[0] - ccoef(0));
+    double alfa = -ccoef(0) * b;
+    ALF[i] = alfa;
+    double beta = P[i] * b;
+    P[i] = beta;
+    for (int k = 1; k < NZM - 1; k++) {
+      double ak = acoef(k), ck = ccoef(k);
+      double e = 1.0 / (eig * rho[k] - ak - ck + ak * alfa);
+      alfa = -ck * e;
+      ALF[k * 64 + i] = alfa;
+      beta = (P[k * PLD + i] - ak * beta) * e;
+      P[k * PLD + i] = beta;
+    }
+    {
+      int k = NZM - 1;
+      double ak = acoef(k);
+      double top = (P[k * PLD + i] - ak * beta) / (eig * rho[k] - ak + ak * alfa);
+      P[k * PLD + i] = top;
+      for (int kk = NZM - 2; kk >= 0; kk--) {
+        top = ALF[kk * 64 + i] * top + P[kk * PLD + i];
+        P[kk * PLD + i] = top;
+      }
+    }
+  }
+  // 4. inverse transform (its leading barrier also ends the Thomas sweep)
+  transform(d.Gt);
+  // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities
+  const double dtdx = d.dtn / d.dx, dtdz = d.dtn / dz, dt3 = d.dt3na;
+  double* uan = d.dudt[na];
+  double* wan = d.dwdt[na];
+  double* unc = d.dudt[nc];
+  double* wnc = d.dwdt[nc];
+  for (int k = wave; k < NZM; k += 4) {
+    size_t o = ((size_t)c * NZM + k) * NX + i;
+    size_t ow = ((size_t)c * nz + k) * NX + i;
+    double pk = P[k * PLD + i];
+    double pim = s.rot(pk, -1);
+    double pkb = P[(k > 0 ? k - 1 : 0) * PLD + i];
+    double rdz = 1. / (dz * adzw[k]);
+    double du_a = ua[o] - (pk - pim) * rdx;
+    double dw_a = wa[ow] - (pk - pkb) * rdz;
+    double uu = d.u[o], ww = d.w[ow];
+    double un = uu + dt3 * (d.at * du_a + d.bt * ub[o] + d.ct * uc[o]);
+    double wn = ww + dt3 * (d.at * dw_a + d.bt * wb[ow] + d.ct * wc[ow]);
+    uan[o] = du_a;
+    wan[ow] = dw_a;
+    unc[o] = un;
+    wnc[ow] = wn;
+    d.u[o] = 0.5 * (uu + un) * (rho[k] * dtdx);
+    d.w[ow] = 0.5 * (ww + wn) * (rhow[k] * dtdz);
+  }
+  if (wave == 0) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
+}
+
+template <int NZM>
+constexpr size_t press64_lds_bytes() {
+  return (size_t)(64 * PLD + NZM * 64) * sizeof(double);
+}
+
+// ------------------------------------------------------------------------------------------
 // MPDATA scalar advection: ADV_MPDATA/advect_scalar2D.F90:8-244 with the statistics of
 // advect_scalar.F90:9-76.  Software-pipelined over levels: stage s processes level K-s.
 // ------------------------------------------------------------------------------------------
@@ -1060,7 +1209,13 @@ __global__ void __launch_bounds__(256) k_precip(Dev d) {
   const double a_pr = A_PR();
   double prec_cfl = __longlong_as_double((long long)d.gmax[1]);
   int nprec = 1;
-  if (prec_cfl > R4(0.9)) nprec = (int)ceil(prec_cfl / R4(0.9));
+  if (!(prec_cfl <= NPREC_MAX * R4(0.9))) {
+    // non-finite or runaway precipitation Courant number: one sweep, and the host reports
+    // the error after the call
+    if (live && i == 0) atomicMax(&d.gmax[2], prec_cfl != prec_cfl ? ERR_FLAG_NAN : ERR_FLAG_PRECIP);
+  } else if (prec_cfl > R4(0.9)) {
+    nprec = (int)ceil(prec_cfl / R4(0.9));
+  }
   const double rnprec = (double)nprec;
   auto Q = [&](double* p, int k) -> double& { return p[((size_t)c * nzm + k) * NX + i]; };
   auto lfac_of = [&](double tabs) {
@@ -1651,6 +1806,9 @@ __global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_sta
     double u2z = s.sum(du * du), w2z = s.sum(0.5 * (wk1 * wk1 + wk * wk));
     double tke = s.sum(Q(d.tke, k)), tkz = s.sum(Q(d.tk, k));
     if (live) {
+      // every prognostic value written back must be finite (CRM_ERR_NAN otherwise)
+      const double chk = u + wk + tabs + Q(d.qt, k) + Q(d.qp, k) + Q(d.qn, k);
+      if (!isfinite(chk)) atomicMax(&d.gmax[2], ERR_FLAG_NAN);
       G4(st.u_wind, k) = u;
       G4(st.v_wind, k) = 0.0;
       G4(st.w_wind, k) = wk;

@@ -1,0 +1,141 @@
+! crm_hip_bindings.F90 -- Fortran ISO_C_BINDING interface to libcrm_hip.so (include/crm_hip.h).
+!
+! The bind(C) derived types mirror the C structs member for member; every array member is a
+! type(c_ptr) to the caller's column-major array (pcols leading), exactly as the Fortran
+! crm_input_type / crm_state_type / crm_rad_type / crm_output_type components are laid out
+! (components/cam/src/physics/crm/crm_input_module.F90:10-47, crm_state_module.F90:12-56,
+! crm_rad_module.F90:17-34, crm_output_module.F90:7-133).
+module crm_hip_bindings
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: CRM_OK = 0, CRM_ERR_ARG = 1, CRM_ERR_CFL = 2, &
+       CRM_ERR_NXRAD = 3, CRM_ERR_DAMPING = 4, CRM_ERR_NAN = 5, CRM_ERR_DEVICE = 6, &
+       CRM_ERR_NOMEM = 7, CRM_ERR_UNSUPPORTED = 8
+  integer(c_int), parameter, public :: CRM_PREC_FP64 = 0, CRM_PREC_MIXED = 1
+
+  type, bind(C), public :: crm_dims
+    integer(c_int) :: ncrms, ld, plev, nx, ny, nz, nx_rad, ny_rad
+    real(c_double) :: dx, dt
+  end type
+
+  type, bind(C), public :: crm_column_info
+    integer(c_int) :: lchnk
+    type(c_ptr)    :: gcol, lat, lon
+    integer(c_int) :: igstep
+  end type
+
+  type, bind(C), public :: crm_input_view
+    type(c_ptr) :: zmid, zint, tl, ql, qccl, qiil, ps, pmid, pint, pdel, phis, ul, vl, &
+                   ocnfrac, tau00, wndls, bflxls, fluxu00, fluxv00, fluxt00, fluxq00
+  end type
+
+  type, bind(C), public :: crm_state_view
+    type(c_ptr) :: u_wind, v_wind, w_wind, temperature, qt, qp, qn
+  end type
+
+  type, bind(C), public :: crm_rad_view
+    type(c_ptr) :: qrad, temperature, qv, qc, qi, cld
+  end type
+
+  type, bind(C), public :: crm_output_view
+    type(c_ptr) :: qcl, qci, qpl, qpi, tk, tkh
+    type(c_ptr) :: prec_crm
+    type(c_ptr) :: cltot, clhgh, clmed, cllow, precc, precl, precsc, precsl
+    type(c_ptr) :: jt_crm, mx_crm, prectend, precstend, taux, tauy, z0m, timing_factor
+    type(c_ptr) :: cldtop, qc_mean, qi_mean, qs_mean, qg_mean, qr_mean
+    type(c_ptr) :: sltend, qltend, qcltend, qiltend
+    type(c_ptr) :: cld, gicewp, gliqwp, mctot, mcup, mcdn, mcuup, mcudn
+    type(c_ptr) :: mu_crm, md_crm, du_crm, eu_crm, ed_crm
+    type(c_ptr) :: flux_qt, fluxsgs_qt, tkez, tkesgsz, tkz, flux_u, flux_v, flux_qp
+    type(c_ptr) :: precflux, qt_ls, qt_trans, qp_trans, qp_fall, qp_src, qp_evp, t_ls
+  end type
+
+  type, bind(C), public :: crm_hip_config
+    integer(c_int) :: device, precision, max_ncrms
+  end type
+
+  type, bind(C), public :: crm_hip_stats
+    integer(c_int) :: nstop, ncycle_max, nprec_max
+    real(c_double) :: ms_device
+  end type
+
+  public :: crm_hip_init, crm_hip_run, crm_hip_run_device, crm_hip_get_stats, &
+            crm_hip_finalize, crm_hip_errstr, crm_hip_message
+
+  interface
+    integer(c_int) function crm_hip_init(ctx, cfg) bind(C, name="crm_hip_init")
+      import :: c_int, c_ptr, crm_hip_config
+      type(c_ptr), intent(out) :: ctx
+      type(crm_hip_config), intent(in) :: cfg
+    end function
+
+    integer(c_int) function crm_hip_run(ctx, dims, col, dt_gl, inp, state, rad, out) &
+        bind(C, name="crm_hip_run")
+      import :: c_int, c_ptr, c_double, crm_dims, crm_column_info, crm_input_view, &
+                crm_state_view, crm_rad_view, crm_output_view
+      type(c_ptr), value :: ctx
+      type(crm_dims), intent(in) :: dims
+      type(crm_column_info), intent(in) :: col
+      real(c_double), value :: dt_gl
+      type(crm_input_view), intent(in) :: inp
+      type(crm_state_view), intent(inout) :: state
+      type(crm_rad_view), intent(inout) :: rad
+      type(crm_output_view), intent(inout) :: out
+    end function
+
+    integer(c_int) function crm_hip_run_device(ctx, dims, col, dt_gl, inp, state, rad, out) &
+        bind(C, name="crm_hip_run_device")
+      import :: c_int, c_ptr, c_double, crm_dims, crm_column_info, crm_input_view, &
+                crm_state_view, crm_rad_view, crm_output_view
+      type(c_ptr), value :: ctx
+      type(crm_dims), intent(in) :: dims
+      type(crm_column_info), intent(in) :: col
+      real(c_double), value :: dt_gl
+      type(crm_input_view), intent(in) :: inp
+      type(crm_state_view), intent(inout) :: state
+      type(crm_rad_view), intent(inout) :: rad
+      type(crm_output_view), intent(inout) :: out
+    end function
+
+    integer(c_int) function crm_hip_get_stats(ctx, stats) bind(C, name="crm_hip_get_stats")
+      import :: c_int, c_ptr, crm_hip_stats
+      type(c_ptr), value :: ctx
+      type(crm_hip_stats), intent(out) :: stats
+    end function
+
+    integer(c_int) function crm_hip_finalize(ctx) bind(C, name="crm_hip_finalize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    type(c_ptr) function crm_hip_errstr(code) bind(C, name="crm_hip_errstr")
+      import :: c_int, c_ptr
+      integer(c_int), value :: code
+    end function
+  end interface
+
+contains
+
+  ! The status text of code as a Fortran string (for endrun messages).
+  function crm_hip_message(code) result(msg)
+    integer(c_int), intent(in) :: code
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    interface
+      integer(c_size_t) function c_strlen(p) bind(C, name="strlen")
+        import :: c_size_t, c_ptr
+        type(c_ptr), value :: p
+      end function
+    end interface
+    type(c_ptr) :: p
+    p = crm_hip_errstr(code)
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function
+
+end module crm_hip_bindings

@@ -1,0 +1,71 @@
+"""The Fortran ISO_C_BINDING layer (acme-ecp_amd/fortran/crm_hip_bindings.F90) against the C
+ABI, without a GPU: it compiles with flang, its bind(C) types have the C structs' sizes, and
+it reaches the library (status strings, argument checks before any device work)."""
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import pytest
+
+import crm_hip
+from crm_views import (CrmColumnInfo, CrmDims, CrmInputView, CrmOutputView, CrmRadView,
+                       CrmStateView)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FLANG = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+
+PROGRAM = r"""
+program probe
+  use iso_c_binding
+  use crm_hip_bindings
+  implicit none
+  type(crm_dims) :: d
+  type(crm_column_info) :: col
+  type(crm_input_view) :: vi
+  type(crm_state_view) :: vs
+  type(crm_rad_view) :: vr
+  type(crm_output_view) :: vo
+  type(crm_hip_config) :: cfg
+  type(crm_hip_stats) :: st
+  integer(c_int) :: rc
+  print '(A,8I6)', 'sizes', c_sizeof(d), c_sizeof(col), c_sizeof(vi), c_sizeof(vs), &
+        c_sizeof(vr), c_sizeof(vo), c_sizeof(cfg), c_sizeof(st)
+  print '(A,A)', 'msg2 ', crm_hip_message(CRM_ERR_CFL)
+  rc = crm_hip_run(c_null_ptr, d, col, 1200.0_c_double, vi, vs, vr, vo)
+  print '(A,I3)', 'run ', rc
+end program
+"""
+
+
+@pytest.fixture(scope="module")
+def probe(tmp_path_factory):
+    if not os.path.exists(FLANG):
+        pytest.skip("flang not available")
+    d = tmp_path_factory.mktemp("fortran")
+    src = os.path.join(ROOT, "acme-ecp_amd", "fortran", "crm_hip_bindings.F90")
+    prog = d / "probe.F90"
+    prog.write_text(PROGRAM)
+    libdir = os.path.dirname(crm_hip.LIB_PATH)
+    subprocess.run([FLANG, "-c", src, "-J", str(d), "-o", str(d / "b.o")], check=True)
+    subprocess.run([FLANG, str(prog), str(d / "b.o"), "-J", str(d), f"-L{libdir}", "-lcrm_hip",
+                    f"-Wl,-rpath,{libdir}", "-o", str(d / "probe")], check=True)
+    out = subprocess.run([str(d / "probe")], check=True, capture_output=True, text=True).stdout
+    return {line.split()[0]: line.split(None, 1)[1].strip() for line in out.splitlines()}
+
+
+def test_struct_sizes_match_c(probe):
+    class Cfg(C.Structure):
+        _fields_ = [("a", C.c_int), ("b", C.c_int), ("c", C.c_int)]
+    want = [C.sizeof(CrmDims), C.sizeof(CrmColumnInfo), C.sizeof(CrmInputView),
+            C.sizeof(CrmStateView), C.sizeof(CrmRadView), C.sizeof(CrmOutputView),
+            C.sizeof(Cfg), C.sizeof(crm_hip.CrmHipStats)]
+    assert [int(v) for v in probe["sizes"].split()] == want
+
+
+def test_status_text_through_fortran(probe):
+    assert probe["msg2"] == crm_hip.errstr(crm_hip.CRM_ERR_CFL)
+
+
+def test_null_context_rejected(probe):
+    assert int(probe["run"]) == crm_hip.CRM_ERR_ARG

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle_lib
-from crm_hip import (CRM_ERR_ARG, CRM_ERR_CFL, CRM_ERR_DAMPING, CRM_ERR_NXRAD,
+from crm_hip import (CRM_ERR_ARG, CRM_ERR_CFL, CRM_ERR_DAMPING, CRM_ERR_NAN, CRM_ERR_NXRAD,
                      CRM_ERR_UNSUPPORTED, CrmError, CrmHip, DeviceCase)
 from crm_inputs import make_case
 from parity import compare_case, convective_case
@@ -92,6 +92,19 @@ def test_status_codes(hip, kw, code):
     case = make_case(args.pop("ncrms"), args.pop("nx"), args.pop("nz"), args.pop("dx"),
                      args.pop("dt"), **args)
     assert hip.run(case, 1200.0, check=False) == code
+
+
+def test_non_finite_state_is_reported(hip):
+    """A NaN in the state propagates silently through the reference; here every loop with a
+    data-dependent trip count is bounded and the write-back flags non-finite values."""
+    case = convective_case(2, seed=7)
+    case.state["temperature"][1, 5, 0, 10] = np.nan
+    assert hip.run(case, 60.0, check=False) == CRM_ERR_NAN
+
+
+def test_bad_time_step_rejected(hip):
+    case = make_case(2, 32, 28, 4000.0, 20.0)
+    assert hip.run(case, 10.0, check=False) == CRM_ERR_ARG  # dt_gl < crm_dt: no substep
 
 
 def test_bad_dims_rejected(hip):
