@@ -90,3 +90,60 @@ def load_golden(path):
             getattr(case, group)[name][...] = z[k]
     expected = {k[4:]: z[k] for k in z.files if k.startswith("ref:")}
     return case, m["dt_gl"], expected
+
+
+# ---- column budgets -------------------------------------------------------------------------
+LC, LF = 2.501e6, 3.337e5
+_F32 = lambda x: float(np.float32(x))  # noqa: E731
+_TBG = (_F32(253.16), _F32(273.16))
+_TPR = (_F32(268.16), _F32(283.16))
+
+
+def _levels(field, n, nz):
+    """(n, plev) GCM-ordered array -> (n, nz) CRM-ordered (CRM level k = GCM level plev-1-k)."""
+    return field[:n, ::-1][:, :nz]
+
+
+def static_energy(case, gamaz):
+    """Liquid-ice static energy / cp of the state (crm_module.F90:508-509 with the
+    temperature ramps of micro_diagnose, microphysics.F90:331-357): (n, nx, nz)."""
+    n = case.ncrms
+    T = case.state["temperature"][:n, :, 0, :]
+    qn, qp = case.state["qn"][:n, :, 0, :], case.state["qp"][:n, :, 0, :]
+    omn = np.clip((T - _TBG[0]) * (1.0 / (_TBG[1] - _TBG[0])), 0, 1)
+    omp = np.clip((T - _TPR[0]) * (1.0 / (_TPR[1] - _TPR[0])), 0, 1)
+    fc, fs = LC / CP, (LC + LF) / CP
+    return T + gamaz[:, None, :] - fc * (qn * omn + qp * omp) - fs * (qn * (1 - omn) + qp * (1 - omp))
+
+
+def budget_residuals(before, after, dt_gl, get):
+    """Relative residuals of the total-water and static-energy budgets of one crm() call.
+
+    Water:  d/dt sum_k m_k <qt + qp>  =  sum_k m_k qt_ls  -  P
+    Energy: d/dt sum_k m_k <t>        =  sum_k m_k t_ls   +  (Lc/cp) P + (Lf/cp) P_snow
+    with m_k = rho adz dz, P the surface precipitation (1000 kg/m3 * prec_crm) and P_snow its
+    frozen part (precsc + precsl).  Energy uses the prognostic static energy t at the end of
+    the call (get("t")): the returned temperature satisfies the saturation adjustment only to
+    its 0.01 K stopping tolerance (cloud.F90).  Returns (water, energy) residuals relative to
+    the column totals."""
+    n, nz, nx = before.ncrms, before.nz, before.nx
+    rho = get("rho", n * nz).reshape(n, nz)
+    adz = get("adz", n * nz).reshape(n, nz)
+    dz = get("dz", n)
+    gamaz = get("gamaz", n * nz).reshape(n, nz)
+    mass = rho * adz * dz[:, None]
+
+    def water(c):
+        q = (c.state["qt"][:n, :, 0, :] + c.state["qp"][:n, :, 0, :]).mean(axis=1)
+        return (mass * q).sum(axis=1)
+
+    P = 1000.0 * after.out["prec_crm"][:n, :, 0].mean(axis=1) * dt_gl
+    Ps = 1000.0 * (after.out["precsc"][:n] + after.out["precsl"][:n]) * dt_gl
+    Fq = (mass * _levels(after.out["qt_ls"], n, nz)).sum(axis=1) * dt_gl
+    Ft = (mass * _levels(after.out["t_ls"], n, nz)).sum(axis=1) * dt_gl
+    Wb, Wa = water(before), water(after)
+    Hb = (mass * static_energy(before, gamaz).mean(axis=1)).sum(axis=1)
+    Ha = (mass * get("t", n * nz * nx).reshape(n, nz, nx).mean(axis=2)).sum(axis=1)
+    rw = np.abs(Wa - Wb - Fq + P).max() / np.abs(Wb).max()
+    rh = np.abs(Ha - Hb - Ft - (LC / CP) * P - (LF / CP) * Ps).max() / np.abs(Hb).max()
+    return rw, rh
