@@ -12,23 +12,52 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-namespace crm {
+// The FP64 form is compiled in namespace crm; the mixed-precision form (FP32 bulk fields) is
+// compiled from the same sources in its own translation unit and namespace.
+#ifndef CRM_NS
+#define CRM_NS crm
+#endif
 
-// Single-precision value of a default-kind (REAL(4)) literal of the reference, widened.
+namespace CRM_NS {
+
+// Single-precision value of a default-kind (REAL(4)) literal of the reference; widened to
+// FP64 in the FP64 translation unit.  The mixed-precision unit keeps it in FP32, which is
+// the same value wherever it meets FP64 operands.
+#ifdef CRM_MIXED_TU
+#define R4(x) (x##f)
+#else
 #define R4(x) ((double)(x##f))
+#endif
+
+// Type of the physical constants: FP64, or FP32 in the mixed-precision unit so that bulk FP32
+// arithmetic does not widen.
+#ifdef CRM_MIXED_TU
+using creal = float;
+#else
+using creal = double;
+#endif
 
 // crm/params.F90:49-69 with cime/src/share/util/shr_const_mod.F90:22-55 under -DCRM
 constexpr double SHR_RGAS = 6.02214e26 * 1.38065e-23;
-constexpr double CP = 1.00464e3;
-constexpr double GGR = 9.80616;
-constexpr double LCOND = 2.501e6;
-constexpr double LFUS = 3.337e5;
-constexpr double LSUB = LCOND + LFUS;
-constexpr double RGAS = SHR_RGAS / 28.966;
-constexpr double RV = SHR_RGAS / 18.016;
-constexpr double FAC_COND = LCOND / CP;
-constexpr double FAC_FUS = LFUS / CP;
-constexpr double FAC_SUB = LSUB / CP;
+constexpr creal CP = 1.00464e3;
+constexpr creal GGR = 9.80616;
+constexpr creal LCOND = 2.501e6;
+constexpr creal LFUS = 3.337e5;
+constexpr creal LSUB = LCOND + LFUS;
+constexpr creal RGAS = SHR_RGAS / 28.966;
+constexpr creal RV = SHR_RGAS / 18.016;
+constexpr creal FAC_COND = LCOND / CP;
+constexpr creal FAC_FUS = LFUS / CP;
+constexpr creal FAC_SUB = LSUB / CP;
+// The same constants in FP64 in every unit (integer ratios are not touched by FP32 literals),
+// for the reference-profile set-up, the state transforms at the call boundary and the outputs.
+constexpr double CP_D = (double)100464 / 100;
+constexpr double GGR_D = (double)980616 / 100000;
+constexpr double LCOND_D = (double)2501000;
+constexpr double LFUS_D = (double)333700;
+constexpr double FAC_COND_D = LCOND_D / CP_D;
+constexpr double FAC_FUS_D = LFUS_D / CP_D;
+constexpr double FAC_SUB_D = (LCOND_D + LFUS_D) / CP_D;
 #define PI_R4 R4(3.141592653589793)
 #define EPSV R4(0.61)
 #define DIFFELQ R4(2.21e-05)
@@ -75,6 +104,7 @@ enum {
 enum Prof {
   P_Z = 0, P_ZI, P_PRES, P_PRESI, P_ADZ, P_ADZW, P_RHO, P_RHOW, P_BET, P_GAMAZ,
   P_GRDFX, P_TAU,                                          // reference state (per call)
+  P_IRHO, P_IADZ, P_IRHOADZ, P_IRHOWADZ, P_DDADZ, P_PRATIO,  // derived, per call
   P_T0, P_Q0, P_QV0, P_TABS0, P_U0, P_QN0, P_QP0,          // domain means (diagnose)
   P_TTEND, P_QTEND, P_UTEND,                               // large-scale tendencies
   P_UWLE, P_UWSB, P_MKWLE1, P_MKWLE2, P_MKWSB1, P_MKWSB2,  // last-substep fluxes
@@ -100,19 +130,23 @@ struct Gam {
   double gamr1, gamr2, gamr3, gams1, gams2, gams3, gamg1, gamg2, gamg3;
 };
 
-// Everything a kernel needs, passed by value.
-struct Dev {
+// Everything a kernel needs, passed by value.  R is the type of the bulk fields (double, or
+// float in the mixed-precision form); level profiles, per-instance scalars, surface
+// precipitation and every accumulated statistic stay FP64.
+template <typename R>
+struct DevT {
   int ncrms, nzm, nz, plev, ld, nx_rad;
   double dx, dt, dtn, dt3na, at, bt, ct, dtfactor;
   int nstep, icycle, nprec_max_slot;
   // prognostic and diagnostic fields [c][k][i]
-  double *u, *w, *t, *qt, *qp, *qn, *tabs, *qv, *qcl, *qci, *qpl, *qpi;
-  double *tk, *tkh, *tke;
-  double *dudt[3], *dwdt[3];  // AB3 history; na/nb/nc resolved on the host
+  R *u, *w, *t, *qt, *qp, *qn, *tabs, *qv, *qcl, *qci, *qpl, *qpi;
+  R *tk, *tkh, *tke;
+  R *dudt[3], *dwdt[3];       // AB3 history; na/nb/nc resolved on the host
   double *precsfc, *precssfc; // [c][i]
   double *prof;               // NPROF blocks of [ncrms][nz]
+  R *profr;                   // the same profiles in R (== prof when R is double)
   double *inst;               // NINST blocks of [ncrms]
-  double *coef;               // NCOEF blocks of [ncrms][nz]
+  R *coef;                    // NCOEF blocks of [ncrms][nz]
   double *rad;                // 5 blocks of [ncrms][nz][nx_rad]: t, qv, qc, qi, cld
   const double *Ft, *Gt;      // transform operators [NX][NX]
   const double *eign;         // [NX] horizontal eigenvalues per slot
@@ -122,14 +156,78 @@ struct Dev {
   double z0c1;                // 3.14159/2 - 3 ln 2 in REAL(4) (crmsurface.F90:125)
 };
 
-__device__ __forceinline__ double* prof(const Dev& d, int p, int c) {
+template <typename R>
+__device__ __forceinline__ double* prof(const DevT<R>& d, int p, int c) {
   return d.prof + ((size_t)p * d.ncrms + c) * d.nz;
 }
-__device__ __forceinline__ double& inst(const Dev& d, int p, int c) {
+template <typename R>
+__device__ __forceinline__ R* profr(const DevT<R>& d, int p, int c) {
+  return d.profr + ((size_t)p * d.ncrms + c) * d.nz;
+}
+template <typename R>
+__device__ __forceinline__ double& inst(const DevT<R>& d, int p, int c) {
   return d.inst[(size_t)p * d.ncrms + c];
 }
 
-// A segment of NX lanes of one wave64 = one CRM instance.
+// ---- cross-lane data movement inside a wave64 ----------------------------------------------
+// DPP (data-parallel primitives) move a register between lanes on the VALU, without the LDS
+// crossbar that ds_bpermute (what __shfl compiles to) occupies.  Controls, with the lane a
+// destination lane l reads:
+//   wave_rol:1  l+1 (mod 64)       wave_ror:1  l-1 (mod 64)
+//   row_ror:15  l+1 within its 16-lane row      row_ror:1  l-1 within its row
+//   quad_perm [1,0,3,2]  l^1       quad_perm [2,3,0,1]  l^2
+//   row_half_mirror  7-l within its 8-lane half-row    row_mirror  15-l within its row
+// and gfx950's v_permlane16_swap / v_permlane32_swap give the partners l^16 and l^32.
+#ifndef CRM_SEG_BPERMUTE
+constexpr int DPP_QP_XOR1 = 0xB1, DPP_QP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+constexpr int DPP_WAVE_ROL1 = 0x134, DPP_WAVE_ROR1 = 0x13C, DPP_ROW_ROR1 = 0x121, DPP_ROW_ROR15 = 0x12F;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  return __hiloint2double(dpp_i<CTRL>(__double2hiint(v)), dpp_i<CTRL>(__double2loint(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL> __device__ __forceinline__ int lane_mv(int v) { return dpp_i<CTRL>(v); }
+template <int CTRL> __device__ __forceinline__ float lane_mv(float v) { return dpp_f<CTRL>(v); }
+template <int CTRL> __device__ __forceinline__ double lane_mv(double v) { return dpp_d<CTRL>(v); }
+// value of lane l^16 (l^32): the swap exchanges odd rows of its first operand with even rows
+// of its second (upper half of the first with lower half of the second); with both operands
+// equal, a lane in an even row (lower half) finds its partner in the second result.
+__device__ __forceinline__ int xor16_i(int v, int lane) {
+  auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+  return (int)((lane & 16) ? r[0] : r[1]);
+}
+__device__ __forceinline__ int xor32_i(int v, int lane) {
+  auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+  return (int)((lane & 32) ? r[0] : r[1]);
+}
+__device__ __forceinline__ double xor16_d(double v, int lane) {
+  return __hiloint2double(xor16_i(__double2hiint(v), lane), xor16_i(__double2loint(v), lane));
+}
+__device__ __forceinline__ double xor32_d(double v, int lane) {
+  return __hiloint2double(xor32_i(__double2hiint(v), lane), xor32_i(__double2loint(v), lane));
+}
+__device__ __forceinline__ int lane_x16(int v, int lane) { return xor16_i(v, lane); }
+__device__ __forceinline__ float lane_x16(float v, int lane) {
+  return __int_as_float(xor16_i(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_x16(double v, int lane) { return xor16_d(v, lane); }
+__device__ __forceinline__ int lane_x32(int v, int lane) { return xor32_i(v, lane); }
+__device__ __forceinline__ float lane_x32(float v, int lane) {
+  return __int_as_float(xor32_i(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_x32(double v, int lane) { return xor32_d(v, lane); }
+#endif
+
+// A segment of NX lanes of one wave64 = one CRM instance.  All lanes of the wave must take
+// part in every call (the kernels keep control flow uniform around them).
 template <int NX>
 struct Seg {
   int lane, i, base;
@@ -138,39 +236,84 @@ struct Seg {
     i = lane & (NX - 1);
     base = lane - i;
   }
+#ifdef CRM_SEG_BPERMUTE
   // value held by column (i + d) mod NX
-  __device__ __forceinline__ double rot(double v, int d) const {
+  template <typename T>
+  __device__ __forceinline__ T rot(T v, int d) const {
     return __shfl(v, base + ((i + d + NX) & (NX - 1)), 64);
   }
-  __device__ __forceinline__ int roti(int v, int d) const {
-    return __shfl(v, base + ((i + d + NX) & (NX - 1)), 64);
-  }
-  __device__ __forceinline__ double sum(double v) const {
-#pragma unroll
-    for (int m = NX / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-  }
-  // sum over aligned groups of g lanes (g a power of two dividing NX)
-  __device__ __forceinline__ double gsum(double v, int g) const {
+  template <typename T, typename Op>
+  __device__ __forceinline__ T reduce(T v, int g, Op op) const {
 #pragma unroll
     for (int m = NX / 2; m >= 1; m >>= 1)
-      if (m < g) v += __shfl_xor(v, m, 64);
+      if (m < g) v = op(v, __shfl_xor(v, m, 64));
     return v;
   }
-  __device__ __forceinline__ double max(double v) const {
-#pragma unroll
-    for (int m = NX / 2; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+#else
+  // value held by column (i + d) mod NX, d = +1 or -1
+  template <typename T>
+  __device__ __forceinline__ T rot(T v, int d) const {
+    if (d > 0) {
+      if constexpr (NX == 64) {
+        return lane_mv<DPP_WAVE_ROL1>(v);
+      } else if constexpr (NX == 32) {
+        T r = lane_mv<DPP_WAVE_ROL1>(v);      // lane 31 (63) got lane 32 (0)
+        T x = lane_x32(r, lane);              // lane 31 (63) now holds what lane 63 (31) got
+        return i == 31 ? x : r;
+      } else if constexpr (NX == 16) {
+        return lane_mv<DPP_ROW_ROR15>(v);
+      } else {
+        T r = lane_mv<DPP_ROW_ROR15>(v);      // NX == 8: lane 7 (15) got lane 8 (0)
+        T m = lane_mv<DPP_HALF_MIRROR>(v);    // lane 7 (15) reads lane 0 (8)
+        return i == 7 ? m : r;
+      }
+    } else {
+      if constexpr (NX == 64) {
+        return lane_mv<DPP_WAVE_ROR1>(v);
+      } else if constexpr (NX == 32) {
+        T r = lane_mv<DPP_WAVE_ROR1>(v);
+        T x = lane_x32(r, lane);
+        return i == 0 ? x : r;
+      } else if constexpr (NX == 16) {
+        return lane_mv<DPP_ROW_ROR1>(v);
+      } else {
+        T r = lane_mv<DPP_ROW_ROR1>(v);
+        T m = lane_mv<DPP_HALF_MIRROR>(v);
+        return i == 0 ? m : r;
+      }
+    }
+  }
+  // Reductions over aligned groups of g lanes pair lanes symmetrically at every step, so every
+  // lane of a group ends with the bitwise same value.
+  template <typename T, typename Op>
+  __device__ __forceinline__ T reduce(T v, int g, Op op) const {
+    if (g > 1) v = op(v, lane_mv<DPP_QP_XOR1>(v));
+    if (g > 2) v = op(v, lane_mv<DPP_QP_XOR2>(v));
+    if (g > 4) v = op(v, lane_mv<DPP_HALF_MIRROR>(v));
+    if (g > 8) v = op(v, lane_mv<DPP_MIRROR>(v));
+    if (g > 16) v = op(v, lane_x16(v, lane));
+    if (g > 32) v = op(v, lane_x32(v, lane));
     return v;
+  }
+#endif
+  template <typename T>
+  __device__ __forceinline__ T sum(T v) const {
+    return reduce(v, NX, [](T a, T b) { return a + b; });
+  }
+  // sum over aligned groups of g lanes (g a power of two dividing NX)
+  template <typename T>
+  __device__ __forceinline__ T gsum(T v, int g) const {
+    return reduce(v, g, [](T a, T b) { return a + b; });
+  }
+  template <typename T>
+  __device__ __forceinline__ T max(T v) const {
+    return reduce(v, NX, [](T a, T b) { return fmax(a, b); });
   }
   __device__ __forceinline__ int imax(int v) const {
-#pragma unroll
-    for (int m = NX / 2; m >= 1; m >>= 1) v = ::max(v, __shfl_xor(v, m, 64));
-    return v;
+    return reduce(v, NX, [](int a, int b) { return ::max(a, b); });
   }
   __device__ __forceinline__ int imin(int v) const {
-#pragma unroll
-    for (int m = NX / 2; m >= 1; m >>= 1) v = ::min(v, __shfl_xor(v, m, 64));
-    return v;
+    return reduce(v, NX, [](int a, int b) { return ::min(a, b); });
   }
 };
 
@@ -193,8 +336,24 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
 // a plain read-modify-write.
 __device__ __forceinline__ void accum(double* p, double v) { unsafeAtomicAdd(p, v); }
 
+// x**y for x >= 0: the library pow in FP64; in FP32 the hardware base-2 logarithm and
+// exponential (about 3e-7 relative for the microphysics exponents 0.1-1.0 over the
+// mixing-ratio range; the library powf evaluates in extended precision and costs ~10x more).
+template <typename T>
+__device__ __forceinline__ T fpow(T x, T y);
+template <>
+__device__ __forceinline__ double fpow<double>(double x, double y) {
+  return pow(x, y);
+}
+template <>
+__device__ __forceinline__ float fpow<float>(float x, float y) {
+  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
 // ---- saturation functions, crm/sat.F90:12-139 (REAL(4) coefficients) ----------------------
-__device__ __forceinline__ double poly8(const float* a, double dt) {
+// T is double in the FP64 form and float for the FP32 fields of the mixed form.
+template <typename T>
+__device__ __forceinline__ T poly8(const float* a, T dt) {
   return a[0] + dt * (a[1] + dt * (a[2] + dt * (a[3] + dt * (a[4] + dt * (a[5] + dt * (a[6] + dt * (a[7] + a[8] * dt)))))));
 }
 __constant__ float c_esatw[9] = {6.105851f, 0.4440316f, 0.1430341e-1f, 0.2641412e-3f, 0.2995057e-5f,
@@ -208,36 +367,43 @@ __constant__ float c_dtesati[9] = {0.503223089f, 0.377174432e-1f, 0.126710138e-2
                                    0.312668753e-6f, 0.255653718e-8f, 0.132073448e-10f, 0.390204672e-13f,
                                    0.497275778e-16f};
 
-__device__ __forceinline__ double esat_low(double t) {
-  return exp(R4(9.550426) - R4(5723.265) / t + R4(3.53068) * log(t) - R4(0.00728332) * t);
+template <typename T>
+__device__ __forceinline__ T esat_low(T t) {
+  return exp((T)R4(9.550426) - (T)R4(5723.265) / t + (T)R4(3.53068) * log(t) - (T)R4(0.00728332) * t);
 }
-__device__ __forceinline__ double esatw(double t) {
-  double dt = t - R4(273.16);
-  if (dt > -80.0) return poly8(c_esatw, dt);
-  return R4(0.01) * 2.0 * esat_low(t);
+template <typename T>
+__device__ __forceinline__ T esatw(T t) {
+  T dt = t - (T)R4(273.16);
+  if (dt > (T)-80.0) return poly8(c_esatw, dt);
+  return (T)R4(0.01) * (T)2.0 * esat_low(t);
 }
-__device__ __forceinline__ double esati(double t) {
-  double dt = t - R4(273.16);
-  if (dt > -80.0) return poly8(c_esati, dt);
-  return R4(0.01) * esat_low(t);
+template <typename T>
+__device__ __forceinline__ T esati(T t) {
+  T dt = t - (T)R4(273.16);
+  if (dt > (T)-80.0) return poly8(c_esati, dt);
+  return (T)R4(0.01) * esat_low(t);
 }
-__device__ __forceinline__ double qsatw(double t, double p) {
-  double e = esatw(t);
-  return R4(0.622) * e / fmax(e, p - e);
+template <typename T>
+__device__ __forceinline__ T qsatw(T t, T p) {
+  T e = esatw(t);
+  return (T)R4(0.622) * e / fmax(e, p - e);
 }
-__device__ __forceinline__ double qsati(double t, double p) {
-  double e = esati(t);
-  return R4(0.622) * e / fmax(e, p - e);
+template <typename T>
+__device__ __forceinline__ T qsati(T t, T p) {
+  T e = esati(t);
+  return (T)R4(0.622) * e / fmax(e, p - e);
 }
-__device__ __forceinline__ double dtqsatw(double t, double p) {
-  double dt = t - R4(273.16);
-  double de = dt > -80.0 ? poly8(c_dtesatw, dt) : esatw(t + 1) - esatw(t);
-  return R4(0.622) * de / p;
+template <typename T>
+__device__ __forceinline__ T dtqsatw(T t, T p) {
+  T dt = t - (T)R4(273.16);
+  T de = dt > (T)-80.0 ? poly8(c_dtesatw, dt) : esatw(t + (T)1) - esatw(t);
+  return (T)R4(0.622) * de / p;
 }
-__device__ __forceinline__ double dtqsati(double t, double p) {
-  double dt = t - R4(273.16);
-  double de = dt > -80.0 ? poly8(c_dtesati, dt) : esati(t + 1.0) - esati(t);
-  return R4(0.622) * de / p;
+template <typename T>
+__device__ __forceinline__ T dtqsati(T t, T p) {
+  T dt = t - (T)R4(273.16);
+  T de = dt > (T)-80.0 ? poly8(c_dtesati, dt) : esati(t + (T)1.0) - esati(t);
+  return (T)R4(0.622) * de / p;
 }
 
-}  // namespace crm
+}  // namespace CRM_NS

@@ -1,0 +1,725 @@
+// crm_driver.hpp -- the device engine of the HIP CRM: context state, device buffers, the
+// substep loop and host staging, for one bulk-field type R.  Included once per translation
+// unit: crm_hip.hip instantiates it for FP64 (namespace crm), crm_hip_mixed.hip for the
+// mixed-precision form with FP32 bulk fields (namespace crm_mixed).
+//
+// The loop mirrors crm() (crm/crm_module.F90:712-1075): per GCM substep a CFL check whose
+// batch-global maximum decides the subcycle count, then per subcycle the phase-group kernels
+// of crm_kernels.hpp, then the per-substep statistics; after the loop the outputs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/crm_hip.h"
+#include "../../include/crm_hip_debug.h"
+#include "crm_engine.hpp"
+#include "crm_kernels.hpp"
+
+namespace crm {
+// Transform operators and eigenvalues of the pressure solve, built on the host in FP64 by the
+// FP64 unit for every engine (crm_hip.hip).
+void pressure_transform_host(int nx, double dx, std::vector<double>& Ft, std::vector<double>& Gt,
+                             std::vector<double>& eign);
+}  // namespace crm
+
+namespace CRM_NS {
+
+namespace {
+
+struct Transform {
+  double *Ft = nullptr, *Gt = nullptr, *eign = nullptr;
+};
+
+enum { NSCAL = 17 };  // u t qt qp qn tabs qv qcl qci qpl qpi tk tkh tke dudt x3
+
+}  // namespace
+
+template <typename R>
+struct Engine : CrmEngine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // device state sized for (cap_ncrms, cap_nx, cap_nzm, cap_nxr)
+  int cap_ncrms = 0, cap_nx = 0, cap_nzm = 0, cap_nxr = 0;
+  char* pool = nullptr;
+  size_t pool_bytes = 0;
+  unsigned long long* gmax = nullptr;
+  int* gcol_dev = nullptr;
+  int gcol_cap = 0;
+  std::map<std::pair<int, double>, Transform> transforms;
+  // staging of host views
+  double* stage = nullptr;
+  size_t stage_bytes = 0;
+  DevT<R> dev{};
+  int na = 0, nb = 1, nc = 2;
+  crm_hip_stats stats{};
+  bool last_ok = false;
+  // per-kernel-class timing with HIP events on `stream` (crm_hip_profile_kernels)
+  unsigned prof_mask = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_kid;  // kernel class of each recorded pair
+  size_t ev_used = 0;       // pairs recorded in the current call
+  double kt_ms[CRM_K_COUNT] = {};
+  long long kt_n[CRM_K_COUNT] = {};
+
+  ~Engine() override;
+  int run(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+          crm_state_view* state, crm_rad_view* rad, crm_output_view* out, bool device_views,
+          int phase_stop) override;
+  int debug_get(const char* name, double* buf) override;
+  void get_stats(crm_hip_stats* s) const override { *s = stats; }
+  int profile_kernels(unsigned mask) override;
+  int kernel_time(int kid, double* ms, long long* n) const override;
+};
+
+// Start timing one launch of kernel class kid; returns the pair index or -1.
+template <typename R>
+static long prof_begin(Engine<R>* ctx, int kid) {
+  if (!((ctx->prof_mask >> kid) & 1u)) return -1;
+  size_t pair = ctx->ev_used;
+  while (ctx->ev_pool.size() < 2 * (pair + 1)) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    ctx->ev_pool.push_back(e);
+  }
+  if (ctx->ev_kid.size() < pair + 1) ctx->ev_kid.resize(pair + 1);
+  ctx->ev_kid[pair] = kid;
+  ctx->ev_used = pair + 1;
+  (void)hipEventRecord(ctx->ev_pool[2 * pair], ctx->stream);
+  return (long)pair;
+}
+template <typename R>
+static void prof_end(Engine<R>* ctx, long pair) {
+  if (pair >= 0) (void)hipEventRecord(ctx->ev_pool[2 * pair + 1], ctx->stream);
+}
+// After the stream has drained: fold the recorded pairs into the per-class totals.
+template <typename R>
+static void prof_harvest(Engine<R>* ctx) {
+  for (size_t q = 0; q < ctx->ev_used; q++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev_pool[2 * q], ctx->ev_pool[2 * q + 1]) == hipSuccess) {
+      ctx->kt_ms[ctx->ev_kid[q]] += ms;
+      ctx->kt_n[ctx->ev_kid[q]] += 1;
+    }
+  }
+  ctx->ev_used = 0;
+}
+#define TIMED(kid, stmt)                   \
+  do {                                     \
+    long pr_ = prof_begin(ctx, kid);       \
+    stmt;                                  \
+    prof_end(ctx, pr_);                    \
+  } while (0)
+
+#define HIPCHECK(x)                                                                 \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      fprintf(stderr, "crm_hip: %s failed: %s\n", #x, hipGetErrorString(e_));       \
+      return CRM_ERR_DEVICE;                                                        \
+    }                                                                               \
+  } while (0)
+
+static bool supported_nx(int nx) { return nx == 8 || nx == 16 || nx == 32 || nx == 64; }
+static bool supported_nz(int nz) { return nz == 28 || nz == 58; }
+
+// REAL(4) Gamma of the reference (gammafff.c:13 via precip_init.F90:24-45)
+static double gammafff(double x_in) {
+  float x = (float)x_in;
+  return (double)(float)std::exp(std::lgamma((double)x));
+}
+
+template <typename R>
+static void host_constants(DevT<R>& d) {
+  const double b_rain = (double)0.8f, b_snow = 0.25, b_grau = 0.5;
+  d.gam.gamr1 = gammafff(3. + b_rain);
+  d.gam.gamr2 = gammafff((5. + b_rain) / 2.);
+  d.gam.gamr3 = gammafff(4. + b_rain);
+  d.gam.gams1 = gammafff(3. + b_snow);
+  d.gam.gams2 = gammafff((5. + b_snow) / 2.);
+  d.gam.gams3 = gammafff(4. + b_snow);
+  d.gam.gamg1 = gammafff(3. + b_grau);
+  d.gam.gamg2 = gammafff((5. + b_grau) / 2.);
+  d.gam.gamg3 = gammafff(4. + b_grau);
+  const double pi = (double)3.141592653589793f;
+  d.tv.crain = b_rain / 4.;
+  d.tv.csnow = b_snow / 4.;
+  d.tv.cgrau = b_grau / 4.;
+  d.tv.vrain = 842.0 * d.gam.gamr3 / 6. / std::pow(pi * 1000.0 * 8.e6, d.tv.crain);
+  d.tv.vsnow = (double)4.84f * d.gam.gams3 / 6. / std::pow(pi * 100.0 * 3.e6, d.tv.csnow);
+  d.tv.vgrau = 94.5 * d.gam.gamg3 / 6. / std::pow(pi * 400.0 * 4.e6, d.tv.cgrau);
+  volatile float two = 2.f;
+  d.z0c1 = (double)(3.14159f / 2.f - 3.f * std::log(two));
+}
+
+template <typename R>
+static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr) {
+  if (ncrms <= ctx->cap_ncrms && nx == ctx->cap_nx && nzm == ctx->cap_nzm && nxr == ctx->cap_nxr) return CRM_OK;
+  int ncap = std::max(ncrms, nx == ctx->cap_nx && nzm == ctx->cap_nzm ? ctx->cap_ncrms : 0);
+  int nz = nzm + 1;
+  const size_t scal = (size_t)ncap * nzm * nx, wl = (size_t)ncap * nz * nx;
+  constexpr bool mirror = !std::is_same<R, double>::value;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t sizes[] = {NSCAL * scal * sizeof(R),                        // bulk scalar fields
+                          4 * wl * sizeof(R),                              // w and dwdt x3
+                          2 * (size_t)ncap * nx * sizeof(double),          // precsfc, precssfc
+                          (size_t)NPROF * ncap * nz * sizeof(double),      // profiles
+                          mirror ? (size_t)NPROF * ncap * nz * sizeof(R) : 0,
+                          (size_t)NINST * ncap * sizeof(double),
+                          (size_t)NCOEF * ncap * nz * sizeof(R),
+                          5 * (size_t)ncap * nzm * nxr * sizeof(double)};  // radiation sums
+  size_t bytes = 0;
+  for (size_t b : sizes) bytes += al(b);
+  if (ctx->pool) HIPCHECK(hipFree(ctx->pool));
+  ctx->pool = nullptr;
+  if (hipMalloc(&ctx->pool, bytes) != hipSuccess) {
+    ctx->cap_ncrms = 0;
+    return CRM_ERR_NOMEM;
+  }
+  HIPCHECK(hipMemset(ctx->pool, 0, bytes));
+  ctx->pool_bytes = bytes;
+  ctx->cap_ncrms = ncap;
+  ctx->cap_nx = nx;
+  ctx->cap_nzm = nzm;
+  ctx->cap_nxr = nxr;
+  char* p = ctx->pool;
+  auto take = [&](size_t b) {
+    char* q = p;
+    p += al(b);
+    return q;
+  };
+  DevT<R>& d = ctx->dev;
+  R* f = (R*)take(sizes[0]);
+  R** sc[NSCAL] = {&d.u, &d.t, &d.qt, &d.qp, &d.qn, &d.tabs, &d.qv, &d.qcl, &d.qci,
+                   &d.qpl, &d.qpi, &d.tk, &d.tkh, &d.tke, &d.dudt[0], &d.dudt[1], &d.dudt[2]};
+  for (int q = 0; q < NSCAL; q++, f += scal) *sc[q] = f;
+  R* wf = (R*)take(sizes[1]);
+  d.w = wf;
+  for (int q = 0; q < 3; q++) d.dwdt[q] = wf + (size_t)(q + 1) * wl;
+  double* ps = (double*)take(sizes[2]);
+  d.precsfc = ps;
+  d.precssfc = ps + (size_t)ncap * nx;
+  d.prof = (double*)take(sizes[3]);
+  if constexpr (mirror) {
+    d.profr = (R*)take(sizes[4]);
+  } else {
+    take(0);
+    d.profr = (R*)d.prof;
+  }
+  d.inst = (double*)take(sizes[5]);
+  d.coef = (R*)take(sizes[6]);
+  d.rad = (double*)take(sizes[7]);
+  return CRM_OK;
+}
+
+template <typename R>
+static int get_transform(Engine<R>* ctx, int nx, double dx, Transform& out) {
+  auto key = std::make_pair(nx, dx);
+  auto it = ctx->transforms.find(key);
+  if (it != ctx->transforms.end()) {
+    out = it->second;
+    return CRM_OK;
+  }
+  std::vector<double> Ft, Gt, eign;
+  crm::pressure_transform_host(nx, dx, Ft, Gt, eign);
+  Transform t;
+  HIPCHECK(hipMalloc(&t.Ft, sizeof(double) * nx * nx));
+  HIPCHECK(hipMalloc(&t.Gt, sizeof(double) * nx * nx));
+  HIPCHECK(hipMalloc(&t.eign, sizeof(double) * nx));
+  HIPCHECK(hipMemcpy(t.Ft, Ft.data(), sizeof(double) * nx * nx, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(t.Gt, Gt.data(), sizeof(double) * nx * nx, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(t.eign, eign.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
+  ctx->transforms[key] = t;
+  out = t;
+  return CRM_OK;
+}
+
+// ---- kernel dispatch over the compile-time column count ------------------------------------
+template <int NX, typename R>
+static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, const DevT<R>& d, int na, int nb, int nc) {
+  if (NX == 64) {  // matrix-core transforms; one instance per block
+    if (nzm == 28)
+      k_press64<28, R><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+    else
+      k_press64<58, R><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+    return;
+  }
+  if (nzm == 28)
+    k_press<NX, 28, R><<<g, b, sh, s>>>(d, na, nb, nc);
+  else
+    k_press<NX, 58, R><<<g, b, sh, s>>>(d, na, nb, nc);
+}
+
+struct Launcher {
+  int nx, nzm, ncrms;
+  hipStream_t s;
+  dim3 grid_seg() const { return dim3((unsigned)(((size_t)ncrms * nx + 255) / 256)); }
+  dim3 grid_inst(int b) const { return dim3((unsigned)((ncrms + b - 1) / b)); }
+};
+
+#define NX_DISPATCH(nx, CALL)       \
+  switch (nx) {                     \
+    case 8: {                       \
+      constexpr int NXV = 8;        \
+      CALL;                         \
+    } break;                        \
+    case 16: {                      \
+      constexpr int NXV = 16;       \
+      CALL;                         \
+    } break;                        \
+    case 32: {                      \
+      constexpr int NXV = 32;       \
+      CALL;                         \
+    } break;                        \
+    default: {                      \
+      constexpr int NXV = 64;       \
+      CALL;                         \
+    } break;                        \
+  }
+
+static void abcoefs(int nstep, const double* dt3, int na, int nb, int nc, double& at, double& bt, double& ct) {
+  if (nstep >= 3) {
+    double alpha = dt3[nb] / dt3[na], beta = dt3[nc] / dt3[na];
+    ct = (2. + 3. * alpha) / (6. * (alpha + beta) * beta);
+    bt = -(1. + 2. * (alpha + beta) * ct) / (2. * alpha);
+    at = 1. - bt - ct;
+  } else if (nstep >= 2) {
+    at = 3. / 2.;
+    bt = -1. / 2.;
+    ct = 0.;
+  } else {
+    at = 1.;
+    bt = 0.;
+    ct = 0.;
+  }
+}
+
+static int validate(const crm_dims* dm) {
+  if (!dm || dm->ncrms < 1 || dm->ld < dm->ncrms || dm->plev < dm->nz + 1) return CRM_ERR_ARG;
+  if (dm->ny != 1 || dm->ny_rad != 1) return CRM_ERR_UNSUPPORTED;
+  if (!supported_nx(dm->nx) || !supported_nz(dm->nz)) return CRM_ERR_UNSUPPORTED;
+  if (dm->nx_rad < 1 || dm->nx % dm->nx_rad != 0) return CRM_ERR_NXRAD;
+  if ((dm->nx / dm->nx_rad) & ((dm->nx / dm->nx_rad) - 1)) return CRM_ERR_UNSUPPORTED;
+  if (60. < 2 * dm->dt) return CRM_ERR_DAMPING;
+  return CRM_OK;
+}
+
+// The whole call on device-resident views.  phase_stop > 0 stops after that phase group of
+// the last substep (debug entry only).
+template <typename R>
+static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* col, double dt_gl,
+                    const crm_input_view* in, crm_state_view* st, crm_rad_view* rad, crm_output_view* out,
+                    int phase_stop) {
+  int rc = validate(dm);
+  if (rc) return rc;
+  const int nx = dm->nx, nzm = dm->nz, ncrms = dm->ncrms;
+  rc = ensure_buffers(ctx, ncrms, nx, nzm, dm->nx_rad);
+  if (rc) return rc;
+  Transform tf;
+  rc = get_transform(ctx, nx, dm->dx, tf);
+  if (rc) return rc;
+  DevT<R>& d = ctx->dev;
+  d.ncrms = ncrms;
+  d.nzm = nzm;
+  d.nz = nzm + 1;
+  d.plev = dm->plev;
+  d.ld = dm->ld;
+  d.nx_rad = dm->nx_rad;
+  d.dx = dm->dx;
+  d.dt = dm->dt;
+  d.Ft = tf.Ft;
+  d.Gt = tf.Gt;
+  d.eign = tf.eign;
+  d.gmax = ctx->gmax;
+  host_constants(d);
+  hipStream_t s = ctx->stream;
+  Launcher L{nx, nzm, ncrms, s};
+  const dim3 b256(256);
+  ctx->last_ok = false;
+  ctx->stats = crm_hip_stats{};
+
+  TIMED(CRM_K_SETUP, (k_setup<R><<<L.grid_inst(64), dim3(64), 0, s>>>(d, *in)));
+  TIMED(CRM_K_INIT, NX_DISPATCH(nx, (k_init<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *out, dt_gl))));
+  if (col && col->igstep <= 1) {
+    if (ctx->gcol_cap < ncrms) {
+      if (ctx->gcol_dev) HIPCHECK(hipFree(ctx->gcol_dev));
+      HIPCHECK(hipMalloc(&ctx->gcol_dev, sizeof(int) * ncrms));
+      ctx->gcol_cap = ncrms;
+    }
+    HIPCHECK(hipMemcpyAsync(ctx->gcol_dev, col->gcol, sizeof(int) * ncrms, hipMemcpyHostToDevice, s));
+    TIMED(CRM_K_SETPERTURB, NX_DISPATCH(nx, (k_setperturb<NXV, R><<<L.grid_inst(64), dim3(64), 0, s>>>(d, ctx->gcol_dev))));
+  }
+  if constexpr (!std::is_same<R, double>::value) {
+    // FP32 copies of the level profiles for the bulk kernels
+    const size_t np = (size_t)NPROF * ncrms * (nzm + 1);
+    k_prof_mirror<R><<<dim3((unsigned)((np + 255) / 256)), b256, 0, s>>>(d.prof, d.profr, np);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemsetAsync(ctx->gmax + 2, 0, sizeof(unsigned long long), s));
+
+  if (!(dt_gl >= dm->dt) || !(dt_gl / dm->dt < 1.0e6)) return CRM_ERR_ARG;
+  const int nstop = (int)(dt_gl / dm->dt);
+  const double dt = dt_gl / nstop;
+  d.dt = dt;
+  ctx->stats.nstop = nstop;
+  double dt3[3] = {0.0, 0.0, 0.0};
+  int na = 0, nb = 1, nc = 2;
+  const dim3 gpress((unsigned)((ncrms + (64 / nx) - 1) / (64 / nx)));
+  const size_t press_lds = (size_t)nzm * 64 * sizeof(double);
+  unsigned long long err_flag = 0;
+  HIPCHECK(hipEventRecord(ctx->ev0, s));
+  for (int nstep = 1; nstep <= nstop; nstep++) {
+    d.nstep = nstep;
+    unsigned long long bits = 0;
+    HIPCHECK(hipMemsetAsync(ctx->gmax, 0, sizeof(unsigned long long), s));
+    TIMED(CRM_K_KURANT, NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+    HIPCHECK(hipMemcpyAsync(&bits, ctx->gmax, sizeof(bits), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    double cfl;
+    std::memcpy(&cfl, &bits, sizeof(cfl));
+    // ncycle = max(1, ceiling(cfl / 0.7)) > 16 is an abort in the reference (kurant.F90:53-75);
+    // the test is made before the conversion so that huge or non-finite values cannot overflow
+    const double q = cfl / (double)0.7f;
+    if (q != q) return CRM_ERR_NAN;
+    if (!(q <= 16.0)) return CRM_ERR_CFL;
+    int ncycle = std::max(1, (int)std::ceil(q));
+    ctx->stats.ncycle_max = std::max(ctx->stats.ncycle_max, ncycle);
+    const bool last = (nstep == nstop);
+    for (int icyc = 1; icyc <= ncycle; icyc++) {
+      d.icycle = icyc;
+      d.dtn = dt / ncycle;
+      dt3[na] = d.dtn;
+      d.dtfactor = d.dtn / dt;
+      abcoefs(nstep, dt3, na, nb, nc, d.at, d.bt, d.ct);
+      d.dt3na = dt3[na];
+      if (icyc == 1)
+        TIMED(CRM_K_PRECIP_INIT,
+              (k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d)));
+      const double* qrad = rad->qrad;
+      TIMED(CRM_K_PRE, NX_DISPATCH(nx, (k_pre<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, na, qrad))));
+      if (last && phase_stop == 7) goto stopped;
+      TIMED(CRM_K_SGS, NX_DISPATCH(nx, (k_sgs<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      if (last && phase_stop == 9) goto stopped;
+      TIMED(CRM_K_MOM, NX_DISPATCH(nx, (k_mom<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, na))));
+      if (last && phase_stop == 11) goto stopped;
+      TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
+      if (last && phase_stop == 13) goto stopped;
+      HIPCHECK(hipMemsetAsync(ctx->gmax + 1, 0, sizeof(unsigned long long), s));
+      TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      if (last && phase_stop == 15) goto stopped;
+      std::swap(d.u, d.dudt[nc]);
+      std::swap(d.w, d.dwdt[nc]);
+      if (last && phase_stop == 17) goto stopped;
+      TIMED(CRM_K_DIFFUSE, NX_DISPATCH(nx, (k_diffuse<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      if (last && phase_stop == 18) goto stopped;
+      TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_micro<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      if (last && phase_stop == 20) goto stopped;
+      int nn = na;
+      na = nc;
+      nc = nb;
+      nb = nn;
+    }
+    TIMED(CRM_K_STATS, NX_DISPATCH(nx, (k_stats<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+    HIPCHECK(hipGetLastError());
+  }
+  TIMED(CRM_K_FINAL, NX_DISPATCH(nx, (k_final<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop))));
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(ctx->ev1, s));
+  HIPCHECK(hipMemcpyAsync(&err_flag, ctx->gmax + 2, sizeof(err_flag), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.ms_device = ms;
+  }
+  prof_harvest(ctx);
+  ctx->na = na;
+  ctx->nb = nb;
+  ctx->nc = nc;
+  if (err_flag) return err_flag == ERR_FLAG_NAN ? CRM_ERR_NAN : CRM_ERR_CFL;
+  ctx->last_ok = true;
+  return CRM_OK;
+stopped:
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  prof_harvest(ctx);
+  ctx->na = na;
+  ctx->nb = nb;
+  ctx->nc = nc;
+  return CRM_OK;
+}
+
+// ---- staging of host views ---------------------------------------------------------------
+namespace {
+struct ViewField {
+  const double* host;
+  double* dev;
+  size_t n;
+  bool in, outw;
+};
+}  // namespace
+
+template <typename R>
+static int stage_views(Engine<R>* ctx, const crm_dims* dm, const crm_input_view* in, crm_state_view* st,
+                       crm_rad_view* rad, crm_output_view* out, crm_input_view& din, crm_state_view& dst,
+                       crm_rad_view& drad, crm_output_view& dout, std::vector<ViewField>& fields) {
+  const size_t ld = dm->ld, plev = dm->plev, g4 = ld * dm->nx * dm->ny * dm->nz, g3 = ld * dm->nx * dm->ny;
+  const size_t r4 = ld * dm->nx_rad * dm->ny_rad * dm->nz;
+  fields.clear();
+  auto add = [&](const double* h, size_t n, bool inb, bool outw) {
+    fields.push_back({h, nullptr, n, inb, outw});
+  };
+  const double* ip[] = {in->zmid, in->zint, in->tl, in->ql, in->qccl, in->qiil, in->ps, in->pmid, in->pint,
+                        in->pdel, in->phis, in->ul, in->vl, in->ocnfrac, in->tau00, in->wndls, in->bflxls,
+                        in->fluxu00, in->fluxv00, in->fluxt00, in->fluxq00};
+  size_t isz[] = {ld * plev, ld * (plev + 1), ld * plev, ld * plev, ld * plev, ld * plev, ld, ld * plev,
+                  ld * (plev + 1), ld * plev, ld, ld * plev, ld * plev, ld, ld, ld, ld, ld, ld, ld, ld};
+  for (int q = 0; q < 21; q++) add(ip[q], ip[q] ? isz[q] : 0, true, false);
+  double* sp[] = {st->u_wind, st->v_wind, st->w_wind, st->temperature, st->qt, st->qp, st->qn};
+  for (double* p : sp) add(p, g4, true, true);
+  add(rad->qrad, r4, true, false);
+  double* rp[] = {rad->temperature, rad->qv, rad->qc, rad->qi, rad->cld};
+  for (double* p : rp) add(p, r4, true, true);
+  double* og[] = {out->qcl, out->qci, out->qpl, out->qpi, out->tk, out->tkh};
+  for (double* p : og) add(p, g4, true, true);
+  add(out->prec_crm, g3, true, true);
+  double* oc[] = {out->cltot, out->clhgh, out->clmed, out->cllow, out->precc, out->precl, out->precsc,
+                  out->precsl, out->jt_crm, out->mx_crm, out->prectend, out->precstend, out->taux,
+                  out->tauy, out->z0m, out->timing_factor};
+  for (double* p : oc) add(p, ld, true, true);
+  double* opl[] = {out->cldtop, out->qc_mean, out->qi_mean, out->qs_mean, out->qg_mean, out->qr_mean,
+                   out->sltend, out->qltend, out->qcltend, out->qiltend, out->cld, out->gicewp,
+                   out->gliqwp, out->mctot, out->mcup, out->mcdn, out->mcuup, out->mcudn, out->mu_crm,
+                   out->md_crm, out->du_crm, out->eu_crm, out->ed_crm, out->flux_qt, out->fluxsgs_qt,
+                   out->tkez, out->tkesgsz, out->tkz, out->flux_u, out->flux_v, out->flux_qp,
+                   out->precflux, out->qt_ls, out->qt_trans, out->qp_trans, out->qp_fall, out->qp_src,
+                   out->qp_evp, out->t_ls};
+  for (double* p : opl) add(p, ld * plev, true, true);
+  size_t total = 0;
+  for (auto& f : fields) {
+    if (!f.host && f.n) {
+      // a missing optional input member (fluxu00.. are unused by crm()) stays absent
+      if (&f - &fields[0] >= 17 && &f - &fields[0] <= 20) {
+        f.n = 0;
+        continue;
+      }
+      return CRM_ERR_ARG;
+    }
+    total += (f.n + 31) & ~size_t(31);
+  }
+  if (total * sizeof(double) > ctx->stage_bytes) {
+    if (ctx->stage) HIPCHECK(hipFree(ctx->stage));
+    ctx->stage = nullptr;
+    if (hipMalloc(&ctx->stage, total * sizeof(double)) != hipSuccess) {
+      ctx->stage_bytes = 0;
+      return CRM_ERR_NOMEM;
+    }
+    ctx->stage_bytes = total * sizeof(double);
+  }
+  double* p = ctx->stage;
+  for (auto& f : fields) {
+    if (!f.n) continue;
+    f.dev = p;
+    p += (f.n + 31) & ~size_t(31);
+    HIPCHECK(hipMemcpyAsync(f.dev, f.host, f.n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  size_t q = 0;
+  const double** di[] = {&din.zmid, &din.zint, &din.tl, &din.ql, &din.qccl, &din.qiil, &din.ps, &din.pmid,
+                         &din.pint, &din.pdel, &din.phis, &din.ul, &din.vl, &din.ocnfrac, &din.tau00,
+                         &din.wndls, &din.bflxls, &din.fluxu00, &din.fluxv00, &din.fluxt00, &din.fluxq00};
+  for (auto pp : di) *pp = fields[q++].dev;
+  double** ds[] = {&dst.u_wind, &dst.v_wind, &dst.w_wind, &dst.temperature, &dst.qt, &dst.qp, &dst.qn};
+  for (auto pp : ds) *pp = fields[q++].dev;
+  drad.qrad = fields[q++].dev;
+  double** dr[] = {&drad.temperature, &drad.qv, &drad.qc, &drad.qi, &drad.cld};
+  for (auto pp : dr) *pp = fields[q++].dev;
+  double** dg[] = {&dout.qcl, &dout.qci, &dout.qpl, &dout.qpi, &dout.tk, &dout.tkh, &dout.prec_crm,
+                   &dout.cltot, &dout.clhgh, &dout.clmed, &dout.cllow, &dout.precc, &dout.precl,
+                   &dout.precsc, &dout.precsl, &dout.jt_crm, &dout.mx_crm, &dout.prectend,
+                   &dout.precstend, &dout.taux, &dout.tauy, &dout.z0m, &dout.timing_factor,
+                   &dout.cldtop, &dout.qc_mean, &dout.qi_mean, &dout.qs_mean, &dout.qg_mean,
+                   &dout.qr_mean, &dout.sltend, &dout.qltend, &dout.qcltend, &dout.qiltend, &dout.cld,
+                   &dout.gicewp, &dout.gliqwp, &dout.mctot, &dout.mcup, &dout.mcdn, &dout.mcuup,
+                   &dout.mcudn, &dout.mu_crm, &dout.md_crm, &dout.du_crm, &dout.eu_crm, &dout.ed_crm,
+                   &dout.flux_qt, &dout.fluxsgs_qt, &dout.tkez, &dout.tkesgsz, &dout.tkz, &dout.flux_u,
+                   &dout.flux_v, &dout.flux_qp, &dout.precflux, &dout.qt_ls, &dout.qt_trans,
+                   &dout.qp_trans, &dout.qp_fall, &dout.qp_src, &dout.qp_evp, &dout.t_ls};
+  for (auto pp : dg) *pp = fields[q++].dev;
+  return CRM_OK;
+}
+
+template <typename R>
+static int unstage(Engine<R>* ctx, std::vector<ViewField>& fields) {
+  for (auto& f : fields)
+    if (f.outw && f.n) HIPCHECK(hipMemcpyAsync((void*)f.host, f.dev, f.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return CRM_OK;
+}
+
+template <typename R>
+static int run_host(Engine<R>* ctx, const crm_dims* dims, const crm_column_info* col, double dt_gl,
+                    const crm_input_view* in, crm_state_view* state, crm_rad_view* rad, crm_output_view* out,
+                    int phase_stop) {
+  if (!ctx || !dims || !in || !state || !rad || !out) return CRM_ERR_ARG;
+  int rc = validate(dims);
+  if (rc) return rc;
+  crm_input_view din{};
+  crm_state_view dst{};
+  crm_rad_view drad{};
+  crm_output_view dout{};
+  std::vector<ViewField> fields;
+  rc = stage_views(ctx, dims, in, state, rad, out, din, dst, drad, dout, fields);
+  if (rc) return rc;
+  rc = run_core(ctx, dims, col, dt_gl, &din, &dst, &drad, &dout, phase_stop);
+  if (rc) return rc;
+  if (phase_stop == 0) rc = unstage(ctx, fields);
+  return rc;
+}
+
+
+template <typename R>
+Engine<R>::~Engine() {
+  (void)hipSetDevice(device);
+  for (auto& kv : transforms) {
+    (void)hipFree(kv.second.Ft);
+    (void)hipFree(kv.second.Gt);
+    (void)hipFree(kv.second.eign);
+  }
+  if (pool) (void)hipFree(pool);
+  if (stage) (void)hipFree(stage);
+  if (gmax) (void)hipFree(gmax);
+  if (gcol_dev) (void)hipFree(gcol_dev);
+  for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename R>
+int Engine<R>::run(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+                   crm_state_view* state, crm_rad_view* rad, crm_output_view* out, bool device_views,
+                   int phase_stop) {
+  if (!dims || !in || !state || !rad || !out) return CRM_ERR_ARG;
+  (void)hipSetDevice(device);
+  if (device_views) return run_core(this, dims, col, dt_gl, in, state, rad, out, phase_stop);
+  return run_host(this, dims, col, dt_gl, in, state, rad, out, phase_stop);
+}
+
+template <typename R>
+int Engine<R>::profile_kernels(unsigned mask) {
+  prof_mask = mask;
+  for (int k = 0; k < CRM_K_COUNT; k++) {
+    kt_ms[k] = 0.0;
+    kt_n[k] = 0;
+  }
+  return CRM_OK;
+}
+
+template <typename R>
+int Engine<R>::kernel_time(int kid, double* ms, long long* n) const {
+  if (kid < 0 || kid >= CRM_K_COUNT || !ms || !n) return CRM_ERR_ARG;
+  *ms = kt_ms[kid];
+  *n = kt_n[kid];
+  return CRM_OK;
+}
+
+template <typename R>
+int Engine<R>::debug_get(const char* name, double* buf) {
+  Engine<R>* ctx = this;
+  if (!ctx->pool || !name || !buf) return -1;
+  const DevT<R>& d = ctx->dev;
+  const int ncrms = d.ncrms, nzm = d.nzm, nz = d.nz, nx = ctx->cap_nx;
+  std::string n(name);
+  const R* grid = nullptr;
+  const double* grid64 = nullptr;
+  int nk = nzm;
+  if (n == "u") grid = d.u;
+  else if (n == "w") { grid = d.w; nk = nz; }
+  else if (n == "t") grid = d.t;
+  else if (n == "tabs") grid = d.tabs;
+  else if (n == "qv") grid = d.qv;
+  else if (n == "qcl") grid = d.qcl;
+  else if (n == "qci") grid = d.qci;
+  else if (n == "qpl") grid = d.qpl;
+  else if (n == "qpi") grid = d.qpi;
+  else if (n == "qt") grid = d.qt;
+  else if (n == "qp") grid = d.qp;
+  else if (n == "qn") grid = d.qn;
+  else if (n == "tk") grid = d.tk;
+  else if (n == "tkh") grid = d.tkh;
+  else if (n == "tke") grid = d.tke;
+  else if (n == "dudt_na") grid = d.dudt[ctx->na];
+  else if (n == "dudt_nb") grid = d.dudt[ctx->nb];
+  else if (n == "dudt_nc") grid = d.dudt[ctx->nc];
+  else if (n == "dwdt_na") { grid = d.dwdt[ctx->na]; nk = nz; }
+  else if (n == "dwdt_nb") { grid = d.dwdt[ctx->nb]; nk = nz; }
+  else if (n == "dwdt_nc") { grid = d.dwdt[ctx->nc]; nk = nz; }
+  else if (n == "precsfc") { grid64 = d.precsfc; nk = 1; }
+  else if (n == "precssfc") { grid64 = d.precssfc; nk = 1; }
+  if (grid64) {
+    size_t cnt = (size_t)ncrms * nk * nx;
+    if (hipMemcpy(buf, grid64, cnt * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return (int)cnt;
+  }
+  if (grid) {
+    size_t cnt = (size_t)ncrms * nk * nx;
+    std::vector<R> tmp(cnt);
+    if (hipMemcpy(tmp.data(), grid, cnt * sizeof(R), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    for (size_t q = 0; q < cnt; q++) buf[q] = (double)tmp[q];
+    return (int)cnt;
+  }
+  static const std::map<std::string, std::pair<int, int>> profs = {
+      {"t0", {P_T0, 0}}, {"q0", {P_Q0, 0}}, {"qv0", {P_QV0, 0}}, {"tabs0", {P_TABS0, 0}}, {"u0", {P_U0, 0}},
+      {"qn0", {P_QN0, 0}}, {"qp0", {P_QP0, 0}}, {"rho", {P_RHO, 0}}, {"rhow", {P_RHOW, 1}},
+      {"adz", {P_ADZ, 0}}, {"adzw", {P_ADZW, 1}}, {"pres", {P_PRES, 0}}, {"presi", {P_PRESI, 1}},
+      {"z", {P_Z, 1}}, {"zi", {P_ZI, 1}}, {"bet", {P_BET, 0}}, {"gamaz", {P_GAMAZ, 0}},
+      {"ttend", {P_TTEND, 0}}, {"qtend", {P_QTEND, 0}}, {"utend", {P_UTEND, 0}}, {"uwle", {P_UWLE, 1}},
+      {"uwsb", {P_UWSB, 1}}, {"mkwle1", {P_MKWLE1, 1}}, {"mkwsb1", {P_MKWSB1, 1}}, {"mkadv1", {P_MKADV1, 1}},
+      {"mkdiff1", {P_MKDIFF1, 1}}, {"qpsrc", {P_QPSRC, 1}}, {"qpevp", {P_QPEVP, 1}},
+      {"qpfall", {P_QPFALL, 1}}, {"precflux", {P_PRECFLUX, 1}}};
+  auto it = profs.find(n);
+  if (it != profs.end()) {
+    int len = it->second.second ? nz : nzm;
+    std::vector<double> tmp((size_t)ncrms * nz);
+    if (hipMemcpy(tmp.data(), d.prof + (size_t)it->second.first * ncrms * nz, tmp.size() * sizeof(double),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      return -1;
+    size_t m = 0;
+    for (int c = 0; c < ncrms; c++)
+      for (int k = 0; k < len; k++) buf[m++] = tmp[(size_t)c * nz + k];
+    return (int)m;
+  }
+  if (n == "dz") {
+    if (hipMemcpy(buf, d.inst + (size_t)I_DZ * ncrms, ncrms * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return -1;
+    return ncrms;
+  }
+  return -1;
+}
+
+
+template <typename R>
+static CrmEngine* make_engine(int device, int* rc) {
+  auto* e = new Engine<R>();
+  e->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess ||
+      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipMalloc(&e->gmax, 3 * sizeof(unsigned long long)) != hipSuccess) {
+    delete e;
+    *rc = CRM_ERR_DEVICE;
+    return nullptr;
+  }
+  *rc = CRM_OK;
+  return e;
+}
+
+}  // namespace CRM_NS

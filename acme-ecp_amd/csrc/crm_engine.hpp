@@ -1,0 +1,24 @@
+// crm_engine.hpp -- the interface between the C ABI (crm_hip.hip) and the device engines,
+// one per bulk-field precision, each compiled in its own translation unit.
+#pragma once
+#include "../../include/crm_hip.h"
+
+struct CrmEngine {
+  virtual ~CrmEngine() {}
+  // One crm() call; device_views selects crm_hip_run_device semantics, phase_stop > 0 stops
+  // inside the last substep (inspection only).
+  virtual int run(const crm_dims* dims, const crm_column_info* col, double dt_gl,
+                  const crm_input_view* in, crm_state_view* state, crm_rad_view* rad,
+                  crm_output_view* out, bool device_views, int phase_stop) = 0;
+  virtual int debug_get(const char* name, double* buf) = 0;
+  virtual void get_stats(crm_hip_stats* s) const = 0;
+  virtual int profile_kernels(unsigned mask) = 0;
+  virtual int kernel_time(int kid, double* ms, long long* n) const = 0;
+};
+
+namespace crm {
+CrmEngine* make_engine_fp64(int device, int* rc);
+}
+namespace crm_mixed {
+CrmEngine* make_engine_mixed(int device, int* rc);
+}

@@ -7,8 +7,9 @@
 #pragma once
 #include "../../include/crm_hip.h"
 #include "crm_common.hpp"
+#include <type_traits>
 
-namespace crm {
+namespace CRM_NS {
 
 // ------------------------------------------------------------------------------------------
 // Reference profiles from crm_input: crm_module.F90:320-375; grid factors sgs.F90:219-228;
@@ -29,7 +30,8 @@ __device__ double z0_est(double z, double bflx, double wnd, double ustar, double
   return z * exp(-lnz);
 }
 
-__global__ void k_setup(Dev d, crm_input_view in) {
+template <typename R>
+__global__ void k_setup(DevT<R> d, crm_input_view in) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.ncrms) return;
   const int ld = d.ld, plev = d.plev, nzm = d.nzm;
@@ -52,8 +54,8 @@ __global__ void k_setup(Dev d, crm_input_view in) {
     zi[k] = IN(in.zint, plev - k) - zs;
     pres[k] = IN(in.pmid, plev - 1 - k) / 100.;
     presi[k] = IN(in.pint, plev - k) / 100.;
-    bet[k] = GGR / IN(in.tl, plev - 1 - k);
-    gamaz[k] = GGR / CP * z[k];
+    bet[k] = GGR_D / IN(in.tl, plev - 1 - k);
+    gamaz[k] = GGR_D / CP_D * z[k];
   }
   zi[nzm] = IN(in.zint, plev - nzm) - zs;
   presi[nzm] = IN(in.pint, plev - nzm) / 100.;
@@ -63,8 +65,8 @@ __global__ void k_setup(Dev d, crm_input_view in) {
   adzw[0] = 1.;
   adzw[nzm] = adzw[nzm - 1];
   for (int k = 0; k < nzm; k++) adz[k] = (zi[k + 1] - zi[k]) / dz;
-  for (int k = 0; k < nzm; k++) rho[k] = IN(in.pdel, plev - 1 - k) / GGR / (adz[k] * dz);
-  for (int k = 1; k < nzm; k++) rhow[k] = (IN(in.pmid, plev - k) - IN(in.pmid, plev - 1 - k)) / GGR / (adzw[k] * dz);
+  for (int k = 0; k < nzm; k++) rho[k] = IN(in.pdel, plev - 1 - k) / GGR_D / (adz[k] * dz);
+  for (int k = 1; k < nzm; k++) rhow[k] = (IN(in.pmid, plev - k) - IN(in.pmid, plev - 1 - k)) / GGR_D / (adzw[k] * dz);
   rhow[0] = 2. * rhow[1] - rhow[2];
   rhow[nzm] = 2. * rhow[nzm - 1] - rhow[nzm - 2];
   for (int k = 0; k < nzm; k++) {
@@ -85,34 +87,58 @@ __global__ void k_setup(Dev d, crm_input_view in) {
       tau[k] = 1. / tt;
     }
   }
+  // per-level factors the sweeps would otherwise divide out per column and substep; the same
+  // IEEE operations, done once per call
+  double* irho = prof(d, P_IRHO, c);
+  double* iadz = prof(d, P_IADZ, c);
+  double* irhoadz = prof(d, P_IRHOADZ, c);
+  double* irhowadz = prof(d, P_IRHOWADZ, c);
+  double* ddadz = prof(d, P_DDADZ, c);
+  double* pratio = prof(d, P_PRATIO, c);
+  for (int k = 0; k < nzm; k++) {
+    irho[k] = 1. / rho[k];
+    iadz[k] = 1. / adz[k];
+    irhoadz[k] = 1. / (rho[k] * adz[k]);
+    irhowadz[k] = 1. / (rhow[k] * adz[k]);
+    const int kc = (k + 1 < nzm) ? k + 1 : k, kb = (k >= 1) ? k - 1 : 0;
+    ddadz[k] = (double)(2.0f / (float)(kc - kb)) / adz[k];  // advect_scalar2D.F90 vertical weights
+    pratio[k] = sqrt(R4(1.29) / rho[k]);
+  }
   double ustar = sqrt(in.tau00[c] / rho[0]);
   double z0 = z0_est(z[0], in.bflxls[c], in.wndls[c], ustar, d.z0c1);
   inst(d, I_Z0, c) = fmax(R4(0.00001), fmin(1.0, z0));
 }
 
-// micro_diagnose (MICRO_SAM1MOM/microphysics.F90:331-357) for one point.
-__device__ __forceinline__ void micro_diag(double qt, double qn, double qp, double tabs, double a_bg,
-                                           double a_pr, double& qv, double& qcl, double& qci,
-                                           double& qpl, double& qpi) {
-  qv = qt - qn;
-  double omn = fmax(0.0, fmin(1.0, (tabs - TBGMIN) * a_bg));
-  qcl = qn * omn;
-  qci = qn * (1. - omn);
-  double omp = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * a_pr));
-  qpl = qp * omp;
-  qpi = qp * (1. - omp);
+// FP32 copies of the level profiles (mixed-precision form only).
+template <typename R>
+__global__ void k_prof_mirror(const double* src, R* dst, size_t n) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) dst[idx] = (R)src[idx];
 }
 
-__device__ __forceinline__ double A_BG() { return 1. / (TBGMAX - TBGMIN); }
-__device__ __forceinline__ double A_PR() { return 1. / (TPRMAX - TPRMIN); }
-__device__ __forceinline__ double A_GR() { return 1. / (TGRMAX - TGRMIN); }
+// micro_diagnose (MICRO_SAM1MOM/microphysics.F90:331-357) for one point.
+template <typename T>
+__device__ __forceinline__ void micro_diag(T qt, T qn, T qp, T tabs, T a_bg, T a_pr, T& qv, T& qcl, T& qci,
+                                           T& qpl, T& qpi) {
+  qv = qt - qn;
+  T omn = fmax((T)0.0, fmin((T)1.0, (tabs - (T)TBGMIN) * a_bg));
+  qcl = qn * omn;
+  qci = qn * ((T)1. - omn);
+  T omp = fmax((T)0.0, fmin((T)1.0, (tabs - (T)TPRMIN) * a_pr));
+  qpl = qp * omp;
+  qpi = qp * ((T)1. - omp);
+}
+
+__device__ __forceinline__ creal A_BG() { return 1. / (TBGMAX - TBGMIN); }
+__device__ __forceinline__ creal A_PR() { return 1. / (TPRMAX - TPRMIN); }
+__device__ __forceinline__ creal A_GR() { return 1. / (TGRMAX - TGRMIN); }
 
 // ------------------------------------------------------------------------------------------
 // Initial fields, means and large-scale tendencies: crm_module.F90:379-594, micro_init
 // (microphysics.F90:160-245), sgs_init (sgs.F90:187-213).  Segment per instance.
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_init(Dev d, crm_input_view in, crm_state_view st, crm_output_view out, double dt_gl) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_init(DevT<R> d, crm_input_view in, crm_state_view st, crm_output_view out, double dt_gl) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -137,14 +163,14 @@ __global__ void __launch_bounds__(256) k_init(Dev d, crm_input_view in, crm_stat
     double qt = G4(st.qt, c, i, k), qp = G4(st.qp, c, i, k), qn = G4(st.qn, c, i, k);
     double qv, qcl, qci, qpl, qpi;
     micro_diag(qt, qn, qp, tabs, a_bg, a_pr, qv, qcl, qci, qpl, qpi);
-    double t = tabs + gamaz[k] - FAC_COND * qcl - FAC_SUB * qci - FAC_COND * qpl - FAC_SUB * qpi;
+    double t = tabs + gamaz[k] - FAC_COND_D * qcl - FAC_SUB_D * qci - FAC_COND_D * qpl - FAC_SUB_D * qpi;
     int l = plev - 1 - k;
     double pd = in.pdel[(size_t)c + (size_t)ld * l];
     colprec += s.sum((qpl + qpi) * pd);
     colprecs += s.sum(qpi * pd);
     double u0 = s.sum(u) * factor_xy;
     double t0 = s.sum(t) * factor_xy;
-    double t00 = s.sum(t + FAC_COND * qpl + FAC_SUB * qpi) * factor_xy;
+    double t00 = s.sum(t + FAC_COND_D * qpl + FAC_SUB_D * qpi) * factor_xy;
     double tabs0 = s.sum(tabs) * factor_xy;
     double q0 = s.sum(qv + qcl + qci) * factor_xy;
     double qv0 = s.sum(qv) * factor_xy;
@@ -184,8 +210,8 @@ __global__ void __launch_bounds__(256) k_init(Dev d, crm_input_view in, crm_stat
         prof(d, P_QV0, c)[k] = qv0;
         prof(d, P_QN0, c)[k] = qn0;
         prof(d, P_QP0, c)[k] = qp0;
-        prof(d, P_TTEND, c)[k] = (IN(in.tl) + gamaz[k] - FAC_COND * (IN(in.qccl) + IN(in.qiil)) -
-                                  FAC_FUS * IN(in.qiil) - t00) * idt_gl;
+        prof(d, P_TTEND, c)[k] = (IN(in.tl) + gamaz[k] - FAC_COND_D * (IN(in.qccl) + IN(in.qiil)) -
+                                  FAC_FUS_D * IN(in.qiil) - t00) * idt_gl;
         prof(d, P_QTEND, c)[k] = (IN(in.ql) + IN(in.qccl) + IN(in.qiil) - q0) * idt_gl;
         prof(d, P_UTEND, c)[k] = (uln - u0) * idt_gl;
       }
@@ -220,8 +246,8 @@ __global__ void __launch_bounds__(256) k_init(Dev d, crm_input_view in, crm_stat
 // cam/RNG_MT.F90:60-139 (whose upper mask is -1147483648).  One thread per instance; the
 // 624-word state lives in thread-private memory.  Only runs when igstep <= 1.
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void k_setperturb(Dev d, const int* gcol) {
+template <int NX, typename R>
+__global__ void k_setperturb(DevT<R> d, const int* gcol) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.ncrms) return;
   uint32_t mt[624];
@@ -258,7 +284,7 @@ __global__ void k_setperturb(Dev d, const int* gcol) {
   for (int k = 0; k < 5 && k < d.nzm; k++) {
     double scaling = (double)(5 - k) / 5.0;
     double t02 = 0.0;
-    double* t = d.t + ((size_t)c * d.nzm + k) * NX;
+    R* t = d.t + ((size_t)c * d.nzm + k) * NX;
     for (int i = 0; i < NX; i++) {
       double r = 1. - 2. * gen();
       t[i] = t[i] + 1 * r * scaling;
@@ -273,8 +299,8 @@ __global__ void k_setperturb(Dev d, const int* gcol) {
 // CFL check: kurant.F90:9-83 and kurant_sgs (sgs.F90:329-372).  Batch-global maximum via an
 // atomic max on the (non-negative) double's bits.
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_kurant(Dev d) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_kurant(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -311,66 +337,66 @@ __global__ void __launch_bounds__(256) k_kurant(Dev d) {
 // zero.F90:8-25, buoyancy.F90:7-38, forcing.F90:7-79, crm_module.F90:766-777,
 // damping.F90:9-117, ice_fall.F90:6-173.  Writes dudt(na), dwdt(na) from scratch.
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_pre(Dev d, int na, const double* qrad) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qrad) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
-  const double dtn = d.dtn, dz = inst(d, I_DZ, c);
-  const double* adz = prof(d, P_ADZ, c);
-  const double* rho = prof(d, P_RHO, c);
-  const double* bet = prof(d, P_BET, c);
-  const double* tabs0 = prof(d, P_TABS0, c);
-  const double* qv0 = prof(d, P_QV0, c);
-  const double* qn0 = prof(d, P_QN0, c);
-  const double* qp0 = prof(d, P_QP0, c);
-  const double* ttend = prof(d, P_TTEND, c);
-  const double* qtend = prof(d, P_QTEND, c);
-  const double* utend = prof(d, P_UTEND, c);
-  const double* tau = prof(d, P_TAU, c);
+  const R dtn = d.dtn, dz = inst(d, I_DZ, c);
+  const R* adz = profr(d, P_ADZ, c);
+  const R* rho = profr(d, P_RHO, c);
+  const R* bet = profr(d, P_BET, c);
+  const R* tabs0 = profr(d, P_TABS0, c);
+  const R* qv0 = profr(d, P_QV0, c);
+  const R* qn0 = profr(d, P_QN0, c);
+  const R* qp0 = profr(d, P_QP0, c);
+  const R* ttend = profr(d, P_TTEND, c);
+  const R* qtend = profr(d, P_QTEND, c);
+  const R* utend = profr(d, P_UTEND, c);
+  const R* tau = profr(d, P_TAU, c);
   const int n_damp = (int)inst(d, I_NDAMP, c);
   const int kmin = (int)inst(d, I_KMIN, c), kmax = (int)inst(d, I_KMAX, c);
   const int nxr = d.nx_rad, ir = i / (NX / nxr);
-  double* dudt = d.dudt[na];
-  double* dwdt = d.dwdt[na];
-  auto F = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  R* dudt = d.dudt[na];
+  R* dwdt = d.dwdt[na];
+  auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
   // cloud-ice flux fz(k) (ice_fall.F90:94-128); zero outside [kmin-1, kmax]
   auto qci_at = [&](int k) { return F(d.qci, k); };
-  auto icefz = [&](int k) -> double {
-    if (k < ::max(0, kmin - 1) || k > kmax || k > nzm - 1) return 0.0;
+  auto icefz = [&](int k) -> R {
+    if (k < ::max(0, kmin - 1) || k > kmax || k > nzm - 1) return R(0.0);
     int kc = ::min(nzm - 1, k + 1), kb = ::max(0, k - 1);
-    double coef = dtn / (0.5 * (adz[kb] + adz[k]) * dz);
-    double qiu = rho[kc] * qci_at(kc), qic = rho[k] * qci_at(k), qid = rho[kb] * qci_at(kb);
-    double vt = fmin(R4(0.4), R4(8.66) * pow(fmax(0.0, qic) + R4(1.e-10), R4(0.24)));
-    double phi;
+    R coef = dtn / (R(0.5) * (adz[kb] + adz[k]) * dz);
+    R qiu = rho[kc] * qci_at(kc), qic = rho[k] * qci_at(k), qid = rho[kb] * qci_at(kb);
+    R vt = fmin(R4(0.4), R4(8.66) * fpow<R>(fmax(R(0.0), qic) + R4(1.e-10), R4(0.24)));
+    R phi;
     if (fabs(qic - qid) < R4(1.0e-25)) {
-      phi = 0.;
+      phi = R(0.);
     } else {
-      double th = (qiu - qic) / (qic - qid);
-      phi = fmax(0.0, fmin(fmin(0.5 * (1. + th), 2.0), 2. * th));
+      R th = (qiu - qic) / (qic - qid);
+      phi = fmax(R(0.0), fmin(fmin(R(0.5) * (R(1.) + th), R(2.0)), R(2.) * th));
     }
-    return -vt * (qic - 0.5 * (1. - coef * vt) * phi * (qic - qid));
+    return -vt * (qic - R(0.5) * (R(1.) - coef * vt) * phi * (qic - qid));
   };
-  double fz_k = icefz(0);
+  R fz_k = icefz(0);
   // level k-1 thermodynamics for the buoyancy average
-  double pb_tabs = 0, pb_qv = 0, pb_qcl = 0, pb_qci = 0, pb_qpl = 0, pb_qpi = 0;
+  R pb_tabs = 0, pb_qv = 0, pb_qcl = 0, pb_qci = 0, pb_qpl = 0, pb_qpi = 0;
   for (int k = 0; k < nzm; k++) {
     size_t o = ((size_t)c * nzm + k) * NX + i;
-    double tabs = d.tabs[o], qv = d.qv[o], qcl = d.qcl[o], qci = d.qci[o], qpl = d.qpl[o], qpi = d.qpi[o];
-    double u = d.u[o], w = d.w[((size_t)c * nz + k) * NX + i];
-    double t = d.t[o], qt = d.qt[o];
+    R tabs = d.tabs[o], qv = d.qv[o], qcl = d.qcl[o], qci = d.qci[o], qpl = d.qpl[o], qpi = d.qpi[o];
+    R u = d.u[o], w = d.w[((size_t)c * nz + k) * NX + i];
+    R t = d.t[o], qt = d.qt[o];
     // buoyancy on w level k from scalar levels k-1 and k
-    double dw = 0.0;
+    R dw = R(0.0);
     if (k >= 1) {
       int kb = k - 1;
-      double betu = adz[kb] / (adz[k] + adz[kb]);
-      double betd = adz[k] / (adz[k] + adz[kb]);
-      double up = tabs0[k] * (EPSV * (qv - qv0[k]) - (qcl + qci - qn0[k] + qpl + qpi - qp0[k])) +
-                  (tabs - tabs0[k]) * (1. + EPSV * qv0[k] - qn0[k] - qp0[k]);
-      double dn = tabs0[kb] * (EPSV * (pb_qv - qv0[kb]) - (pb_qcl + pb_qci - qn0[kb] + pb_qpl + pb_qpi - qp0[kb])) +
-                  (pb_tabs - tabs0[kb]) * (1. + EPSV * qv0[kb] - qn0[kb] - qp0[kb]);
-      dw = 0.0 + bet[k] * betu * up + bet[kb] * betd * dn;
+      R betu = adz[kb] / (adz[k] + adz[kb]);
+      R betd = adz[k] / (adz[k] + adz[kb]);
+      R up = tabs0[k] * (EPSV * (qv - qv0[k]) - (qcl + qci - qn0[k] + qpl + qpi - qp0[k])) +
+                  (tabs - tabs0[k]) * (R(1.) + EPSV * qv0[k] - qn0[k] - qp0[k]);
+      R dn = tabs0[kb] * (EPSV * (pb_qv - qv0[kb]) - (pb_qcl + pb_qci - qn0[kb] + pb_qpl + pb_qpi - qp0[kb])) +
+                  (pb_tabs - tabs0[kb]) * (R(1.) + EPSV * qv0[kb] - qn0[kb] - qp0[kb]);
+      dw = R(0.0) + bet[k] * betu * up + bet[kb] * betd * dn;
     }
     pb_tabs = tabs;
     pb_qv = qv;
@@ -381,33 +407,33 @@ __global__ void __launch_bounds__(256) k_pre(Dev d, int na, const double* qrad) 
     // large-scale forcing with level-wise refill of negative vapour
     t = t + ttend[k] * dtn;
     qt = qt + qtend[k] * dtn;
-    bool neg = qt < 0.;
-    double nneg = s.sum(neg ? 1.0 : 0.0);
-    double qneg = s.sum(neg ? qt : 0.0);
-    double qpoz = s.sum(neg ? 0.0 : qt);
-    double du = 0.0 + utend[k];
-    if (nneg > 0.5 && qpoz + qneg > 0.) qt = fmax(0.0, qt * (1. + qneg / qpoz));
+    bool neg = qt < R(0.);
+    R nneg = s.sum(neg ? R(1.0) : R(0.0));
+    R qneg = s.sum(neg ? qt : R(0.0));
+    R qpoz = s.sum(neg ? R(0.0) : qt);
+    R du = R(0.0) + utend[k];
+    if (nneg > R(0.5) && qpoz + qneg > R(0.)) qt = fmax(R(0.0), qt * (R(1.) + qneg / qpoz));
     // radiative heating on column groups
     t = t + qrad[(size_t)c + (size_t)d.ld * ((size_t)ir + (size_t)nxr * k)] * dtn;
     // sponge layer
     if (k + 1 >= nzm - n_damp) {
-      double tk = tau[k];
-      double u0l = s.sum(u / (double)NX), t0l = s.sum(t / (double)NX);
+      R tk = tau[k];
+      R u0l = s.sum(u / (R)NX), t0l = s.sum(t / (R)NX);
       du = du - (u - u0l) * tk;
       dw = dw - w * tk;
       t = t - dtn * (t - t0l) * tk;
       qt = qt - dtn * (qv - qv0[k]) * tk;
     }
     // cloud-ice sedimentation
-    double fz_kp = icefz(k + 1);
+    R fz_kp = icefz(k + 1);
     if (k >= ::max(0, kmin - 2) && k <= kmax) {
-      double coef = dtn / (dz * adz[k] * rho[k]);
-      double dqi = coef * (fz_k - fz_kp);
+      R coef = dtn / (dz * adz[k] * rho[k]);
+      R dqi = coef * (fz_k - fz_kp);
       qt = qt + dqi;
       t = t - (FAC_COND + FAC_FUS) * dqi;
     }
     if (k == 0 && live) {
-      double dqi = -(dtn / dz) * fz_k;
+      R dqi = -(dtn / dz) * fz_k;
       d.precsfc[(size_t)c * NX + i] += dqi;
       d.precssfc[(size_t)c * NX + i] += dqi;
     }
@@ -419,96 +445,96 @@ __global__ void __launch_bounds__(256) k_pre(Dev d, int na, const double* qrad) 
       dwdt[((size_t)c * nz + k) * NX + i] = dw;
     }
   }
-  if (live) dwdt[((size_t)c * nz + nzm) * NX + i] = 0.0;
+  if (live) dwdt[((size_t)c * nz + nzm) * NX + i] = R(0.0);
 }
 
 // ------------------------------------------------------------------------------------------
 // Smagorinsky eddy diffusivity: shear_prod2D.F90:7-96 and tke_full.F90:11-317 (dosmagor).
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_sgs(Dev d) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
-  const double dz = inst(d, I_DZ, c), dx = d.dx;
-  const double* adz = prof(d, P_ADZ, c);
-  const double* adzw = prof(d, P_ADZW, c);
-  const double* bet = prof(d, P_BET, c);
-  const double* presi = prof(d, P_PRESI, c);
-  const double* u0 = prof(d, P_U0, c);
-  const double* grdf = prof(d, P_GRDFX, c);
-  const double Cs = R4(0.15), Ck = R4(0.1), Pr = 1.;
-  const double Ce = Ck * Ck * Ck / (Cs * Cs * Cs * Cs);
-  const double rdx0 = 1. / dx;
-  auto F = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  const R dz = inst(d, I_DZ, c), dx = d.dx;
+  const R* adz = profr(d, P_ADZ, c);
+  const R* adzw = profr(d, P_ADZW, c);
+  const R* bet = profr(d, P_BET, c);
+  const R* presi = profr(d, P_PRESI, c);
+  const R* u0 = profr(d, P_U0, c);
+  const R* grdf = profr(d, P_GRDFX, c);
+  const R Cs = R4(0.15), Ck = R4(0.1), Pr = R(1.);
+  const R Ce = Ck * Ck * Ck / (Cs * Cs * Cs * Cs);
+  const R rdx0 = R(1.) / dx;
+  auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
   auto W = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
   // window: u at k-1,k,k+1 and w at k,k+1
-  double um = 0, u = F(d.u, 0), up = F(d.u, 1);
-  double w0 = W(0), w1 = W(1);
-  double bsv_prev = 0.0;
+  R um = 0, u = F(d.u, 0), up = F(d.u, 1);
+  R w0 = W(0), w1 = W(1);
+  R bsv_prev = R(0.0);
   // thermodynamics at k (carried) and k+1
-  double th_tabs = F(d.tabs, 0), th_qcl = F(d.qcl, 0), th_qci = F(d.qci, 0), th_qv = F(d.qv, 0);
-  double th_qpl = F(d.qpl, 0), th_qpi = F(d.qpi, 0), th_t = F(d.t, 0), th_tkh = F(d.tkh, 0);
+  R th_tabs = F(d.tabs, 0), th_qcl = F(d.qcl, 0), th_qci = F(d.qci, 0), th_qv = F(d.qv, 0);
+  R th_qpl = F(d.qpl, 0), th_qpi = F(d.qpi, 0), th_t = F(d.t, 0), th_tkh = F(d.tkh, 0);
   for (int k = 0; k < nzm - 1; k++) {
     int kc = k + 1, kb = k - 1;
-    double up_next = (kc + 1 < nzm) ? F(d.u, kc + 1) : 0.0;
-    double w2 = W(kc + 1);
+    R up_next = (kc + 1 < nzm) ? F(d.u, kc + 1) : R(0.0);
+    R w2 = W(kc + 1);
     // shear production def2 at level k
-    double rdz = 1. / (dz * adz[k]);
-    double rdx = rdx0 * sqrt(dx * rdz);
-    double u_ip = s.rot(u, 1);
-    double du = (u_ip - u) * rdx;
-    double dwz = (w1 - w0) * rdz;
-    double def2 = 2. * (du * du + dwz * dwz);
-    double w0m = s.rot(w0, -1), w0p = s.rot(w0, 1), w1m = s.rot(w1, -1), w1p = s.rot(w1, 1);
-    double up_ip = s.rot(up, 1);
+    R rdz = R(1.) / (dz * adz[k]);
+    R rdx = rdx0 * sqrt(dx * rdz);
+    R u_ip = s.rot(u, 1);
+    R du = (u_ip - u) * rdx;
+    R dwz = (w1 - w0) * rdz;
+    R def2 = R(2.) * (du * du + dwz * dwz);
+    R w0m = s.rot(w0, -1), w0p = s.rot(w0, 1), w1m = s.rot(w1, -1), w1p = s.rot(w1, 1);
+    R up_ip = s.rot(up, 1);
     {
-      double rdzw_up = 1. / (dz * adzw[kc]);
-      double rdx_up = rdx0 * sqrt(dx * rdzw_up);
-      double a1 = (up_ip - u0[kc] - u_ip + u0[k]) * rdzw_up + (w1p - w1) * rdx_up;
-      double a2 = (up - u0[kc] - u + u0[k]) * rdzw_up + (w1 - w1m) * rdx_up;
+      R rdzw_up = R(1.) / (dz * adzw[kc]);
+      R rdx_up = rdx0 * sqrt(dx * rdzw_up);
+      R a1 = (up_ip - u0[kc] - u_ip + u0[k]) * rdzw_up + (w1p - w1) * rdx_up;
+      R a2 = (up - u0[kc] - u + u0[k]) * rdzw_up + (w1 - w1m) * rdx_up;
       if (k >= 1) {
-        double rdzw_dn = 1. / (dz * adzw[k]);
-        double rdx_dn = rdx0 * sqrt(dx * rdzw_dn);
-        double um_ip = s.rot(um, 1);
-        double a3 = (u_ip - u0[k] - um_ip + u0[kb]) * rdzw_dn + (w0p - w0) * rdx_dn;
-        double a4 = (u - u0[k] - um + u0[kb]) * rdzw_dn + (w0 - w0m) * rdx_dn;
-        def2 = def2 + 0.5 * (a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4);
+        R rdzw_dn = R(1.) / (dz * adzw[k]);
+        R rdx_dn = rdx0 * sqrt(dx * rdzw_dn);
+        R um_ip = s.rot(um, 1);
+        R a3 = (u_ip - u0[k] - um_ip + u0[kb]) * rdzw_dn + (w0p - w0) * rdx_dn;
+        R a4 = (u - u0[k] - um + u0[kb]) * rdzw_dn + (w0 - w0m) * rdx_dn;
+        def2 = def2 + R(0.5) * (a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4);
       } else {
-        def2 = def2 + 0.5 * (a1 * a1 + a2 * a2);
+        def2 = def2 + R(0.5) * (a1 * a1 + a2 * a2);
       }
     }
     // SGS buoyancy flux at the interface between k and k+1 (tke_full.F90:120-237)
-    double n_tabs = F(d.tabs, kc), n_qcl = F(d.qcl, kc), n_qci = F(d.qci, kc), n_qv = F(d.qv, kc);
-    double n_qpl = F(d.qpl, kc), n_qpi = F(d.qpi, kc), n_t = F(d.t, kc), n_tkh = F(d.tkh, kc);
-    double bsv;
+    R n_tabs = F(d.tabs, kc), n_qcl = F(d.qcl, kc), n_qci = F(d.qci, kc), n_qv = F(d.qv, kc);
+    R n_qpl = F(d.qpl, kc), n_qpi = F(d.qpi, kc), n_t = F(d.t, kc), n_tkh = F(d.tkh, kc);
+    R bsv;
     {
-      double betdz = 0.5 * (bet[kc] + bet[k]) / dz / adzw[k + 1];
-      double tabs_if = 0.5 * (n_tabs + FAC_COND * n_qcl + FAC_SUB * n_qci + th_tabs + FAC_COND * th_qcl + FAC_SUB * th_qci);
-      double qtot_if = 0.5 * (n_qv + n_qcl + n_qci + th_qv + th_qcl + th_qci);
-      double qp_if = 0.5 * (n_qpl + n_qpi + th_qpl + th_qpi);
-      double bbb = 1. + EPSV * qtot_if - qp_if;
-      double dtt = n_t - th_t;
-      double dqt = n_qv + n_qcl + n_qci - th_qv - th_qcl - th_qci;
+      R betdz = R(0.5) * (bet[kc] + bet[k]) / dz / adzw[k + 1];
+      R tabs_if = R(0.5) * (n_tabs + FAC_COND * n_qcl + FAC_SUB * n_qci + th_tabs + FAC_COND * th_qcl + FAC_SUB * th_qci);
+      R qtot_if = R(0.5) * (n_qv + n_qcl + n_qci + th_qv + th_qcl + th_qci);
+      R qp_if = R(0.5) * (n_qpl + n_qpi + th_qpl + th_qpi);
+      R bbb = R(1.) + EPSV * qtot_if - qp_if;
+      R dtt = n_t - th_t;
+      R dqt = n_qv + n_qcl + n_qci - th_qv - th_qcl - th_qci;
       bsv = betdz * (bbb * dtt + EPSV * tabs_if * dqt + (bbb * FAC_COND - tabs_if) * (n_qpl - th_qpl) +
                      (bbb * FAC_SUB - tabs_if) * (n_qpi - th_qpi));
-      double qctot = n_qcl + n_qci + th_qcl + th_qci;
-      if (qctot > 0.) {
-        double omn = (n_qcl + th_qcl) / (qctot + R4(1.e-20));
-        double pp = presi[k + 1];
-        double qsat_check = omn * qsatw(tabs_if, pp) + (1. - omn) * qsati(tabs_if, pp);
+      R qctot = n_qcl + n_qci + th_qcl + th_qci;
+      if (qctot > R(0.)) {
+        R omn = (n_qcl + th_qcl) / (qctot + R4(1.e-20));
+        R pp = presi[k + 1];
+        R qsat_check = omn * qsatw(tabs_if, pp) + (R(1.) - omn) * qsati(tabs_if, pp);
         if (qtot_if > qsat_check) {
-          double lstarn = FAC_COND + (1. - omn) * FAC_FUS;
-          double tif = 0.5 * (n_tabs + th_tabs);
-          double dqsat = omn * dtqsatw(tif, pp) + (1. - omn) * dtqsati(tif, pp);
-          double qsatt = omn * qsatw(tif, pp) + (1. - omn) * qsati(tif, pp);
-          double b2 = 1. + EPSV * qsatt + qsatt - qtot_if - qp_if + R4(1.61) * tif * dqsat;
-          b2 = b2 / (1. + lstarn * dqsat);
-          double tk_ = th_tabs;
-          bsv = betdz * (b2 * dtt + (b2 * lstarn - (1. + lstarn * dqsat) * tif) * dqt +
-                         (b2 * FAC_COND - (1. + FAC_COND * dqsat) * tk_) * (n_qpl - th_qpl) +
-                         (b2 * FAC_SUB - (1. + FAC_SUB * dqsat) * tk_) * (n_qpi - th_qpi));
+          R lstarn = FAC_COND + (R(1.) - omn) * FAC_FUS;
+          R tif = R(0.5) * (n_tabs + th_tabs);
+          R dqsat = omn * dtqsatw(tif, pp) + (R(1.) - omn) * dtqsati(tif, pp);
+          R qsatt = omn * qsatw(tif, pp) + (R(1.) - omn) * qsati(tif, pp);
+          R b2 = R(1.) + EPSV * qsatt + qsatt - qtot_if - qp_if + R4(1.61) * tif * dqsat;
+          b2 = b2 / (R(1.) + lstarn * dqsat);
+          R tk_ = th_tabs;
+          bsv = betdz * (b2 * dtt + (b2 * lstarn - (R(1.) + lstarn * dqsat) * tif) * dqt +
+                         (b2 * FAC_COND - (R(1.) + FAC_COND * dqsat) * tk_) * (n_qpl - th_qpl) +
+                         (b2 * FAC_SUB - (R(1.) + FAC_SUB * dqsat) * tk_) * (n_qpi - th_qpi));
         }
       }
     }
@@ -516,24 +542,24 @@ __global__ void __launch_bounds__(256) k_sgs(Dev d) {
     (void)n_tkh;
     // eddy viscosity at level k (tke_full.F90:246-308)
     {
-      double grd = dz * adz[k];
-      double Ce1 = Ce / R4(0.7) * R4(0.19);
-      double Ce2 = Ce / R4(0.7) * R4(0.51);
-      double cx = dx * dx / d.dt / grdf[k];
-      double dzm = dz * fmin(adzw[k], adzw[k + 1]);
-      double cz = dzm * dzm / d.dt / 1.0;
-      double tkmax = R4(0.09) / (1. / cx + 1. / cx + 1. / cz);
-      double buoy = 0.5 * (bsv_prev + bsv);
+      R grd = dz * adz[k];
+      R Ce1 = Ce / R4(0.7) * R4(0.19);
+      R Ce2 = Ce / R4(0.7) * R4(0.51);
+      R cx = dx * dx / (R)d.dt / grdf[k];
+      R dzm = dz * fmin(adzw[k], adzw[k + 1]);
+      R cz = dzm * dzm / (R)d.dt / (R)1.0;
+      R tkmax = R4(0.09) / (R(1.) / cx + R(1.) / cx + R(1.) / cz);
+      R buoy = R(0.5) * (bsv_prev + bsv);
       size_t o = ((size_t)c * nzm + k) * NX + i;
-      double smix;
-      if (buoy <= 0.)
+      R smix;
+      if (buoy <= R(0.))
         smix = grd;
       else
         smix = fmin(grd, fmax(R4(0.1) * grd, sqrt(R4(0.76) * d.tk[o] / Ck / sqrt(buoy + R4(1.e-10)))));
-      double ratio = smix / grd;
-      double Cee = Ce1 + Ce2 * ratio;
-      double tk = sqrt(Ck * Ck * Ck / Cee * fmax(0.0, def2 - Pr * buoy)) * (smix * smix);
-      double tt = tk / (Ck * smix);
+      R ratio = smix / grd;
+      R Cee = Ce1 + Ce2 * ratio;
+      R tk = sqrt(Ck * Ck * Ck / Cee * fmax(R(0.0), def2 - Pr * buoy)) * (smix * smix);
+      R tt = tk / (Ck * smix);
       tk = fmin(tk, tkmax);
       if (live) {
         d.tke[o] = tt * tt;
@@ -563,91 +589,91 @@ __global__ void __launch_bounds__(256) k_sgs(Dev d) {
 // advect2_mom_z.F90:7-118, diffuse_mom2D.F90:7-161.  Each level's four contributions are
 // added in the reference's order.
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_mom(Dev d, int na) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_mom(DevT<R> d, int na) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
-  const double dz = inst(d, I_DZ, c), dx = d.dx;
-  const double* adz = prof(d, P_ADZ, c);
-  const double* adzw = prof(d, P_ADZW, c);
-  const double* rho = prof(d, P_RHO, c);
-  const double* rhow = prof(d, P_RHOW, c);
-  const double* grdf = prof(d, P_GRDFX, c);
-  double* dudt = d.dudt[na];
-  double* dwdt = d.dwdt[na];
-  const double dx25 = 0.25 / dx, dz25 = 1. / (4. * dz);
-  const double rdx2 = 1. / dx / dx, rdx25 = 0.25 * rdx2;
-  const double rdz = 1. / dz, dzx = dz / dx;
-  auto F = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  const R dz = inst(d, I_DZ, c), dx = d.dx;
+  const R* adz = profr(d, P_ADZ, c);
+  const R* adzw = profr(d, P_ADZW, c);
+  const R* rho = profr(d, P_RHO, c);
+  const R* rhow = profr(d, P_RHOW, c);
+  const R* grdf = profr(d, P_GRDFX, c);
+  R* dudt = d.dudt[na];
+  R* dwdt = d.dwdt[na];
+  const R dx25 = R(0.25) / dx, dz25 = R(1.) / (R(4.) * dz);
+  const R rdx2 = R(1.) / dx / dx, rdx25 = R(0.25) * rdx2;
+  const R rdz = R(1.) / dz, dzx = dz / dx;
+  auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
   auto W = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
-  double u_k = F(d.u, 0), u_n = F(d.u, 1);
-  double w_k = W(0), w_n = W(1);
-  double tk_k = F(d.tk, 0), tk_n = F(d.tk, 1);
-  double fuz_k = 0.0;    // fuz at level k (0 at the surface)
-  double fwz_prev = 0.0; // fwz(k-1)
-  double fuv_k = 0.0;    // vertical SGS u flux at the bottom of level k
-  double fwv_k = 0.0;    // vertical SGS w flux at w level k
-  double pend_adv = 0.0, pend_dif = 0.0;  // horizontal contributions for w level k
-  double uwsb_sum = 0.0;
+  R u_k = F(d.u, 0), u_n = F(d.u, 1);
+  R w_k = W(0), w_n = W(1);
+  R tk_k = F(d.tk, 0), tk_n = F(d.tk, 1);
+  R fuz_k = R(0.0);    // fuz at level k (0 at the surface)
+  R fwz_prev = R(0.0); // fwz(k-1)
+  R fuv_k = R(0.0);    // vertical SGS u flux at the bottom of level k
+  R fwv_k = R(0.0);    // vertical SGS w flux at w level k
+  R pend_adv = R(0.0), pend_dif = R(0.0);  // horizontal contributions for w level k
+  R uwsb_sum = R(0.0);
   for (int k = 0; k < nzm; k++) {
     int kn = k + 1;
     bool has_n = kn < nzm;
     int kcu = has_n ? kn : k;
-    double u_nn = (kn + 1 < nzm) ? F(d.u, kn + 1) : 0.0;
-    double w_nn = (kn + 1 < nz) ? W(kn + 1) : 0.0;
-    double tk_nn = (kn + 1 < nzm) ? F(d.tk, kn + 1) : 0.0;
-    double u_m = s.rot(u_k, -1), u_p = s.rot(u_k, 1);
-    double ucu = has_n ? u_n : u_k, ucu_p = s.rot(ucu, 1);
-    double wn_m = s.rot(w_n, -1), wn_p = s.rot(w_n, 1);
-    double tkcu = has_n ? tk_n : tk_k;
-    double tk_p = s.rot(tk_k, 1), tkcu_p = s.rot(tkcu, 1);
-    double tk_m = s.rot(tk_k, -1), tkn_m = s.rot(tk_n, -1);
+    R u_nn = (kn + 1 < nzm) ? F(d.u, kn + 1) : R(0.0);
+    R w_nn = (kn + 1 < nz) ? W(kn + 1) : R(0.0);
+    R tk_nn = (kn + 1 < nzm) ? F(d.tk, kn + 1) : R(0.0);
+    R u_m = s.rot(u_k, -1), u_p = s.rot(u_k, 1);
+    R ucu = has_n ? u_n : u_k, ucu_p = s.rot(ucu, 1);
+    R wn_m = s.rot(w_n, -1), wn_p = s.rot(w_n, 1);
+    R tkcu = has_n ? tk_n : tk_k;
+    R tk_p = s.rot(tk_k, 1), tkcu_p = s.rot(tkcu, 1);
+    R tk_m = s.rot(tk_k, -1), tkn_m = s.rot(tk_n, -1);
     // ---- dudt(k)
     size_t o = ((size_t)c * nzm + k) * NX + i;
-    double du = dudt[o];
-    double fu1 = dx25 * (u_k + u_m) * (u_m + u_k);
-    double fu2 = dx25 * (u_p + u_k) * (u_k + u_p);
+    R du = dudt[o];
+    R fu1 = dx25 * (u_k + u_m) * (u_m + u_k);
+    R fu2 = dx25 * (u_p + u_k) * (u_k + u_p);
     du = du - (fu2 - fu1);
-    double fuz_n = 0.0;
+    R fuz_n = R(0.0);
     if (has_n) {
-      double rhoi = dz25 * rhow[kn];
-      double www = rhoi * (w_n + wn_m);
+      R rhoi = dz25 * rhow[kn];
+      R www = rhoi * (w_n + wn_m);
       fuz_n = www * (u_n + u_k);
     }
-    double rhoi_k = 1. / (rho[k] * adz[k]);
+    R rhoi_k = R(1.) / (rho[k] * adz[k]);
     du = du - (fuz_n - fuz_k) * rhoi_k;
-    double fuh = -2. * (rdx2 * grdf[k] * tk_k) * (u_p - u_k);
-    double fuh_m = s.rot(fuh, -1);
+    R fuh = -R(2.) * (rdx2 * grdf[k] * tk_k) * (u_p - u_k);
+    R fuh_m = s.rot(fuh, -1);
     du = du - (fuh - fuh_m);
     // vertical SGS fluxes at the top of level k (w level k+1)
-    double fuv_n = 0.0, fwv_n;
+    R fuv_n = R(0.0), fwv_n;
     if (has_n) {
-      double rdz2 = rdz * rdz * 1.0;
-      double rdz25 = 0.25 * rdz2;
-      double iadz = 1. / adz[k], iadzw = 1. / adzw[kn];
-      double tkz = rdz2 * tk_k;
-      fwv_n = -2. * tkz * (w_n - w_k) * rho[k] * iadz;
+      R rdz2 = rdz * rdz * R(1.0);
+      R rdz25 = R(0.25) * rdz2;
+      R iadz = R(1.) / adz[k], iadzw = R(1.) / adzw[kn];
+      R tkz = rdz2 * tk_k;
+      fwv_n = -R(2.) * tkz * (w_n - w_k) * rho[k] * iadz;
       tkz = rdz25 * (tk_k + tk_m + tk_n + tkn_m);
       fuv_n = -tkz * ((u_n - u_k) * iadzw + (w_n - wn_m) * dzx) * rhow[kn];
       uwsb_sum = s.sum(fuv_n);
       if (live && i == 0) prof(d, P_UWSB, c)[kn] = uwsb_sum;
     } else {
-      double rdz2 = rdz * rdz * 1.0;
-      double tkz = rdz2 * 1.0 * tk_k;
-      fwv_n = -2. * tkz * (w_n - w_k) / adz[k] * rho[k];
+      R rdz2 = rdz * rdz * R(1.0);
+      R tkz = rdz2 * R(1.0) * tk_k;
+      fwv_n = -R(2.) * tkz * (w_n - w_k) / adz[k] * rho[k];
     }
     du = du - (fuv_n - fuv_k) * rhoi_k;
     if (live) dudt[o] = du;
-    double fuz_sum = s.sum(fuz_n);
+    R fuz_sum = s.sum(fuz_n);
     if (has_n && live && i == 0) prof(d, P_UWLE, c)[kn] = fuz_sum;
     // ---- dwdt(k) for w levels 1..nzm-1
-    double fwz_k = dz25 * (w_n * rhow[kn] + w_k * rhow[k]) * (w_n + w_k);
+    R fwz_k = dz25 * (w_n * rhow[kn] + w_k * rhow[k]) * (w_n + w_k);
     if (k >= 1) {
       size_t ow = ((size_t)c * nz + k) * NX + i;
-      double dw = dwdt[ow];
-      double rhoiw = 1. / (rhow[k] * adzw[k]);
+      R dw = dwdt[ow];
+      R rhoiw = R(1.) / (rhow[k] * adzw[k]);
       dw = dw - pend_adv;
       dw = dw - (fwz_k - fwz_prev) * rhoiw;
       dw = dw - pend_dif;
@@ -656,14 +682,14 @@ __global__ void __launch_bounds__(256) k_mom(Dev d, int na) {
     }
     // ---- horizontal contributions to w level k+1 (applied next iteration)
     {
-      double irho = 1. / (rhow[kn] * adzw[kn]);
-      double fw1 = dx25 * (u_k * rho[k] * adz[k] + ucu * rho[kcu] * adz[kcu]) * (wn_m + w_n);
-      double fw2 = dx25 * (u_p * rho[k] * adz[k] + ucu_p * rho[kcu] * adz[kcu]) * (w_n + wn_p);
+      R irho = R(1.) / (rhow[kn] * adzw[kn]);
+      R fw1 = dx25 * (u_k * rho[k] * adz[k] + ucu * rho[kcu] * adz[kcu]) * (wn_m + w_n);
+      R fw2 = dx25 * (u_p * rho[k] * adz[k] + ucu_p * rho[kcu] * adz[kcu]) * (w_n + wn_p);
       pend_adv = irho * (fw2 - fw1);
-      double dxz = dx / (dz * adzw[kn]);
-      double tkx = rdx25 * grdf[k] * (tk_k + tk_p + tkcu + tkcu_p);
-      double fwh = -tkx * (wn_p - w_n + (ucu_p - u_p) * dxz);
-      double fwh_m = s.rot(fwh, -1);
+      R dxz = dx / (dz * adzw[kn]);
+      R tkx = rdx25 * grdf[k] * (tk_k + tk_p + tkcu + tkcu_p);
+      R fwh = -tkx * (wn_p - w_n + (ucu_p - u_p) * dxz);
+      R fwh_m = s.rot(fwh, -1);
       pend_dif = fwh - fwh_m;
     }
     fuz_k = fuz_n;
@@ -680,13 +706,13 @@ __global__ void __launch_bounds__(256) k_mom(Dev d, int na) {
   // lid (w level nzm): horizontal contributions only
   if (live) {
     size_t ow = ((size_t)c * nz + nzm) * NX + i;
-    double dw = dwdt[ow];
+    R dw = dwdt[ow];
     dw = dw - pend_adv;
     dw = dw - pend_dif;
     dwdt[ow] = dw;
     if (i == 0) {
-      prof(d, P_UWLE, c)[0] = 0.0;
-      prof(d, P_UWSB, c)[0] = 0.0;
+      prof(d, P_UWLE, c)[0] = R(0.0);
+      prof(d, P_UWSB, c)[0] = R(0.0);
     }
   }
 }
@@ -698,8 +724,8 @@ __global__ void __launch_bounds__(256) k_mom(Dev d, int na) {
 // one wave; levels are dealt round-robin to the waves; the slab lives in LDS; the Thomas
 // sweep runs on wave 0 with its factor in registers (NZM compile-time).
 // ------------------------------------------------------------------------------------------
-template <int NX, int NZM>
-__global__ void __launch_bounds__(256) k_press(Dev d, int na, int nb, int nc) {
+template <int NX, int NZM, typename R>
+__global__ void __launch_bounds__(256) k_press(DevT<R> d, int na, int nb, int nc) {
   extern __shared__ double slab[];  // [64/NX instances][NZM][NX] == [NZM][64] per lane column
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane & (NX - 1);
@@ -716,8 +742,8 @@ __global__ void __launch_bounds__(256) k_press(Dev d, int na, int nb, int nc) {
   auto P = [&](int k, int col) -> double& { return slab[(size_t)k * 64 + (lane - i) + col]; };
   const double rdx = 1. / d.dx, btat = d.bt / d.at, ctat = d.ct / d.at;
   const double dta = 1. / d.dt3na / d.at;
-  const double *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
-  const double *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
+  const R *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
+  const R *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
   // 1. right-hand side
   for (int k = wave; k < NZM; k += 4) {
     size_t o = ((size_t)c * NZM + k) * NX + i;
@@ -798,10 +824,10 @@ __global__ void __launch_bounds__(256) k_press(Dev d, int na, int nb, int nc) {
   __syncthreads();
   // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities
   const double dtdx = d.dtn / d.dx, dtdz = d.dtn / dz, dt3 = d.dt3na;
-  double* uan = d.dudt[na];
-  double* wan = d.dwdt[na];
-  double* unc = d.dudt[nc];
-  double* wnc = d.dwdt[nc];
+  R* uan = d.dudt[na];
+  R* wan = d.dwdt[na];
+  R* unc = d.dudt[nc];
+  R* wnc = d.dwdt[nc];
   for (int k = wave; k < NZM; k += 4) {
     size_t o = ((size_t)c * NZM + k) * NX + i;
     size_t ow = ((size_t)c * nz + k) * NX + i;
@@ -841,8 +867,8 @@ __global__ void __launch_bounds__(256) k_press(Dev d, int na, int nb, int nc) {
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PLD = 66;  // slab row stride in doubles: conflict-free A-fragment reads
 
-template <int NZM>
-__global__ void __launch_bounds__(256) k_press64(Dev d, int na, int nb, int nc) {
+template <int NZM, typename R>
+__global__ void __launch_bounds__(256) k_press64(DevT<R> d, int na, int nb, int nc) {
   constexpr int NX = 64, NK = 64;
   extern __shared__ double lds[];
   double* P = lds;              // [NK][PLD] slab: rhs, coefficients, pressure
@@ -859,8 +885,8 @@ __global__ void __launch_bounds__(256) k_press64(Dev d, int na, int nb, int nc) 
   const double* rhow = prof(d, P_RHOW, c);
   const double rdx = 1. / d.dx, btat = d.bt / d.at, ctat = d.ct / d.at;
   const double dta = 1. / d.dt3na / d.at;
-  const double *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
-  const double *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
+  const R *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
+  const R *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
   const int r16 = lane & 15, q4 = lane >> 4;
   // 1. right-hand side (rows >= NZM of the slab are zero)
   for (int k = wave; k < NK; k += 4) {
@@ -944,10 +970,10 @@ __global__ void __launch_bounds__(256) k_press64(Dev d, int na, int nb, int nc) 
   transform(d.Gt);
   // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities
   const double dtdx = d.dtn / d.dx, dtdz = d.dtn / dz, dt3 = d.dt3na;
-  double* uan = d.dudt[na];
-  double* wan = d.dwdt[na];
-  double* unc = d.dudt[nc];
-  double* wnc = d.dwdt[nc];
+  R* uan = d.dudt[na];
+  R* wan = d.dwdt[na];
+  R* unc = d.dudt[nc];
+  R* wnc = d.dwdt[nc];
   for (int k = wave; k < NZM; k += 4) {
     size_t o = ((size_t)c * NZM + k) * NX + i;
     size_t ow = ((size_t)c * nz + k) * NX + i;
@@ -979,30 +1005,35 @@ constexpr size_t press64_lds_bytes() {
 // MPDATA scalar advection: ADV_MPDATA/advect_scalar2D.F90:8-244 with the statistics of
 // advect_scalar.F90:9-76.  Software-pipelined over levels: stage s processes level K-s.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ double andiff(double x1, double x2, double a, double b) {
-  return (fabs(a) - a * a * b) * 0.5 * (x2 - x1);
+template <typename T>
+__device__ __forceinline__ T andiff(T x1, T x2, T a, T b) {
+  return (fabs(a) - a * a * b) * (T)0.5 * (x2 - x1);
 }
-__device__ __forceinline__ double across(double x1, double a1, double a2) { return 0.03125 * a1 * a2 * x1; }
-__device__ __forceinline__ double ppos(double y) { return fmax(0.0, y); }
-__device__ __forceinline__ double pneg(double y) { return -fmin(0.0, y); }
+template <typename T>
+__device__ __forceinline__ T across(T x1, T a1, T a2) { return (T)0.03125 * a1 * a2 * x1; }
+template <typename T>
+__device__ __forceinline__ T ppos(T y) { return fmax((T)0.0, y); }
+template <typename T>
+__device__ __forceinline__ T pneg(T y) { return -fmin((T)0.0, y); }
 
 // terminal velocity of precipitation, microphysics.F90:453-486
-__device__ __forceinline__ double term_vel(const TermVel& tv, double q, double rho, double tabs) {
-  double v = 0.;
+template <typename T>
+__device__ __forceinline__ T term_vel(const TermVel& tv, T q, T rho, T tabs) {
+  T v = 0;
   if (q > QP_THRESHOLD) {
-    double omp = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * A_PR()));
-    if (omp == 1.) {
-      v = tv.vrain * pow(rho * q, tv.crain);
-    } else if (omp == 0.) {
-      double omg = fmax(0.0, fmin(1.0, (tabs - TGRMIN) * A_GR()));
-      double qgg = omg * q, qss = q - qgg;
-      v = (omg * tv.vgrau * pow(rho * qgg, tv.cgrau) + (1. - omg) * tv.vsnow * pow(rho * qss, tv.csnow));
+    T omp = fmax(T(0.0), fmin(T(1.0), (tabs - TPRMIN) * (T)A_PR()));
+    if (omp == T(1.)) {
+      v = (T)tv.vrain * fpow<T>(rho * q, (T)tv.crain);
+    } else if (omp == T(0.)) {
+      T omg = fmax(T(0.0), fmin(T(1.0), (tabs - TGRMIN) * (T)A_GR()));
+      T qgg = omg * q, qss = q - qgg;
+      v = (omg * (T)tv.vgrau * fpow<T>(rho * qgg, (T)tv.cgrau) + (T(1.) - omg) * (T)tv.vsnow * fpow<T>(rho * qss, (T)tv.csnow));
     } else {
-      double omg = fmax(0.0, fmin(1.0, (tabs - TGRMIN) * A_GR()));
-      double qrr = omp * q, qss = q - qrr, qgg = omg * qss;
+      T omg = fmax(T(0.0), fmin(T(1.0), (tabs - TGRMIN) * (T)A_GR()));
+      T qrr = omp * q, qss = q - qrr, qgg = omg * qss;
       qss = qss - qgg;
-      v = (omp * tv.vrain * pow(rho * qrr, tv.crain) +
-           (1. - omp) * (omg * tv.vgrau * pow(rho * qgg, tv.cgrau) + (1. - omg) * tv.vsnow * pow(rho * qss, tv.csnow)));
+      v = (omp * (T)tv.vrain * fpow<T>(rho * qrr, (T)tv.crain) +
+           (T(1.) - omp) * (omg * (T)tv.vgrau * fpow<T>(rho * qgg, (T)tv.cgrau) + (T(1.) - omg) * (T)tv.vsnow * fpow<T>(rho * qss, (T)tv.csnow)));
     }
   }
   return v;
@@ -1014,35 +1045,39 @@ __device__ __forceinline__ double term_vel(const TermVel& tv, double q, double r
 // iteration.  Statistics: fadv (advect_scalar.F90:62-72) and the vertical flux of
 // advect_scalar2D.F90:113-118,204-206.  With want_cfl, also the precipitation Courant number
 // of precip_fall (microphysics.F90:596-611) on the advected field.
-template <int NX>
-__device__ void mpdata_one(const Dev& d, const Seg<NX>& s, int c, bool live, double* f, double* fadv,
+template <int NX, typename R>
+__device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live, R* f, double* fadv,
                            double* flux, bool want_cfl) {
   const int nzm = d.nzm, nz = d.nz, i = s.i;
-  const double* adz = prof(d, P_ADZ, c);
-  const double* rho = prof(d, P_RHO, c);
-  const double* rhow = prof(d, P_RHOW, c);
-  const double eps = R4(1.e-10);
-  auto F = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  const R* adz = profr(d, P_ADZ, c);
+  const R* rho = profr(d, P_RHO, c);
+  const R* irho = profr(d, P_IRHO, c);
+  const R* iadz = profr(d, P_IADZ, c);
+  const R* irhowadz = profr(d, P_IRHOWADZ, c);
+  const R* ddadz = profr(d, P_DDADZ, c);
+  const R* pratio = profr(d, P_PRATIO, c);
+  const R eps = R4(1.e-10);
+  auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
   auto Wv = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
-  const double dz = inst(d, I_DZ, c);
-  double pcfl = 0.0;
-  double f_0 = 0, f_1 = 0, f_2 = 0, f_3 = 0, f_4 = 0, f_5 = 0;
+  const R dz = inst(d, I_DZ, c);
+  R pcfl = R(0.0);
+  R f_0 = 0, f_1 = 0, f_2 = 0, f_3 = 0, f_4 = 0, f_5 = 0;
   // inputs of the next two levels, loaded two iterations before use
-  double nf1 = F(f, 0), nu1 = F(d.u, 0), nw1 = Wv(0);
-  double nf2 = F(f, 1), nu2 = F(d.u, 1), nw2 = Wv(1);
+  R nf1 = F(f, 0), nu1 = F(d.u, 0), nw1 = Wv(0);
+  R nf2 = F(f, 1), nu2 = F(d.u, 1), nw2 = Wv(1);
   // tabs for the precipitation CFL of stage 5, loaded two iterations before use
-  double tq_a = 0, tq_b = 0;
-  double U_0 = 0, U_1 = 0, U_2 = 0, U_3 = 0;
-  double W_0 = 0, W_1 = 0, W_2 = 0;
-  double uu1_0 = 0, uu1_1 = 0, ww1_0 = 0, ww1_1 = 0;
-  double f1_a = 0, f1_b = 0, f1_c = 0, f1_d = 0;
-  double mx0_a = 0, mn0_a = 0;
-  double uu2_a = 0, uu2_b = 0, ww2_a = 0, ww2_b = 0;
-  double mx1_a = 0, mn1_a = 0;
-  double mx_a = 0, mx_b = 0, mn_a = 0, mn_b = 0;
-  double uu3_a = 0, ww3_a = 0;
+  R tq_a = 0, tq_b = 0;
+  R U_0 = 0, U_1 = 0, U_2 = 0, U_3 = 0;
+  R W_0 = 0, W_1 = 0, W_2 = 0;
+  R uu1_0 = 0, uu1_1 = 0, ww1_0 = 0, ww1_1 = 0;
+  R f1_a = 0, f1_b = 0, f1_c = 0, f1_d = 0;
+  R mx0_a = 0, mn0_a = 0;
+  R uu2_a = 0, uu2_b = 0, ww2_a = 0, ww2_b = 0;
+  R mx1_a = 0, mn1_a = 0;
+  R mx_a = 0, mx_b = 0, mn_a = 0, mn_b = 0;
+  R uu3_a = 0, ww3_a = 0;
   for (int K = 0; K <= nzm + 4; K++) {
-    const double cf = nf1, cu = nu1, cw = nw1;
+    const R cf = nf1, cu = nu1, cw = nw1;
     nf1 = nf2;
     nu1 = nu2;
     nw1 = nw2;
@@ -1051,91 +1086,91 @@ __device__ void mpdata_one(const Dev& d, const Seg<NX>& s, int c, bool live, dou
       nu2 = F(d.u, K + 2);
       nw2 = Wv(K + 2);
     }
-    double tq_new = 0.0;
+    R tq_new = R(0.0);
     if (want_cfl && K >= 3 && K - 3 < nzm) tq_new = F(d.tabs, K - 3);
     // stage 0: donor-cell fluxes at level K
     if (K < nzm) {
       f_0 = cf;
       U_0 = cu;
       W_0 = cw;
-      double fkb = (K >= 1) ? f_1 : f_0;
-      uu1_0 = fmax(0.0, U_0) * s.rot(f_0, -1) + fmin(0.0, U_0) * f_0;
-      ww1_0 = fmax(0.0, W_0) * fkb + fmin(0.0, W_0) * f_0;
+      R fkb = (K >= 1) ? f_1 : f_0;
+      uu1_0 = fmax(R(0.0), U_0) * s.rot(f_0, -1) + fmin(R(0.0), U_0) * f_0;
+      ww1_0 = fmax(R(0.0), W_0) * fkb + fmin(R(0.0), W_0) * f_0;
     } else {
-      f_0 = U_0 = W_0 = uu1_0 = ww1_0 = 0.0;
+      f_0 = U_0 = W_0 = uu1_0 = ww1_0 = R(0.0);
     }
     // stage 1: donor-cell update and first bounds at level K-1
     const int L1 = K - 1;
-    double f1_new = 0, mx0_new = 0, mn0_new = 0;
+    R f1_new = 0, mx0_new = 0, mn0_new = 0;
     if (L1 >= 0 && L1 < nzm) {
-      double fkb = (L1 >= 1) ? f_2 : f_1, fkc = (L1 + 1 < nzm) ? f_0 : f_1;
-      f1_new = f_1 - (s.rot(uu1_1, 1) - uu1_1 + (ww1_0 - ww1_1) * (1. / adz[L1])) * (1. / rho[L1]);
-      double fm = s.rot(f_1, -1), fp = s.rot(f_1, 1);
+      R fkb = (L1 >= 1) ? f_2 : f_1, fkc = (L1 + 1 < nzm) ? f_0 : f_1;
+      f1_new = f_1 - (s.rot(uu1_1, 1) - uu1_1 + (ww1_0 - ww1_1) * iadz[L1]) * irho[L1];
+      R fm = s.rot(f_1, -1), fp = s.rot(f_1, 1);
       mx0_new = fmax(fmax(fmax(fmax(fm, fp), fkb), fkc), f_1);
       mn0_new = fmin(fmin(fmin(fmin(fm, fp), fkb), fkc), f_1);
-      double s1 = s.sum(ww1_1);
+      R s1 = s.sum(ww1_1);
       if (flux && live && i == 0) flux[L1] = s1;
     }
     // stage 2: antidiffusive velocities and second bounds at level K-2
     const int L2 = K - 2;
-    double uu2_new = 0, ww2_new = 0, mx1_new = 0, mn1_new = 0;
+    R uu2_new = 0, ww2_new = 0, mx1_new = 0, mn1_new = 0;
     if (L2 >= 0 && L2 < nzm) {
       int kc = (L2 + 1 < nzm) ? L2 + 1 : L2, kb = (L2 >= 1) ? L2 - 1 : 0;
-      double f1L = f1_a;
-      double f1c = (kc == L2) ? f1_a : f1_new;
-      double f1b = (kb == L2) ? f1_a : f1_b;
-      double Uc = U_2, Ub = (kb == L2) ? U_2 : U_3;
-      double Wc = W_2, Wkc = (kc == L2) ? W_2 : W_1;
-      double irho = 1. / rho[L2], irhow = 1. / (rhow[L2] * adz[L2]);
-      double dd = (double)(2.0f / (float)(kc - kb)) / adz[L2];
-      double f1L_m = s.rot(f1L, -1), f1L_p = s.rot(f1L, 1);
-      double f1c_m = s.rot(f1c, -1), f1b_m = s.rot(f1b, -1), f1b_p = s.rot(f1b, 1);
-      double Wc_m = s.rot(Wc, -1), Wkc_m = s.rot(Wkc, -1);
-      double Uc_p = s.rot(Uc, 1), Ub_p = s.rot(Ub, 1);
-      uu2_new = andiff(f1L_m, f1L, Uc, irho) -
-                across(dd * (f1c_m + f1c - f1b_m - f1b), Uc, Wc_m + Wkc_m + Wc + Wkc) * irho;
+      R f1L = f1_a;
+      R f1c = (kc == L2) ? f1_a : f1_new;
+      R f1b = (kb == L2) ? f1_a : f1_b;
+      R Uc = U_2, Ub = (kb == L2) ? U_2 : U_3;
+      R Wc = W_2, Wkc = (kc == L2) ? W_2 : W_1;
+      const R irho2 = irho[L2], irhow = irhowadz[L2];
+      const R dd = ddadz[L2];
+      R f1L_m = s.rot(f1L, -1), f1L_p = s.rot(f1L, 1);
+      R f1c_m = s.rot(f1c, -1), f1b_m = s.rot(f1b, -1), f1b_p = s.rot(f1b, 1);
+      R Wc_m = s.rot(Wc, -1), Wkc_m = s.rot(Wkc, -1);
+      R Uc_p = s.rot(Uc, 1), Ub_p = s.rot(Ub, 1);
+      uu2_new = andiff(f1L_m, f1L, Uc, irho2) -
+                across(dd * (f1c_m + f1c - f1b_m - f1b), Uc, Wc_m + Wkc_m + Wc + Wkc) * irho2;
       ww2_new = andiff(f1b, f1L, Wc, irhow) -
-                across(f1b_p + f1L_p - f1b_m - f1L_m, Wc, Ub + Uc + Uc_p + Ub_p) * irho;
-      if (L2 == 0) ww2_new = 0.;
+                across(f1b_p + f1L_p - f1b_m - f1L_m, Wc, Ub + Uc + Uc_p + Ub_p) * irho2;
+      if (L2 == 0) ww2_new = R(0.);
       mx1_new = fmax(fmax(fmax(fmax(fmax(f1L_m, f1L_p), f1b), f1c), f1L), mx0_a);
       mn1_new = fmin(fmin(fmin(fmin(fmin(f1L_m, f1L_p), f1b), f1c), f1L), mn0_a);
     }
     // stage 3: normalised limiters at level K-3
     const int L3 = K - 3;
-    double mx_new = 0, mn_new = 0;
+    R mx_new = 0, mn_new = 0;
     if (L3 >= 0 && L3 < nzm) {
-      double uu2p = s.rot(uu2_a, 1);
-      double ww2c = (L3 + 1 < nzm) ? ww2_new : ww2_a;
-      double iadz = 1. / adz[L3];
-      mx_new = rho[L3] * (mx1_a - f1_b) / (pneg(uu2p) + ppos(uu2_a) + iadz * (pneg(ww2c) + ppos(ww2_a)) + eps);
-      mn_new = rho[L3] * (f1_b - mn1_a) / (ppos(uu2p) + pneg(uu2_a) + iadz * (ppos(ww2c) + pneg(ww2_a)) + eps);
+      R uu2p = s.rot(uu2_a, 1);
+      R ww2c = (L3 + 1 < nzm) ? ww2_new : ww2_a;
+      const R iadz3 = iadz[L3];
+      mx_new = rho[L3] * (mx1_a - f1_b) / (pneg(uu2p) + ppos(uu2_a) + iadz3 * (pneg(ww2c) + ppos(ww2_a)) + eps);
+      mn_new = rho[L3] * (f1_b - mn1_a) / (ppos(uu2p) + pneg(uu2_a) + iadz3 * (ppos(ww2c) + pneg(ww2_a)) + eps);
     }
     // stage 4: limited antidiffusive fluxes at level K-4
     const int L4 = K - 4;
-    double uu3_new = 0, ww3_new = 0;
+    R uu3_new = 0, ww3_new = 0;
     if (L4 >= 0 && L4 < nzm) {
-      double mxm = s.rot(mx_a, -1), mnm = s.rot(mn_a, -1);
-      double mxb = (L4 >= 1) ? mx_b : mx_a, mnb = (L4 >= 1) ? mn_b : mn_a;
-      uu3_new = ppos(uu2_b) * fmin(fmin(1.0, mx_a), mnm) - pneg(uu2_b) * fmin(fmin(1.0, mxm), mn_a);
-      ww3_new = ppos(ww2_b) * fmin(fmin(1.0, mx_a), mnb) - pneg(ww2_b) * fmin(fmin(1.0, mxb), mn_a);
-      double s3 = s.sum(ww3_new);
+      R mxm = s.rot(mx_a, -1), mnm = s.rot(mn_a, -1);
+      R mxb = (L4 >= 1) ? mx_b : mx_a, mnb = (L4 >= 1) ? mn_b : mn_a;
+      uu3_new = ppos(uu2_b) * fmin(fmin(R(1.0), mx_a), mnm) - pneg(uu2_b) * fmin(fmin(R(1.0), mxm), mn_a);
+      ww3_new = ppos(ww2_b) * fmin(fmin(R(1.0), mx_a), mnb) - pneg(ww2_b) * fmin(fmin(R(1.0), mxb), mn_a);
+      R s3 = s.sum(ww3_new);
       if (flux && live && i == 0) flux[L4] = flux[L4] + s3;
     }
     // stage 5: corrected update at level K-5
     const int L5 = K - 5;
     if (L5 >= 0 && L5 < nzm) {
-      double ww3up = (L5 + 1 < nzm) ? ww3_new : 0.0;
-      double fn = fmax(0.0, f1_d - (s.rot(uu3_a, 1) - uu3_a + (ww3up - ww3_a) * (1. / adz[L5])) * (1. / rho[L5]));
+      R ww3up = (L5 + 1 < nzm) ? ww3_new : R(0.0);
+      R fn = fmax(R(0.0), f1_d - (s.rot(uu3_a, 1) - uu3_a + (ww3up - ww3_a) * iadz[L5]) * irho[L5]);
       size_t o = ((size_t)c * nzm + L5) * NX + i;
-      double adv = s.sum(fn - f_5);
+      R adv = s.sum(fn - f_5);
       if (live) {
         f[o] = fn;
         if (fadv && i == 0) fadv[L5] = adv;
       }
       if (want_cfl) {
-        double wmax = dz * adz[L5 >= 1 ? L5 - 1 : 0] / d.dtn;
-        double wp = sqrt(R4(1.29) / rho[L5]) * term_vel(d.tv, fn, rho[L5], tq_b);
-        pcfl = fmax(pcfl, wp * (1. / wmax));
+        R wmax = dz * adz[L5 >= 1 ? L5 - 1 : 0] / (R)d.dtn;
+        R wp = pratio[L5] * term_vel(d.tv, fn, rho[L5], tq_b);
+        pcfl = fmax(pcfl, wp * (R(1.) / wmax));
       }
     }
     f_5 = f_4;
@@ -1178,14 +1213,14 @@ __device__ void mpdata_one(const Dev& d, const Seg<NX>& s, int c, bool live, dou
 }
 
 // advect_all_scalars.F90:7-94: t, then the two microphysics fields.
-template <int NX>
-__global__ void __launch_bounds__(256) k_mpdata(Dev d) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_mpdata(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
-  mpdata_one<NX>(d, s, c, live, d.t, nullptr, nullptr, false);
-  mpdata_one<NX>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
-  mpdata_one<NX>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
+  mpdata_one<NX, R>(d, s, c, live, d.t, nullptr, nullptr, false);
+  mpdata_one<NX, R>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
+  mpdata_one<NX, R>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1193,21 +1228,23 @@ __global__ void __launch_bounds__(256) k_mpdata(Dev d) {
 // sub-step count nprec read from the maximum Courant number gathered in k_mpdata.
 // Per column, one pipelined sweep per sub-step.
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_precip(Dev d) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_precip(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, i = s.i;
-  const double dz = inst(d, I_DZ, c), dtn = d.dtn;
-  const double* adz = prof(d, P_ADZ, c);
-  const double* rho = prof(d, P_RHO, c);
-  const double* rhow = prof(d, P_RHOW, c);
+  const R dz = inst(d, I_DZ, c), dtn = d.dtn;
+  const R* adz = profr(d, P_ADZ, c);
+  const R* rho = profr(d, P_RHO, c);
+  const R* rhow = profr(d, P_RHOW, c);
   double* qpfall = prof(d, P_QPFALL, c);
   double* precflux = prof(d, P_PRECFLUX, c);
-  const double eps = R4(1.e-10);
-  const double a_pr = A_PR();
-  double prec_cfl = __longlong_as_double((long long)d.gmax[1]);
+  const R* pratio = profr(d, P_PRATIO, c);
+  const R* irhoadz_p = profr(d, P_IRHOADZ, c);
+  const R eps = R4(1.e-10);
+  const R a_pr = A_PR();
+  R prec_cfl = __longlong_as_double((long long)d.gmax[1]);
   int nprec = 1;
   if (!(prec_cfl <= NPREC_MAX * R4(0.9))) {
     // non-finite or runaway precipitation Courant number: one sweep, and the host reports
@@ -1216,81 +1253,81 @@ __global__ void __launch_bounds__(256) k_precip(Dev d) {
   } else if (prec_cfl > R4(0.9)) {
     nprec = (int)ceil(prec_cfl / R4(0.9));
   }
-  const double rnprec = (double)nprec;
-  auto Q = [&](double* p, int k) -> double& { return p[((size_t)c * nzm + k) * NX + i]; };
-  auto lfac_of = [&](double tabs) {
-    double om = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * a_pr));
+  const R rnprec = (R)nprec;
+  auto Q = [&](R* p, int k) -> R& { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto lfac_of = [&](R tabs) {
+    R om = fmax(R(0.0), fmin(R(1.0), (tabs - TPRMIN) * a_pr));
     return FAC_COND + (1 - om) * FAC_FUS;
   };
-  const double omega0 = fmax(0.0, fmin(1.0, (Q(d.tabs, 0) - TPRMIN) * a_pr));
+  const R omega0 = fmax(R(0.0), fmin(R(1.0), (Q(d.tabs, 0) - TPRMIN) * a_pr));
   for (int iprec = 1; iprec <= nprec; iprec++) {
-    double tq_0 = 0, tq_1 = 0, tq_2 = 0, tq_3 = 0;
-    double ta_0 = 0, ta_1 = 0, ta_2 = 0, ta_3 = 0;  // tabs ring
-    double tt_a = 0, tt_b = 0;                       // t of level K-2, K-3
+    R tq_0 = 0, tq_1 = 0, tq_2 = 0, tq_3 = 0;
+    R ta_0 = 0, ta_1 = 0, ta_2 = 0, ta_3 = 0;  // tabs ring
+    R tt_a = 0, tt_b = 0;                       // t of level K-2, K-3
     // qp and tabs of the next two levels, loaded two iterations before use
-    double nq1 = Q(d.qp, 0), na1 = Q(d.tabs, 0), nq2 = Q(d.qp, 1), na2 = Q(d.tabs, 1);
-    double wp_0 = 0, wp_1 = 0, wp_2 = 0;
-    double fz0_0 = 0, fz0_1 = 0, fz0_2 = 0;
-    double tq1_a = 0, tq1_b = 0;
-    double mx0_a = 0, mn0_a = 0;
-    double www_a = 0;
-    double mx_a = 0, mn_a = 0;
-    double fz_a = 0;
+    R nq1 = Q(d.qp, 0), na1 = Q(d.tabs, 0), nq2 = Q(d.qp, 1), na2 = Q(d.tabs, 1);
+    R wp_0 = 0, wp_1 = 0, wp_2 = 0;
+    R fz0_0 = 0, fz0_1 = 0, fz0_2 = 0;
+    R tq1_a = 0, tq1_b = 0;
+    R mx0_a = 0, mn0_a = 0;
+    R www_a = 0;
+    R mx_a = 0, mn_a = 0;
+    R fz_a = 0;
     for (int K = 0; K <= nzm + 2; K++) {
-      const double cq = nq1, ca = na1;
+      const R cq = nq1, ca = na1;
       nq1 = nq2;
       na1 = na2;
       if (K + 2 < nzm) {
         nq2 = Q(d.qp, K + 2);
         na2 = Q(d.tabs, K + 2);
       }
-      const double tt_new = (K >= 1 && K - 1 < nzm) ? Q(d.t, K - 1) : 0.0;
+      const R tt_new = (K >= 1 && K - 1 < nzm) ? Q(d.t, K - 1) : R(0.0);
       // a: level K, upwind flux
       if (K < nzm) {
         tq_0 = cq;
         ta_0 = ca;
-        double v = sqrt(R4(1.29) / rho[K]) * term_vel(d.tv, tq_0, rho[K], ta_0);
+        R v = pratio[K] * term_vel(d.tv, tq_0, rho[K], ta_0);
         wp_0 = -v * rhow[K] * dtn / dz / rnprec;
         fz0_0 = tq_0 * wp_0;
       } else {
-        tq_0 = wp_0 = fz0_0 = 0.0;
+        tq_0 = wp_0 = fz0_0 = R(0.0);
       }
       // b: level K-1, upwind update, first bounds, antidiffusive flux
       const int B = K - 1;
-      double tq1_new = 0, mx0_new = 0, mn0_new = 0, www_new = 0;
+      R tq1_new = 0, mx0_new = 0, mn0_new = 0, www_new = 0;
       if (B >= 0 && B < nzm) {
-        double irhoadz = 1. / (rho[B] * adz[B]);
+        R irhoadz = irhoadz_p[B];
         tq1_new = tq_1 - (fz0_0 - fz0_1) * irhoadz;
-        double tkb = (B >= 1) ? tq_2 : tq_1, tkc = (B + 1 < nzm) ? tq_0 : tq_1;
+        R tkb = (B >= 1) ? tq_2 : tq_1, tkc = (B + 1 < nzm) ? tq_0 : tq_1;
         mx0_new = fmax(fmax(tkb, tkc), tq_1);
         mn0_new = fmin(fmin(tkb, tkc), tq_1);
-        double t1b = (B >= 1) ? tq1_a : tq1_new, wpb = (B >= 1) ? wp_2 : wp_1;
-        www_new = 0.5 * (1. + wp_1 * irhoadz) * (t1b * wpb - tq1_new * wp_1);
+        R t1b = (B >= 1) ? tq1_a : tq1_new, wpb = (B >= 1) ? wp_2 : wp_1;
+        www_new = R(0.5) * (R(1.) + wp_1 * irhoadz) * (t1b * wpb - tq1_new * wp_1);
       }
       // d: level K-2, limiter and corrected flux
       const int D = K - 2;
-      double mx_new = 0, mn_new = 0, fz_new = 0;
+      R mx_new = 0, mn_new = 0, fz_new = 0;
       if (D >= 0 && D < nzm) {
-        double t1b = (D >= 1) ? tq1_b : tq1_a, t1c = (D + 1 < nzm) ? tq1_new : tq1_a;
-        double mx1 = fmax(fmax(fmax(t1b, t1c), tq1_a), mx0_a);
-        double mn1 = fmin(fmin(fmin(t1b, t1c), tq1_a), mn0_a);
-        double wwc = (D + 1 < nzm) ? www_new : www_a;
-        double ra = rho[D] * adz[D];
+        R t1b = (D >= 1) ? tq1_b : tq1_a, t1c = (D + 1 < nzm) ? tq1_new : tq1_a;
+        R mx1 = fmax(fmax(fmax(t1b, t1c), tq1_a), mx0_a);
+        R mn1 = fmin(fmin(fmin(t1b, t1c), tq1_a), mn0_a);
+        R wwc = (D + 1 < nzm) ? www_new : www_a;
+        R ra = rho[D] * adz[D];
         mx_new = ra * (mx1 - tq1_a) / (pneg(wwc) + ppos(www_a) + eps);
         mn_new = ra * (tq1_a - mn1) / (ppos(wwc) + pneg(www_a) + eps);
-        double mxb = (D >= 1) ? mx_a : mx_new, mnb = (D >= 1) ? mn_a : mn_new;
-        fz_new = fz0_2 + ppos(www_a) * fmin(fmin(1.0, mx_new), mnb) - pneg(www_a) * fmin(fmin(1.0, mxb), mn_new);
+        R mxb = (D >= 1) ? mx_a : mx_new, mnb = (D >= 1) ? mn_a : mn_new;
+        fz_new = fz0_2 + ppos(www_a) * fmin(fmin(R(1.0), mx_new), mnb) - pneg(www_a) * fmin(fmin(R(1.0), mxb), mn_new);
       }
       // f: level K-3, update of qp and t, statistics
       const int Fl = K - 3;
       if (Fl >= 0 && Fl < nzm) {
-        double irhoadz = 1. / (rho[Fl] * adz[Fl]);
-        double fzF = fz_a, fzF1 = (Fl + 1 < nzm) ? fz_new : 0.0;
-        double lf = lfac_of(ta_3), lf1 = (Fl + 1 < nzm) ? lfac_of(ta_2) : 0.0;
-        double dq = -(fzF1 - fzF) * irhoadz;
-        double lat_heat = -(lf1 * fzF1 - lf * fzF) * irhoadz;
-        double sfall = s.sum(dq * 1.);
-        double sflux = s.sum(fzF * 1.);
+        R irhoadz = irhoadz_p[Fl];
+        R fzF = fz_a, fzF1 = (Fl + 1 < nzm) ? fz_new : R(0.0);
+        R lf = lfac_of(ta_3), lf1 = (Fl + 1 < nzm) ? lfac_of(ta_2) : R(0.0);
+        R dq = -(fzF1 - fzF) * irhoadz;
+        R lat_heat = -(lf1 * fzF1 - lf * fzF) * irhoadz;
+        R sfall = s.sum(dq * R(1.));
+        R sflux = s.sum(fzF * R(1.));
         if (live) {
           Q(d.qp, Fl) = tq_3 - (fzF1 - fzF) * irhoadz;
           Q(d.t, Fl) = tt_b - lat_heat;
@@ -1299,8 +1336,8 @@ __global__ void __launch_bounds__(256) k_precip(Dev d) {
             accum(&precflux[Fl], -sflux);
           }
           if (Fl == 0) {
-            d.precsfc[(size_t)c * NX + i] -= fzF * 1.;
-            d.precssfc[(size_t)c * NX + i] -= fzF * (1. - omega0) * 1.;
+            d.precsfc[(size_t)c * NX + i] -= fzF * R(1.);
+            d.precssfc[(size_t)c * NX + i] -= fzF * (R(1.) - omega0) * R(1.);
           }
         }
       }
@@ -1332,42 +1369,43 @@ __global__ void __launch_bounds__(256) k_precip(Dev d) {
 // SGS diffusion of t, qt, qp: diffuse_scalar.F90:8-72 and diffuse_scalar2D.F90:5-141 (zero
 // surface and top fluxes in the CRM).
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__device__ void diffuse_one(const Dev& d, const Seg<NX>& s, int c, bool live, double* f, double* fdiff,
+template <int NX, typename R>
+__device__ void diffuse_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live, R* f, double* fdiff,
                             double* flux) {
   const int nzm = d.nzm, i = s.i;
-  const double dz = inst(d, I_DZ, c), dtn = d.dtn;
-  const double* adz = prof(d, P_ADZ, c);
-  const double* adzw = prof(d, P_ADZW, c);
-  const double* rho = prof(d, P_RHO, c);
-  const double* rhow = prof(d, P_RHOW, c);
-  const double* grdf = prof(d, P_GRDFX, c);
-  const double rdx2 = 1. / (d.dx * d.dx), rdz2 = 1. / (dz * dz);
-  auto Q = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
-  double f_k = Q(f, 0), f_n = Q(f, 1), tkh_k = Q(d.tkh, 0), tkh_n = Q(d.tkh, 1);
-  double flx_prev = 0.0 * (1. / dz) * rhow[0];
+  const R dz = inst(d, I_DZ, c), dtn = d.dtn;
+  const R* adz = profr(d, P_ADZ, c);
+  const R* adzw = profr(d, P_ADZW, c);
+  const R* rho = profr(d, P_RHO, c);
+  const R* rhow = profr(d, P_RHOW, c);
+  const R* grdf = profr(d, P_GRDFX, c);
+  const R* irhoadz = profr(d, P_IRHOADZ, c);
+  const R rdx2 = (R)(R(1.) / (d.dx * d.dx)), rdz2 = (R)1. / (dz * dz);
+  auto Q = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  R f_k = Q(f, 0), f_n = Q(f, 1), tkh_k = Q(d.tkh, 0), tkh_n = Q(d.tkh, 1);
+  R flx_prev = R(0.0) * (R(1.) / dz) * rhow[0];
   for (int k = 0; k < nzm; k++) {
-    double f_nn = (k + 2 < nzm) ? Q(f, k + 2) : 0.0;
-    double tkh_nn = (k + 2 < nzm) ? Q(d.tkh, k + 2) : 0.0;
-    double rdx5 = 0.5 * rdx2 * grdf[k];
-    double tkx = rdx5 * (tkh_k + s.rot(tkh_k, 1));
-    double flxh = -tkx * (s.rot(f_k, 1) - f_k);
-    double dfh = 0. - (flxh - s.rot(flxh, -1));
-    double flxv;
+    R f_nn = (k + 2 < nzm) ? Q(f, k + 2) : R(0.0);
+    R tkh_nn = (k + 2 < nzm) ? Q(d.tkh, k + 2) : R(0.0);
+    R rdx5 = R(0.5) * rdx2 * grdf[k];
+    R tkx = rdx5 * (tkh_k + s.rot(tkh_k, 1));
+    R flxh = -tkx * (s.rot(f_k, 1) - f_k);
+    R dfh = R(0.) - (flxh - s.rot(flxh, -1));
+    R flxv;
     if (k <= nzm - 2) {
-      double rhoi = rhow[k + 1] / adzw[k + 1];
-      double rdz5 = 0.5 * rdz2 * 1.0;
-      double tkz = rdz5 * (tkh_k + tkh_n);
+      R rhoi = rhow[k + 1] / adzw[k + 1];
+      R rdz5 = R(0.5) * rdz2 * R(1.0);
+      R tkz = rdz5 * (tkh_k + tkh_n);
       flxv = -tkz * (f_n - f_k) * rhoi;
-      double sv = s.sum(flxv);
+      R sv = s.sum(flxv);
       if (flux && live && i == 0) flux[k + 1] = sv;
     } else {
-      flxv = 0.0 * (1. / dz) * (1. / adzw[nzm]) * rhow[nzm];
+      flxv = R(0.0) * (R(1.) / dz) * (R(1.) / adzw[nzm]) * rhow[nzm];
     }
-    double rhoi = 1. / (adz[k] * rho[k]);
-    double dfdt = dtn * (dfh - (flxv - flx_prev) * rhoi);
-    double fnew = f_k + dfdt;
-    double sd = s.sum(fnew - f_k);
+    R rhoi = irhoadz[k];
+    R dfdt = dtn * (dfh - (flxv - flx_prev) * rhoi);
+    R fnew = f_k + dfdt;
+    R sd = s.sum(fnew - f_k);
     if (live) {
       f[((size_t)c * nzm + k) * NX + i] = fnew;
       if (fdiff && i == 0) fdiff[k] = sd;
@@ -1378,28 +1416,29 @@ __device__ void diffuse_one(const Dev& d, const Seg<NX>& s, int c, bool live, do
     tkh_k = tkh_n;
     tkh_n = tkh_nn;
   }
-  if (flux && live && i == 0) flux[0] = 0.0;
+  if (flux && live && i == 0) flux[0] = R(0.0);
 }
 
-template <int NX>
-__global__ void __launch_bounds__(256) k_diffuse(Dev d) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_diffuse(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
-  diffuse_one<NX>(d, s, c, live, d.t, nullptr, nullptr);
-  diffuse_one<NX>(d, s, c, live, d.qt, prof(d, P_MKDIFF1, c), prof(d, P_MKWSB1, c));
-  diffuse_one<NX>(d, s, c, live, d.qp, prof(d, P_MKDIFF2, c), prof(d, P_MKWSB2, c));
+  diffuse_one<NX, R>(d, s, c, live, d.t, nullptr, nullptr);
+  diffuse_one<NX, R>(d, s, c, live, d.qt, prof(d, P_MKDIFF1, c), prof(d, P_MKWSB1, c));
+  diffuse_one<NX, R>(d, s, c, live, d.qp, prof(d, P_MKDIFF2, c), prof(d, P_MKWSB2, c));
 }
 
 // ------------------------------------------------------------------------------------------
 // Bulk microphysics coefficients per level: precip_init.F90:47-91 (one thread per level).
 // ------------------------------------------------------------------------------------------
-__global__ void k_precip_init(Dev d) {
+template <typename R>
+__global__ void k_precip_init(DevT<R> d) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= d.ncrms * d.nzm) return;
   int c = idx / d.nzm, k = idx - c * d.nzm;
   double rho = prof(d, P_RHO, c)[k], t0 = prof(d, P_TABS0, c)[k], pres = prof(d, P_PRES, c)[k];
-  auto C = [&](int q) -> double& { return d.coef[((size_t)q * d.ncrms + c) * d.nz + k]; };
+  auto C = [&](int q) -> R& { return d.coef[((size_t)q * d.ncrms + c) * d.nz + k]; };
   const Gam& g = d.gam;
   double pratio = sqrt(R4(1.29) / rho);
   double rrr1 = 393. / (t0 + 120.) * pow(t0 / 273., 1.5);
@@ -1439,34 +1478,34 @@ __global__ void k_precip_init(Dev d) {
 // per level followed by the level means.  Also finds the cloud-ice level range for the next
 // ice fall (ice_fall.F90:34-46).
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_micro(Dev d) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_micro(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, i = s.i;
-  const double dtn = d.dtn, factor = 1. / (double)NX;
-  const double* gamaz = prof(d, P_GAMAZ, c);
-  const double* presp = prof(d, P_PRES, c);
-  const double an = 1. / (TBGMAX - TBGMIN), bn = TBGMIN * an;
-  const double ap = 1. / (TPRMAX - TPRMIN), bp = TPRMIN * ap;
-  const double fac1 = FAC_COND + (1 + bp) * FAC_FUS, fac2 = FAC_FUS * ap;
-  const double a_bg = A_BG(), a_pr = A_PR(), a_gr = A_GR();
-  const double powr1 = (3 + B_RAIN) / 4., powr2 = (5 + B_RAIN) / 8.;
-  const double pows1 = (3 + B_SNOW) / 4., pows2 = (5 + B_SNOW) / 8.;
-  const double powg1 = (3 + B_GRAU) / 4., powg2 = (5 + B_GRAU) / 8.;
+  const R dtn = d.dtn, factor = R(1.) / (R)NX;
+  const R* gamaz = profr(d, P_GAMAZ, c);
+  const R* presp = profr(d, P_PRES, c);
+  const R an = R(1.) / (TBGMAX - TBGMIN), bn = TBGMIN * an;
+  const R ap = R(1.) / (TPRMAX - TPRMIN), bp = TPRMIN * ap;
+  const R fac1 = FAC_COND + (1 + bp) * FAC_FUS, fac2 = FAC_FUS * ap;
+  const R a_bg = A_BG(), a_pr = A_PR(), a_gr = A_GR();
+  const R powr1 = (3 + B_RAIN) / R(4.), powr2 = (5 + B_RAIN) / R(8.);
+  const R pows1 = (3 + B_SNOW) / R(4.), pows2 = (5 + B_SNOW) / R(8.);
+  const R powg1 = (3 + B_GRAU) / R(4.), powg2 = (5 + B_GRAU) / R(8.);
   int kmin = nzm, kmax = -1;
   for (int k = 0; k < nzm; k++) {
     size_t o = ((size_t)c * nzm + k) * NX + i;
     auto CO = [&](int q) { return d.coef[((size_t)q * d.ncrms + c) * d.nz + k]; };
-    const double pres = presp[k];
-    double t = d.t[o];
-    double q = fmax(0.0, d.qt[o]);
-    double qp = d.qp[o];
+    const R pres = presp[k];
+    R t = d.t[o];
+    R q = fmax(R(0.0), d.qt[o]);
+    R qp = d.qp[o];
     // cloud(): saturation adjustment with Newton iterations
-    double tabs = t - gamaz[k];
-    double tabs1 = (tabs + fac1 * qp) / (1. + fac2 * qp);
-    double qsatt, om;
+    R tabs = t - gamaz[k];
+    R tabs1 = (tabs + fac1 * qp) / (R(1.) + fac2 * qp);
+    R qsatt, om;
     if (tabs1 >= TBGMAX) {
       tabs1 = tabs + FAC_COND * qp;
       qsatt = qsatw(tabs1, pres);
@@ -1475,133 +1514,133 @@ __global__ void __launch_bounds__(256) k_micro(Dev d) {
       qsatt = qsati(tabs1, pres);
     } else {
       om = an * tabs1 - bn;
-      qsatt = om * qsatw(tabs1, pres) + (1. - om) * qsati(tabs1, pres);
+      qsatt = om * qsatw(tabs1, pres) + (R(1.) - om) * qsati(tabs1, pres);
     }
-    double qn;
+    R qn;
     if (q > qsatt) {
       int niter = 0;
-      double dtabs = 100., dqsat = 0.;
+      R dtabs = R(100.), dqsat = R(0.);
       while (fabs(dtabs) > R4(0.01) && niter < 10) {
-        double lstarn, dlstarn, lstarp, dlstarp;
+        R lstarn, dlstarn, lstarp, dlstarp;
         if (tabs1 >= TBGMAX) {
           lstarn = FAC_COND;
-          dlstarn = 0.;
+          dlstarn = R(0.);
           qsatt = qsatw(tabs1, pres);
           dqsat = dtqsatw(tabs1, pres);
         } else if (tabs1 <= TBGMIN) {
           lstarn = FAC_SUB;
-          dlstarn = 0.;
+          dlstarn = R(0.);
           qsatt = qsati(tabs1, pres);
           dqsat = dtqsati(tabs1, pres);
         } else {
           om = an * tabs1 - bn;
-          lstarn = FAC_COND + (1. - om) * FAC_FUS;
+          lstarn = FAC_COND + (R(1.) - om) * FAC_FUS;
           dlstarn = an * FAC_FUS;
-          qsatt = om * qsatw(tabs1, pres) + (1. - om) * qsati(tabs1, pres);
-          dqsat = om * dtqsatw(tabs1, pres) + (1. - om) * dtqsati(tabs1, pres);
+          qsatt = om * qsatw(tabs1, pres) + (R(1.) - om) * qsati(tabs1, pres);
+          dqsat = om * dtqsatw(tabs1, pres) + (R(1.) - om) * dtqsati(tabs1, pres);
         }
         if (tabs1 >= TPRMAX) {
           lstarp = FAC_COND;
-          dlstarp = 0.;
+          dlstarp = R(0.);
         } else if (tabs1 <= TPRMIN) {
           lstarp = FAC_SUB;
-          dlstarp = 0.;
+          dlstarp = R(0.);
         } else {
-          double omp = ap * tabs1 - bp;
-          lstarp = FAC_COND + (1. - omp) * FAC_FUS;
+          R omp = ap * tabs1 - bp;
+          lstarp = FAC_COND + (R(1.) - omp) * FAC_FUS;
           dlstarp = ap * FAC_FUS;
         }
-        double fff = tabs - tabs1 + lstarn * (q - qsatt) + lstarp * qp;
-        double dfff = dlstarn * (q - qsatt) + dlstarp * qp - lstarn * dqsat - 1.;
+        R fff = tabs - tabs1 + lstarn * (q - qsatt) + lstarp * qp;
+        R dfff = dlstarn * (q - qsatt) + dlstarp * qp - lstarn * dqsat - R(1.);
         dtabs = -fff / dfff;
         niter++;
         tabs1 = tabs1 + dtabs;
       }
       qsatt = qsatt + dqsat * dtabs;
-      qn = fmax(0.0, q - qsatt);
+      qn = fmax(R(0.0), q - qsatt);
     } else {
-      qn = 0.;
+      qn = R(0.);
     }
     tabs = tabs1;
-    qp = fmax(0.0, qp);
+    qp = fmax(R(0.0), qp);
     // precip_proc()
-    double src = 0.0, evp = 0.0;
-    if (qn + qp > 0.) {
-      double omn = fmax(0.0, fmin(1.0, (tabs - TBGMIN) * a_bg));
-      double omp = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * a_pr));
-      double omg = fmax(0.0, fmin(1.0, (tabs - TGRMIN) * a_gr));
-      if (qn > 0.) {
-        double qcc = qn * omn, qii = qn * (1. - omn);
-        double autor = (qcc > QCW0) ? ALPHAELQ : 0.;
-        double autos = (qii > QCI0) ? BETAELQ * CO(C_COEFICE) : 0.;
-        double accrr = 0.;
-        if (omp > R4(0.001)) accrr = CO(C_ACCRRC) * pow(qp * omp, powr1);
-        double accrcs = 0., accris = 0.;
+    R src = R(0.0), evp = R(0.0);
+    if (qn + qp > R(0.)) {
+      R omn = fmax(R(0.0), fmin(R(1.0), (tabs - TBGMIN) * a_bg));
+      R omp = fmax(R(0.0), fmin(R(1.0), (tabs - TPRMIN) * a_pr));
+      R omg = fmax(R(0.0), fmin(R(1.0), (tabs - TGRMIN) * a_gr));
+      if (qn > R(0.)) {
+        R qcc = qn * omn, qii = qn * (R(1.) - omn);
+        R autor = (qcc > QCW0) ? ALPHAELQ : R(0.);
+        R autos = (qii > QCI0) ? BETAELQ * CO(C_COEFICE) : R(0.);
+        R accrr = R(0.);
+        if (omp > R4(0.001)) accrr = CO(C_ACCRRC) * fpow<R>(qp * omp, powr1);
+        R accrcs = R(0.), accris = R(0.);
         if (omp < R4(0.999) && omg < R4(0.999)) {
-          double tmp = pow(qp * (1. - omp) * (1. - omg), pows1);
+          R tmp = fpow<R>(qp * (R(1.) - omp) * (R(1.) - omg), pows1);
           accrcs = CO(C_ACCRSC) * tmp;
           accris = CO(C_ACCRSI) * tmp;
         }
-        double accrcg = 0., accrig = 0.;
+        R accrcg = R(0.), accrig = R(0.);
         if (omp < R4(0.999) && omg > R4(0.001)) {
-          double tmp = pow(qp * (1. - omp) * omg, powg1);
+          R tmp = fpow<R>(qp * (R(1.) - omp) * omg, powg1);
           accrcg = CO(C_ACCRGC) * tmp;
           accrig = CO(C_ACCRGI) * tmp;
         }
-        qcc = (qcc + dtn * autor * QCW0) / (1. + dtn * (accrr + accrcs + accrcg + autor));
-        qii = (qii + dtn * autos * QCI0) / (1. + dtn * (accris + accrig + autos));
-        double dq = dtn * (accrr * qcc + autor * (qcc - QCW0) + (accris + accrig) * qii + (accrcs + accrcg) * qcc + autos * (qii - QCI0));
+        qcc = (qcc + dtn * autor * QCW0) / (R(1.) + dtn * (accrr + accrcs + accrcg + autor));
+        qii = (qii + dtn * autos * QCI0) / (R(1.) + dtn * (accris + accrig + autos));
+        R dq = dtn * (accrr * qcc + autor * (qcc - QCW0) + (accris + accrig) * qii + (accrcs + accrcg) * qcc + autos * (qii - QCI0));
         dq = fmin(dq, qn);
         qp = qp + dq;
         q = q - dq;
         qn = qn - dq;
         src = dq;
-      } else if (qp > QP_THRESHOLD && qn == 0.) {
-        double qs = 0.;
+      } else if (qp > QP_THRESHOLD && qn == R(0.)) {
+        R qs = R(0.);
         if (omn > R4(0.001)) qs = qs + omn * qsatw(tabs, pres);
-        if (omn < R4(0.999)) qs = qs + (1. - omn) * qsati(tabs, pres);
-        double dq = 0.;
+        if (omn < R4(0.999)) qs = qs + (R(1.) - omn) * qsati(tabs, pres);
+        R dq = R(0.);
         if (omp > R4(0.001)) {
-          double qrr = qp * omp;
-          dq = dq + CO(C_EVAPR1) * sqrt(qrr) + CO(C_EVAPR2) * pow(qrr, powr2);
+          R qrr = qp * omp;
+          dq = dq + CO(C_EVAPR1) * sqrt(qrr) + CO(C_EVAPR2) * fpow<R>(qrr, powr2);
         }
         if (omp < R4(0.999) && omg < R4(0.999)) {
-          double qss = qp * (1. - omp) * (1. - omg);
-          dq = dq + CO(C_EVAPS1) * sqrt(qss) + CO(C_EVAPS2) * pow(qss, pows2);
+          R qss = qp * (R(1.) - omp) * (R(1.) - omg);
+          dq = dq + CO(C_EVAPS1) * sqrt(qss) + CO(C_EVAPS2) * fpow<R>(qss, pows2);
         }
         if (omp < R4(0.999) && omg > R4(0.001)) {
-          double qgg = qp * (1. - omp) * omg;
-          dq = dq + CO(C_EVAPG1) * sqrt(qgg) + CO(C_EVAPG2) * pow(qgg, powg2);
+          R qgg = qp * (R(1.) - omp) * omg;
+          dq = dq + CO(C_EVAPG1) * sqrt(qgg) + CO(C_EVAPG2) * fpow<R>(qgg, powg2);
         }
-        dq = dq * dtn * (q / qs - 1.);
-        dq = fmax(-0.5 * qp, dq);
+        dq = dq * dtn * (q / qs - R(1.));
+        dq = fmax(-R(0.5) * qp, dq);
         qp = qp + dq;
         q = q - dq;
         evp = dq;
       } else {
         q = q + qp;
         evp = -qp;
-        qp = 0.;
+        qp = R(0.);
       }
     }
     {
-      double dq = qp;
-      qp = fmax(0.0, qp);
+      R dq = qp;
+      qp = fmax(R(0.0), qp);
       q = q + (dq - qp);
     }
-    double ssrc = s.sum(src), sevp = s.sum(evp);
+    R ssrc = s.sum(src), sevp = s.sum(evp);
     // micro_diagnose, then diagnose
-    double qv, qcl, qci, qpl, qpi;
+    R qv, qcl, qci, qpl, qpi;
     micro_diag(q, qn, qp, tabs, a_bg, a_pr, qv, qcl, qci, qpl, qpi);
     tabs = t - gamaz[k] + FAC_COND * (qcl + qpl) + FAC_SUB * (qci + qpi);
-    double u = d.u[o];
-    double u0 = s.sum(u) * factor;
-    double t0 = s.sum(t) * factor;
-    double tabs0 = s.sum(tabs) * factor;
-    double q0 = s.sum(qv + qcl + qci) * factor;
-    double qn0 = s.sum(qcl + qci) * factor;
-    double qp0 = s.sum(qpl + qpi) * factor;
-    if (qcl + qci > 0. && tabs < R4(273.15)) {
+    R u = d.u[o];
+    R u0 = s.sum(u) * factor;
+    R t0 = s.sum(t) * factor;
+    R tabs0 = s.sum(tabs) * factor;
+    R q0 = s.sum(qv + qcl + qci) * factor;
+    R qn0 = s.sum(qcl + qci) * factor;
+    R qp0 = s.sum(qpl + qpi) * factor;
+    if (qcl + qci > R(0.) && tabs < R4(273.15)) {
       kmin = ::min(kmin, k);
       kmax = ::max(kmax, k);
     }
@@ -1625,6 +1664,15 @@ __global__ void __launch_bounds__(256) k_micro(Dev d) {
         prof(d, P_QN0, c)[k] = qn0;
         prof(d, P_QP0, c)[k] = qp0;
         prof(d, P_QV0, c)[k] = q0 - qn0;
+        if constexpr (!std::is_same<R, double>::value) {  // FP32 copies read by the bulk kernels
+          profr(d, P_U0, c)[k] = u0;
+          profr(d, P_T0, c)[k] = t0;
+          profr(d, P_TABS0, c)[k] = tabs0;
+          profr(d, P_Q0, c)[k] = q0;
+          profr(d, P_QN0, c)[k] = qn0;
+          profr(d, P_QP0, c)[k] = qp0;
+          profr(d, P_QV0, c)[k] = q0 - qn0;
+        }
       }
     }
   }
@@ -1640,8 +1688,8 @@ __global__ void __launch_bounds__(256) k_micro(Dev d) {
 // Per-GCM-substep statistics: crm_module.F90:896-1073.  Accumulated on the device in
 // profiles indexed by CRM level (w level for the mass fluxes) and per-instance sums.
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_stats(Dev d) {
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_stats(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -1653,8 +1701,8 @@ __global__ void __launch_bounds__(256) k_stats(Dev d) {
   const double* pres = prof(d, P_PRES, c);
   const double thr = R4(0.001), wmin = 2.;
   const int nxr = d.nx_rad, g = NX / nxr;
-  auto Q = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
-  auto Wv = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
+  auto Q = [&](const R* p, int k) -> double { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto Wv = [&](int k) -> double { return d.w[((size_t)c * nz + k) * NX + i]; };
   double cwp = 0, cwph = 0, cwpm = 0, cwpl = 0, clt = 0, cmt = 0, cht = 0, ctt = 0;
   bool flag_top = true;
   // level k+1 values (above) for the downdraft test, level k-1 read ahead for the updraft test
@@ -1770,8 +1818,8 @@ __global__ void __launch_bounds__(256) k_stats(Dev d) {
 // ------------------------------------------------------------------------------------------
 // End of call: crm_module.F90:1078-1659.  State write-back and every crm_output member.
 // ------------------------------------------------------------------------------------------
-template <int NX>
-__global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_state_view st, crm_rad_view rad, crm_output_view out,
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_final(DevT<R> d, crm_input_view in, crm_state_view st, crm_rad_view rad, crm_output_view out,
                         double dt_gl, int nstop) {
   Seg<NX> s;
   int c;
@@ -1786,7 +1834,7 @@ __global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_sta
   auto IN = [&](const double* p, int l) { return p[(size_t)c + (size_t)ld * l]; };
   auto O2 = [&](double* p, int l) -> double& { return p[(size_t)c + (size_t)ld * l]; };
   auto G4 = [&](double* p, int k) -> double& { return p[(size_t)c + (size_t)ld * ((size_t)i + (size_t)NX * k)]; };
-  auto Q = [&](const double* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto Q = [&](const R* p, int k) -> double { return p[((size_t)c * nzm + k) * NX + i]; };
   auto Wv = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
   const double a_gr = A_GR();
   double colprec = 0, colprecs = 0;
@@ -1824,7 +1872,7 @@ __global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_sta
       G4(out.qpi, k) = qpi;
       if (i == 0) {
         bool top2 = (l == ptop0 || l == ptop0 + 1);
-        O2(out.sltend, l) = top2 ? 0.0 : CP * (tln - IN(in.tl, l)) * icrm_run_time;
+        O2(out.sltend, l) = top2 ? 0.0 : CP_D * (tln - IN(in.tl, l)) * icrm_run_time;
         O2(out.qltend, l) = top2 ? 0.0 : (qln - IN(in.ql, l)) * icrm_run_time;
         O2(out.qcltend, l) = top2 ? 0.0 : (qccln - IN(in.qccl, l)) * icrm_run_time;
         O2(out.qiltend, l) = top2 ? 0.0 : (qiiln - IN(in.qiil, l)) * icrm_run_time;
@@ -1835,8 +1883,8 @@ __global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_sta
         O2(out.qs_mean, l) = qs_mean;
         O2(out.cld, l) = fmin(1.0, prof(d, P_CLD, c)[k] * factor_xyt);
         O2(out.cldtop, l) = fmin(1.0, prof(d, P_CLDTOP, c)[k] * factor_xyt);
-        O2(out.gicewp, l) = prof(d, P_GICE, c)[k] * pd * 1000. / GGR * factor_xyt;
-        O2(out.gliqwp, l) = prof(d, P_GLIQ, c)[k] * pd * 1000. / GGR * factor_xyt;
+        O2(out.gicewp, l) = prof(d, P_GICE, c)[k] * pd * 1000. / GGR_D * factor_xyt;
+        O2(out.gliqwp, l) = prof(d, P_GLIQ, c)[k] * pd * 1000. / GGR_D * factor_xyt;
         double mcup = prof(d, P_MCUP, c)[k] * factor_xyt, mcdn = prof(d, P_MCDN, c)[k] * factor_xyt;
         double mcuup = prof(d, P_MCUUP, c)[k] * factor_xyt, mcudn = prof(d, P_MCUDN, c)[k] * factor_xyt;
         O2(out.mcup, l) = mcup;
@@ -1896,18 +1944,18 @@ __global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_sta
     for (int l = 0; l < plev; l++) {
       double pd = IN(in.pdel, l);
       double mu_a = MU(l), mu_b = MU(l + 1), md_a = MD(l), md_b = MD(l + 1);
-      double mu = 0.5 * (mu_a + mu_b) * GGR / 100.;
-      double md = 0.5 * (md_a + md_b) * GGR / 100.;
+      double mu = 0.5 * (mu_a + mu_b) * GGR_D / 100.;
+      double md = 0.5 * (md_a + md_b) * GGR_D / 100.;
       O2(out.mu_crm, l) = mu;
       O2(out.md_crm, l) = md;
       O2(out.eu_crm, l) = 0.;
       O2(out.du_crm, l) = 0.;
       O2(out.ed_crm, l) = 0.;
       if (mu_a - mu_b > 0)
-        O2(out.eu_crm, l) = (mu_a - mu_b) * GGR / pd;
+        O2(out.eu_crm, l) = (mu_a - mu_b) * GGR_D / pd;
       else
-        O2(out.du_crm, l) = -1.0 * (mu_a - mu_b) * GGR / pd;
-      if (md_b - md_a < 0) O2(out.ed_crm, l) = (md_a - md_b) * GGR / pd;
+        O2(out.du_crm, l) = -1.0 * (mu_a - mu_b) * GGR_D / pd;
+      if (md_b - md_a < 0) O2(out.ed_crm, l) = (md_a - md_b) * GGR_D / pd;
       if (fabs(mu) > R4(1.0e-15) || fabs(md) > R4(1.0e-15)) {
         jt = fmin((double)(l + 1), jt);
         mx = fmax((double)(l + 1), mx);
@@ -1915,8 +1963,8 @@ __global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_sta
     }
     out.jt_crm[c] = jt;
     out.mx_crm[c] = mx;
-    out.prectend[c] = (colprec - inst(d, I_COLPREC, c)) / GGR * factor_xy * icrm_run_time;
-    out.precstend[c] = (colprecs - inst(d, I_COLPRECS, c)) / GGR * factor_xy * icrm_run_time;
+    out.prectend[c] = (colprec - inst(d, I_COLPREC, c)) / GGR_D * factor_xy * icrm_run_time;
+    out.precstend[c] = (colprecs - inst(d, I_COLPRECS, c)) / GGR_D * factor_xy * icrm_run_time;
     out.cltot[c] = inst(d, I_CLTOT, c) * factor_xyt;
     out.clhgh[c] = inst(d, I_CLHGH, c) * factor_xyt;
     out.clmed[c] = inst(d, I_CLMED, c) * factor_xyt;
@@ -1956,4 +2004,4 @@ __global__ void __launch_bounds__(256) k_final(Dev d, crm_input_view in, crm_sta
   }
 }
 
-}  // namespace crm
+}  // namespace CRM_NS

@@ -48,8 +48,25 @@ KERNEL_WORDS = {
 STATE_WORDS_PER_LEVEL = 16
 
 
+# HBM bytes per launch measured with rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE), each
+# scaled by what the same counter reads for a device copy of known size; written next to the
+# command that produced it.  Lines: "<kernel class> <fetch bytes> <write bytes>".
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_4096x64x58.txt")
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel class from the committed counter summary, or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        parts = line.split()
+        if len(parts) == 3 and not line.startswith("#") and parts[0] == kernel:
+            return float(parts[1]) + float(parts[2])
+    return None
 
 
 def cpu_baseline(ncrms_sample):
@@ -76,43 +93,24 @@ def cpu_baseline(ncrms_sample):
                       f"({nstop} substeps), one process, {sec:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--ncrms", type=int, default=WORKLOAD["ncrms"])
-    ap.add_argument("--cpu-ncrms", type=int, default=32,
-                    help="instances in the CPU baseline sample (0 skips it)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(instances are independent: a multi-GPU run is N copies of this one)",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import torch
+def measure(prec, case, args, torch):
+    """Warmup and timed crm() calls of one precision on a fresh device copy of case."""
     from crm_hip import KERNELS, CrmHip, DeviceCase
-
     w = WORKLOAD
-    ncrms = args.ncrms
     nstop = int(w["dt_gl"] / w["dt"])
-    log(f"building inputs: {ncrms} x {w['nx']}x{w['nz']}")
-    case = make_case(ncrms, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True)
     dcase = DeviceCase(case, "cuda:0")
     torch.cuda.synchronize()
-    hip = CrmHip(0, "fp64")
-
+    hip = CrmHip(0, prec)
     # warmup: the first call also sizes the device pool; time every kernel class to find the
     # dominant one
     hip.profile_kernels(KERNELS)
     for n in range(args.warmup):
         hip.run_device(dcase, w["dt_gl"])
-        log(f"warmup step {n + 1}/{args.warmup} done")
+        log(f"[{prec}] warmup step {n + 1}/{args.warmup} done")
     warm = hip.kernel_times()
     dominant = max((k for k in warm if k in KERNEL_WORDS), key=lambda k: warm[k][0])
-    log("warmup kernel time [ms]: " + ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in warm.items()))
-
+    log(f"[{prec}] warmup kernel time [ms]: " +
+        ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in warm.items()))
     # timed region: only the dominant kernel carries events
     hip.profile_kernels([dominant])
     torch.cuda.synchronize()
@@ -121,19 +119,96 @@ def main():
     for n in range(args.steps):
         hip.run_device(dcase, w["dt_gl"])
         ncyc.append(hip.stats().ncycle_max)
-        log(f"step {n + 1}/{args.steps} done")
+        log(f"[{prec}] step {n + 1}/{args.steps} done")
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
-    ms_step = 1e3 * sec / args.steps
-    colsteps = ncrms * w["nx"] * nstop
-    value = colsteps * args.steps / sec
-
     kms, klaunch = hip.kernel_times()[dominant]
-    cells = ncrms * w["nx"] * w["nz"]
-    bytes_launch = KERNEL_WORDS[dominant] * 8 * cells
+    hip.close()
+    value = case.ncrms * w["nx"] * nstop * args.steps / sec
+    word = 8 if prec == "fp64" else 4
+    bytes_launch = KERNEL_WORDS[dominant] * word * case.ncrms * w["nx"] * w["nz"]
     avg_ms = kms / klaunch
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    return {"value": value, "ms_per_step": 1e3 * sec / args.steps, "max_ncycle": max(ncyc),
+            "dcase": dcase,
+            "kernel_ms": {k: v[0] / max(1, args.warmup) for k, v in warm.items()},
+            "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": None, "avg_launch_ms": avg_ms, "launches": klaunch,
+                         "algorithmic_bytes_per_launch": bytes_launch}}
+
+
+def state_error(d64, d32, torch):
+    """Normwise relative difference max|a-b|/max|a| of the mixed-precision state and
+    domain-mean outputs against the FP64 ones, computed on the device."""
+    out = {}
+    for group, names in (("state", ("u_wind", "w_wind", "temperature", "qt", "qp", "qn")),
+                         ("out", ("sltend", "qltend", "precc", "precl", "cltot"))):
+        for n in names:
+            a, b = d64.arrays[group][n], d32.arrays[group][n]
+            scale = a.abs().max().item()
+            diff = (a - b).abs().max().item()
+            out[f"{group}.{n}"] = diff / scale if scale > 0 else diff
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ncrms", type=int, default=WORKLOAD["ncrms"])
+    ap.add_argument("--cpu-ncrms", type=int, default=32,
+                    help="instances in the CPU baseline sample (0 skips it)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="skip the mixed-precision measurement")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (scale for the HBM "
+                         "byte counters)")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 1 << 27
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 5}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(instances are independent: a multi-GPU run is N copies of this one)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    w = WORKLOAD
+    ncrms = args.ncrms
+    nstop = int(w["dt_gl"] / w["dt"])
+    log(f"building inputs: {ncrms} x {w['nx']}x{w['nz']}")
+    case = make_case(ncrms, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True)
+
+    res = {}
+    for prec in ("fp64", "mixed"):
+        if prec == "mixed" and args.fp64_only:
+            continue
+        res[prec] = measure(prec, case, args, torch)
+    r64 = res["fp64"]
+    value = r64["value"]
     whole = STATE_WORDS_PER_LEVEL * 8 * w["nz"] * value / 1e9
+    mixed = None
+    if "mixed" in res:
+        r32 = res["mixed"]
+        mixed = {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
+                 "speedup_vs_fp64": r32["value"] / value, "roofline": r32["roofline"],
+                 "error_vs_fp64": state_error(r64["dcase"], r32["dcase"], torch),
+                 "kernel_ms_per_step_warmup": r32["kernel_ms"],
+                 "note": "FP32 bulk fields, advection, SGS and microphysics; FP64 pressure solve, "
+                         "reference profiles, statistics and outputs; error after the same "
+                         f"{args.warmup + args.steps} GCM steps from the same initial state"}
 
     cpu = None
     if args.cpu_ncrms > 0:
@@ -147,7 +222,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -156,18 +231,15 @@ def main():
         "config": {"workload": f"{ncrms} 2D CRMs of 64x58, dx 1 km, dt 5 s, one 1200 s GCM "
                                f"step ({nstop} substeps) per call, inputs resident in HBM",
                    "ncrms": ncrms, "nx": w["nx"], "nz": w["nz"], "substeps": nstop,
-                   "precision": "fp64", "max_ncycle": max(ncyc)},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_ms, "launches": klaunch,
-                     "algorithmic_bytes_per_launch": bytes_launch},
+                   "precision": "fp64", "max_ncycle": r64["max_ncycle"]},
+        "roofline": dict(r64["roofline"], traffic=measured_traffic(r64["roofline"]["kernel"])),
         "whole_step_hbm": {"achieved": whole, "frac": whole / HBM_PEAK_GBS, "unit": "GB/s",
                            "bytes_per_column_substep": STATE_WORDS_PER_LEVEL * 8 * w["nz"]},
-        "kernel_ms_per_step_warmup": {k: v[0] / max(1, args.warmup) for k, v in warm.items()},
+        "kernel_ms_per_step_warmup": r64["kernel_ms"],
+        "mixed_precision": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    hip.close()
 
 
 if __name__ == "__main__":
