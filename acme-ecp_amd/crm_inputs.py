@@ -121,6 +121,8 @@ def make_case(ncrms, nx, nz, dx, dt, ny=1, dt_gl=1200.0, sst=300.0, rh_sfc=0.8, 
         c.inp["tl"][n, :] = tm
         c.inp["ql"][n, :] = qv
         c.inp["ul"][n, :] = u
+        if ny > 1:  # 3D: a cross wind of half the along-x shear
+            c.inp["vl"][n, :] = 0.5 * u
         c.inp["ps"][n] = pi[-1]
         c.inp["ocnfrac"][n] = 1.0
         c.inp["wndls"][n] = abs(u[-1])
@@ -136,6 +138,8 @@ def make_case(ncrms, nx, nz, dx, dt, ny=1, dt_gl=1200.0, sst=300.0, rh_sfc=0.8, 
             c.state["temperature"][n, :, :, k] = tcrm[k]
             c.state["qt"][n, :, :, k] = qcrm[k]
             c.state["u_wind"][n, :, :, k] = ucrm[k]
+            if ny > 1:
+                c.state["v_wind"][n, :, :, k] = 0.5 * ucrm[k]
         for k in range(min(5, nz)):
             c.state["temperature"][n, :, :, k] += noise[k].T
     return c
@@ -168,4 +172,10 @@ def add_convection(c, seed=1):
                 c.state["qp"][n, i, 0, k] = 5.0e-4
         c.state["u_wind"][n] += rng.uniform(-1.0, 1.0, size=c.state["u_wind"][n].shape)
         c.state["w_wind"][n, :, :, 1:] += rng.uniform(-0.5, 0.5, size=(nx, c.ny, nz - 1))
+        if c.ny > 1:
+            # 3D: the plume and the ice cloud span a few rows in y, and v gets noise too
+            for j in range(1, max(2, c.ny // 3)):
+                for f in ("qn", "qt", "qp", "w_wind"):
+                    c.state[f][n, :, j, :] = c.state[f][n, :, 0, :]
+            c.state["v_wind"][n] += rng.uniform(-1.0, 1.0, size=c.state["v_wind"][n].shape)
     return c

@@ -106,8 +106,11 @@ enum Prof {
   P_GRDFX, P_TAU,                                          // reference state (per call)
   P_IRHO, P_IADZ, P_IRHOADZ, P_IRHOWADZ, P_DDADZ, P_PRATIO,  // derived, per call
   P_T0, P_Q0, P_QV0, P_TABS0, P_U0, P_QN0, P_QP0,          // domain means (diagnose)
+  P_V0,                                                    // (3D only)
   P_TTEND, P_QTEND, P_UTEND,                               // large-scale tendencies
+  P_VTEND,                                                 // (3D only)
   P_UWLE, P_UWSB, P_MKWLE1, P_MKWLE2, P_MKWSB1, P_MKWSB2,  // last-substep fluxes
+  P_VWLE, P_VWSB,                                          // (3D only)
   P_MKADV1, P_MKADV2, P_MKDIFF1, P_MKDIFF2, P_QPSRC, P_QPEVP,
   P_QPFALL, P_PRECFLUX,                                    // accumulated over the call
   P_CLD, P_CLDTOP, P_MCUP, P_MCDN, P_MCUUP, P_MCUDN, P_GLIQ, P_GICE, P_MUI, P_MDI,
@@ -136,18 +139,21 @@ struct Gam {
 template <typename R>
 struct DevT {
   int ncrms, nzm, nz, plev, ld, nx_rad;
+  int ny;                     // 1 for 2D; 8 for the 3D 8 x 8 CRM (column index 8 j + i)
   double dx, dt, dtn, dt3na, at, bt, ct, dtfactor;
   int nstep, icycle, nprec_max_slot;
   // prognostic and diagnostic fields [c][k][i]
   R *u, *w, *t, *qt, *qp, *qn, *tabs, *qv, *qcl, *qci, *qpl, *qpi;
   R *tk, *tkh, *tke;
   R *dudt[3], *dwdt[3];       // AB3 history; na/nb/nc resolved on the host
+  R *v, *dvdt[3];             // 3D only (v is identically zero in 2D and not stored)
   double *precsfc, *precssfc; // [c][i]
   double *prof;               // NPROF blocks of [ncrms][nz]
   R *profr;                   // the same profiles in R (== prof when R is double)
   double *inst;               // NINST blocks of [ncrms]
   R *coef;                    // NCOEF blocks of [ncrms][nz]
-  double *rad;                // 5 blocks of [ncrms][nz][nx_rad]: t, qv, qc, qi, cld
+  double *rad;                // 5 blocks of [ncrms][nzm][nx_rad]: t, qv, qc, qi, cld (in 3D
+                              // nx_rad counts the nx_rad*ny_rad radiation columns)
   const double *Ft, *Gt;      // transform operators [NX][NX]
   const double *eign;         // [NX] horizontal eigenvalues per slot
   unsigned long long *gmax;   // [0] kurant cfl bits, [1] precipitation cfl bits, [2] error flag
@@ -230,6 +236,7 @@ __device__ __forceinline__ double lane_x32(double v, int lane) { return xor32_d(
 // part in every call (the kernels keep control flow uniform around them).
 template <int NX>
 struct Seg {
+  static constexpr bool Y3 = false;  // 2D: one row of NX columns per instance
   int lane, i, base;
   __device__ __forceinline__ Seg() {
     lane = threadIdx.x & 63;
@@ -315,6 +322,33 @@ struct Seg {
   __device__ __forceinline__ int imin(int v) const {
     return reduce(v, NX, [](int a, int b) { return ::min(a, b); });
   }
+};
+
+// A 3D CRM of 8 x 8 columns on one wave: lane = 8 j + i.  i here is the column index within
+// the instance (0..63, the storage index), as for the 2D segments; x neighbours come from
+// rotations inside each aligned group of 8 lanes, y neighbours from a rotation by 8 lanes,
+// and reductions run over all 64 columns (nx*ny).
+struct Seg3 {
+  static constexpr bool Y3 = true;
+  Seg<64> all;
+  Seg<8> row;
+  int lane, i;
+  __device__ __forceinline__ Seg3() {
+    lane = threadIdx.x & 63;
+    i = lane;
+  }
+  // value held by the column at x offset d (+1 or -1), periodic in x
+  template <typename T>
+  __device__ __forceinline__ T rot(T v, int d) const { return row.rot(v, d); }
+  // value held by the column at y offset d (+1 or -1), periodic in y
+  template <typename T>
+  __device__ __forceinline__ T roty(T v, int d) const { return __shfl(v, (lane + 8 * d) & 63, 64); }
+  template <typename T>
+  __device__ __forceinline__ T sum(T v) const { return all.sum(v); }
+  template <typename T>
+  __device__ __forceinline__ T max(T v) const { return all.max(v); }
+  __device__ __forceinline__ int imax(int v) const { return all.imax(v); }
+  __device__ __forceinline__ int imin(int v) const { return all.imin(v); }
 };
 
 // Instance of this thread's segment; false when past ncrms (the segment then idles but still

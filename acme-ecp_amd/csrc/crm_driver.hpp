@@ -25,8 +25,8 @@
 
 namespace crm {
 // Transform operators and eigenvalues of the pressure solve, built on the host in FP64 by the
-// FP64 unit for every engine (crm_hip.hip).
-void pressure_transform_host(int nx, double dx, std::vector<double>& Ft, std::vector<double>& Gt,
+// FP64 unit for every engine (crm_hip.hip).  ny > 1: the x-y operators of a 3D CRM.
+void pressure_transform_host(int nx, int ny, double dx, std::vector<double>& Ft, std::vector<double>& Gt,
                              std::vector<double>& eign);
 }  // namespace crm
 
@@ -39,6 +39,7 @@ struct Transform {
 };
 
 enum { NSCAL = 17 };  // u t qt qp qn tabs qv qcl qci qpl qpi tk tkh tke dudt x3
+enum { NSCAL3 = 4 };  // 3D only: v dvdt x3
 
 }  // namespace
 
@@ -47,8 +48,10 @@ struct Engine : CrmEngine {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // device state sized for (cap_ncrms, cap_nx, cap_nzm, cap_nxr)
+  // device state sized for (cap_ncrms, cap_nx, cap_nzm, cap_nxr, cap_y3); cap_nx counts the
+  // columns of an instance (nx * ny) and cap_nxr its radiation columns (nx_rad * ny_rad)
   int cap_ncrms = 0, cap_nx = 0, cap_nzm = 0, cap_nxr = 0;
+  bool cap_y3 = false;
   char* pool = nullptr;
   size_t pool_bytes = 0;
   unsigned long long* gmax = nullptr;
@@ -161,8 +164,10 @@ static void host_constants(DevT<R>& d) {
 }
 
 template <typename R>
-static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr) {
-  if (ncrms <= ctx->cap_ncrms && nx == ctx->cap_nx && nzm == ctx->cap_nzm && nxr == ctx->cap_nxr) return CRM_OK;
+static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr, bool y3) {
+  if (ncrms <= ctx->cap_ncrms && nx == ctx->cap_nx && nzm == ctx->cap_nzm && nxr == ctx->cap_nxr &&
+      y3 == ctx->cap_y3)
+    return CRM_OK;
   int ncap = std::max(ncrms, nx == ctx->cap_nx && nzm == ctx->cap_nzm ? ctx->cap_ncrms : 0);
   int nz = nzm + 1;
   const size_t scal = (size_t)ncap * nzm * nx, wl = (size_t)ncap * nz * nx;
@@ -175,7 +180,8 @@ static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr) {
                           mirror ? (size_t)NPROF * ncap * nz * sizeof(R) : 0,
                           (size_t)NINST * ncap * sizeof(double),
                           (size_t)NCOEF * ncap * nz * sizeof(R),
-                          5 * (size_t)ncap * nzm * nxr * sizeof(double)};  // radiation sums
+                          5 * (size_t)ncap * nzm * nxr * sizeof(double),   // radiation sums
+                          y3 ? NSCAL3 * scal * sizeof(R) : 0};             // v, dvdt x3
   size_t bytes = 0;
   for (size_t b : sizes) bytes += al(b);
   if (ctx->pool) HIPCHECK(hipFree(ctx->pool));
@@ -190,6 +196,7 @@ static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr) {
   ctx->cap_nx = nx;
   ctx->cap_nzm = nzm;
   ctx->cap_nxr = nxr;
+  ctx->cap_y3 = y3;
   char* p = ctx->pool;
   auto take = [&](size_t b) {
     char* q = p;
@@ -217,39 +224,43 @@ static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr) {
   d.inst = (double*)take(sizes[5]);
   d.coef = (R*)take(sizes[6]);
   d.rad = (double*)take(sizes[7]);
+  R* vf = (R*)take(sizes[8]);
+  d.v = y3 ? vf : nullptr;
+  for (int q = 0; q < 3; q++) d.dvdt[q] = y3 ? vf + (size_t)(q + 1) * scal : nullptr;
   return CRM_OK;
 }
 
 template <typename R>
-static int get_transform(Engine<R>* ctx, int nx, double dx, Transform& out) {
-  auto key = std::make_pair(nx, dx);
+static int get_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform& out) {
+  auto key = std::make_pair(nx * 1000 + ny, dx);
   auto it = ctx->transforms.find(key);
   if (it != ctx->transforms.end()) {
     out = it->second;
     return CRM_OK;
   }
   std::vector<double> Ft, Gt, eign;
-  crm::pressure_transform_host(nx, dx, Ft, Gt, eign);
+  crm::pressure_transform_host(nx, ny, dx, Ft, Gt, eign);
+  const size_t n = (size_t)nx * ny;  // modes = columns
   Transform t;
-  HIPCHECK(hipMalloc(&t.Ft, sizeof(double) * nx * nx));
-  HIPCHECK(hipMalloc(&t.Gt, sizeof(double) * nx * nx));
-  HIPCHECK(hipMalloc(&t.eign, sizeof(double) * nx));
-  HIPCHECK(hipMemcpy(t.Ft, Ft.data(), sizeof(double) * nx * nx, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(t.Gt, Gt.data(), sizeof(double) * nx * nx, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(t.eign, eign.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
+  HIPCHECK(hipMalloc(&t.Ft, sizeof(double) * n * n));
+  HIPCHECK(hipMalloc(&t.Gt, sizeof(double) * n * n));
+  HIPCHECK(hipMalloc(&t.eign, sizeof(double) * n));
+  HIPCHECK(hipMemcpy(t.Ft, Ft.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(t.Gt, Gt.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(t.eign, eign.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   ctx->transforms[key] = t;
   out = t;
   return CRM_OK;
 }
 
 // ---- kernel dispatch over the compile-time column count ------------------------------------
-template <int NX, typename R>
+template <int NX, typename R, typename G>
 static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, const DevT<R>& d, int na, int nb, int nc) {
   if (NX == 64) {  // matrix-core transforms; one instance per block
     if (nzm == 28)
-      k_press64<28, R><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+      k_press64<28, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
     else
-      k_press64<58, R><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+      k_press64<58, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
     return;
   }
   if (nzm == 28)
@@ -265,25 +276,36 @@ struct Launcher {
   dim3 grid_inst(int b) const { return dim3((unsigned)((ncrms + b - 1) / b)); }
 };
 
+// NXV: columns per instance; GEO: the lane geometry (a 2D row of NXV columns, or the 8 x 8
+// columns of a 3D CRM on one wave).
 #define NX_DISPATCH(nx, CALL)       \
-  switch (nx) {                     \
-    case 8: {                       \
-      constexpr int NXV = 8;        \
-      CALL;                         \
-    } break;                        \
-    case 16: {                      \
-      constexpr int NXV = 16;       \
-      CALL;                         \
-    } break;                        \
-    case 32: {                      \
-      constexpr int NXV = 32;       \
-      CALL;                         \
-    } break;                        \
-    default: {                      \
-      constexpr int NXV = 64;       \
-      CALL;                         \
-    } break;                        \
-  }
+  if (y3) {                         \
+    constexpr int NXV = 64;         \
+    using GEO = Seg3;               \
+    CALL;                           \
+  } else                            \
+    switch (nx) {                   \
+      case 8: {                     \
+        constexpr int NXV = 8;      \
+        using GEO = Seg<8>;         \
+        CALL;                       \
+      } break;                      \
+      case 16: {                    \
+        constexpr int NXV = 16;     \
+        using GEO = Seg<16>;        \
+        CALL;                       \
+      } break;                      \
+      case 32: {                    \
+        constexpr int NXV = 32;     \
+        using GEO = Seg<32>;        \
+        CALL;                       \
+      } break;                      \
+      default: {                    \
+        constexpr int NXV = 64;     \
+        using GEO = Seg<64>;        \
+        CALL;                       \
+      } break;                      \
+    }
 
 static void abcoefs(int nstep, const double* dt3, int na, int nb, int nc, double& at, double& bt, double& ct) {
   if (nstep >= 3) {
@@ -304,8 +326,18 @@ static void abcoefs(int nstep, const double* dt3, int na, int nb, int nc, double
 
 static int validate(const crm_dims* dm) {
   if (!dm || dm->ncrms < 1 || dm->ld < dm->ncrms || dm->plev < dm->nz + 1) return CRM_ERR_ARG;
-  if (dm->ny != 1 || dm->ny_rad != 1) return CRM_ERR_UNSUPPORTED;
-  if (!supported_nx(dm->nx) || !supported_nz(dm->nz)) return CRM_ERR_UNSUPPORTED;
+  if (!supported_nz(dm->nz)) return CRM_ERR_UNSUPPORTED;
+  if (dm->ny != 1) {
+    // 3D: 8 x 8 columns (dy == dx); radiation on every column or on the domain mean
+    if (dm->nx != 8 || dm->ny != 8) return CRM_ERR_UNSUPPORTED;
+    if (dm->nx_rad < 1 || dm->ny_rad < 1 || dm->nx % dm->nx_rad || dm->ny % dm->ny_rad) return CRM_ERR_NXRAD;
+    if (!((dm->nx_rad == dm->nx && dm->ny_rad == dm->ny) || (dm->nx_rad == 1 && dm->ny_rad == 1)))
+      return CRM_ERR_UNSUPPORTED;
+    if (60. < 2 * dm->dt) return CRM_ERR_DAMPING;
+    return CRM_OK;
+  }
+  if (dm->ny_rad != 1) return CRM_ERR_NXRAD;
+  if (!supported_nx(dm->nx)) return CRM_ERR_UNSUPPORTED;
   if (dm->nx_rad < 1 || dm->nx % dm->nx_rad != 0) return CRM_ERR_NXRAD;
   if ((dm->nx / dm->nx_rad) & ((dm->nx / dm->nx_rad) - 1)) return CRM_ERR_UNSUPPORTED;
   if (60. < 2 * dm->dt) return CRM_ERR_DAMPING;
@@ -320,11 +352,13 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
                     int phase_stop) {
   int rc = validate(dm);
   if (rc) return rc;
-  const int nx = dm->nx, nzm = dm->nz, ncrms = dm->ncrms;
-  rc = ensure_buffers(ctx, ncrms, nx, nzm, dm->nx_rad);
+  const bool y3 = dm->ny > 1;
+  const int nx = dm->nx * dm->ny, nzm = dm->nz, ncrms = dm->ncrms;  // nx: columns per instance
+  const int nxr = dm->nx_rad * dm->ny_rad;
+  rc = ensure_buffers(ctx, ncrms, nx, nzm, nxr, y3);
   if (rc) return rc;
   Transform tf;
-  rc = get_transform(ctx, nx, dm->dx, tf);
+  rc = get_transform(ctx, dm->nx, dm->ny, dm->dx, tf);
   if (rc) return rc;
   DevT<R>& d = ctx->dev;
   d.ncrms = ncrms;
@@ -332,7 +366,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   d.nz = nzm + 1;
   d.plev = dm->plev;
   d.ld = dm->ld;
-  d.nx_rad = dm->nx_rad;
+  d.nx_rad = nxr;
+  d.ny = dm->ny;
   d.dx = dm->dx;
   d.dt = dm->dt;
   d.Ft = tf.Ft;
@@ -406,20 +441,21 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       const double* qrad = rad->qrad;
       TIMED(CRM_K_PRE, NX_DISPATCH(nx, (k_pre<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, na, qrad))));
       if (last && phase_stop == 7) goto stopped;
-      TIMED(CRM_K_SGS, NX_DISPATCH(nx, (k_sgs<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      TIMED(CRM_K_SGS, NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 9) goto stopped;
-      TIMED(CRM_K_MOM, NX_DISPATCH(nx, (k_mom<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, na))));
+      TIMED(CRM_K_MOM, NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d, na))));
       if (last && phase_stop == 11) goto stopped;
-      TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
+      TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
       if (last && phase_stop == 13) goto stopped;
       HIPCHECK(hipMemsetAsync(ctx->gmax + 1, 0, sizeof(unsigned long long), s));
-      TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d))));
       TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 15) goto stopped;
       std::swap(d.u, d.dudt[nc]);
       std::swap(d.w, d.dwdt[nc]);
+      if (y3) std::swap(d.v, d.dvdt[nc]);
       if (last && phase_stop == 17) goto stopped;
-      TIMED(CRM_K_DIFFUSE, NX_DISPATCH(nx, (k_diffuse<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      TIMED(CRM_K_DIFFUSE, NX_DISPATCH(nx, (k_diffuse<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 18) goto stopped;
       TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_micro<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 20) goto stopped;
@@ -660,6 +696,10 @@ int Engine<R>::debug_get(const char* name, double* buf) {
   else if (n == "dudt_na") grid = d.dudt[ctx->na];
   else if (n == "dudt_nb") grid = d.dudt[ctx->nb];
   else if (n == "dudt_nc") grid = d.dudt[ctx->nc];
+  else if (n == "v" && d.v) grid = d.v;
+  else if (n == "dvdt_na" && d.v) grid = d.dvdt[ctx->na];
+  else if (n == "dvdt_nb" && d.v) grid = d.dvdt[ctx->nb];
+  else if (n == "dvdt_nc" && d.v) grid = d.dvdt[ctx->nc];
   else if (n == "dwdt_na") { grid = d.dwdt[ctx->na]; nk = nz; }
   else if (n == "dwdt_nb") { grid = d.dwdt[ctx->nb]; nk = nz; }
   else if (n == "dwdt_nc") { grid = d.dwdt[ctx->nc]; nk = nz; }
@@ -683,6 +723,7 @@ int Engine<R>::debug_get(const char* name, double* buf) {
       {"adz", {P_ADZ, 0}}, {"adzw", {P_ADZW, 1}}, {"pres", {P_PRES, 0}}, {"presi", {P_PRESI, 1}},
       {"z", {P_Z, 1}}, {"zi", {P_ZI, 1}}, {"bet", {P_BET, 0}}, {"gamaz", {P_GAMAZ, 0}},
       {"ttend", {P_TTEND, 0}}, {"qtend", {P_QTEND, 0}}, {"utend", {P_UTEND, 0}}, {"uwle", {P_UWLE, 1}},
+      {"v0", {P_V0, 0}}, {"vtend", {P_VTEND, 0}}, {"vwle", {P_VWLE, 1}}, {"vwsb", {P_VWSB, 1}},
       {"uwsb", {P_UWSB, 1}}, {"mkwle1", {P_MKWLE1, 1}}, {"mkwsb1", {P_MKWSB1, 1}}, {"mkadv1", {P_MKADV1, 1}},
       {"mkdiff1", {P_MKDIFF1, 1}}, {"qpsrc", {P_QPSRC, 1}}, {"qpevp", {P_QPEVP, 1}},
       {"qpfall", {P_QPFALL, 1}}, {"precflux", {P_PRECFLUX, 1}}};

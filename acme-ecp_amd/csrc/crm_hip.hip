@@ -7,15 +7,41 @@
 namespace crm {
 CrmEngine* make_engine_fp64(int device, int* rc) { return make_engine<double>(device, rc); }
 
-void pressure_transform_host(int nx, double dx, std::vector<double>& Ft, std::vector<double>& Gt,
+void pressure_transform_host(int nx, int ny, double dx, std::vector<double>& Ft, std::vector<double>& Gt,
                              std::vector<double>& eign) {
-  fft991_operators(nx, Ft, Gt);
-  eign.assign(nx, 0.0);
   // pressure.F90:172-198
   const double pii = 3.14159265358979323846, xnx = pii / nx, ddx2 = 1.0 / (dx * dx);
-  for (int s = 0; s < nx; s++) {
-    double xi = fft991_slot_mode(nx, s);
-    eign[s] = (2.0 * std::cos(2.0 * xnx * xi) - 2.0) * ddx2 + 0.0;
+  if (ny <= 1) {
+    fft991_operators(nx, Ft, Gt);
+    eign.assign(nx, 0.0);
+    for (int s = 0; s < nx; s++) {
+      double xi = fft991_slot_mode(nx, s);
+      eign[s] = (2.0 * std::cos(2.0 * xnx * xi) - 2.0) * ddx2 + 0.0;
+    }
+    return;
+  }
+  // 3D (dy == dx): the x transform of every row, then the y transform of every x
+  // coefficient (pressure.F90:122-147), is one linear map of the nx*ny slab: the Kronecker
+  // product of the two one-dimensional operators.  Column q = nx*jq + iq, mode s = nx*sy + sx.
+  std::vector<double> fx, gx, fy, gy;
+  fft991_operators(nx, fx, gx);
+  fft991_operators(ny, fy, gy);
+  const int n = nx * ny;
+  const double xny = pii / ny, ddy2 = 1.0 / (dx * dx);
+  Ft.assign((size_t)n * n, 0.0);
+  Gt.assign((size_t)n * n, 0.0);
+  eign.assign(n, 0.0);
+  for (int q = 0; q < n; q++) {
+    const int iq = q % nx, jq = q / nx;
+    for (int s = 0; s < n; s++) {
+      const int sx = s % nx, sy = s / nx;
+      Ft[(size_t)q * n + s] = fx[(size_t)iq * nx + sx] * fy[(size_t)jq * ny + sy];
+      Gt[(size_t)s * n + q] = gx[(size_t)sx * nx + iq] * gy[(size_t)sy * ny + jq];
+    }
+  }
+  for (int s = 0; s < n; s++) {
+    const double xi = fft991_slot_mode(nx, s % nx), xj = fft991_slot_mode(ny, s / nx);
+    eign[s] = (2.0 * std::cos(2.0 * xnx * xi) - 2.0) * ddx2 + (2.0 * std::cos(2.0 * xny * xj) - 2.0) * ddy2;
   }
 }
 }  // namespace crm

@@ -144,6 +144,7 @@ __global__ void __launch_bounds__(256) k_init(DevT<R> d, crm_input_view in, crm_
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, ld = d.ld, plev = d.plev, i = s.i;
   const double umax = 0.5 * d.dx / d.dt, factor_xy = 1. / (double)NX, idt_gl = 1. / dt_gl;
+  const bool y3 = d.ny > 1;  // 3D: NX == 64 columns = 8 x 8, lane 8 j + i is the state's (i, j)
   auto G4 = [&](const double* p, int cc, int ii, int k) {
     return p[(size_t)cc + (size_t)ld * ((size_t)ii + (size_t)NX * k)];
   };
@@ -158,6 +159,11 @@ __global__ void __launch_bounds__(256) k_init(DevT<R> d, crm_input_view in, crm_
     size_t ow = ((size_t)c * nz + k) * NX + i;
     double u = G4(st.u_wind, c, i, k);
     if (clamp) u = fmin(umax, fmax(-umax, u));
+    double v = 0.0;
+    if (y3) {
+      v = G4(st.v_wind, c, i, k);
+      if (clamp) v = fmin(umax, fmax(-umax, v));
+    }
     double w = G4(st.w_wind, c, i, k);
     double tabs = G4(st.temperature, c, i, k);
     double qt = G4(st.qt, c, i, k), qp = G4(st.qp, c, i, k), qn = G4(st.qn, c, i, k);
@@ -169,6 +175,7 @@ __global__ void __launch_bounds__(256) k_init(DevT<R> d, crm_input_view in, crm_
     colprec += s.sum((qpl + qpi) * pd);
     colprecs += s.sum(qpi * pd);
     double u0 = s.sum(u) * factor_xy;
+    double v0 = y3 ? s.sum(v) * factor_xy : 0.0;
     double t0 = s.sum(t) * factor_xy;
     double t00 = s.sum(t + FAC_COND_D * qpl + FAC_SUB_D * qpi) * factor_xy;
     double tabs0 = s.sum(tabs) * factor_xy;
@@ -200,9 +207,18 @@ __global__ void __launch_bounds__(256) k_init(DevT<R> d, crm_input_view in, crm_
         d.dudt[sl][o] = 0.0;
         d.dwdt[sl][ow] = 0.0;
       }
+      if (y3) {
+        d.v[o] = v;
+        for (int sl = 0; sl < 3; sl++) d.dvdt[sl][o] = 0.0;
+      }
       if (i == 0) {
         auto IN = [&](const double* p) { return p[(size_t)c + (size_t)ld * l]; };
         double uln = fmin(umax, fmax(-umax, IN(in.ul)));
+        if (y3) {
+          double vln = fmin(umax, fmax(-umax, IN(in.vl)));
+          prof(d, P_V0, c)[k] = v0;
+          prof(d, P_VTEND, c)[k] = (vln - v0) * idt_gl;
+        }
         prof(d, P_U0, c)[k] = u0;
         prof(d, P_T0, c)[k] = t0;
         prof(d, P_TABS0, c)[k] = tabs0;
@@ -308,15 +324,16 @@ __global__ void __launch_bounds__(256) k_kurant(DevT<R> d) {
   const double dz = inst(d, I_DZ, c);
   const double* adzw = prof(d, P_ADZW, c);
   const double* grdf = prof(d, P_GRDFX, c);
-  const double dxf = sqrt((1. / d.dx) * (1. / d.dx) + 0.0);
+  const bool y3 = d.ny > 1;  // 3D: dy == dx
+  const double dxf = sqrt((1. / d.dx) * (1. / d.dx) + (y3 ? (1. / d.dx) * (1. / d.dx) : 0.0));
   double cfl = 0.0, wm_prev = 0.0, uhm_prev = 0.0;
   for (int k = 0; k <= nzm; k++) {
     double wm = 0.0, uhm = 0.0;
     if (k < nzm) {
       size_t o = ((size_t)c * nzm + k) * NX + i;
       wm = s.max(fabs(d.w[((size_t)c * nz + k) * NX + i]));
-      double uu = d.u[o];
-      uhm = s.max(sqrt(uu * uu + 0.0));
+      double uu = d.u[o], vv = y3 ? (double)d.v[o] : 0.0;
+      uhm = s.max(sqrt(uu * uu + vv * vv));
       double tkhmax = s.max(d.tkh[o]);
       double dzz = dz * adzw[k];
       double sgs = fmax(fmax(0.5 * tkhmax * 1.0 * d.dt / (dzz * dzz), 0.5 * tkhmax * grdf[k] * d.dt / (d.dx * d.dx)), 0.0);
@@ -354,7 +371,9 @@ __global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qr
   const R* ttend = profr(d, P_TTEND, c);
   const R* qtend = profr(d, P_QTEND, c);
   const R* utend = profr(d, P_UTEND, c);
+  const R* vtend = profr(d, P_VTEND, c);
   const R* tau = profr(d, P_TAU, c);
+  const bool y3 = d.ny > 1;
   const int n_damp = (int)inst(d, I_NDAMP, c);
   const int kmin = (int)inst(d, I_KMIN, c), kmax = (int)inst(d, I_KMAX, c);
   const int nxr = d.nx_rad, ir = i / (NX / nxr);
@@ -412,6 +431,11 @@ __global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qr
     R qneg = s.sum(neg ? qt : R(0.0));
     R qpoz = s.sum(neg ? R(0.0) : qt);
     R du = R(0.0) + utend[k];
+    R v = R(0.0), dv = R(0.0);
+    if (y3) {
+      v = d.v[o];
+      dv = dv + vtend[k];
+    }
     if (nneg > R(0.5) && qpoz + qneg > R(0.)) qt = fmax(R(0.0), qt * (R(1.) + qneg / qpoz));
     // radiative heating on column groups
     t = t + qrad[(size_t)c + (size_t)d.ld * ((size_t)ir + (size_t)nxr * k)] * dtn;
@@ -420,6 +444,10 @@ __global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qr
       R tk = tau[k];
       R u0l = s.sum(u / (R)NX), t0l = s.sum(t / (R)NX);
       du = du - (u - u0l) * tk;
+      if (y3) {
+        R v0l = s.sum(v / (R)NX);
+        dv = dv - (v - v0l) * tk;
+      }
       dw = dw - w * tk;
       t = t - dtn * (t - t0l) * tk;
       qt = qt - dtn * (qv - qv0[k]) * tk;
@@ -442,6 +470,7 @@ __global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qr
       d.t[o] = t;
       d.qt[o] = qt;
       dudt[o] = du;
+      if (y3) d.dvdt[na][o] = dv;
       dwdt[((size_t)c * nz + k) * NX + i] = dw;
     }
   }
@@ -449,11 +478,59 @@ __global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qr
 }
 
 // ------------------------------------------------------------------------------------------
-// Smagorinsky eddy diffusivity: shear_prod2D.F90:7-96 and tke_full.F90:11-317 (dosmagor).
+// Deformation squared of a 3D CRM at level k < nzm-1 (shear_prod3D.F90:26-80 for k >= 1,
+// :81-106 at the surface level), dy == dx.  Window: u, v at k-1, k, k+1 and w at k, k+1.
+// The four horizontal shear terms are one staggered quantity H at the (i+1/2, j+1/2) corners
+// read at the four corners of the cell.
 // ------------------------------------------------------------------------------------------
-template <int NX, typename R>
+template <typename R, typename G>
+__device__ __forceinline__ R shear3(const G& s, int k, R dz, R dx, const R* adz, const R* adzw, const R* u0,
+                                    const R* v0, R um, R u, R up, R vm, R v, R vp, R w0, R w1) {
+  const int kc = k + 1, kb = k - 1;
+  const R rdx0 = R(1.) / dx;
+  const R rdz = R(1.) / (dz * adz[k]);
+  const R rdzw_up = R(1.) / (dz * adzw[kc]);
+  const R rdx = rdx0 * sqrt(dx * rdz);
+  const R rdx_up = rdx0 * sqrt(dx * rdzw_up);
+  // rdy* == rdx* for dy == dx
+  R u_ip = s.rot(u, 1), v_jp = s.roty(v, 1);
+  R a = (u_ip - u) * rdx, b = (v_jp - v) * rdx, cz = (w1 - w0) * rdz;
+  R def2 = R(2.) * (a * a + b * b + cz * cz);
+  R H = (s.roty(u_ip, 1) - u_ip) * rdx + (s.rot(v_jp, 1) - v_jp) * rdx;  // corner (i+1/2, j+1/2)
+  R H_im = s.rot(H, -1), H_jm = s.roty(H, -1), H_ijm = s.roty(H_im, -1);
+  def2 = def2 + R(0.25) * (H * H + H_im * H_im + H_jm * H_jm + H_ijm * H_ijm);
+  // u-w and v-w shear at the upper interface
+  R up_ip = s.rot(up, 1), w1p = s.rot(w1, 1), w1m = s.rot(w1, -1);
+  R a1 = (up_ip - u0[kc] - u_ip + u0[k]) * rdzw_up + (w1p - w1) * rdx_up;
+  R a2 = (up - u0[kc] - u + u0[k]) * rdzw_up + (w1 - w1m) * rdx_up;
+  R vp_jp = s.roty(vp, 1), w1jp = s.roty(w1, 1), w1jm = s.roty(w1, -1);
+  R b1 = (vp_jp - v0[kc] - v_jp + v0[k]) * rdzw_up + (w1jp - w1) * rdx_up;
+  R b2 = (vp - v0[kc] - v + v0[k]) * rdzw_up + (w1 - w1jm) * rdx_up;
+  if (k >= 1) {
+    const R rdzw_dn = R(1.) / (dz * adzw[k]);
+    const R rdx_dn = rdx0 * sqrt(dx * rdzw_dn);
+    R um_ip = s.rot(um, 1), w0p = s.rot(w0, 1), w0m = s.rot(w0, -1);
+    R a3 = (u_ip - u0[k] - um_ip + u0[kb]) * rdzw_dn + (w0p - w0) * rdx_dn;
+    R a4 = (u - u0[k] - um + u0[kb]) * rdzw_dn + (w0 - w0m) * rdx_dn;
+    R vm_jp = s.roty(vm, 1), w0jp = s.roty(w0, 1), w0jm = s.roty(w0, -1);
+    R b3 = (v_jp - v0[k] - vm_jp + v0[kb]) * rdzw_dn + (w0jp - w0) * rdx_dn;
+    R b4 = (v - v0[k] - vm + v0[kb]) * rdzw_dn + (w0 - w0jm) * rdx_dn;
+    def2 = def2 + R(0.25) * (a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4);
+    def2 = def2 + R(0.25) * (b1 * b1 + b2 * b2 + b3 * b3 + b4 * b4);
+  } else {
+    def2 = def2 + R(0.5) * (b1 * b1 + b2 * b2);
+    def2 = def2 + R(0.5) * (a1 * a1 + a2 * a2);
+  }
+  return def2;
+}
+
+// ------------------------------------------------------------------------------------------
+// Smagorinsky eddy diffusivity: shear_prod2D.F90:7-96 (3D: shear_prod3D.F90:7-140) and
+// tke_full.F90:11-317 (dosmagor).
+// ------------------------------------------------------------------------------------------
+template <int NX, typename R, typename G = Seg<NX>>
 __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
-  Seg<NX> s;
+  G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
@@ -469,8 +546,14 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
   const R rdx0 = R(1.) / dx;
   auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
   auto W = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
-  // window: u at k-1,k,k+1 and w at k,k+1
+  // window: u (and v in 3D) at k-1,k,k+1 and w at k,k+1
   R um = 0, u = F(d.u, 0), up = F(d.u, 1);
+  R vm = 0, v = 0, vp = 0;
+  const R* v0 = profr(d, P_V0, c);
+  if constexpr (G::Y3) {
+    v = F(d.v, 0);
+    vp = F(d.v, 1);
+  }
   R w0 = W(0), w1 = W(1);
   R bsv_prev = R(0.0);
   // thermodynamics at k (carried) and k+1
@@ -479,30 +562,37 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
   for (int k = 0; k < nzm - 1; k++) {
     int kc = k + 1, kb = k - 1;
     R up_next = (kc + 1 < nzm) ? F(d.u, kc + 1) : R(0.0);
+    R vp_next = R(0.0);
+    if constexpr (G::Y3) vp_next = (kc + 1 < nzm) ? F(d.v, kc + 1) : R(0.0);
     R w2 = W(kc + 1);
     // shear production def2 at level k
-    R rdz = R(1.) / (dz * adz[k]);
-    R rdx = rdx0 * sqrt(dx * rdz);
-    R u_ip = s.rot(u, 1);
-    R du = (u_ip - u) * rdx;
-    R dwz = (w1 - w0) * rdz;
-    R def2 = R(2.) * (du * du + dwz * dwz);
-    R w0m = s.rot(w0, -1), w0p = s.rot(w0, 1), w1m = s.rot(w1, -1), w1p = s.rot(w1, 1);
-    R up_ip = s.rot(up, 1);
-    {
-      R rdzw_up = R(1.) / (dz * adzw[kc]);
-      R rdx_up = rdx0 * sqrt(dx * rdzw_up);
-      R a1 = (up_ip - u0[kc] - u_ip + u0[k]) * rdzw_up + (w1p - w1) * rdx_up;
-      R a2 = (up - u0[kc] - u + u0[k]) * rdzw_up + (w1 - w1m) * rdx_up;
-      if (k >= 1) {
-        R rdzw_dn = R(1.) / (dz * adzw[k]);
-        R rdx_dn = rdx0 * sqrt(dx * rdzw_dn);
-        R um_ip = s.rot(um, 1);
-        R a3 = (u_ip - u0[k] - um_ip + u0[kb]) * rdzw_dn + (w0p - w0) * rdx_dn;
-        R a4 = (u - u0[k] - um + u0[kb]) * rdzw_dn + (w0 - w0m) * rdx_dn;
-        def2 = def2 + R(0.5) * (a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4);
-      } else {
-        def2 = def2 + R(0.5) * (a1 * a1 + a2 * a2);
+    R def2;
+    if constexpr (G::Y3) {
+      def2 = shear3(s, k, dz, dx, adz, adzw, u0, v0, um, u, up, vm, v, vp, w0, w1);
+    } else {
+      R rdz = R(1.) / (dz * adz[k]);
+      R rdx = rdx0 * sqrt(dx * rdz);
+      R u_ip = s.rot(u, 1);
+      R du = (u_ip - u) * rdx;
+      R dwz = (w1 - w0) * rdz;
+      def2 = R(2.) * (du * du + dwz * dwz);
+      R w0m = s.rot(w0, -1), w0p = s.rot(w0, 1), w1m = s.rot(w1, -1), w1p = s.rot(w1, 1);
+      R up_ip = s.rot(up, 1);
+      {
+        R rdzw_up = R(1.) / (dz * adzw[kc]);
+        R rdx_up = rdx0 * sqrt(dx * rdzw_up);
+        R a1 = (up_ip - u0[kc] - u_ip + u0[k]) * rdzw_up + (w1p - w1) * rdx_up;
+        R a2 = (up - u0[kc] - u + u0[k]) * rdzw_up + (w1 - w1m) * rdx_up;
+        if (k >= 1) {
+          R rdzw_dn = R(1.) / (dz * adzw[k]);
+          R rdx_dn = rdx0 * sqrt(dx * rdzw_dn);
+          R um_ip = s.rot(um, 1);
+          R a3 = (u_ip - u0[k] - um_ip + u0[kb]) * rdzw_dn + (w0p - w0) * rdx_dn;
+          R a4 = (u - u0[k] - um + u0[kb]) * rdzw_dn + (w0 - w0m) * rdx_dn;
+          def2 = def2 + R(0.5) * (a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4);
+        } else {
+          def2 = def2 + R(0.5) * (a1 * a1 + a2 * a2);
+        }
       }
     }
     // SGS buoyancy flux at the interface between k and k+1 (tke_full.F90:120-237)
@@ -579,6 +669,9 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
     um = u;
     u = up;
     up = up_next;
+    vm = v;
+    v = vp;
+    vp = vp_next;
     w0 = w1;
     w1 = w2;
   }
@@ -590,7 +683,7 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
 // added in the reference's order.
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_mom(DevT<R> d, int na) {
+__device__ __forceinline__ void mom2(const DevT<R>& d, int na) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -715,6 +808,199 @@ __global__ void __launch_bounds__(256) k_mom(DevT<R> d, int na) {
       prof(d, P_UWSB, c)[0] = R(0.0);
     }
   }
+}
+
+// 3D momentum (8 x 8 columns, one instance per wave): advect2_mom_xy.F90:22-58,
+// advect2_mom_z.F90:7-118 (3D branch :47-63), diffuse_mom3D.F90:7-207, dy == dx.  Each
+// tendency receives its contributions in the reference's order: x then y advection, vertical
+// advection, x then y diffusion, vertical diffusion.
+template <typename R>
+__device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
+  constexpr int NX = 64;
+  Seg3 s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, nz = d.nz, i = s.i;
+  const R dz = inst(d, I_DZ, c), dx = d.dx;
+  const R* adz = profr(d, P_ADZ, c);
+  const R* adzw = profr(d, P_ADZW, c);
+  const R* rho = profr(d, P_RHO, c);
+  const R* rhow = profr(d, P_RHOW, c);
+  const R* grdf = profr(d, P_GRDFX, c);  // grdf_y == grdf_x
+  R* dudt = d.dudt[na];
+  R* dvdt = d.dvdt[na];
+  R* dwdt = d.dwdt[na];
+  const R dx25 = R(0.25) / dx, dz25 = R(1.) / (R(4.) * dz);
+  const R rdx2 = R(1.) / (dx * dx), rdx25 = R(0.25) * rdx2;
+  const R rdz = R(1.) / dz, dzx = dz / dx, dxy = dx / dx;
+  auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
+  auto W = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
+  R u_k = F(d.u, 0), u_n = F(d.u, 1);
+  R v_k = F(d.v, 0), v_n = F(d.v, 1);
+  R w_k = W(0), w_n = W(1);
+  R tk_k = F(d.tk, 0), tk_n = F(d.tk, 1);
+  R fuz_k = R(0.0), fvz_k = R(0.0);  // vertical advective fluxes at the bottom of level k
+  R fwz_prev = R(0.0);
+  R fuv_k = R(0.0), fvv_k = R(0.0);  // vertical SGS fluxes at the bottom of level k
+  R fwv_k = R(0.0);
+  R pax = R(0.0), pay = R(0.0), pdx = R(0.0), pdy = R(0.0);  // horizontal terms for w level k
+  for (int k = 0; k < nzm; k++) {
+    const int kn = k + 1;
+    const bool has_n = kn < nzm;
+    const int kcu = has_n ? kn : k;
+    R u_nn = (kn + 1 < nzm) ? F(d.u, kn + 1) : R(0.0);
+    R v_nn = (kn + 1 < nzm) ? F(d.v, kn + 1) : R(0.0);
+    R w_nn = (kn + 1 < nz) ? W(kn + 1) : R(0.0);
+    R tk_nn = (kn + 1 < nzm) ? F(d.tk, kn + 1) : R(0.0);
+    // neighbours at level k
+    R u_m = s.rot(u_k, -1), u_p = s.rot(u_k, 1), u_jm = s.roty(u_k, -1), u_jp = s.roty(u_k, 1);
+    R u_p_jm = s.roty(u_p, -1);
+    R v_m = s.rot(v_k, -1), v_p = s.rot(v_k, 1), v_jm = s.roty(v_k, -1), v_jp = s.roty(v_k, 1);
+    R v_m_jp = s.roty(v_m, 1);
+    R tk_m = s.rot(tk_k, -1), tk_p = s.rot(tk_k, 1), tk_jm = s.roty(tk_k, -1), tk_jp = s.roty(tk_k, 1);
+    R tk_p_jm = s.roty(tk_p, -1), tk_m_jp = s.roty(tk_m, 1);
+    R ucu = has_n ? u_n : u_k, vcu = has_n ? v_n : v_k, tkcu = has_n ? tk_n : tk_k;
+    R ucu_p = s.rot(ucu, 1), vcu_jp = s.roty(vcu, 1);
+    R tkcu_p = s.rot(tkcu, 1), tkcu_jp = s.roty(tkcu, 1);
+    R tkn_m = s.rot(tk_n, -1), tkn_jm = s.roty(tk_n, -1);
+    R wn_m = s.rot(w_n, -1), wn_p = s.rot(w_n, 1), wn_jm = s.roty(w_n, -1), wn_jp = s.roty(w_n, 1);
+    const R rhoi_k = R(1.) / (rho[k] * adz[k]);
+    // ---- vertical fluxes at w level k+1
+    R fuz_n = R(0.0), fvz_n = R(0.0), fuv_n = R(0.0), fvv_n = R(0.0), fwv_n;
+    if (has_n) {
+      R rhoi = dz25 * rhow[kn];
+      fuz_n = rhoi * (w_n + wn_m) * (u_n + u_k);
+      fvz_n = rhoi * (w_n + wn_jm) * (v_n + v_k);
+      R rdz2 = rdz * rdz * R(1.0);
+      R rdz25 = R(0.25) * rdz2;
+      R iadz = R(1.) / adz[k], iadzw = R(1.) / adzw[kn];
+      R tkz = rdz2 * tk_k;
+      fwv_n = -R(2.) * tkz * (w_n - w_k) * rho[k] * iadz;
+      tkz = rdz25 * (tk_k + tk_m + tk_n + tkn_m);
+      fuv_n = -tkz * ((u_n - u_k) * iadzw + (w_n - wn_m) * dzx) * rhow[kn];
+      tkz = rdz25 * (tk_k + tk_jm + tk_n + tkn_jm);
+      fvv_n = -tkz * ((v_n - v_k) * iadzw + (w_n - wn_jm) * dzx) * rhow[kn];
+      R s_uwle = s.sum(fuz_n), s_vwle = s.sum(fvz_n), s_uwsb = s.sum(fuv_n), s_vwsb = s.sum(fvv_n);
+      if (live && i == 0) {
+        prof(d, P_UWLE, c)[kn] = s_uwle;
+        prof(d, P_VWLE, c)[kn] = s_vwle;
+        prof(d, P_UWSB, c)[kn] = s_uwsb;
+        prof(d, P_VWSB, c)[kn] = s_vwsb;
+      }
+    } else {
+      R rdz2 = rdz * rdz * R(1.0);
+      R tkz = rdz2 * R(1.0) * tk_k;
+      fwv_n = -R(2.) * tkz * (w_n - w_k) / adz[k] * rho[k];
+    }
+    // ---- dudt(k)
+    size_t o = ((size_t)c * nzm + k) * NX + i;
+    R du = dudt[o];
+    {
+      R fu1 = dx25 * (u_k + u_m) * (u_m + u_k);
+      R fu2 = dx25 * (u_p + u_k) * (u_k + u_p);
+      du = du - (fu2 - fu1);
+      fu1 = dx25 * (v_k + v_m) * (u_jm + u_k);
+      fu2 = dx25 * (v_jp + v_m_jp) * (u_k + u_jp);
+      du = du - (fu2 - fu1);
+      du = du - (fuz_n - fuz_k) * rhoi_k;
+      R fx = -R(2.) * (rdx2 * grdf[k] * tk_k) * (u_p - u_k);
+      du = du - (fx - s.rot(fx, -1));
+      R tky = rdx25 * grdf[k] * (tk_k + tk_m + tk_jp + tk_m_jp);
+      R fy = -tky * (u_jp - u_k + (v_jp - v_m_jp) * dxy);
+      du = du - (fy - s.roty(fy, -1));
+      du = du - (fuv_n - fuv_k) * rhoi_k;
+    }
+    // ---- dvdt(k)
+    R dv = dvdt[o];
+    {
+      R fv1 = dx25 * (u_k + u_jm) * (v_m + v_k);
+      R fv2 = dx25 * (u_p + u_p_jm) * (v_k + v_p);
+      dv = dv - (fv2 - fv1);
+      fv1 = dx25 * (v_k + v_jm) * (v_jm + v_k);
+      fv2 = dx25 * (v_jp + v_k) * (v_k + v_jp);
+      dv = dv - (fv2 - fv1);
+      dv = dv - (fvz_n - fvz_k) * rhoi_k;
+      R tkx = rdx25 * grdf[k] * (tk_k + tk_jm + tk_p + tk_p_jm);
+      R fx = -tkx * (v_p - v_k + (u_p - u_p_jm) * dxy);
+      dv = dv - (fx - s.rot(fx, -1));
+      R fy = -R(2.) * (rdx2 * grdf[k] * tk_k) * (v_jp - v_k);
+      dv = dv - (fy - s.roty(fy, -1));
+      dv = dv - (fvv_n - fvv_k) * rhoi_k;
+    }
+    if (live) {
+      dudt[o] = du;
+      dvdt[o] = dv;
+    }
+    // ---- dwdt(k) for w levels 1..nzm-1
+    R fwz_k = dz25 * (w_n * rhow[kn] + w_k * rhow[k]) * (w_n + w_k);
+    if (k >= 1) {
+      size_t ow = ((size_t)c * nz + k) * NX + i;
+      R dw = dwdt[ow];
+      R rhoiw = R(1.) / (rhow[k] * adzw[k]);
+      dw = dw - pax;
+      dw = dw - pay;
+      dw = dw - (fwz_k - fwz_prev) * rhoiw;
+      dw = dw - pdx;
+      dw = dw - pdy;
+      dw = dw - (fwv_n - fwv_k) * rhoiw;
+      if (live) dwdt[ow] = dw;
+    }
+    // ---- horizontal terms of w level k+1 (applied next iteration)
+    {
+      R irho = R(1.) / (rhow[kn] * adzw[kn]);
+      R fw1 = dx25 * (u_k * rho[k] * adz[k] + ucu * rho[kcu] * adz[kcu]) * (wn_m + w_n);
+      R fw2 = dx25 * (u_p * rho[k] * adz[k] + ucu_p * rho[kcu] * adz[kcu]) * (w_n + wn_p);
+      pax = irho * (fw2 - fw1);
+      fw1 = dx25 * (v_k * rho[k] * adz[k] + vcu * rho[kcu] * adz[kcu]) * (wn_jm + w_n);
+      fw2 = dx25 * (v_jp * rho[k] * adz[k] + vcu_jp * rho[kcu] * adz[kcu]) * (w_n + wn_jp);
+      pay = irho * (fw2 - fw1);
+      R dxz = dx / (dz * adzw[kn]);
+      R tkx = rdx25 * grdf[k] * (tk_k + tk_p + tkcu + tkcu_p);
+      R fx = -tkx * (wn_p - w_n + (ucu_p - u_p) * dxz);
+      pdx = fx - s.rot(fx, -1);
+      R tky = rdx25 * grdf[k] * (tk_k + tk_jp + tkcu + tkcu_jp);
+      R fy = -tky * (wn_jp - w_n + (vcu_jp - v_jp) * dxz);
+      pdy = fy - s.roty(fy, -1);
+    }
+    fuz_k = fuz_n;
+    fvz_k = fvz_n;
+    fwz_prev = fwz_k;
+    fuv_k = fuv_n;
+    fvv_k = fvv_n;
+    fwv_k = fwv_n;
+    u_k = u_n;
+    u_n = u_nn;
+    v_k = v_n;
+    v_n = v_nn;
+    w_k = w_n;
+    w_n = w_nn;
+    tk_k = tk_n;
+    tk_n = tk_nn;
+  }
+  // lid (w level nzm): horizontal terms only
+  if (live) {
+    size_t ow = ((size_t)c * nz + nzm) * NX + i;
+    R dw = dwdt[ow];
+    dw = dw - pax;
+    dw = dw - pay;
+    dw = dw - pdx;
+    dw = dw - pdy;
+    dwdt[ow] = dw;
+    if (i == 0) {
+      prof(d, P_UWLE, c)[0] = R(0.0);
+      prof(d, P_UWSB, c)[0] = R(0.0);
+      prof(d, P_VWLE, c)[0] = R(0.0);
+      prof(d, P_VWSB, c)[0] = R(0.0);
+    }
+  }
+}
+
+template <int NX, typename R, typename G = Seg<NX>>
+__global__ void __launch_bounds__(256) k_mom(DevT<R> d, int na) {
+  if constexpr (G::Y3)
+    mom3<R>(d, na);
+  else
+    mom2<NX, R>(d, na);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -867,7 +1153,7 @@ __global__ void __launch_bounds__(256) k_press(DevT<R> d, int na, int nb, int nc
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PLD = 66;  // slab row stride in doubles: conflict-free A-fragment reads
 
-template <int NZM, typename R>
+template <int NZM, typename R, typename G = Seg<64>>
 __global__ void __launch_bounds__(256) k_press64(DevT<R> d, int na, int nb, int nc) {
   constexpr int NX = 64, NK = 64;
   extern __shared__ double lds[];
@@ -876,7 +1162,7 @@ __global__ void __launch_bounds__(256) k_press64(DevT<R> d, int na, int nb, int 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = blockIdx.x;  // grid is exactly ncrms blocks
   const int i = lane;
-  Seg<NX> s;
+  G s;  // 3D (8 x 8 columns): the transforms are the x-y Kronecker operators, dy == dx
   const int nz = NZM + 1;
   const double dz = inst(d, I_DZ, c);
   const double* adz = prof(d, P_ADZ, c);
@@ -899,10 +1185,18 @@ __global__ void __launch_bounds__(256) k_press64(DevT<R> d, int na, int nb, int 
       double rup = rhow[k + 1] / rho[k] * rdz;
       double rdn = rhow[k] / rho[k] * rdz;
       double uu = d.u[o], uap = ua[o], ubp = ub[o], ucp = uc[o];
-      pv = (rdx * (s.rot(uu, 1) - uu) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
-           (rdx * (s.rot(uap, 1) - uap) + (wa[own] * rup - wa[ow] * rdn)) +
-           btat * (rdx * (s.rot(ubp, 1) - ubp) + (wb[own] * rup - wb[ow] * rdn)) +
-           ctat * (rdx * (s.rot(ucp, 1) - ucp) + (wc[own] * rup - wc[ow] * rdn));
+      if constexpr (G::Y3) {  // press_rhs.F90:45-69
+        double vv = d.v[o], vap = d.dvdt[na][o], vbp = d.dvdt[nb][o], vcp = d.dvdt[nc][o];
+        pv = (rdx * (s.rot(uu, 1) - uu) + rdx * (s.roty(vv, 1) - vv) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
+             (rdx * (s.rot(uap, 1) - uap) + rdx * (s.roty(vap, 1) - vap) + (wa[own] * rup - wa[ow] * rdn)) +
+             btat * (rdx * (s.rot(ubp, 1) - ubp) + rdx * (s.roty(vbp, 1) - vbp) + (wb[own] * rup - wb[ow] * rdn)) +
+             ctat * (rdx * (s.rot(ucp, 1) - ucp) + rdx * (s.roty(vcp, 1) - vcp) + (wc[own] * rup - wc[ow] * rdn));
+      } else {
+        pv = (rdx * (s.rot(uu, 1) - uu) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
+             (rdx * (s.rot(uap, 1) - uap) + (wa[own] * rup - wa[ow] * rdn)) +
+             btat * (rdx * (s.rot(ubp, 1) - ubp) + (wb[own] * rup - wb[ow] * rdn)) +
+             ctat * (rdx * (s.rot(ucp, 1) - ucp) + (wc[own] * rup - wc[ow] * rdn));
+      }
       pv = pv * rho[k];
     }
     P[k * PLD + i] = pv;
@@ -986,6 +1280,15 @@ __global__ void __launch_bounds__(256) k_press64(DevT<R> d, int na, int nb, int 
     double uu = d.u[o], ww = d.w[ow];
     double un = uu + dt3 * (d.at * du_a + d.bt * ub[o] + d.ct * uc[o]);
     double wn = ww + dt3 * (d.at * dw_a + d.bt * wb[ow] + d.ct * wc[ow]);
+    if constexpr (G::Y3) {  // press_grad.F90:29, adams.F90:29,32
+      double pjm = s.roty(pk, -1);
+      double dv_a = d.dvdt[na][o] - (pk - pjm) * rdx;
+      double vv = d.v[o];
+      double vn = vv + dt3 * (d.at * dv_a + d.bt * d.dvdt[nb][o] + d.ct * d.dvdt[nc][o]);
+      d.dvdt[na][o] = dv_a;
+      d.dvdt[nc][o] = vn;
+      d.v[o] = 0.5 * (vv + vn) * (rho[k] * dtdx);
+    }
     uan[o] = du_a;
     wan[ow] = dw_a;
     unc[o] = un;
@@ -1045,9 +1348,10 @@ __device__ __forceinline__ T term_vel(const TermVel& tv, T q, T rho, T tabs) {
 // iteration.  Statistics: fadv (advect_scalar.F90:62-72) and the vertical flux of
 // advect_scalar2D.F90:113-118,204-206.  With want_cfl, also the precipitation Courant number
 // of precip_fall (microphysics.F90:596-611) on the advected field.
-template <int NX, typename R>
-__device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live, R* f, double* fadv,
+template <int NX, typename R, typename G>
+__device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f, double* fadv,
                            double* flux, bool want_cfl) {
+  constexpr bool Y3 = G::Y3;  // 3D: advect_scalar3D.F90:8-349 (dy == dx)
   const int nzm = d.nzm, nz = d.nz, i = s.i;
   const R* adz = profr(d, P_ADZ, c);
   const R* rho = profr(d, P_RHO, c);
@@ -1065,6 +1369,13 @@ __device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live,
   // inputs of the next two levels, loaded two iterations before use
   R nf1 = F(f, 0), nu1 = F(d.u, 0), nw1 = Wv(0);
   R nf2 = F(f, 1), nu2 = F(d.u, 1), nw2 = Wv(1);
+  R nv1 = 0, nv2 = 0;
+  if constexpr (Y3) {
+    nv1 = F(d.v, 0);
+    nv2 = F(d.v, 1);
+  }
+  R V_0 = 0, V_1 = 0, V_2 = 0, V_3 = 0;
+  R vv1_0 = 0, vv1_1 = 0, vv2_a = 0, vv2_b = 0, vv3_a = 0;
   // tabs for the precipitation CFL of stage 5, loaded two iterations before use
   R tq_a = 0, tq_b = 0;
   R U_0 = 0, U_1 = 0, U_2 = 0, U_3 = 0;
@@ -1077,14 +1388,16 @@ __device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live,
   R mx_a = 0, mx_b = 0, mn_a = 0, mn_b = 0;
   R uu3_a = 0, ww3_a = 0;
   for (int K = 0; K <= nzm + 4; K++) {
-    const R cf = nf1, cu = nu1, cw = nw1;
+    const R cf = nf1, cu = nu1, cw = nw1, cv = nv1;
     nf1 = nf2;
     nu1 = nu2;
     nw1 = nw2;
+    nv1 = nv2;
     if (K + 2 < nzm) {
       nf2 = F(f, K + 2);
       nu2 = F(d.u, K + 2);
       nw2 = Wv(K + 2);
+      if constexpr (Y3) nv2 = F(d.v, K + 2);
     }
     R tq_new = R(0.0);
     if (want_cfl && K >= 3 && K - 3 < nzm) tq_new = F(d.tabs, K - 3);
@@ -1096,24 +1409,36 @@ __device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live,
       R fkb = (K >= 1) ? f_1 : f_0;
       uu1_0 = fmax(R(0.0), U_0) * s.rot(f_0, -1) + fmin(R(0.0), U_0) * f_0;
       ww1_0 = fmax(R(0.0), W_0) * fkb + fmin(R(0.0), W_0) * f_0;
+      if constexpr (Y3) {
+        V_0 = cv;
+        vv1_0 = fmax(R(0.0), V_0) * s.roty(f_0, -1) + fmin(R(0.0), V_0) * f_0;
+      }
     } else {
       f_0 = U_0 = W_0 = uu1_0 = ww1_0 = R(0.0);
+      V_0 = vv1_0 = R(0.0);
     }
     // stage 1: donor-cell update and first bounds at level K-1
     const int L1 = K - 1;
     R f1_new = 0, mx0_new = 0, mn0_new = 0;
     if (L1 >= 0 && L1 < nzm) {
       R fkb = (L1 >= 1) ? f_2 : f_1, fkc = (L1 + 1 < nzm) ? f_0 : f_1;
-      f1_new = f_1 - (s.rot(uu1_1, 1) - uu1_1 + (ww1_0 - ww1_1) * iadz[L1]) * irho[L1];
       R fm = s.rot(f_1, -1), fp = s.rot(f_1, 1);
-      mx0_new = fmax(fmax(fmax(fmax(fm, fp), fkb), fkc), f_1);
-      mn0_new = fmin(fmin(fmin(fmin(fm, fp), fkb), fkc), f_1);
+      if constexpr (Y3) {
+        f1_new = f_1 - (s.rot(uu1_1, 1) - uu1_1 + s.roty(vv1_1, 1) - vv1_1 + (ww1_0 - ww1_1) * iadz[L1]) * irho[L1];
+        R fjm = s.roty(f_1, -1), fjp = s.roty(f_1, 1);
+        mx0_new = fmax(fmax(fmax(fmax(fmax(fmax(fm, fp), fjm), fjp), fkb), fkc), f_1);
+        mn0_new = fmin(fmin(fmin(fmin(fmin(fmin(fm, fp), fjm), fjp), fkb), fkc), f_1);
+      } else {
+        f1_new = f_1 - (s.rot(uu1_1, 1) - uu1_1 + (ww1_0 - ww1_1) * iadz[L1]) * irho[L1];
+        mx0_new = fmax(fmax(fmax(fmax(fm, fp), fkb), fkc), f_1);
+        mn0_new = fmin(fmin(fmin(fmin(fm, fp), fkb), fkc), f_1);
+      }
       R s1 = s.sum(ww1_1);
       if (flux && live && i == 0) flux[L1] = s1;
     }
     // stage 2: antidiffusive velocities and second bounds at level K-2
     const int L2 = K - 2;
-    R uu2_new = 0, ww2_new = 0, mx1_new = 0, mn1_new = 0;
+    R uu2_new = 0, ww2_new = 0, mx1_new = 0, mn1_new = 0, vv2_new = 0;
     if (L2 >= 0 && L2 < nzm) {
       int kc = (L2 + 1 < nzm) ? L2 + 1 : L2, kb = (L2 >= 1) ? L2 - 1 : 0;
       R f1L = f1_a;
@@ -1127,13 +1452,35 @@ __device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live,
       R f1c_m = s.rot(f1c, -1), f1b_m = s.rot(f1b, -1), f1b_p = s.rot(f1b, 1);
       R Wc_m = s.rot(Wc, -1), Wkc_m = s.rot(Wkc, -1);
       R Uc_p = s.rot(Uc, 1), Ub_p = s.rot(Ub, 1);
-      uu2_new = andiff(f1L_m, f1L, Uc, irho2) -
-                across(dd * (f1c_m + f1c - f1b_m - f1b), Uc, Wc_m + Wkc_m + Wc + Wkc) * irho2;
-      ww2_new = andiff(f1b, f1L, Wc, irhow) -
-                across(f1b_p + f1L_p - f1b_m - f1L_m, Wc, Ub + Uc + Uc_p + Ub_p) * irho2;
+      if constexpr (Y3) {
+        R Vc = V_2, Vb = (kb == L2) ? V_2 : V_3;
+        R f1L_jm = s.roty(f1L, -1), f1L_jp = s.roty(f1L, 1);
+        R f1L_m_jm = s.roty(f1L_m, -1), f1L_m_jp = s.roty(f1L_m, 1);
+        R f1L_p_jm = s.roty(f1L_p, -1);
+        R f1c_jm = s.roty(f1c, -1), f1b_jm = s.roty(f1b, -1), f1b_jp = s.roty(f1b, 1);
+        R Vc_m = s.rot(Vc, -1), Vc_jp = s.roty(Vc, 1), Vc_m_jp = s.roty(Vc_m, 1), Vb_jp = s.roty(Vb, 1);
+        R Uc_jm = s.roty(Uc, -1), Uc_p_jm = s.roty(Uc_p, -1);
+        R Wc_jm = s.roty(Wc, -1), Wkc_jm = s.roty(Wkc, -1);
+        uu2_new = andiff(f1L_m, f1L, Uc, irho2) -
+                  (across(f1L_m_jp + f1L_jp - f1L_m_jm - f1L_jm, Uc, Vc_m + Vc_m_jp + Vc_jp + Vc) +
+                   across(dd * (f1c_m + f1c - f1b_m - f1b), Uc, Wc_m + Wkc_m + Wc + Wkc)) * irho2;
+        vv2_new = andiff(f1L_jm, f1L, Vc, irho2) -
+                  (across(f1L_p_jm + f1L_p - f1L_m_jm - f1L_m, Vc, Uc_jm + Uc + Uc_p + Uc_p_jm) +
+                   across(dd * (f1c_jm + f1c - f1b_jm - f1b), Vc, Wc_jm + Wc + Wkc + Wkc_jm)) * irho2;
+        ww2_new = andiff(f1b, f1L, Wc, irhow) -
+                  (across(f1b_p + f1L_p - f1b_m - f1L_m, Wc, Ub + Uc + Uc_p + Ub_p) +
+                   across(f1L_jp + f1b_jp - f1L_jm - f1b_jm, Wc, Vb + Vb_jp + Vc_jp + Vc)) * irho2;
+        mx1_new = fmax(fmax(fmax(fmax(fmax(fmax(fmax(f1L_m, f1L_p), f1L_jm), f1L_jp), f1b), f1c), f1L), mx0_a);
+        mn1_new = fmin(fmin(fmin(fmin(fmin(fmin(fmin(f1L_m, f1L_p), f1L_jm), f1L_jp), f1b), f1c), f1L), mn0_a);
+      } else {
+        uu2_new = andiff(f1L_m, f1L, Uc, irho2) -
+                  across(dd * (f1c_m + f1c - f1b_m - f1b), Uc, Wc_m + Wkc_m + Wc + Wkc) * irho2;
+        ww2_new = andiff(f1b, f1L, Wc, irhow) -
+                  across(f1b_p + f1L_p - f1b_m - f1L_m, Wc, Ub + Uc + Uc_p + Ub_p) * irho2;
+        mx1_new = fmax(fmax(fmax(fmax(fmax(f1L_m, f1L_p), f1b), f1c), f1L), mx0_a);
+        mn1_new = fmin(fmin(fmin(fmin(fmin(f1L_m, f1L_p), f1b), f1c), f1L), mn0_a);
+      }
       if (L2 == 0) ww2_new = R(0.);
-      mx1_new = fmax(fmax(fmax(fmax(fmax(f1L_m, f1L_p), f1b), f1c), f1L), mx0_a);
-      mn1_new = fmin(fmin(fmin(fmin(fmin(f1L_m, f1L_p), f1b), f1c), f1L), mn0_a);
     }
     // stage 3: normalised limiters at level K-3
     const int L3 = K - 3;
@@ -1142,17 +1489,29 @@ __device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live,
       R uu2p = s.rot(uu2_a, 1);
       R ww2c = (L3 + 1 < nzm) ? ww2_new : ww2_a;
       const R iadz3 = iadz[L3];
-      mx_new = rho[L3] * (mx1_a - f1_b) / (pneg(uu2p) + ppos(uu2_a) + iadz3 * (pneg(ww2c) + ppos(ww2_a)) + eps);
-      mn_new = rho[L3] * (f1_b - mn1_a) / (ppos(uu2p) + pneg(uu2_a) + iadz3 * (ppos(ww2c) + pneg(ww2_a)) + eps);
+      if constexpr (Y3) {
+        R vv2p = s.roty(vv2_a, 1);
+        mx_new = rho[L3] * (mx1_a - f1_b) /
+                 (pneg(uu2p) + ppos(uu2_a) + pneg(vv2p) + ppos(vv2_a) + iadz3 * (pneg(ww2c) + ppos(ww2_a)) + eps);
+        mn_new = rho[L3] * (f1_b - mn1_a) /
+                 (ppos(uu2p) + pneg(uu2_a) + ppos(vv2p) + pneg(vv2_a) + iadz3 * (ppos(ww2c) + pneg(ww2_a)) + eps);
+      } else {
+        mx_new = rho[L3] * (mx1_a - f1_b) / (pneg(uu2p) + ppos(uu2_a) + iadz3 * (pneg(ww2c) + ppos(ww2_a)) + eps);
+        mn_new = rho[L3] * (f1_b - mn1_a) / (ppos(uu2p) + pneg(uu2_a) + iadz3 * (ppos(ww2c) + pneg(ww2_a)) + eps);
+      }
     }
     // stage 4: limited antidiffusive fluxes at level K-4
     const int L4 = K - 4;
-    R uu3_new = 0, ww3_new = 0;
+    R uu3_new = 0, ww3_new = 0, vv3_new = 0;
     if (L4 >= 0 && L4 < nzm) {
       R mxm = s.rot(mx_a, -1), mnm = s.rot(mn_a, -1);
       R mxb = (L4 >= 1) ? mx_b : mx_a, mnb = (L4 >= 1) ? mn_b : mn_a;
       uu3_new = ppos(uu2_b) * fmin(fmin(R(1.0), mx_a), mnm) - pneg(uu2_b) * fmin(fmin(R(1.0), mxm), mn_a);
       ww3_new = ppos(ww2_b) * fmin(fmin(R(1.0), mx_a), mnb) - pneg(ww2_b) * fmin(fmin(R(1.0), mxb), mn_a);
+      if constexpr (Y3) {
+        R mxjm = s.roty(mx_a, -1), mnjm = s.roty(mn_a, -1);
+        vv3_new = ppos(vv2_b) * fmin(fmin(R(1.0), mx_a), mnjm) - pneg(vv2_b) * fmin(fmin(R(1.0), mxjm), mn_a);
+      }
       R s3 = s.sum(ww3_new);
       if (flux && live && i == 0) flux[L4] = flux[L4] + s3;
     }
@@ -1160,7 +1519,11 @@ __device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live,
     const int L5 = K - 5;
     if (L5 >= 0 && L5 < nzm) {
       R ww3up = (L5 + 1 < nzm) ? ww3_new : R(0.0);
-      R fn = fmax(R(0.0), f1_d - (s.rot(uu3_a, 1) - uu3_a + (ww3up - ww3_a) * iadz[L5]) * irho[L5]);
+      R fn;
+      if constexpr (Y3)
+        fn = fmax(R(0.0), f1_d - (s.rot(uu3_a, 1) - uu3_a + s.roty(vv3_a, 1) - vv3_a + (ww3up - ww3_a) * iadz[L5]) * irho[L5]);
+      else
+        fn = fmax(R(0.0), f1_d - (s.rot(uu3_a, 1) - uu3_a + (ww3up - ww3_a) * iadz[L5]) * irho[L5]);
       size_t o = ((size_t)c * nzm + L5) * NX + i;
       R adv = s.sum(fn - f_5);
       if (live) {
@@ -1183,6 +1546,13 @@ __device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live,
     U_3 = U_2;
     U_2 = U_1;
     U_1 = U_0;
+    V_3 = V_2;
+    V_2 = V_1;
+    V_1 = V_0;
+    vv1_1 = vv1_0;
+    vv2_b = vv2_a;
+    vv2_a = vv2_new;
+    vv3_a = vv3_new;
     W_2 = W_1;
     W_1 = W_0;
     uu1_1 = uu1_0;
@@ -1213,14 +1583,14 @@ __device__ void mpdata_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live,
 }
 
 // advect_all_scalars.F90:7-94: t, then the two microphysics fields.
-template <int NX, typename R>
+template <int NX, typename R, typename G = Seg<NX>>
 __global__ void __launch_bounds__(256) k_mpdata(DevT<R> d) {
-  Seg<NX> s;
+  G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
-  mpdata_one<NX, R>(d, s, c, live, d.t, nullptr, nullptr, false);
-  mpdata_one<NX, R>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
-  mpdata_one<NX, R>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
+  mpdata_one<NX, R, G>(d, s, c, live, d.t, nullptr, nullptr, false);
+  mpdata_one<NX, R, G>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
+  mpdata_one<NX, R, G>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1366,11 +1736,11 @@ __global__ void __launch_bounds__(256) k_precip(DevT<R> d) {
 }
 
 // ------------------------------------------------------------------------------------------
-// SGS diffusion of t, qt, qp: diffuse_scalar.F90:8-72 and diffuse_scalar2D.F90:5-141 (zero
-// surface and top fluxes in the CRM).
+// SGS diffusion of t, qt, qp: diffuse_scalar.F90:8-72 and diffuse_scalar2D.F90:5-141 (3D:
+// diffuse_scalar3D.F90:6-218, dy == dx; zero surface and top fluxes in the CRM).
 // ------------------------------------------------------------------------------------------
-template <int NX, typename R>
-__device__ void diffuse_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live, R* f, double* fdiff,
+template <int NX, typename R, typename G>
+__device__ void diffuse_one(const DevT<R>& d, const G& s, int c, bool live, R* f, double* fdiff,
                             double* flux) {
   const int nzm = d.nzm, i = s.i;
   const R dz = inst(d, I_DZ, c), dtn = d.dtn;
@@ -1391,6 +1761,11 @@ __device__ void diffuse_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live
     R tkx = rdx5 * (tkh_k + s.rot(tkh_k, 1));
     R flxh = -tkx * (s.rot(f_k, 1) - f_k);
     R dfh = R(0.) - (flxh - s.rot(flxh, -1));
+    if constexpr (G::Y3) {
+      R tky = rdx5 * (tkh_k + s.roty(tkh_k, 1));
+      R flxy = -tky * (s.roty(f_k, 1) - f_k);
+      dfh = dfh - (flxy - s.roty(flxy, -1));
+    }
     R flxv;
     if (k <= nzm - 2) {
       R rhoi = rhow[k + 1] / adzw[k + 1];
@@ -1419,14 +1794,14 @@ __device__ void diffuse_one(const DevT<R>& d, const Seg<NX>& s, int c, bool live
   if (flux && live && i == 0) flux[0] = R(0.0);
 }
 
-template <int NX, typename R>
+template <int NX, typename R, typename G = Seg<NX>>
 __global__ void __launch_bounds__(256) k_diffuse(DevT<R> d) {
-  Seg<NX> s;
+  G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
-  diffuse_one<NX, R>(d, s, c, live, d.t, nullptr, nullptr);
-  diffuse_one<NX, R>(d, s, c, live, d.qt, prof(d, P_MKDIFF1, c), prof(d, P_MKWSB1, c));
-  diffuse_one<NX, R>(d, s, c, live, d.qp, prof(d, P_MKDIFF2, c), prof(d, P_MKWSB2, c));
+  diffuse_one<NX, R, G>(d, s, c, live, d.t, nullptr, nullptr);
+  diffuse_one<NX, R, G>(d, s, c, live, d.qt, prof(d, P_MKDIFF1, c), prof(d, P_MKWSB1, c));
+  diffuse_one<NX, R, G>(d, s, c, live, d.qp, prof(d, P_MKDIFF2, c), prof(d, P_MKWSB2, c));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1484,6 +1859,7 @@ __global__ void __launch_bounds__(256) k_micro(DevT<R> d) {
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, i = s.i;
+  const bool y3 = d.ny > 1;
   const R dtn = d.dtn, factor = R(1.) / (R)NX;
   const R* gamaz = profr(d, P_GAMAZ, c);
   const R* presp = profr(d, P_PRES, c);
@@ -1635,6 +2011,7 @@ __global__ void __launch_bounds__(256) k_micro(DevT<R> d) {
     tabs = t - gamaz[k] + FAC_COND * (qcl + qpl) + FAC_SUB * (qci + qpi);
     R u = d.u[o];
     R u0 = s.sum(u) * factor;
+    R v0 = y3 ? s.sum(d.v[o]) * factor : R(0.0);
     R t0 = s.sum(t) * factor;
     R tabs0 = s.sum(tabs) * factor;
     R q0 = s.sum(qv + qcl + qci) * factor;
@@ -1664,7 +2041,9 @@ __global__ void __launch_bounds__(256) k_micro(DevT<R> d) {
         prof(d, P_QN0, c)[k] = qn0;
         prof(d, P_QP0, c)[k] = qp0;
         prof(d, P_QV0, c)[k] = q0 - qn0;
+        if (y3) prof(d, P_V0, c)[k] = v0;
         if constexpr (!std::is_same<R, double>::value) {  // FP32 copies read by the bulk kernels
+          if (y3) profr(d, P_V0, c)[k] = v0;
           profr(d, P_U0, c)[k] = u0;
           profr(d, P_T0, c)[k] = t0;
           profr(d, P_TABS0, c)[k] = tabs0;
@@ -1830,6 +2209,8 @@ __global__ void __launch_bounds__(256) k_final(DevT<R> d, crm_input_view in, crm
   const double* rho = prof(d, P_RHO, c);
   const double* rhow = prof(d, P_RHOW, c);
   const double* u0 = prof(d, P_U0, c);
+  const double* v0 = prof(d, P_V0, c);
+  const bool y3 = d.ny > 1;
   const int ptop0 = plev - nzm;  // 0-based GCM index of the top CRM level
   auto IN = [&](const double* p, int l) { return p[(size_t)c + (size_t)ld * l]; };
   auto O2 = [&](double* p, int l) -> double& { return p[(size_t)c + (size_t)ld * l]; };
@@ -1852,13 +2233,19 @@ __global__ void __launch_bounds__(256) k_final(DevT<R> d, crm_input_view in, crm
     double qg_mean = s.sum(qpi * omg) * factor_xy, qs_mean = s.sum(qpi * (1. - omg)) * factor_xy;
     double du = u - u0[k];
     double u2z = s.sum(du * du), w2z = s.sum(0.5 * (wk1 * wk1 + wk * wk));
+    double v = 0.0, v2z = 0.0;
+    if (y3) {
+      v = Q(d.v, k);
+      double dv = v - v0[k];
+      v2z = s.sum(dv * dv);
+    }
     double tke = s.sum(Q(d.tke, k)), tkz = s.sum(Q(d.tk, k));
     if (live) {
       // every prognostic value written back must be finite (CRM_ERR_NAN otherwise)
-      const double chk = u + wk + tabs + Q(d.qt, k) + Q(d.qp, k) + Q(d.qn, k);
+      const double chk = u + v + wk + tabs + Q(d.qt, k) + Q(d.qp, k) + Q(d.qn, k);
       if (!isfinite(chk)) atomicMax(&d.gmax[2], ERR_FLAG_NAN);
       G4(st.u_wind, k) = u;
-      G4(st.v_wind, k) = 0.0;
+      G4(st.v_wind, k) = v;
       G4(st.w_wind, k) = wk;
       G4(st.temperature, k) = tabs;
       G4(st.qt, k) = Q(d.qt, k);
@@ -1902,7 +2289,7 @@ __global__ void __launch_bounds__(256) k_final(DevT<R> d, crm_input_view in, crm
         double mkdiff1 = prof(d, P_MKDIFF1, c)[k] * factor_xy * icrm_run_time;
         double mkdiff2 = prof(d, P_MKDIFF2, c)[k] * factor_xy * icrm_run_time;
         O2(out.flux_u, l) = (prof(d, P_UWLE, c)[k] + prof(d, P_UWSB, c)[k]) * t1 * factor_xy / nstop;
-        O2(out.flux_v, l) = 0.0 * t1 * factor_xy / nstop;
+        O2(out.flux_v, l) = (y3 ? prof(d, P_VWLE, c)[k] + prof(d, P_VWSB, c)[k] : 0.0) * t1 * factor_xy / nstop;
         O2(out.flux_qt, l) = mkwle1 + mkwsb1;
         O2(out.fluxsgs_qt, l) = mkwsb1;
         O2(out.flux_qp, l) = mkwle2 + mkwsb2;
@@ -1910,7 +2297,7 @@ __global__ void __launch_bounds__(256) k_final(DevT<R> d, crm_input_view in, crm
         O2(out.qp_trans, l) = mkadv2 + mkdiff2;
         double tkesgsz = rho[k] * tke * factor_xy;
         O2(out.tkesgsz, l) = tkesgsz;
-        O2(out.tkez, l) = rho[k] * 0.5 * (u2z + 0.0 + w2z) * factor_xy + tkesgsz;
+        O2(out.tkez, l) = rho[k] * 0.5 * (u2z + v2z + w2z) * factor_xy + tkesgsz;
         O2(out.tkz, l) = tkz * factor_xy;
         O2(out.precflux, l) = prof(d, P_PRECFLUX, c)[k] * factor_xy * dz / d.dt / nstop / 1000.;
         O2(out.qp_fall, l) = prof(d, P_QPFALL, c)[k] * factor_xy * icrm_run_time;

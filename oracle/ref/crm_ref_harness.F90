@@ -923,6 +923,9 @@ contains
     case ('dwdt_na'); call grid3(dwdt(:,1:nx,1:ny,1:nz,na))
     case ('dwdt_nb'); call grid3(dwdt(:,1:nx,1:ny,1:nz,nb))
     case ('dwdt_nc'); call grid3(dwdt(:,1:nx,1:ny,1:nz,nc))
+    case ('dvdt_na'); call grid3(dvdt(:,1:nx,1:ny,1:nzm,na))
+    case ('dvdt_nb'); call grid3(dvdt(:,1:nx,1:ny,1:nzm,nb))
+    case ('dvdt_nc'); call grid3(dvdt(:,1:nx,1:ny,1:nzm,nc))
     case ('precsfc');  call grid2(precsfc(:,1:nx,1:ny))
     case ('precssfc'); call grid2(precssfc(:,1:nx,1:ny))
     case ('t0');    call prof(t0)
@@ -930,6 +933,7 @@ contains
     case ('qv0');   call prof(qv0)
     case ('tabs0'); call prof(tabs0)
     case ('u0');    call prof(u0)
+    case ('v0');    call prof(v0)
     case ('qn0');   call prof(qn0)
     case ('qp0');   call prof(qp0)
     case ('p0');    call prof(p0)
@@ -946,6 +950,9 @@ contains
     case ('ttend'); call prof(ttend)
     case ('qtend'); call prof(qtend)
     case ('utend'); call prof(utend)
+    case ('vtend'); call prof(vtend)
+    case ('vwle');  call prof(vwle)
+    case ('vwsb');  call prof(vwsb)
     case ('uwle');  call prof(uwle)
     case ('uwsb');  call prof(uwsb)
     case ('mkwle1'); call prof(mkwle(:,:,1))

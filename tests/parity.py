@@ -4,7 +4,7 @@ import numpy as np
 from crm_inputs import add_convection, make_case
 
 # Fields of crm_state / crm_rad / crm_output compared after a call.
-STATE = ["u_wind", "w_wind", "temperature", "qt", "qp", "qn"]
+STATE = ["u_wind", "v_wind", "w_wind", "temperature", "qt", "qp", "qn"]
 RAD = ["temperature", "qv", "qc", "qi", "cld"]
 OUT = ["qcl", "qci", "qpl", "qpi", "tk", "tkh", "prec_crm", "cltot", "clhgh", "clmed", "cllow",
        "precc", "precl", "precsc", "precsl", "jt_crm", "mx_crm", "prectend", "precstend", "z0m",
@@ -20,6 +20,9 @@ GRID = ["u", "w", "t", "tabs", "qv", "qcl", "qci", "qpl", "qpi", "qt", "qp", "qn
         "precssfc"]
 PROF = ["t0", "q0", "qv0", "tabs0", "u0", "qn0", "qp0", "rho", "rhow", "adz", "adzw", "pres",
         "presi", "z", "zi", "bet", "gamaz", "ttend", "qtend", "utend", "dz"]
+# 3D CRMs only (v and its history, means, tendencies and fluxes)
+GRID3 = ["v", "dvdt_na", "dvdt_nb", "dvdt_nc"]
+PROF3 = ["v0", "vtend", "uwle", "uwsb", "vwle", "vwsb"]
 
 
 def relerr(a, b):
@@ -106,14 +109,15 @@ def _levels(field, n, nz):
 
 def static_energy(case, gamaz):
     """Liquid-ice static energy / cp of the state (crm_module.F90:508-509 with the
-    temperature ramps of micro_diagnose, microphysics.F90:331-357): (n, nx, nz)."""
+    temperature ramps of micro_diagnose, microphysics.F90:331-357): (n, nx, ny, nz)."""
     n = case.ncrms
-    T = case.state["temperature"][:n, :, 0, :]
-    qn, qp = case.state["qn"][:n, :, 0, :], case.state["qp"][:n, :, 0, :]
+    T = case.state["temperature"][:n]
+    qn, qp = case.state["qn"][:n], case.state["qp"][:n]
     omn = np.clip((T - _TBG[0]) * (1.0 / (_TBG[1] - _TBG[0])), 0, 1)
     omp = np.clip((T - _TPR[0]) * (1.0 / (_TPR[1] - _TPR[0])), 0, 1)
     fc, fs = LC / CP, (LC + LF) / CP
-    return T + gamaz[:, None, :] - fc * (qn * omn + qp * omp) - fs * (qn * (1 - omn) + qp * (1 - omp))
+    return (T + gamaz[:, None, None, :] - fc * (qn * omn + qp * omp)
+            - fs * (qn * (1 - omn) + qp * (1 - omp)))
 
 
 def budget_residuals(before, after, dt_gl, get):
@@ -126,7 +130,7 @@ def budget_residuals(before, after, dt_gl, get):
     the call (get("t")): the returned temperature satisfies the saturation adjustment only to
     its 0.01 K stopping tolerance (cloud.F90).  Returns (water, energy) residuals relative to
     the column totals."""
-    n, nz, nx = before.ncrms, before.nz, before.nx
+    n, nz, nx = before.ncrms, before.nz, before.nx * before.ny
     rho = get("rho", n * nz).reshape(n, nz)
     adz = get("adz", n * nz).reshape(n, nz)
     dz = get("dz", n)
@@ -134,15 +138,15 @@ def budget_residuals(before, after, dt_gl, get):
     mass = rho * adz * dz[:, None]
 
     def water(c):
-        q = (c.state["qt"][:n, :, 0, :] + c.state["qp"][:n, :, 0, :]).mean(axis=1)
+        q = (c.state["qt"][:n] + c.state["qp"][:n]).mean(axis=(1, 2))
         return (mass * q).sum(axis=1)
 
-    P = 1000.0 * after.out["prec_crm"][:n, :, 0].mean(axis=1) * dt_gl
+    P = 1000.0 * after.out["prec_crm"][:n].mean(axis=(1, 2)) * dt_gl
     Ps = 1000.0 * (after.out["precsc"][:n] + after.out["precsl"][:n]) * dt_gl
     Fq = (mass * _levels(after.out["qt_ls"], n, nz)).sum(axis=1) * dt_gl
     Ft = (mass * _levels(after.out["t_ls"], n, nz)).sum(axis=1) * dt_gl
     Wb, Wa = water(before), water(after)
-    Hb = (mass * static_energy(before, gamaz).mean(axis=1)).sum(axis=1)
+    Hb = (mass * static_energy(before, gamaz).mean(axis=(1, 2))).sum(axis=1)
     Ha = (mass * get("t", n * nz * nx).reshape(n, nz, nx).mean(axis=2)).sum(axis=1)
     rw = np.abs(Wa - Wb - Fq + P).max() / np.abs(Wb).max()
     rh = np.abs(Ha - Hb - Ft - (LC / CP) * P - (LF / CP) * Ps).max() / np.abs(Hb).max()

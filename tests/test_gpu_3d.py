@@ -1,0 +1,156 @@
+"""Parity of the HIP CRM's 3D form (8 x 8 columns, 28 levels, dx = dy = 4 km, dt 20 s) with
+the reference CRM compiled from its own sources for that size class (oracle/_ref).  The 3D
+routines (shear_prod3D, advect2_mom_xy/_z 3D branches, diffuse_mom3D, advect_scalar3D,
+diffuse_scalar3D, the x-y pressure transform) have no C restatement: the compiled reference
+is the oracle here, at the same bars as the 2D tests (1e-10 after one substep, 1e-8 over ten,
+the ensemble spread over a GCM step, budgets to FP64 round-off)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from crm_hip import CrmHip
+from parity import (GRID, GRID3, PROF, PROF3, budget_residuals, compare_case, convective_case,
+                    relerr)
+
+pytestmark = pytest.mark.gpu
+
+TOL_1 = 1e-10
+TOL_N = 1e-8
+SHAPE = dict(nx=8, ny=8, nz=28, dx=4000.0, dt=20.0)
+PHASES = [7, 9, 11, 13, 15, 17, 18, 20]
+
+
+def _case(n, seed):
+    return convective_case(n, seed=seed, **SHAPE)
+
+
+def _reference(case):
+    path = oracle_lib.ref_so_path(case)
+    if not os.path.exists(path):
+        pytest.skip(f"reference class not built: {path}")
+    return oracle_lib.reference(case)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    h = CrmHip(0)
+    yield h
+    h.close()
+
+
+def _worst(errs):
+    return sorted(errs.items(), key=lambda kv: -kv[1])[:4]
+
+
+@pytest.mark.parametrize("phase", PHASES)
+def test_phase_parity_3d(hip, phase):
+    n = 5
+    case = _case(n, seed=30 + phase)
+    ref = _reference(case)
+    assert ref.run(case.copy(), SHAPE["dt"], phase) == 0
+    assert hip.debug_run(case.copy(), SHAPE["dt"], phase) == 0
+    size = n * (SHAPE["nz"] + 1) * 64
+    errs = {}
+    for name in GRID + GRID3 + PROF + PROF3:
+        # after uvw the library swaps buffers where the reference copies
+        if phase >= 17 and name in ("dudt_nc", "dwdt_nc", "dvdt_nc"):
+            continue
+        errs[name] = relerr(hip.debug_get(name, size), ref.field(name, size))
+    print(phase, _worst(errs))
+    assert max(errs.values()) <= TOL_1, _worst(errs)
+
+
+@pytest.mark.parametrize("nsub", [1, 10])
+def test_full_call_parity_3d(hip, nsub):
+    dt = SHAPE["dt"]
+    case = _case(6, seed=40 + nsub)
+    want, got = case.copy(), case.copy()
+    assert _reference(case).run(want, dt * nsub) == 0
+    hip.run(got, dt * nsub)
+    errs = compare_case(got, want, dt_gl=dt * nsub)
+    print(nsub, _worst(errs))
+    assert errs["state.v_wind"] <= (TOL_1 if nsub == 1 else TOL_N)
+    assert max(errs.values()) <= (TOL_1 if nsub == 1 else TOL_N), _worst(errs)
+
+
+def test_first_gcm_step_perturbation_3d(hip):
+    """setperturb on the 64 columns of each instance, rows of x inside y (setperturb.F90:53-81)."""
+    case = _case(4, seed=51)
+    case.igstep = 1
+    case.gcol = np.array([3, 17, 4242, 9], dtype=np.int32)
+    want, got = case.copy(), case.copy()
+    assert _reference(case).run(want, SHAPE["dt"]) == 0
+    hip.run(got, SHAPE["dt"])
+    errs = compare_case(got, want, dt_gl=SHAPE["dt"])
+    assert max(errs.values()) <= TOL_1, _worst(errs)
+
+
+def test_radiation_columns_3d(hip):
+    """Radiative heating and the crm_rad averages on every (i, j) column."""
+    case = _case(3, seed=52)
+    case.rad["qrad"][...] = np.random.default_rng(2).uniform(-2e-5, 2e-5, case.rad["qrad"].shape)
+    want, got = case.copy(), case.copy()
+    assert _reference(case).run(want, 100.0) == 0
+    hip.run(got, 100.0)
+    errs = compare_case(got, want, dt_gl=100.0)
+    assert max(errs.values()) <= TOL_N, _worst(errs)
+
+
+def test_budgets_close_3d(hip):
+    before = _case(3, seed=53)
+    after = before.copy()
+    assert hip.debug_run(after, 1200.0, 0) == 0
+    rw, rh = budget_residuals(before, after, 1200.0, hip.debug_get)
+    print("water %.2e energy %.2e" % (rw, rh))
+    assert rw < 1e-14 and rh < 1e-13
+
+
+def test_gcm_step_inside_ensemble_spread_3d(hip):
+    """A full 1200 s GCM step: domain-mean heating, moistening and surface precipitation of
+    the GPU run against a 16-member reference ensemble with initial temperature perturbed by
+    a relative 1e-14 (same criterion as for the 2D CRM: RMS z-score at most 3)."""
+    case = _case(8, seed=54)
+    got = case.copy()
+    hip.run(got, 1200.0)
+    ref = _reference(case)
+    rng = np.random.default_rng(9)
+
+    def means(c):
+        return {"heating": c.out["sltend"][: c.ncrms] / 1004.64, "moistening": c.out["qltend"][: c.ncrms],
+                "precip": (c.out["precc"] + c.out["precl"])[: c.ncrms]}
+
+    members = []
+    for _ in range(16):
+        c = case.copy()
+        t = c.state["temperature"]
+        t[...] = t * (1.0 + 1e-14 * rng.uniform(-1.0, 1.0, size=t.shape))
+        assert ref.run(c, 1200.0) == 0
+        members.append(means(c))
+    g = means(got)
+    for key in g:
+        ens = np.stack([m[key] for m in members])
+        mean, sd = ens.mean(axis=0), ens.std(axis=0, ddof=1)
+        dev = np.abs(g[key] - mean)
+        live = sd > 0
+        z = dev[live] / sd[live]
+        rms_z = float(np.sqrt(np.mean(z * z))) if z.size else 0.0
+        print(key, "rms z %.2f" % rms_z)
+        assert rms_z <= 3.0, key
+        assert np.all(dev[~live] <= 1e-13 * max(np.abs(mean).max(), 1e-300)), key
+
+
+def test_mixed_precision_3d_one_substep():
+    """The mixed form on the 3D CRM: every state field within 1e-5 after one substep."""
+    h = CrmHip(0, "mixed")
+    try:
+        case = _case(4, seed=55)
+        want, got = case.copy(), case.copy()
+        assert _reference(case).run(want, SHAPE["dt"]) == 0
+        h.run(got, SHAPE["dt"])
+        errs = compare_case(got, want, groups=("state",), dt_gl=SHAPE["dt"])
+        print(_worst(errs))
+        assert max(errs.values()) < 1e-5, _worst(errs)
+    finally:
+        h.close()
