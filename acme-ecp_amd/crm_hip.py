@@ -20,12 +20,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libcrm_hip.so")
 
 CRM_OK, CRM_ERR_ARG, CRM_ERR_CFL, CRM_ERR_NXRAD, CRM_ERR_DAMPING = 0, 1, 2, 3, 4
-CRM_ERR_NAN, CRM_ERR_DEVICE, CRM_ERR_NOMEM, CRM_ERR_UNSUPPORTED = 5, 6, 7, 8
+CRM_ERR_NAN, CRM_ERR_DEVICE, CRM_ERR_NOMEM, CRM_ERR_UNSUPPORTED, CRM_ERR_ACCEL = 5, 6, 7, 8, 9
 PRECISION = {"fp64": 0, "mixed": 1}
 
 # Every function include/crm_hip.h declares.
-ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_get_stats",
-               "crm_hip_finalize", "crm_hip_errstr"]
+ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_accel_init",
+               "crm_hip_get_stats", "crm_hip_finalize", "crm_hip_errstr"]
 # include/crm_hip_debug.h
 DEBUG_SYMBOLS = ["crm_hip_debug_run", "crm_hip_debug_get", "crm_hip_fft991_operators",
                  "crm_hip_profile_kernels", "crm_hip_kernel_time", "crm_hip_kernel_name"]
@@ -40,7 +40,7 @@ class CrmHipConfig(C.Structure):
 
 class CrmHipStats(C.Structure):
     _fields_ = [("nstop", C.c_int), ("ncycle_max", C.c_int), ("nprec_max", C.c_int),
-                ("ms_device", C.c_double)]
+                ("ms_device", C.c_double), ("accel_ceased", C.c_int)]
 
 
 class CrmError(RuntimeError):
@@ -78,6 +78,7 @@ def load():
     lib.crm_hip_debug_run.argtypes = views + [C.c_int]
     lib.crm_hip_debug_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
     lib.crm_hip_get_stats.argtypes = [P, C.POINTER(CrmHipStats)]
+    lib.crm_hip_accel_init.argtypes = [P, C.c_int, C.c_double, C.c_int]
     lib.crm_hip_finalize.argtypes = [P]
     lib.crm_hip_errstr.argtypes = [C.c_int]
     lib.crm_hip_errstr.restype = C.c_char_p
@@ -156,6 +157,13 @@ class CrmHip:
         if rc and check:
             raise CrmError(rc, "crm_hip_run_device")
         return rc
+
+    def accel_init(self, use_crm_accel, crm_accel_factor=0.0, crm_accel_uv=False):
+        """Mean-state acceleration for later calls (crm_accel_init, accelerate_crm.F90:16-42)."""
+        rc = self.lib.crm_hip_accel_init(self.ctx, int(bool(use_crm_accel)), float(crm_accel_factor),
+                                         int(bool(crm_accel_uv)))
+        if rc:
+            raise CrmError(rc, "crm_hip_accel_init")
 
     def debug_run(self, case, dt_gl, phase_stop):
         return self._call(self.lib.crm_hip_debug_run, case.dims(), case.column_info(), dt_gl,

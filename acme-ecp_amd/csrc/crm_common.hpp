@@ -114,6 +114,7 @@ enum Prof {
   P_MKADV1, P_MKADV2, P_MKDIFF1, P_MKDIFF2, P_QPSRC, P_QPEVP,
   P_QPFALL, P_PRECFLUX,                                    // accumulated over the call
   P_CLD, P_CLDTOP, P_MCUP, P_MCDN, P_MCUUP, P_MCUDN, P_GLIQ, P_GICE, P_MUI, P_MDI,
+  P_ACCT, P_ACCQ, P_ACCU, P_ACCV,                          // mean-state acceleration, per subcycle
   NPROF
 };
 
@@ -156,7 +157,8 @@ struct DevT {
                               // nx_rad counts the nx_rad*ny_rad radiation columns)
   const double *Ft, *Gt;      // transform operators [NX][NX]
   const double *eign;         // [NX] horizontal eigenvalues per slot
-  unsigned long long *gmax;   // [0] kurant cfl bits, [1] precipitation cfl bits, [2] error flag
+  unsigned long long *gmax;   // [0] kurant cfl bits, [1] precipitation cfl bits, [2] error flag,
+                              // [3] mean-state acceleration cease flag
   TermVel tv;
   Gam gam;
   double z0c1;                // 3.14159/2 - 3 ln 2 in REAL(4) (crmsurface.F90:125)

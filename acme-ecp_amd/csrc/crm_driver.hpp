@@ -72,6 +72,9 @@ struct Engine : CrmEngine {
   size_t ev_used = 0;       // pairs recorded in the current call
   double kt_ms[CRM_K_COUNT] = {};
   long long kt_n[CRM_K_COUNT] = {};
+  // mean-state acceleration (crm_hip_accel_init)
+  int acc_use = 0, acc_uv = 0;
+  double acc_factor = 0.0;
 
   ~Engine() override;
   int run(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
@@ -80,6 +83,13 @@ struct Engine : CrmEngine {
   int debug_get(const char* name, double* buf) override;
   void get_stats(crm_hip_stats* s) const override { *s = stats; }
   int profile_kernels(unsigned mask) override;
+  int accel_init(int use, double factor, int uv) override {
+    if (use && !(factor >= 0.0)) return CRM_ERR_ARG;
+    acc_use = use;
+    acc_factor = factor;
+    acc_uv = uv;
+    return CRM_OK;
+  }
   int kernel_time(int kid, double* ms, long long* n) const override;
 };
 
@@ -398,28 +408,57 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     k_prof_mirror<R><<<dim3((unsigned)((np + 255) / 256)), b256, 0, s>>>(d.prof, d.profr, np);
   }
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipMemsetAsync(ctx->gmax + 2, 0, sizeof(unsigned long long), s));
+  HIPCHECK(hipMemsetAsync(ctx->gmax + 2, 0, 2 * sizeof(unsigned long long), s));
 
   if (!(dt_gl >= dm->dt) || !(dt_gl / dm->dt < 1.0e6)) return CRM_ERR_ARG;
-  const int nstop = (int)(dt_gl / dm->dt);
+  int nstop = (int)(dt_gl / dm->dt);
   const double dt = dt_gl / nstop;
+  // mean-state acceleration: crm_accel_nstop (accelerate_crm.F90:44-62, crm_module.F90:718-720)
+  const bool acc = ctx->acc_use != 0;
+  const double acc_f = ctx->acc_factor;
+  bool ceased = false;
+  if (acc) {
+    const int a = (int)(1 + acc_f);
+    if (a < 1 || nstop % a != 0) return CRM_ERR_ACCEL;
+    nstop = (int)(nstop / (1 + acc_f));
+  }
   d.dt = dt;
-  ctx->stats.nstop = nstop;
   double dt3[3] = {0.0, 0.0, 0.0};
   int na = 0, nb = 1, nc = 2;
   const dim3 gpress((unsigned)((ncrms + (64 / nx) - 1) / (64 / nx)));
   const size_t press_lds = (size_t)nzm * 64 * sizeof(double);
   unsigned long long err_flag = 0;
   HIPCHECK(hipEventRecord(ctx->ev0, s));
-  for (int nstep = 1; nstep <= nstop; nstep++) {
+  // A mean t change above 5 K flagged by the acceleration during substep n stops it for the
+  // rest of the call and lengthens it (accelerate_crm.F90:255-278); the flag is read with the
+  // next CFL maximum, or after the last substep.
+  auto cease = [&](unsigned long long flag, int nstep_flagged) {
+    if (acc && !ceased && flag) {
+      ceased = true;
+      nstop = (int)(nstop + (nstop - nstep_flagged + 1) * acc_f);
+    }
+  };
+  int nstep = 0;
+  while (true) {
+    if (nstep >= nstop) {
+      if (acc && !ceased) {
+        unsigned long long flag = 0;
+        HIPCHECK(hipMemcpyAsync(&flag, ctx->gmax + 3, sizeof(flag), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        cease(flag, nstep);
+      }
+      if (nstep >= nstop) break;
+    }
+    nstep++;
     d.nstep = nstep;
-    unsigned long long bits = 0;
+    unsigned long long bits[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemsetAsync(ctx->gmax, 0, sizeof(unsigned long long), s));
     TIMED(CRM_K_KURANT, NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
-    HIPCHECK(hipMemcpyAsync(&bits, ctx->gmax, sizeof(bits), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(bits, ctx->gmax, sizeof(bits), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+    cease(bits[3], nstep - 1);
     double cfl;
-    std::memcpy(&cfl, &bits, sizeof(cfl));
+    std::memcpy(&cfl, &bits[0], sizeof(cfl));
     // ncycle = max(1, ceiling(cfl / 0.7)) > 16 is an abort in the reference (kurant.F90:53-75);
     // the test is made before the conversion so that huge or non-finite values cannot overflow
     const double q = cfl / (double)0.7f;
@@ -457,7 +496,13 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       if (last && phase_stop == 17) goto stopped;
       TIMED(CRM_K_DIFFUSE, NX_DISPATCH(nx, (k_diffuse<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d))));
       if (last && phase_stop == 18) goto stopped;
-      TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_micro<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      {
+        const int accel = (acc && !ceased) ? 1 : 0;
+        TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_micro<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, accel))));
+        if (accel && last && phase_stop == 19) goto stopped;  // before the acceleration
+        if (accel)  // crm_module.F90:872-876
+          TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_accel<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, acc_f, ctx->acc_uv))));
+      }
       if (last && phase_stop == 20) goto stopped;
       int nn = na;
       na = nc;
@@ -467,6 +512,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     TIMED(CRM_K_STATS, NX_DISPATCH(nx, (k_stats<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
     HIPCHECK(hipGetLastError());
   }
+  ctx->stats.nstop = nstop;
+  ctx->stats.accel_ceased = ceased ? 1 : 0;
   TIMED(CRM_K_FINAL, NX_DISPATCH(nx, (k_final<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop))));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(ctx->ev1, s));
@@ -754,7 +801,7 @@ static CrmEngine* make_engine(int device, int* rc) {
   e->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess ||
       hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
-      hipMalloc(&e->gmax, 3 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&e->gmax, 4 * sizeof(unsigned long long)) != hipSuccess) {
     delete e;
     *rc = CRM_ERR_DEVICE;
     return nullptr;

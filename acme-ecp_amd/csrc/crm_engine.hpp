@@ -14,6 +14,7 @@ struct CrmEngine {
   virtual void get_stats(crm_hip_stats* s) const = 0;
   virtual int profile_kernels(unsigned mask) = 0;
   virtual int kernel_time(int kid, double* ms, long long* n) const = 0;
+  virtual int accel_init(int use, double factor, int uv) = 0;
 };
 
 namespace crm {

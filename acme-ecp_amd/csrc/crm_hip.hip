@@ -88,6 +88,11 @@ int crm_hip_run_device(crm_hip_ctx* ctx, const crm_dims* dims, const crm_column_
   return ctx->eng->run(dims, col, dt_gl, in, state, rad, out, true, 0);
 }
 
+int crm_hip_accel_init(crm_hip_ctx* ctx, int use_crm_accel, double crm_accel_factor, int crm_accel_uv) {
+  if (!ctx || !ctx->eng) return CRM_ERR_ARG;
+  return ctx->eng->accel_init(use_crm_accel, crm_accel_factor, crm_accel_uv);
+}
+
 int crm_hip_get_stats(const crm_hip_ctx* ctx, crm_hip_stats* stats) {
   if (!ctx || !stats) return CRM_ERR_ARG;
   ctx->eng->get_stats(stats);
@@ -112,6 +117,7 @@ const char* crm_hip_errstr(int code) {
     case CRM_ERR_DEVICE: return "HIP runtime error";
     case CRM_ERR_NOMEM: return "device allocation failed";
     case CRM_ERR_UNSUPPORTED: return "size class or option not provided by this build";
+    case CRM_ERR_ACCEL: return "bad crm_accel_factor and nstop pair: (1+crm_accel_factor) does not divide nstop";
     default: return "unknown error";
   }
 }

@@ -1853,8 +1853,81 @@ __global__ void k_precip_init(DevT<R> d) {
 // per level followed by the level means.  Also finds the cloud-ice level range for the next
 // ice fall (ice_fall.F90:34-46).
 // ------------------------------------------------------------------------------------------
+// diagnose (diagnose.F90:8-177) at level k: tabs from t and the condensate split, the level
+// means, the cloud-ice level range of ice_fall; writes the microphysics fields and tabs.
+template <int NX, typename R, typename S_>
+__device__ __forceinline__ void diagnose_level(const DevT<R>& d, const S_& s, int c, bool live, int k, size_t o,
+                                               bool y3, R t, R q, R qp, R qn, R qv, R qcl, R qci, R qpl, R qpi,
+                                               R u, R v, int& kmin, int& kmax) {
+  const R factor = R(1.) / (R)NX;
+  const R* gamaz = profr(d, P_GAMAZ, c);
+  R tabs = t - gamaz[k] + FAC_COND * (qcl + qpl) + FAC_SUB * (qci + qpi);
+  R u0 = s.sum(u) * factor;
+  R v0 = y3 ? s.sum(v) * factor : R(0.0);
+  R t0 = s.sum(t) * factor;
+  R tabs0 = s.sum(tabs) * factor;
+  R q0 = s.sum(qv + qcl + qci) * factor;
+  R qn0 = s.sum(qcl + qci) * factor;
+  R qp0 = s.sum(qpl + qpi) * factor;
+  if (qcl + qci > R(0.) && tabs < R4(273.15)) {
+    kmin = ::min(kmin, k);
+    kmax = ::max(kmax, k);
+  }
+  if (live) {
+    d.qt[o] = q;
+    d.qp[o] = qp;
+    d.qn[o] = qn;
+    d.tabs[o] = tabs;
+    d.qv[o] = qv;
+    d.qcl[o] = qcl;
+    d.qci[o] = qci;
+    d.qpl[o] = qpl;
+    d.qpi[o] = qpi;
+    if (s.i == 0) {
+      prof(d, P_U0, c)[k] = u0;
+      prof(d, P_T0, c)[k] = t0;
+      prof(d, P_TABS0, c)[k] = tabs0;
+      prof(d, P_Q0, c)[k] = q0;
+      prof(d, P_QN0, c)[k] = qn0;
+      prof(d, P_QP0, c)[k] = qp0;
+      prof(d, P_QV0, c)[k] = q0 - qn0;
+      if (y3) prof(d, P_V0, c)[k] = v0;
+      if constexpr (!std::is_same<R, double>::value) {  // FP32 copies read by the bulk kernels
+        if (y3) profr(d, P_V0, c)[k] = v0;
+        profr(d, P_U0, c)[k] = u0;
+        profr(d, P_T0, c)[k] = t0;
+        profr(d, P_TABS0, c)[k] = tabs0;
+        profr(d, P_Q0, c)[k] = q0;
+        profr(d, P_QN0, c)[k] = qn0;
+        profr(d, P_QP0, c)[k] = qp0;
+        profr(d, P_QV0, c)[k] = q0 - qn0;
+      }
+    }
+  }
+}
+
+// The level means of accelerate_crm.F90:196-250 after micro_proc, minus those of the last
+// diagnose; flags a mean t change above 5 K (ceaseflag) in gmax[3].
+template <int NX, typename R, typename S_>
+__device__ __forceinline__ void accel_means(const DevT<R>& d, const S_& s, int c, bool live, int k, size_t o,
+                                            bool y3, R t, R qv, R qcl, R qci) {
+  const double coef = 1. / (double)NX;
+  double tb = s.sum((double)t * coef);
+  double qtb = s.sum(((double)qcl + (double)qci + (double)qv) * coef);
+  double ub = s.sum((double)d.u[o] * coef);
+  double vb = y3 ? s.sum((double)d.v[o] * coef) : 0.0;
+  if (live && s.i == 0) {
+    double tt = tb - prof(d, P_T0, c)[k];
+    prof(d, P_ACCT, c)[k] = tt;
+    prof(d, P_ACCQ, c)[k] = qtb - prof(d, P_Q0, c)[k];
+    prof(d, P_ACCU, c)[k] = ub - prof(d, P_U0, c)[k];
+    if (y3) prof(d, P_ACCV, c)[k] = vb - prof(d, P_V0, c)[k];
+    if (fabs(tt) > 5.) atomicMax(&d.gmax[3], 1ull);
+  }
+}
+
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_micro(DevT<R> d) {
+__global__ void __launch_bounds__(256) k_micro(DevT<R> d, int accel) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -2005,55 +2078,97 @@ __global__ void __launch_bounds__(256) k_micro(DevT<R> d) {
       q = q + (dq - qp);
     }
     R ssrc = s.sum(src), sevp = s.sum(evp);
-    // micro_diagnose, then diagnose
+    if (live && i == 0) {
+      prof(d, P_QPSRC, c)[k] = ssrc;
+      prof(d, P_QPEVP, c)[k] = sevp;
+    }
+    // micro_diagnose, then diagnose (or, with mean-state acceleration pending, the means it
+    // needs; k_accel then applies it and diagnoses)
     R qv, qcl, qci, qpl, qpi;
     micro_diag(q, qn, qp, tabs, a_bg, a_pr, qv, qcl, qci, qpl, qpi);
-    tabs = t - gamaz[k] + FAC_COND * (qcl + qpl) + FAC_SUB * (qci + qpi);
-    R u = d.u[o];
-    R u0 = s.sum(u) * factor;
-    R v0 = y3 ? s.sum(d.v[o]) * factor : R(0.0);
-    R t0 = s.sum(t) * factor;
-    R tabs0 = s.sum(tabs) * factor;
-    R q0 = s.sum(qv + qcl + qci) * factor;
-    R qn0 = s.sum(qcl + qci) * factor;
-    R qp0 = s.sum(qpl + qpi) * factor;
-    if (qcl + qci > R(0.) && tabs < R4(273.15)) {
-      kmin = ::min(kmin, k);
-      kmax = ::max(kmax, k);
+    if (accel) {
+      accel_means<NX, R>(d, s, c, live, k, o, y3, t, qv, qcl, qci);
+      if (live) {
+        d.qt[o] = q;
+        d.qp[o] = qp;
+        d.qn[o] = qn;
+        d.qv[o] = qv;
+        d.qcl[o] = qcl;
+        d.qci[o] = qci;
+        d.qpl[o] = qpl;
+        d.qpi[o] = qpi;
+      }
+    } else {
+      diagnose_level<NX, R>(d, s, c, live, k, o, y3, t, q, qp, qn, qv, qcl, qci, qpl, qpi, d.u[o],
+                            y3 ? d.v[o] : R(0.0), kmin, kmax);
     }
-    if (live) {
-      d.qt[o] = q;
-      d.qp[o] = qp;
-      d.qn[o] = qn;
-      d.tabs[o] = tabs;
-      d.qv[o] = qv;
-      d.qcl[o] = qcl;
-      d.qci[o] = qci;
-      d.qpl[o] = qpl;
-      d.qpi[o] = qpi;
-      if (i == 0) {
-        prof(d, P_QPSRC, c)[k] = ssrc;
-        prof(d, P_QPEVP, c)[k] = sevp;
-        prof(d, P_U0, c)[k] = u0;
-        prof(d, P_T0, c)[k] = t0;
-        prof(d, P_TABS0, c)[k] = tabs0;
-        prof(d, P_Q0, c)[k] = q0;
-        prof(d, P_QN0, c)[k] = qn0;
-        prof(d, P_QP0, c)[k] = qp0;
-        prof(d, P_QV0, c)[k] = q0 - qn0;
-        if (y3) prof(d, P_V0, c)[k] = v0;
-        if constexpr (!std::is_same<R, double>::value) {  // FP32 copies read by the bulk kernels
-          if (y3) profr(d, P_V0, c)[k] = v0;
-          profr(d, P_U0, c)[k] = u0;
-          profr(d, P_T0, c)[k] = t0;
-          profr(d, P_TABS0, c)[k] = tabs0;
-          profr(d, P_Q0, c)[k] = q0;
-          profr(d, P_QN0, c)[k] = qn0;
-          profr(d, P_QP0, c)[k] = qp0;
-          profr(d, P_QV0, c)[k] = q0 - qn0;
+  }
+  if (accel) return;
+  kmin = s.imin(kmin);
+  kmax = s.imax(kmax);
+  if (live && i == 0) {
+    inst(d, I_KMIN, c) = kmin;
+    inst(d, I_KMAX, c) = kmax;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Mean-state acceleration: accelerate_crm.F90:109-366 (called between micro_proc and
+// diagnose, crm_module.F90:872-876), then diagnose.  Unless a mean t change above 5 K was
+// flagged in this call (gmax[3]; accelerate_crm.F90:255-278), t and qt (and u, v with
+// crm_accel_uv) get crm_accel_factor times the change of their level mean since the last
+// diagnose (computed by k_micro, profiles P_ACCT..P_ACCV); negative qt is clipped level by
+// level with the excess removed from the positive values, and qv, qcl, qci re-split.
+// ------------------------------------------------------------------------------------------
+template <int NX, typename R>
+__global__ void __launch_bounds__(256) k_accel(DevT<R> d, double factor, int uv) {
+  Seg<NX> s;
+  int c;
+  bool live = seg_instance<NX>(d.ncrms, c);
+  const int nzm = d.nzm, i = s.i;
+  const bool y3 = d.ny > 1;
+  const bool apply = *(volatile unsigned long long*)&d.gmax[3] == 0;
+  const double tmin = 50.;
+  int kmin = nzm, kmax = -1;
+  for (int k = 0; k < nzm; k++) {
+    size_t o = ((size_t)c * nzm + k) * NX + i;
+    R t = d.t[o], q = d.qt[o], qp = d.qp[o], qn = d.qn[o];
+    R qv = d.qv[o], qcl = d.qcl[o], qci = d.qci[o], qpl = d.qpl[o], qpi = d.qpi[o];
+    R u = d.u[o], v = y3 ? d.v[o] : R(0.0);
+    if (apply) {
+      t = (R)fmax(tmin, (double)t + factor * prof(d, P_ACCT, c)[k]);
+      if (uv) {
+        u = (R)((double)u + factor * prof(d, P_ACCU, c)[k]);
+        if (y3) v = (R)((double)v + factor * prof(d, P_ACCV, c)[k]);
+      }
+      double qt = (double)q + factor * prof(d, P_ACCQ, c)[k];
+      double qneg = s.sum(qt < 0. ? qt : 0.0), qpoz = s.sum(qt < 0. ? 0.0 : qt);
+      if (qpoz + qneg <= 0.) {
+        qt = 0.;
+        qv = qcl = qci = R(0.);
+      } else {
+        double f = 1. + qneg / qpoz;
+        qt = fmax(0., qt * f);
+        if (qt <= 0.) {
+          qv = qcl = qci = R(0.);
+        } else {
+          double qt_res = qt - (double)qcl - (double)qci;
+          qv = (R)fmax(0., qt_res);
+          if (qt_res < 0.) {
+            f = 1. + qt_res / ((double)qcl + (double)qci);
+            qcl = (R)((double)qcl * f);
+            qci = (R)((double)qci * f);
+          }
         }
       }
+      q = (R)qt;
+      if (live) {
+        d.t[o] = t;
+        d.u[o] = u;
+        if (y3) d.v[o] = v;
+      }
     }
+    diagnose_level<NX, R>(d, s, c, live, k, o, y3, t, q, qp, qn, qv, qcl, qci, qpl, qpi, u, v, kmin, kmax);
   }
   kmin = s.imin(kmin);
   kmax = s.imax(kmax);

@@ -12,7 +12,7 @@ module crm_hip_bindings
 
   integer(c_int), parameter, public :: CRM_OK = 0, CRM_ERR_ARG = 1, CRM_ERR_CFL = 2, &
        CRM_ERR_NXRAD = 3, CRM_ERR_DAMPING = 4, CRM_ERR_NAN = 5, CRM_ERR_DEVICE = 6, &
-       CRM_ERR_NOMEM = 7, CRM_ERR_UNSUPPORTED = 8
+       CRM_ERR_NOMEM = 7, CRM_ERR_UNSUPPORTED = 8, CRM_ERR_ACCEL = 9
   integer(c_int), parameter, public :: CRM_PREC_FP64 = 0, CRM_PREC_MIXED = 1
 
   type, bind(C), public :: crm_dims
@@ -59,9 +59,10 @@ module crm_hip_bindings
   type, bind(C), public :: crm_hip_stats
     integer(c_int) :: nstop, ncycle_max, nprec_max
     real(c_double) :: ms_device
+    integer(c_int) :: accel_ceased
   end type
 
-  public :: crm_hip_init, crm_hip_run, crm_hip_run_device, crm_hip_get_stats, &
+  public :: crm_hip_init, crm_hip_run, crm_hip_run_device, crm_hip_accel_init, crm_hip_get_stats, &
             crm_hip_finalize, crm_hip_errstr, crm_hip_message
 
   interface
@@ -97,6 +98,15 @@ module crm_hip_bindings
       type(crm_state_view), intent(inout) :: state
       type(crm_rad_view), intent(inout) :: rad
       type(crm_output_view), intent(inout) :: out
+    end function
+
+    ! crm_accel_init replacement (accelerate_crm.F90:16-42)
+    integer(c_int) function crm_hip_accel_init(ctx, use_crm_accel, crm_accel_factor, crm_accel_uv) &
+        bind(C, name="crm_hip_accel_init")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: use_crm_accel, crm_accel_uv
+      real(c_double), value :: crm_accel_factor
     end function
 
     integer(c_int) function crm_hip_get_stats(ctx, stats) bind(C, name="crm_hip_get_stats")

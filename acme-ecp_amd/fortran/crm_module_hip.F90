@@ -13,7 +13,8 @@
 ! Built inside an E3SM tree (it uses CAM's ppgrid, phys_grid, time_manager, cam_abortutils
 ! and the CRM type modules); crm_hip_bindings.F90 holds the ISO_C_BINDING layer and builds
 ! on its own.  Not covered: CLUBB_CRM, MAML, ECPP statistics (crm_ecpp_output is accepted
-! and left untouched), sp_crm_accel.
+! and left untouched).  Mean-state acceleration takes its settings from the physics namelist
+! as crm_accel_init does (accelerate_crm.F90:16-42).
 module crm_module
   use iso_c_binding
   use shr_kind_mod,          only: r8 => shr_kind_r8
@@ -40,6 +41,7 @@ contains
     use time_manager,   only: get_nstep
     use cam_abortutils, only: endrun
     use crmdims,        only: crm_nx, crm_ny, crm_nz, crm_nx_rad, crm_ny_rad, crm_dx, crm_dt
+    use phys_control,   only: phys_getopts
 
     integer,  intent(in) :: lchnk, ncrms, plev
     integer,  intent(in) :: icol(ncrms)
@@ -61,11 +63,21 @@ contains
     real(c_double), target :: lat(ncrms), lon(ncrms)
     integer(c_int) :: rc
     integer :: i
+    logical :: use_crm_accel, crm_accel_uv
+    real(r8) :: crm_accel_factor
 
     if (.not. c_associated(ctx)) then
       cfg = crm_hip_config(0, CRM_PREC_FP64, pcols)
       rc = crm_hip_init(ctx, cfg)
       if (rc /= CRM_OK) call endrun('crm_hip_init: '//crm_hip_message(rc))
+      use_crm_accel = .false.
+      crm_accel_factor = 0._r8
+      crm_accel_uv = .false.
+      call phys_getopts(use_crm_accel_out = use_crm_accel, crm_accel_factor_out = crm_accel_factor, &
+                        crm_accel_uv_out = crm_accel_uv)
+      rc = crm_hip_accel_init(ctx, merge(1_c_int, 0_c_int, use_crm_accel), &
+                              real(crm_accel_factor, c_double), merge(1_c_int, 0_c_int, crm_accel_uv))
+      if (rc /= CRM_OK) call endrun('crm_hip_accel_init: '//crm_hip_message(rc))
     end if
 
     do i = 1, ncrms

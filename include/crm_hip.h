@@ -44,7 +44,8 @@ enum {
   CRM_ERR_NAN = 5,        /* non-finite prognostic field detected after the call            */
   CRM_ERR_DEVICE = 6,     /* HIP runtime error                                              */
   CRM_ERR_NOMEM = 7,      /* device allocation failed                                       */
-  CRM_ERR_UNSUPPORTED = 8 /* configuration not provided by this build                       */
+  CRM_ERR_UNSUPPORTED = 8, /* configuration not provided by this build                      */
+  CRM_ERR_ACCEL = 9       /* 1 + crm_accel_factor does not divide nstop (accelerate_crm.F90:51-58) */
 };
 
 /* Arithmetic of the bulk stencils. The pressure solve (DFT + Thomas sweep), the hydrostatic
@@ -136,6 +137,7 @@ typedef struct crm_hip_stats {
   int ncycle_max;       /* largest kurant subcycle count seen in the last call              */
   int nprec_max;        /* largest precipitation sub-step count seen in the last call       */
   double ms_device;     /* device time of the last call's substep loop [ms] (HIP events)    */
+  int accel_ceased;     /* mean-state acceleration stopped by its 5 K test (accelerate_crm.F90:255-278) */
 } crm_hip_stats;
 
 typedef struct crm_hip_ctx crm_hip_ctx;
@@ -157,6 +159,14 @@ int crm_hip_run(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *c
 int crm_hip_run_device(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col,
                        double dt_gl, const crm_input_view *in, crm_state_view *state,
                        crm_rad_view *rad, crm_output_view *out);
+
+/* Mean-state acceleration settings of a context, the replacement of crm_accel_init
+ * (accelerate_crm.F90:16-42, which reads them from the physics namelist): with
+ * use_crm_accel != 0 every later call integrates nstop / (1 + crm_accel_factor) substeps and
+ * amplifies the change of the horizontal-mean t and qt (and u, v when crm_accel_uv != 0) of
+ * each subcycle by crm_accel_factor (accelerate_crm.F90:109-366, crm_module.F90:718-720,
+ * 872-876). Default: off. */
+int crm_hip_accel_init(crm_hip_ctx *ctx, int use_crm_accel, double crm_accel_factor, int crm_accel_uv);
 
 int crm_hip_get_stats(const crm_hip_ctx *ctx, crm_hip_stats *stats);
 int crm_hip_finalize(crm_hip_ctx *ctx);

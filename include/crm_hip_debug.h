@@ -24,7 +24,9 @@ extern "C" {
  *   15 scalar advection and sedimentation     (crm_module.F90:842)
  *   17 uvw                                    (crm_module.F90:846-850)
  *   18 sgs_scalars                            (crm_module.F90:854)
- *   20 micro_proc + diagnose                  (crm_module.F90:865-880)
+ *   19 micro_proc, only with mean-state acceleration on: before accelerate_crm
+ *                                             (crm_module.F90:865-870)
+ *   20 micro_proc + accelerate_crm + diagnose (crm_module.F90:865-880)
  * 0 runs the whole call. Views are host pointers as for crm_hip_run. */
 int crm_hip_debug_run(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col,
                       double dt_gl, const crm_input_view *in, crm_state_view *state,

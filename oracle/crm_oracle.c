@@ -1956,6 +1956,94 @@ static void diagnose(void) {
     }
 }
 
+/* Mean-state acceleration, accelerate_crm.F90 (Jones, Bretherton and Pritchard 2015).
+ * Settings of crm_accel_init (accelerate_crm.F90:16-42): use_crm_accel, crm_accel_factor
+ * (1 + factor = 'a'), crm_accel_uv. */
+static struct {
+  int use;
+  double factor;
+  int uv;
+} ACC = {0, 0.0, 0};
+
+void oracle_accel_init(int use_crm_accel, double crm_accel_factor, int crm_accel_uv) {
+  ACC.use = use_crm_accel;
+  ACC.factor = crm_accel_factor;
+  ACC.uv = crm_accel_uv;
+}
+
+/* accelerate_crm.F90:109-366 for one call inside the subcycle loop (after micro_proc, before
+ * diagnose): the change of the horizontal means of t and qt (and u, v) since the last
+ * diagnose, amplified by crm_accel_factor; qt clipped at zero level by level with the
+ * negative excess removed proportionally from the positive values; qv, qcl, qci re-split.
+ * Raises *ceaseflag (and changes nothing) when the mean t change exceeds 5 K anywhere. */
+static void accelerate_crm(int *ceaseflag) {
+  const int ncrms = S.ncrms, nx = S.nx, nzm = S.nzm;
+  const double coef = 1.0 / (double)(nx * 1);
+  const double ttend_threshold = 5., tmin = 50.;
+  const size_t n = (size_t)ncrms * (nzm + 1);
+  double *tt = (double *)calloc(n, sizeof(double)), *qq = (double *)calloc(n, sizeof(double));
+  double *uu = (double *)calloc(n, sizeof(double));
+  for (int c = 0; c < ncrms; c++)
+    for (int k = 1; k <= nzm; k++) {
+      double tb = 0, qtb = 0, ub = 0;
+      for (int i = 1; i <= nx; i++) {
+        tb += FLD(S.t, c, i, k) * coef;
+        qtb += (FLD(S.qcl, c, i, k) + FLD(S.qci, c, i, k) + FLD(S.qv, c, i, k)) * coef;
+        if (ACC.uv) ub += FLD(S.u, c, i, k) * coef;
+      }
+      size_t o = (size_t)c * (nzm + 1) + k;
+      tt[o] = tb - PR(S.t0, c, k);
+      qq[o] = qtb - PR(S.q0, c, k);
+      if (ACC.uv) uu[o] = ub - PR(S.u0, c, k);
+      if (fabs(tt[o]) > ttend_threshold) *ceaseflag = 1;
+    }
+  if (*ceaseflag) goto done;
+  for (int c = 0; c < ncrms; c++)
+    for (int k = 1; k <= nzm; k++) {
+      size_t o = (size_t)c * (nzm + 1) + k;
+      double qpoz = 0., qneg = 0.;
+      for (int i = 1; i <= nx; i++) {
+        FLD(S.t, c, i, k) = MAX(tmin, FLD(S.t, c, i, k) + ACC.factor * tt[o]);
+        if (ACC.uv) FLD(S.u, c, i, k) = FLD(S.u, c, i, k) + ACC.factor * uu[o];
+        FLD(S.qt, c, i, k) = FLD(S.qt, c, i, k) + ACC.factor * qq[o];
+      }
+      for (int i = 1; i <= nx; i++) {
+        if (FLD(S.qt, c, i, k) < 0.)
+          qneg = qneg + FLD(S.qt, c, i, k);
+        else
+          qpoz = qpoz + FLD(S.qt, c, i, k);
+      }
+      for (int i = 1; i <= nx; i++) {
+        if (qpoz + qneg <= 0.) {
+          FLD(S.qt, c, i, k) = 0.;
+          FLD(S.qv, c, i, k) = 0.;
+          FLD(S.qcl, c, i, k) = 0.;
+          FLD(S.qci, c, i, k) = 0.;
+        } else {
+          double factor = 1. + qneg / qpoz;
+          FLD(S.qt, c, i, k) = MAX(0., FLD(S.qt, c, i, k) * factor);
+          if (FLD(S.qt, c, i, k) <= 0.) {
+            FLD(S.qv, c, i, k) = 0.;
+            FLD(S.qcl, c, i, k) = 0.;
+            FLD(S.qci, c, i, k) = 0.;
+          } else {
+            double qt_res = FLD(S.qt, c, i, k) - FLD(S.qcl, c, i, k) - FLD(S.qci, c, i, k);
+            FLD(S.qv, c, i, k) = MAX(0., qt_res);
+            if (qt_res < 0.) {
+              factor = 1. + qt_res / (FLD(S.qcl, c, i, k) + FLD(S.qci, c, i, k));
+              FLD(S.qcl, c, i, k) = FLD(S.qcl, c, i, k) * factor;
+              FLD(S.qci, c, i, k) = FLD(S.qci, c, i, k) * factor;
+            }
+          }
+        }
+      }
+    }
+done:
+  free(tt);
+  free(qq);
+  free(uu);
+}
+
 /* crmsurface.F90:104-131 */
 static double z0_est(double z, double bflx, double wnd, double ustar) {
   const double vonk = R4(0.4), am = R4(4.8), bm = R4(19.3), eps = R4(1.e-10);
@@ -2164,6 +2252,15 @@ int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, cons
   S.nstop = (int)(dt_gl / S.dt);
   S.dt = dt_gl / S.nstop;
   double icrm_run_time = 1. / dt_gl;
+  /* crm_accel_nstop (accelerate_crm.F90:44-62), crm_module.F90:718-720 */
+  int accel_cease = 0;
+  if (ACC.use) {
+    if (S.nstop % (int)(1 + ACC.factor) != 0) {
+      status = CRM_ERR_ACCEL;
+      goto cleanup;
+    }
+    S.nstop = (int)(S.nstop / (1 + ACC.factor));
+  }
 
   /* main loop, crm_module.F90:727-1075 */
   S.nstep = 0;
@@ -2253,6 +2350,11 @@ int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, cons
       precip_proc();
       micro_diagnose();
       STOP_AT(19);
+      /* crm_module.F90:872-876 */
+      if (ACC.use && !accel_cease) {
+        accelerate_crm(&accel_cease);
+        if (accel_cease) S.nstop = (int)(S.nstop + (S.nstop - S.nstep + 1) * ACC.factor);
+      }
       diagnose();
       STOP_AT(20);
       int nn = S.na;
