@@ -17,7 +17,7 @@ from crm_views import (CrmColumnInfo, CrmDims, CrmInputView, CrmOutputView, CrmR
                        CrmStateView, views_from_addresses)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libcrm_hip.so")
+LIB_PATH = os.environ.get("CRM_HIP_LIB") or os.path.join(HERE, "lib", "libcrm_hip.so")
 
 CRM_OK, CRM_ERR_ARG, CRM_ERR_CFL, CRM_ERR_NXRAD, CRM_ERR_DAMPING = 0, 1, 2, 3, 4
 CRM_ERR_NAN, CRM_ERR_DEVICE, CRM_ERR_NOMEM, CRM_ERR_UNSUPPORTED, CRM_ERR_ACCEL = 5, 6, 7, 8, 9

@@ -11,6 +11,14 @@
 
 namespace CRM_NS {
 
+// Launch bounds of the segment kernels: 256 threads, and at least CRM_WAVES_PER_EU waves per
+// SIMD resident: the level sweeps are latency bound, and a second wave hides more than the
+// few spilled registers cost where the FP64 sweeps exceed the 256-register file of two waves.
+#ifndef CRM_WAVES_PER_EU
+#define CRM_WAVES_PER_EU 2
+#endif
+#define CRM_LB __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRM_WAVES_PER_EU)))
+
 // ------------------------------------------------------------------------------------------
 // Reference profiles from crm_input: crm_module.F90:320-375; grid factors sgs.F90:219-228;
 // sponge profile damping.F90:44-61; roughness crm_module.F90:583-590 (z0_est
@@ -138,7 +146,7 @@ __device__ __forceinline__ creal A_GR() { return 1. / (TGRMAX - TGRMIN); }
 // (microphysics.F90:160-245), sgs_init (sgs.F90:187-213).  Segment per instance.
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_init(DevT<R> d, crm_input_view in, crm_state_view st, crm_output_view out, double dt_gl) {
+__global__ void CRM_LB k_init(DevT<R> d, crm_input_view in, crm_state_view st, crm_output_view out, double dt_gl) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -316,7 +324,7 @@ __global__ void k_setperturb(DevT<R> d, const int* gcol) {
 // atomic max on the (non-negative) double's bits.
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_kurant(DevT<R> d) {
+__global__ void CRM_LB k_kurant(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -326,17 +334,29 @@ __global__ void __launch_bounds__(256) k_kurant(DevT<R> d) {
   const double* grdf = prof(d, P_GRDFX, c);
   const bool y3 = d.ny > 1;  // 3D: dy == dx
   const double dxf = sqrt((1. / d.dx) * (1. / d.dx) + (y3 ? (1. / d.dx) * (1. / d.dx) : 0.0));
+  // Every term is a monotone function of a level maximum over the columns, and rounding is
+  // monotone, so the maximum over columns is taken once, of the per-column maximum over
+  // levels: the same value as per-level column maxima (kurant.F90:30-60).
   double cfl = 0.0, wm_prev = 0.0, uhm_prev = 0.0;
+  auto W = [&](int k) { return (double)d.w[((size_t)c * nz + k) * NX + i]; };
+  auto U = [&](int k) { return (double)d.u[((size_t)c * nzm + k) * NX + i]; };
+  auto V = [&](int k) { return y3 ? (double)d.v[((size_t)c * nzm + k) * NX + i] : 0.0; };
+  auto T = [&](int k) { return (double)d.tkh[((size_t)c * nzm + k) * NX + i]; };
+  double w_n = W(0), u_n = U(0), v_n = V(0), t_n = T(0);  // read one level ahead
   for (int k = 0; k <= nzm; k++) {
     double wm = 0.0, uhm = 0.0;
     if (k < nzm) {
-      size_t o = ((size_t)c * nzm + k) * NX + i;
-      wm = s.max(fabs(d.w[((size_t)c * nz + k) * NX + i]));
-      double uu = d.u[o], vv = y3 ? (double)d.v[o] : 0.0;
-      uhm = s.max(sqrt(uu * uu + vv * vv));
-      double tkhmax = s.max(d.tkh[o]);
+      const double ww = w_n, uu = u_n, vv = v_n, tkh = t_n;
+      if (k + 1 < nzm) {
+        w_n = W(k + 1);
+        u_n = U(k + 1);
+        v_n = V(k + 1);
+        t_n = T(k + 1);
+      }
+      wm = fabs(ww);
+      uhm = sqrt(uu * uu + vv * vv);
       double dzz = dz * adzw[k];
-      double sgs = fmax(fmax(0.5 * tkhmax * 1.0 * d.dt / (dzz * dzz), 0.5 * tkhmax * grdf[k] * d.dt / (d.dx * d.dx)), 0.0);
+      double sgs = fmax(fmax(0.5 * tkh * 1.0 * d.dt / (dzz * dzz), 0.5 * tkh * grdf[k] * d.dt / (d.dx * d.dx)), 0.0);
       cfl = fmax(cfl, sgs);
     }
     if (k >= 1) {
@@ -346,6 +366,7 @@ __global__ void __launch_bounds__(256) k_kurant(DevT<R> d) {
     wm_prev = wm;
     uhm_prev = uhm;
   }
+  cfl = s.max(cfl);
   if (live && i == 0) atomic_max_nonneg(&d.gmax[0], cfl);
 }
 
@@ -355,7 +376,7 @@ __global__ void __launch_bounds__(256) k_kurant(DevT<R> d) {
 // damping.F90:9-117, ice_fall.F90:6-173.  Writes dudt(na), dwdt(na) from scratch.
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qrad) {
+__global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -400,11 +421,35 @@ __global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qr
   R fz_k = icefz(0);
   // level k-1 thermodynamics for the buoyancy average
   R pb_tabs = 0, pb_qv = 0, pb_qcl = 0, pb_qci = 0, pb_qpl = 0, pb_qpi = 0;
+  // the fields of one level, read one level ahead of their use
+  struct Lvl {
+    R tabs, qv, qcl, qci, qpl, qpi, u, w, t, qt, v;
+    double qr;
+  };
+  auto load = [&](int k, Lvl& L) {
+    size_t o = ((size_t)c * nzm + k) * NX + i;
+    L.tabs = d.tabs[o];
+    L.qv = d.qv[o];
+    L.qcl = d.qcl[o];
+    L.qci = d.qci[o];
+    L.qpl = d.qpl[o];
+    L.qpi = d.qpi[o];
+    L.u = d.u[o];
+    L.w = d.w[((size_t)c * nz + k) * NX + i];
+    L.t = d.t[o];
+    L.qt = d.qt[o];
+    L.v = y3 ? d.v[o] : R(0.0);
+    L.qr = qrad[(size_t)c + (size_t)d.ld * ((size_t)ir + (size_t)nxr * k)];
+  };
+  Lvl nxt;
+  load(0, nxt);
   for (int k = 0; k < nzm; k++) {
     size_t o = ((size_t)c * nzm + k) * NX + i;
-    R tabs = d.tabs[o], qv = d.qv[o], qcl = d.qcl[o], qci = d.qci[o], qpl = d.qpl[o], qpi = d.qpi[o];
-    R u = d.u[o], w = d.w[((size_t)c * nz + k) * NX + i];
-    R t = d.t[o], qt = d.qt[o];
+    const Lvl L = nxt;
+    if (k + 1 < nzm) load(k + 1, nxt);
+    R tabs = L.tabs, qv = L.qv, qcl = L.qcl, qci = L.qci, qpl = L.qpl, qpi = L.qpi;
+    R u = L.u, w = L.w;
+    R t = L.t, qt = L.qt;
     // buoyancy on w level k from scalar levels k-1 and k
     R dw = R(0.0);
     if (k >= 1) {
@@ -426,19 +471,20 @@ __global__ void __launch_bounds__(256) k_pre(DevT<R> d, int na, const double* qr
     // large-scale forcing with level-wise refill of negative vapour
     t = t + ttend[k] * dtn;
     qt = qt + qtend[k] * dtn;
-    bool neg = qt < R(0.);
-    R nneg = s.sum(neg ? R(1.0) : R(0.0));
-    R qneg = s.sum(neg ? qt : R(0.0));
-    R qpoz = s.sum(neg ? R(0.0) : qt);
-    R du = R(0.0) + utend[k];
-    R v = R(0.0), dv = R(0.0);
-    if (y3) {
-      v = d.v[o];
-      dv = dv + vtend[k];
+    // the level sums only matter when some column went negative; the vote is over the whole
+    // wave, so every lane of a segment takes the same branch
+    const bool neg = qt < R(0.);
+    if (__any(neg)) {
+      R nneg = s.sum(neg ? R(1.0) : R(0.0));
+      R qneg = s.sum(neg ? qt : R(0.0));
+      R qpoz = s.sum(neg ? R(0.0) : qt);
+      if (nneg > R(0.5) && qpoz + qneg > R(0.)) qt = fmax(R(0.0), qt * (R(1.) + qneg / qpoz));
     }
-    if (nneg > R(0.5) && qpoz + qneg > R(0.)) qt = fmax(R(0.0), qt * (R(1.) + qneg / qpoz));
+    R du = R(0.0) + utend[k];
+    R v = L.v, dv = R(0.0);
+    if (y3) dv = dv + vtend[k];
     // radiative heating on column groups
-    t = t + qrad[(size_t)c + (size_t)d.ld * ((size_t)ir + (size_t)nxr * k)] * dtn;
+    t = t + L.qr * dtn;
     // sponge layer
     if (k + 1 >= nzm - n_damp) {
       R tk = tau[k];
@@ -529,7 +575,7 @@ __device__ __forceinline__ R shear3(const G& s, int k, R dz, R dx, const R* adz,
 // tke_full.F90:11-317 (dosmagor).
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R, typename G = Seg<NX>>
-__global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
+__global__ void CRM_LB k_sgs(DevT<R> d) {
   G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -556,11 +602,27 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
   }
   R w0 = W(0), w1 = W(1);
   R bsv_prev = R(0.0);
-  // thermodynamics at k (carried) and k+1
+  // thermodynamics at k (carried) and k+1; level k+2 and tk at k+1 are read one iteration
+  // ahead of their use
   R th_tabs = F(d.tabs, 0), th_qcl = F(d.qcl, 0), th_qci = F(d.qci, 0), th_qv = F(d.qv, 0);
-  R th_qpl = F(d.qpl, 0), th_qpi = F(d.qpi, 0), th_t = F(d.t, 0), th_tkh = F(d.tkh, 0);
+  R th_qpl = F(d.qpl, 0), th_qpi = F(d.qpi, 0), th_t = F(d.t, 0);
+  R nx_tabs = F(d.tabs, 1), nx_qcl = F(d.qcl, 1), nx_qci = F(d.qci, 1), nx_qv = F(d.qv, 1);
+  R nx_qpl = F(d.qpl, 1), nx_qpi = F(d.qpi, 1), nx_t = F(d.t, 1);
+  R tk_cur = F(d.tk, 0);
   for (int k = 0; k < nzm - 1; k++) {
     int kc = k + 1, kb = k - 1;
+    const R n_tabs = nx_tabs, n_qcl = nx_qcl, n_qci = nx_qci, n_qv = nx_qv, n_qpl = nx_qpl, n_qpi = nx_qpi;
+    const R n_t = nx_t, tk_k = tk_cur;
+    if (kc + 1 < nzm) {
+      nx_tabs = F(d.tabs, kc + 1);
+      nx_qcl = F(d.qcl, kc + 1);
+      nx_qci = F(d.qci, kc + 1);
+      nx_qv = F(d.qv, kc + 1);
+      nx_qpl = F(d.qpl, kc + 1);
+      nx_qpi = F(d.qpi, kc + 1);
+      nx_t = F(d.t, kc + 1);
+    }
+    tk_cur = F(d.tk, kc);
     R up_next = (kc + 1 < nzm) ? F(d.u, kc + 1) : R(0.0);
     R vp_next = R(0.0);
     if constexpr (G::Y3) vp_next = (kc + 1 < nzm) ? F(d.v, kc + 1) : R(0.0);
@@ -596,8 +658,6 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
       }
     }
     // SGS buoyancy flux at the interface between k and k+1 (tke_full.F90:120-237)
-    R n_tabs = F(d.tabs, kc), n_qcl = F(d.qcl, kc), n_qci = F(d.qci, kc), n_qv = F(d.qv, kc);
-    R n_qpl = F(d.qpl, kc), n_qpi = F(d.qpi, kc), n_t = F(d.t, kc), n_tkh = F(d.tkh, kc);
     R bsv;
     {
       R betdz = R(0.5) * (bet[kc] + bet[k]) / dz / adzw[k + 1];
@@ -628,8 +688,6 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
         }
       }
     }
-    (void)th_tkh;
-    (void)n_tkh;
     // eddy viscosity at level k (tke_full.F90:246-308)
     {
       R grd = dz * adz[k];
@@ -645,7 +703,7 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
       if (buoy <= R(0.))
         smix = grd;
       else
-        smix = fmin(grd, fmax(R4(0.1) * grd, sqrt(R4(0.76) * d.tk[o] / Ck / sqrt(buoy + R4(1.e-10)))));
+        smix = fmin(grd, fmax(R4(0.1) * grd, sqrt(R4(0.76) * tk_k / Ck / sqrt(buoy + R4(1.e-10)))));
       R ratio = smix / grd;
       R Cee = Ce1 + Ce2 * ratio;
       R tk = sqrt(Ck * Ck * Ck / Cee * fmax(R(0.0), def2 - Pr * buoy)) * (smix * smix);
@@ -665,7 +723,6 @@ __global__ void __launch_bounds__(256) k_sgs(DevT<R> d) {
     th_qpl = n_qpl;
     th_qpi = n_qpi;
     th_t = n_t;
-    th_tkh = n_tkh;
     um = u;
     u = up;
     up = up_next;
@@ -710,10 +767,16 @@ __device__ __forceinline__ void mom2(const DevT<R>& d, int na) {
   R fwv_k = R(0.0);    // vertical SGS w flux at w level k
   R pend_adv = R(0.0), pend_dif = R(0.0);  // horizontal contributions for w level k
   R uwsb_sum = R(0.0);
+  // tendencies of the next level, read one iteration ahead
+  R du_pre = dudt[((size_t)c * nzm) * NX + i];
+  R dw_pre = dwdt[((size_t)c * nz + 1) * NX + i];
   for (int k = 0; k < nzm; k++) {
     int kn = k + 1;
     bool has_n = kn < nzm;
     int kcu = has_n ? kn : k;
+    const R du_k = du_pre, dw_k = dw_pre;
+    if (has_n) du_pre = dudt[((size_t)c * nzm + kn) * NX + i];
+    if (k >= 1 && has_n) dw_pre = dwdt[((size_t)c * nz + kn) * NX + i];
     R u_nn = (kn + 1 < nzm) ? F(d.u, kn + 1) : R(0.0);
     R w_nn = (kn + 1 < nz) ? W(kn + 1) : R(0.0);
     R tk_nn = (kn + 1 < nzm) ? F(d.tk, kn + 1) : R(0.0);
@@ -725,7 +788,7 @@ __device__ __forceinline__ void mom2(const DevT<R>& d, int na) {
     R tk_m = s.rot(tk_k, -1), tkn_m = s.rot(tk_n, -1);
     // ---- dudt(k)
     size_t o = ((size_t)c * nzm + k) * NX + i;
-    R du = dudt[o];
+    R du = du_k;
     R fu1 = dx25 * (u_k + u_m) * (u_m + u_k);
     R fu2 = dx25 * (u_p + u_k) * (u_k + u_p);
     du = du - (fu2 - fu1);
@@ -765,7 +828,7 @@ __device__ __forceinline__ void mom2(const DevT<R>& d, int na) {
     R fwz_k = dz25 * (w_n * rhow[kn] + w_k * rhow[k]) * (w_n + w_k);
     if (k >= 1) {
       size_t ow = ((size_t)c * nz + k) * NX + i;
-      R dw = dwdt[ow];
+      R dw = dw_k;
       R rhoiw = R(1.) / (rhow[k] * adzw[k]);
       dw = dw - pend_adv;
       dw = dw - (fwz_k - fwz_prev) * rhoiw;
@@ -844,10 +907,19 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
   R fuv_k = R(0.0), fvv_k = R(0.0);  // vertical SGS fluxes at the bottom of level k
   R fwv_k = R(0.0);
   R pax = R(0.0), pay = R(0.0), pdx = R(0.0), pdy = R(0.0);  // horizontal terms for w level k
+  // tendencies of the next level, read one iteration ahead
+  R du_pre = dudt[((size_t)c * nzm) * NX + i], dv_pre = dvdt[((size_t)c * nzm) * NX + i];
+  R dw_pre = dwdt[((size_t)c * nz + 1) * NX + i];
   for (int k = 0; k < nzm; k++) {
     const int kn = k + 1;
     const bool has_n = kn < nzm;
     const int kcu = has_n ? kn : k;
+    const R du_k = du_pre, dv_k = dv_pre, dw_k = dw_pre;
+    if (has_n) {
+      du_pre = dudt[((size_t)c * nzm + kn) * NX + i];
+      dv_pre = dvdt[((size_t)c * nzm + kn) * NX + i];
+    }
+    if (k >= 1 && has_n) dw_pre = dwdt[((size_t)c * nz + kn) * NX + i];
     R u_nn = (kn + 1 < nzm) ? F(d.u, kn + 1) : R(0.0);
     R v_nn = (kn + 1 < nzm) ? F(d.v, kn + 1) : R(0.0);
     R w_nn = (kn + 1 < nz) ? W(kn + 1) : R(0.0);
@@ -894,7 +966,7 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
     }
     // ---- dudt(k)
     size_t o = ((size_t)c * nzm + k) * NX + i;
-    R du = dudt[o];
+    R du = du_k;
     {
       R fu1 = dx25 * (u_k + u_m) * (u_m + u_k);
       R fu2 = dx25 * (u_p + u_k) * (u_k + u_p);
@@ -911,7 +983,7 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
       du = du - (fuv_n - fuv_k) * rhoi_k;
     }
     // ---- dvdt(k)
-    R dv = dvdt[o];
+    R dv = dv_k;
     {
       R fv1 = dx25 * (u_k + u_jm) * (v_m + v_k);
       R fv2 = dx25 * (u_p + u_p_jm) * (v_k + v_p);
@@ -935,7 +1007,7 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
     R fwz_k = dz25 * (w_n * rhow[kn] + w_k * rhow[k]) * (w_n + w_k);
     if (k >= 1) {
       size_t ow = ((size_t)c * nz + k) * NX + i;
-      R dw = dwdt[ow];
+      R dw = dw_k;
       R rhoiw = R(1.) / (rhow[k] * adzw[k]);
       dw = dw - pax;
       dw = dw - pay;
@@ -996,7 +1068,7 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
 }
 
 template <int NX, typename R, typename G = Seg<NX>>
-__global__ void __launch_bounds__(256) k_mom(DevT<R> d, int na) {
+__global__ void CRM_LB k_mom(DevT<R> d, int na) {
   if constexpr (G::Y3)
     mom3<R>(d, na);
   else
@@ -1011,7 +1083,7 @@ __global__ void __launch_bounds__(256) k_mom(DevT<R> d, int na) {
 // sweep runs on wave 0 with its factor in registers (NZM compile-time).
 // ------------------------------------------------------------------------------------------
 template <int NX, int NZM, typename R>
-__global__ void __launch_bounds__(256) k_press(DevT<R> d, int na, int nb, int nc) {
+__global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
   extern __shared__ double slab[];  // [64/NX instances][NZM][NX] == [NZM][64] per lane column
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane & (NX - 1);
@@ -1154,7 +1226,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PLD = 66;  // slab row stride in doubles: conflict-free A-fragment reads
 
 template <int NZM, typename R, typename G = Seg<64>>
-__global__ void __launch_bounds__(256) k_press64(DevT<R> d, int na, int nb, int nc) {
+__global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   constexpr int NX = 64, NK = 64;
   extern __shared__ double lds[];
   double* P = lds;              // [NK][PLD] slab: rhs, coefficients, pressure
@@ -1584,7 +1656,7 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
 
 // advect_all_scalars.F90:7-94: t, then the two microphysics fields.
 template <int NX, typename R, typename G = Seg<NX>>
-__global__ void __launch_bounds__(256) k_mpdata(DevT<R> d) {
+__global__ void CRM_LB k_mpdata(DevT<R> d) {
   G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -1599,7 +1671,7 @@ __global__ void __launch_bounds__(256) k_mpdata(DevT<R> d) {
 // Per column, one pipelined sweep per sub-step.
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_precip(DevT<R> d) {
+__global__ void CRM_LB k_precip(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -1795,7 +1867,7 @@ __device__ void diffuse_one(const DevT<R>& d, const G& s, int c, bool live, R* f
 }
 
 template <int NX, typename R, typename G = Seg<NX>>
-__global__ void __launch_bounds__(256) k_diffuse(DevT<R> d) {
+__global__ void CRM_LB k_diffuse(DevT<R> d) {
   G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -1927,7 +1999,7 @@ __device__ __forceinline__ void accel_means(const DevT<R>& d, const S_& s, int c
 }
 
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_micro(DevT<R> d, int accel) {
+__global__ void CRM_LB k_micro(DevT<R> d, int accel) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -1944,13 +2016,25 @@ __global__ void __launch_bounds__(256) k_micro(DevT<R> d, int accel) {
   const R pows1 = (3 + B_SNOW) / R(4.), pows2 = (5 + B_SNOW) / R(8.);
   const R powg1 = (3 + B_GRAU) / R(4.), powg2 = (5 + B_GRAU) / R(8.);
   int kmin = nzm, kmax = -1;
+  // t, qt, qp (and the winds for the level means) of the next level, read one level ahead
+  const size_t o0 = ((size_t)c * nzm) * NX + i;
+  R t_pre = d.t[o0], qt_pre = d.qt[o0], qp_pre = d.qp[o0], u_pre = d.u[o0];
+  R v_pre = y3 ? d.v[o0] : R(0.0);
   for (int k = 0; k < nzm; k++) {
     size_t o = ((size_t)c * nzm + k) * NX + i;
     auto CO = [&](int q) { return d.coef[((size_t)q * d.ncrms + c) * d.nz + k]; };
     const R pres = presp[k];
-    R t = d.t[o];
-    R q = fmax(R(0.0), d.qt[o]);
-    R qp = d.qp[o];
+    R t = t_pre;
+    R q = fmax(R(0.0), qt_pre);
+    R qp = qp_pre;
+    const R u_k = u_pre, v_k = v_pre;
+    if (k + 1 < nzm) {
+      t_pre = d.t[o + NX];
+      qt_pre = d.qt[o + NX];
+      qp_pre = d.qp[o + NX];
+      u_pre = d.u[o + NX];
+      if (y3) v_pre = d.v[o + NX];
+    }
     // cloud(): saturation adjustment with Newton iterations
     R tabs = t - gamaz[k];
     R tabs1 = (tabs + fac1 * qp) / (R(1.) + fac2 * qp);
@@ -2099,8 +2183,7 @@ __global__ void __launch_bounds__(256) k_micro(DevT<R> d, int accel) {
         d.qpi[o] = qpi;
       }
     } else {
-      diagnose_level<NX, R>(d, s, c, live, k, o, y3, t, q, qp, qn, qv, qcl, qci, qpl, qpi, d.u[o],
-                            y3 ? d.v[o] : R(0.0), kmin, kmax);
+      diagnose_level<NX, R>(d, s, c, live, k, o, y3, t, q, qp, qn, qv, qcl, qci, qpl, qpi, u_k, v_k, kmin, kmax);
     }
   }
   if (accel) return;
@@ -2121,7 +2204,7 @@ __global__ void __launch_bounds__(256) k_micro(DevT<R> d, int accel) {
 // level with the excess removed from the positive values, and qv, qcl, qci re-split.
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_accel(DevT<R> d, double factor, int uv) {
+__global__ void CRM_LB k_accel(DevT<R> d, double factor, int uv) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -2183,7 +2266,7 @@ __global__ void __launch_bounds__(256) k_accel(DevT<R> d, double factor, int uv)
 // profiles indexed by CRM level (w level for the mass fluxes) and per-instance sums.
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_stats(DevT<R> d) {
+__global__ void CRM_LB k_stats(DevT<R> d) {
   Seg<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -2197,14 +2280,37 @@ __global__ void __launch_bounds__(256) k_stats(DevT<R> d) {
   const int nxr = d.nx_rad, g = NX / nxr;
   auto Q = [&](const R* p, int k) -> double { return p[((size_t)c * nzm + k) * NX + i]; };
   auto Wv = [&](int k) -> double { return d.w[((size_t)c * nz + k) * NX + i]; };
+  // the fields of one level; the sweep runs down from the top and reads level k-1 while it
+  // works on level k
+  struct Lvl {
+    double qcl, qci, qpl, qpi, tabs, qv, w;
+  };
+  auto load = [&](int k, Lvl& L) {
+    L.qcl = Q(d.qcl, k);
+    L.qci = Q(d.qci, k);
+    L.qpl = Q(d.qpl, k);
+    L.qpi = Q(d.qpi, k);
+    L.tabs = Q(d.tabs, k);
+    L.qv = Q(d.qv, k);
+    L.w = Wv(k);
+  };
   double cwp = 0, cwph = 0, cwpm = 0, cwpl = 0, clt = 0, cmt = 0, cht = 0, ctt = 0;
   bool flag_top = true;
-  // level k+1 values (above) for the downdraft test, level k-1 read ahead for the updraft test
   double w_up = Wv(nzm);  // lid
-  double qc_up = 0, qi_up = 0, qpl_up = 0, qpi_up = 0, tabs_up = 0;
+  Lvl up{}, cur, nxt{};
+  load(nzm - 1, cur);
+  // radiation sums: exactly one lane owns each (instance, level, radiation column), so they
+  // are plain read-modify-writes; the reads are issued at the top of the level
+  const bool rad_owner = live && (i & (g - 1)) == 0;
+  const size_t rad_blk = (size_t)d.ncrms * nzm * nxr;
   for (int k = nzm - 1; k >= 0; k--) {
-    double qcl = Q(d.qcl, k), qci = Q(d.qci, k), qpl = Q(d.qpl, k), qpi = Q(d.qpi, k), tabs = Q(d.tabs, k);
-    double wk = Wv(k);
+    if (k >= 1) load(k - 1, nxt);
+    const size_t rad_base = (size_t)c * nzm * nxr + (size_t)k * nxr + i / g;
+    double ra[5] = {0, 0, 0, 0, 0};
+    if (rad_owner)
+      for (int q = 0; q < 5; q++) ra[q] = d.rad[q * rad_blk + rad_base];
+    const double qcl = cur.qcl, qci = cur.qci, qpl = cur.qpl, qpi = cur.qpi, tabs = cur.tabs;
+    const double wk = cur.w;
     // column water paths from the top (the reference's level nz-k)
     double tmp1 = rho[k] * adz[k] * dz * (qcl + qci);
     cwp += tmp1;
@@ -2246,28 +2352,31 @@ __global__ void __launch_bounds__(256) k_stats(DevT<R> d) {
     // mass fluxes through w level k (crm_module.F90:1017-1049)
     double mui = 0, mdi = 0;
     if (wk > 0.) {
-      int kx = k >= 1 ? k - 1 : 0;
-      double qcx = k >= 1 ? Q(d.qcl, kx) : qcl, qix = k >= 1 ? Q(d.qci, kx) : qci;
-      double tx = k >= 1 ? Q(d.tabs, kx) : tabs;
-      double qs = qsatw(tx, pres[kx]);
-      if (qcx + qix > fmin(R4(1.e-5), 0.01 * qs)) mui = rhow[k] * wk;
+      const Lvl& X = k >= 1 ? nxt : cur;
+      const int kx = k >= 1 ? k - 1 : 0;
+      double qs = qsatw(X.tabs, pres[kx]);
+      if (X.qcl + X.qci > fmin(R4(1.e-5), 0.01 * qs)) mui = rhow[k] * wk;
     } else if (wk < 0.) {
-      int kx = k + 1 < nzm ? k + 1 : nzm - 1;
-      double qcx = k + 1 < nzm ? qc_up : qcl, qix = k + 1 < nzm ? qi_up : qci;
-      double qplx = k + 1 < nzm ? qpl_up : qpl, qpix = k + 1 < nzm ? qpi_up : qpi;
-      double tx = k + 1 < nzm ? tabs_up : tabs;
-      double qs = qsatw(tx, pres[kx]);
-      if (qcx + qix > fmin(R4(1.e-5), 0.01 * qs))
+      const Lvl& X = k + 1 < nzm ? up : cur;
+      const int kx = k + 1 < nzm ? k + 1 : nzm - 1;
+      double qs = qsatw(X.tabs, pres[kx]);
+      if (X.qcl + X.qci > fmin(R4(1.e-5), 0.01 * qs))
         mdi = rhow[k] * wk;
-      else if (qplx + qpix > R4(1.0e-4))
+      else if (X.qpl + X.qpi > R4(1.0e-4))
         mdi = rhow[k] * wk;
     }
     double s_top = s.sum(top_hit), s_cld = s.sum(cld), s_mcup = s.sum(mcup), s_mcdn = s.sum(mcdn);
     double s_mcuup = s.sum(mcuup), s_mcudn = s.sum(mcudn), s_liq = s.sum(qcl), s_ice = s.sum(qci);
     double s_mui = s.sum(mui), s_mdi = s.sum(mdi);
-    // radiation-group sums (crm_module.F90:979-1015)
-    double r_t = s.gsum(tabs, g), r_qv = s.gsum(fmax(0.0, Q(d.qv, k)), g), r_qc = s.gsum(qcl, g);
-    double r_qi = s.gsum(qci, g), r_cld = s.gsum((qcl + qci > 0) ? 1.0 : 0.0, g);
+    // radiation-column sums (crm_module.F90:979-1015)
+    double r_t = tabs, r_qv = fmax(0.0, cur.qv), r_qc = qcl, r_qi = qci, r_cld = (qcl + qci > 0) ? 1.0 : 0.0;
+    if (g > 1) {
+      r_t = s.gsum(r_t, g);
+      r_qv = s.gsum(r_qv, g);
+      r_qc = s.gsum(r_qc, g);
+      r_qi = s.gsum(r_qi, g);
+      r_cld = s.gsum(r_cld, g);
+    }
     if (live) {
       if (i == 0) {
         accum(&prof(d, P_CLDTOP, c)[nzm - 1 - k], s_top);  // the reference's index mix (crm_module.F90:918-925)
@@ -2281,23 +2390,17 @@ __global__ void __launch_bounds__(256) k_stats(DevT<R> d) {
         accum(&prof(d, P_MUI, c)[k], s_mui);
         accum(&prof(d, P_MDI, c)[k], s_mdi);
       }
-      if ((i & (g - 1)) == 0) {
-        int ir = i / g;
-        size_t base = (size_t)c * nzm * nxr + (size_t)k * nxr + ir;
-        size_t blk = (size_t)d.ncrms * nzm * nxr;
-        accum(&d.rad[0 * blk + base], r_t);
-        accum(&d.rad[1 * blk + base], r_qv);
-        accum(&d.rad[2 * blk + base], r_qc);
-        accum(&d.rad[3 * blk + base], r_qi);
-        accum(&d.rad[4 * blk + base], r_cld);
+      if (rad_owner) {
+        d.rad[0 * rad_blk + rad_base] = ra[0] + r_t;
+        d.rad[1 * rad_blk + rad_base] = ra[1] + r_qv;
+        d.rad[2 * rad_blk + rad_base] = ra[2] + r_qc;
+        d.rad[3 * rad_blk + rad_base] = ra[3] + r_qi;
+        d.rad[4 * rad_blk + rad_base] = ra[4] + r_cld;
       }
     }
     w_up = wk;
-    qc_up = qcl;
-    qi_up = qci;
-    qpl_up = qpl;
-    qpi_up = qpi;
-    tabs_up = tabs;
+    up = cur;
+    cur = nxt;
   }
   double a = s.sum(cwp > thr ? ctt : 0.0), b = s.sum(cwph > thr ? cht : 0.0);
   double m = s.sum(cwpm > thr ? cmt : 0.0), l = s.sum(cwpl > thr ? clt : 0.0);
@@ -2313,7 +2416,7 @@ __global__ void __launch_bounds__(256) k_stats(DevT<R> d) {
 // End of call: crm_module.F90:1078-1659.  State write-back and every crm_output member.
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
-__global__ void __launch_bounds__(256) k_final(DevT<R> d, crm_input_view in, crm_state_view st, crm_rad_view rad, crm_output_view out,
+__global__ void CRM_LB k_final(DevT<R> d, crm_input_view in, crm_state_view st, crm_rad_view rad, crm_output_view out,
                         double dt_gl, int nstop) {
   Seg<NX> s;
   int c;
