@@ -25,10 +25,13 @@ PRECISION = {"fp64": 0, "mixed": 1}
 
 # Every function include/crm_hip.h declares.
 ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_accel_init",
-               "crm_hip_get_stats", "crm_hip_finalize", "crm_hip_errstr"]
+               "crm_hip_run_many", "crm_hip_set_schedule", "crm_hip_get_schedule",
+               "crm_hip_autotune", "crm_hip_set_graphs", "crm_hip_get_stats", "crm_hip_finalize",
+               "crm_hip_errstr"]
 # include/crm_hip_debug.h
 DEBUG_SYMBOLS = ["crm_hip_debug_run", "crm_hip_debug_get", "crm_hip_fft991_operators",
-                 "crm_hip_profile_kernels", "crm_hip_kernel_time", "crm_hip_kernel_name"]
+                 "crm_hip_profile_kernels", "crm_hip_kernel_time", "crm_hip_kernel_name",
+                 "crm_hip_graph_launches"]
 # Kernel classes of one call (CRM_K_* of include/crm_hip_debug.h), in launch order.
 KERNELS = ["setup", "init", "setperturb", "kurant", "precip_init", "pre", "sgs", "mom", "press",
            "mpdata", "precip", "diffuse", "micro", "stats", "final"]
@@ -41,6 +44,11 @@ class CrmHipConfig(C.Structure):
 class CrmHipStats(C.Structure):
     _fields_ = [("nstop", C.c_int), ("ncycle_max", C.c_int), ("nprec_max", C.c_int),
                 ("ms_device", C.c_double), ("accel_ceased", C.c_int)]
+
+
+class CrmClassDesc(C.Structure):
+    _fields_ = [("dims", CrmDims), ("col", CrmColumnInfo), ("inp", CrmInputView),
+                ("state", CrmStateView), ("rad", CrmRadView), ("out", CrmOutputView)]
 
 
 class CrmError(RuntimeError):
@@ -79,6 +87,15 @@ def load():
     lib.crm_hip_debug_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
     lib.crm_hip_get_stats.argtypes = [P, C.POINTER(CrmHipStats)]
     lib.crm_hip_accel_init.argtypes = [P, C.c_int, C.c_double, C.c_int]
+    IA = C.c_int * len(KERNELS)
+    lib.crm_hip_set_schedule.argtypes = [P, IA]
+    lib.crm_hip_get_schedule.argtypes = [P, IA]
+    lib.crm_hip_autotune.argtypes = views + [IA]
+    lib.crm_hip_set_graphs.argtypes = [P, C.c_int]
+    lib.crm_hip_graph_launches.argtypes = [P]
+    lib.crm_hip_graph_launches.restype = C.c_longlong
+    lib.crm_hip_run_many.argtypes = [C.c_int, C.POINTER(P), C.POINTER(CrmClassDesc), C.c_double,
+                                     C.POINTER(C.c_int)]
     lib.crm_hip_finalize.argtypes = [P]
     lib.crm_hip_errstr.argtypes = [C.c_int]
     lib.crm_hip_errstr.restype = C.c_char_p
@@ -165,6 +182,36 @@ class CrmHip:
         if rc:
             raise CrmError(rc, "crm_hip_accel_init")
 
+    def schedule(self):
+        """{kernel class: threads per workgroup} of this context."""
+        a = (C.c_int * len(KERNELS))()
+        self.lib.crm_hip_get_schedule(self.ctx, a)
+        return dict(zip(KERNELS, list(a)))
+
+    def set_schedule(self, sched):
+        cur = self.schedule()
+        cur.update(sched)
+        a = (C.c_int * len(KERNELS))(*[cur[k] for k in KERNELS])
+        rc = self.lib.crm_hip_set_schedule(self.ctx, a)
+        if rc:
+            raise CrmError(rc, "crm_hip_set_schedule")
+
+    def autotune(self, dcase, dt_gl):
+        """Tune the workgroup size of every kernel class on a DeviceCase (advanced in place by
+        the trial calls: pass a scratch copy); returns the chosen schedule."""
+        a = (C.c_int * len(KERNELS))()
+        rc = self._call(self.lib.crm_hip_autotune, dcase.case.dims(), dcase.case.column_info(),
+                        dt_gl, dcase.views(), a)
+        if rc:
+            raise CrmError(rc, "crm_hip_autotune")
+        return dict(zip(KERNELS, list(a)))
+
+    def set_graphs(self, on):
+        self.lib.crm_hip_set_graphs(self.ctx, int(bool(on)))
+
+    def graph_launches(self):
+        return int(self.lib.crm_hip_graph_launches(self.ctx))
+
     def debug_run(self, case, dt_gl, phase_stop):
         return self._call(self.lib.crm_hip_debug_run, case.dims(), case.column_info(), dt_gl,
                           case.views(), C.c_int(phase_stop))
@@ -232,3 +279,24 @@ class DeviceCase:
             for k, t in self.arrays[group].items():
                 dst[k][...] = t.cpu().numpy().T
         return case
+
+
+def run_many(hips, dcases, dt_gl, check=True):
+    """Advance several DeviceCases (one per CrmHip context, uneven size classes) by dt_gl
+    concurrently (crm_hip_run_many); returns the per-class status codes."""
+    lib = load()
+    n = len(hips)
+    descs = (CrmClassDesc * n)()
+    keep = []
+    for j, (h, dc) in enumerate(zip(hips, dcases)):
+        dims, col = dc.case.dims(), dc.case.column_info()
+        vi, vs, vr, vo = dc.views()
+        keep.append((dims, col, vi, vs, vr, vo))
+        descs[j].dims, descs[j].col = dims, col
+        descs[j].inp, descs[j].state, descs[j].rad, descs[j].out = vi, vs, vr, vo
+    ctxs = (C.c_void_p * n)(*[h.ctx.value for h in hips])
+    status = (C.c_int * n)()
+    rc = lib.crm_hip_run_many(n, ctxs, descs, C.c_double(dt_gl), status)
+    if rc and check:
+        raise CrmError(rc, "crm_hip_run_many")
+    return list(status)

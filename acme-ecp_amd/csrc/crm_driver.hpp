@@ -75,6 +75,19 @@ struct Engine : CrmEngine {
   // mean-state acceleration (crm_hip_accel_init)
   int acc_use = 0, acc_uv = 0;
   double acc_factor = 0.0;
+  // launch schedule: threads per workgroup of each segment-kernel class (a multiple of 64 up
+  // to 256; instances per workgroup = block / columns), set by the autotuner
+  int block[CRM_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256};
+  // HIP graphs of the steady substep (one subcycle, constant Adams-Bashforth coefficients),
+  // keyed by the kernel arguments: the buffer rotation of the Adams-Bashforth scheme repeats
+  // every 12 substeps, so a call reuses at most 12 graphs, and later calls reuse them too
+  bool use_graphs = true;
+  std::map<std::string, hipGraphExec_t> graphs;
+  long long graph_launches = 0;
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
 
   ~Engine() override;
   int run(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
@@ -91,6 +104,23 @@ struct Engine : CrmEngine {
     return CRM_OK;
   }
   int kernel_time(int kid, double* ms, long long* n) const override;
+  int set_schedule(const int* blk) override {
+    for (int k = 0; k < CRM_K_COUNT; k++)
+      if (blk[k] < 64 || blk[k] > 256 || blk[k] % 64) return CRM_ERR_ARG;
+    for (int k = 0; k < CRM_K_COUNT; k++) block[k] = blk[k];
+    drop_graphs();
+    return CRM_OK;
+  }
+  void get_schedule(int* blk) const override {
+    for (int k = 0; k < CRM_K_COUNT; k++) blk[k] = block[k];
+  }
+  void set_graphs(bool on) override {
+    use_graphs = on;
+    if (!on) drop_graphs();
+  }
+  long long graph_count() const override { return graph_launches; }
+  int autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+               crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) override;
 };
 
 // Start timing one launch of kernel class kid; returns the pair index or -1.
@@ -194,6 +224,7 @@ static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr, b
                           y3 ? NSCAL3 * scal * sizeof(R) : 0};             // v, dvdt x3
   size_t bytes = 0;
   for (size_t b : sizes) bytes += al(b);
+  ctx->drop_graphs();  // their kernel arguments name the old buffers
   if (ctx->pool) HIPCHECK(hipFree(ctx->pool));
   ctx->pool = nullptr;
   if (hipMalloc(&ctx->pool, bytes) != hipSuccess) {
@@ -282,8 +313,11 @@ static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, cons
 struct Launcher {
   int nx, nzm, ncrms;
   hipStream_t s;
-  dim3 grid_seg() const { return dim3((unsigned)(((size_t)ncrms * nx + 255) / 256)); }
+  const int* block;  // threads per workgroup of each kernel class
   dim3 grid_inst(int b) const { return dim3((unsigned)((ncrms + b - 1) / b)); }
+  // grid and workgroup of segment-kernel class kid
+  dim3 g(int kid) const { return dim3((unsigned)(((size_t)ncrms * nx + block[kid] - 1) / block[kid])); }
+  dim3 b(int kid) const { return dim3((unsigned)block[kid]); }
 };
 
 // NXV: columns per instance; GEO: the lane geometry (a 2D row of NXV columns, or the 8 x 8
@@ -386,13 +420,13 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   d.gmax = ctx->gmax;
   host_constants(d);
   hipStream_t s = ctx->stream;
-  Launcher L{nx, nzm, ncrms, s};
+  Launcher L{nx, nzm, ncrms, s, ctx->block};
   const dim3 b256(256);
   ctx->last_ok = false;
   ctx->stats = crm_hip_stats{};
 
   TIMED(CRM_K_SETUP, (k_setup<R><<<L.grid_inst(64), dim3(64), 0, s>>>(d, *in)));
-  TIMED(CRM_K_INIT, NX_DISPATCH(nx, (k_init<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *out, dt_gl))));
+  TIMED(CRM_K_INIT, NX_DISPATCH(nx, (k_init<NXV, R><<<L.g(CRM_K_INIT), L.b(CRM_K_INIT), 0, s>>>(d, *in, *st, *out, dt_gl))));
   if (col && col->igstep <= 1) {
     if (ctx->gcol_cap < ncrms) {
       if (ctx->gcol_dev) HIPCHECK(hipFree(ctx->gcol_dev));
@@ -453,7 +487,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     d.nstep = nstep;
     unsigned long long bits[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemsetAsync(ctx->gmax, 0, sizeof(unsigned long long), s));
-    TIMED(CRM_K_KURANT, NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+    TIMED(CRM_K_KURANT, NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.g(CRM_K_KURANT), L.b(CRM_K_KURANT), 0, s>>>(d))));
     HIPCHECK(hipMemcpyAsync(bits, ctx->gmax, sizeof(bits), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     cease(bits[3], nstep - 1);
@@ -467,6 +501,70 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     int ncycle = std::max(1, (int)std::ceil(q));
     ctx->stats.ncycle_max = std::max(ctx->stats.ncycle_max, ncycle);
     const bool last = (nstep == nstop);
+    // Steady substeps (one subcycle, constant Adams-Bashforth coefficients from the third
+    // substep on) replay a captured HIP graph of the whole substep, statistics included.
+    if (ctx->use_graphs && ncycle == 1 && nstep >= 3 && !acc && ctx->prof_mask == 0 && phase_stop == 0) {
+      d.icycle = 1;
+      d.dtn = dt;
+      dt3[na] = d.dtn;
+      d.dtfactor = 1.0;
+      abcoefs(nstep, dt3, na, nb, nc, d.at, d.bt, d.ct);
+      d.dt3na = dt3[na];
+      const double* qrad = rad->qrad;
+      auto body = [&]() {
+        k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d);
+        NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad)));
+        NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
+        NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na)));
+        NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
+        k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + 1);
+        NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d)));
+        NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d)));
+        std::swap(d.u, d.dudt[nc]);
+        std::swap(d.w, d.dwdt[nc]);
+        if (y3) std::swap(d.v, d.dvdt[nc]);
+        NX_DISPATCH(nx, (k_diffuse<NXV, R, GEO><<<L.g(CRM_K_DIFFUSE), L.b(CRM_K_DIFFUSE), 0, s>>>(d)));
+        NX_DISPATCH(nx, (k_micro<NXV, R><<<L.g(CRM_K_MICRO), L.b(CRM_K_MICRO), 0, s>>>(d, 0)));
+        int nn = na;
+        na = nc;
+        nc = nb;
+        nb = nn;
+        NX_DISPATCH(nx, (k_stats<NXV, R><<<L.g(CRM_K_STATS), L.b(CRM_K_STATS), 0, s>>>(d)));
+      };
+      DevT<R> dk;  // the key: every kernel argument of the substep
+      std::memcpy(&dk, &d, sizeof(d));
+      dk.nstep = 0;
+      dk.icycle = 0;
+      std::string key((const char*)&dk, sizeof(dk));
+      const int ints[4] = {na, nb, nc, nx};
+      key.append((const char*)ints, sizeof(ints));
+      key.append((const char*)&qrad, sizeof(qrad));
+      key.append((const char*)ctx->block, sizeof(ctx->block));
+      auto it = ctx->graphs.find(key);
+      hipGraphExec_t exec = nullptr;
+      if (it == ctx->graphs.end()) {
+        hipGraph_t graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        body();
+        HIPCHECK(hipStreamEndCapture(s, &graph));
+        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        ctx->graphs[key] = exec;
+      } else {
+        exec = it->second;
+        // the host side of the substep: buffer swap of uvw and the index rotation
+        std::swap(d.u, d.dudt[nc]);
+        std::swap(d.w, d.dwdt[nc]);
+        if (y3) std::swap(d.v, d.dvdt[nc]);
+        int nn = na;
+        na = nc;
+        nc = nb;
+        nb = nn;
+      }
+      HIPCHECK(hipGraphLaunch(exec, s));
+      ctx->graph_launches++;
+      continue;
+    }
     for (int icyc = 1; icyc <= ncycle; icyc++) {
       d.icycle = icyc;
       d.dtn = dt / ncycle;
@@ -478,30 +576,30 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         TIMED(CRM_K_PRECIP_INIT,
               (k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d)));
       const double* qrad = rad->qrad;
-      TIMED(CRM_K_PRE, NX_DISPATCH(nx, (k_pre<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, na, qrad))));
+      TIMED(CRM_K_PRE, NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad))));
       if (last && phase_stop == 7) goto stopped;
-      TIMED(CRM_K_SGS, NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d))));
+      TIMED(CRM_K_SGS, NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d))));
       if (last && phase_stop == 9) goto stopped;
-      TIMED(CRM_K_MOM, NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d, na))));
+      TIMED(CRM_K_MOM, NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na))));
       if (last && phase_stop == 11) goto stopped;
       TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
       if (last && phase_stop == 13) goto stopped;
-      HIPCHECK(hipMemsetAsync(ctx->gmax + 1, 0, sizeof(unsigned long long), s));
-      TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d))));
-      TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+      k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + 1);
+      TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d))));
+      TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d))));
       if (last && phase_stop == 15) goto stopped;
       std::swap(d.u, d.dudt[nc]);
       std::swap(d.w, d.dwdt[nc]);
       if (y3) std::swap(d.v, d.dvdt[nc]);
       if (last && phase_stop == 17) goto stopped;
-      TIMED(CRM_K_DIFFUSE, NX_DISPATCH(nx, (k_diffuse<NXV, R, GEO><<<L.grid_seg(), b256, 0, s>>>(d))));
+      TIMED(CRM_K_DIFFUSE, NX_DISPATCH(nx, (k_diffuse<NXV, R, GEO><<<L.g(CRM_K_DIFFUSE), L.b(CRM_K_DIFFUSE), 0, s>>>(d))));
       if (last && phase_stop == 18) goto stopped;
       {
         const int accel = (acc && !ceased) ? 1 : 0;
-        TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_micro<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, accel))));
+        TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_micro<NXV, R><<<L.g(CRM_K_MICRO), L.b(CRM_K_MICRO), 0, s>>>(d, accel))));
         if (accel && last && phase_stop == 19) goto stopped;  // before the acceleration
         if (accel)  // crm_module.F90:872-876
-          TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_accel<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, acc_f, ctx->acc_uv))));
+          TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_accel<NXV, R><<<L.g(CRM_K_MICRO), L.b(CRM_K_MICRO), 0, s>>>(d, acc_f, ctx->acc_uv))));
       }
       if (last && phase_stop == 20) goto stopped;
       int nn = na;
@@ -509,12 +607,12 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       nc = nb;
       nb = nn;
     }
-    TIMED(CRM_K_STATS, NX_DISPATCH(nx, (k_stats<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d))));
+    TIMED(CRM_K_STATS, NX_DISPATCH(nx, (k_stats<NXV, R><<<L.g(CRM_K_STATS), L.b(CRM_K_STATS), 0, s>>>(d))));
     HIPCHECK(hipGetLastError());
   }
   ctx->stats.nstop = nstop;
   ctx->stats.accel_ceased = ceased ? 1 : 0;
-  TIMED(CRM_K_FINAL, NX_DISPATCH(nx, (k_final<NXV, R><<<L.grid_seg(), b256, 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop))));
+  TIMED(CRM_K_FINAL, NX_DISPATCH(nx, (k_final<NXV, R><<<L.g(CRM_K_FINAL), L.b(CRM_K_FINAL), 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop))));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(ctx->ev1, s));
   HIPCHECK(hipMemcpyAsync(&err_flag, ctx->gmax + 2, sizeof(err_flag), hipMemcpyDeviceToHost, s));
@@ -677,6 +775,7 @@ Engine<R>::~Engine() {
     (void)hipFree(kv.second.Gt);
     (void)hipFree(kv.second.eign);
   }
+  drop_graphs();
   if (pool) (void)hipFree(pool);
   if (stage) (void)hipFree(stage);
   if (gmax) (void)hipFree(gmax);
@@ -794,6 +893,38 @@ int Engine<R>::debug_get(const char* name, double* buf) {
   return -1;
 }
 
+
+// Workgroup sizes per kernel class: one call per candidate size with every class timed on
+// the stream, then the fastest size of each class.  The views are scratch: each call advances
+// them.
+template <typename R>
+int Engine<R>::autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+                        crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) {
+  static const int cand[] = {64, 128, 256};
+  const unsigned saved_mask = prof_mask;
+  double t[3][CRM_K_COUNT] = {};
+  for (int c = 0; c < 3; c++) {
+    for (int k = 0; k < CRM_K_COUNT; k++) block[k] = cand[c];
+    drop_graphs();
+    profile_kernels((1u << CRM_K_COUNT) - 1);
+    int rc = run(dims, col, dt_gl, in, state, rad, out, true, 0);
+    if (rc) {
+      profile_kernels(saved_mask);
+      return rc;
+    }
+    for (int k = 0; k < CRM_K_COUNT; k++) t[c][k] = kt_n[k] ? kt_ms[k] / (double)kt_n[k] : 0.0;
+  }
+  for (int k = 0; k < CRM_K_COUNT; k++) {
+    int bc = 2;  // 256 unless a smaller workgroup measured faster
+    for (int c = 0; c < 3; c++)
+      if (t[c][k] > 0.0 && t[c][k] < t[bc][k]) bc = c;
+    block[k] = cand[bc];
+    if (best) best[k] = block[k];
+  }
+  drop_graphs();
+  profile_kernels(saved_mask);
+  return CRM_OK;
+}
 
 template <typename R>
 static CrmEngine* make_engine(int device, int* rc) {

@@ -15,6 +15,12 @@ struct CrmEngine {
   virtual int profile_kernels(unsigned mask) = 0;
   virtual int kernel_time(int kid, double* ms, long long* n) const = 0;
   virtual int accel_init(int use, double factor, int uv) = 0;
+  virtual int set_schedule(const int* block) = 0;
+  virtual void get_schedule(int* block) const = 0;
+  virtual void set_graphs(bool on) = 0;
+  virtual long long graph_count() const = 0;
+  virtual int autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+                       crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) = 0;
 };
 
 namespace crm {

@@ -1,6 +1,9 @@
 // crm_hip.hip -- the C ABI of include/crm_hip.h and include/crm_hip_debug.h, and the FP64
 // engine (crm_driver.hpp instantiated with FP64 bulk fields).  The mixed-precision engine is
 // compiled separately (crm_hip_mixed.hip); a context owns one engine of its precision.
+#include <thread>
+#include <vector>
+
 #include "crm_driver.hpp"
 #include "fft991_operator.hpp"
 
@@ -91,6 +94,68 @@ int crm_hip_run_device(crm_hip_ctx* ctx, const crm_dims* dims, const crm_column_
 int crm_hip_accel_init(crm_hip_ctx* ctx, int use_crm_accel, double crm_accel_factor, int crm_accel_uv) {
   if (!ctx || !ctx->eng) return CRM_ERR_ARG;
   return ctx->eng->accel_init(use_crm_accel, crm_accel_factor, crm_accel_uv);
+}
+
+static_assert(CRM_HIP_NKERNEL == CRM_K_COUNT, "kernel class count");
+
+int crm_hip_run_many(int n, crm_hip_ctx* const* ctxs, const crm_class_desc* classes, double dt_gl, int* status) {
+  if (n < 1 || !ctxs || !classes) return CRM_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i] || !ctxs[i]->eng) return CRM_ERR_ARG;
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return CRM_ERR_ARG;
+  }
+  std::vector<int> rcs(n, CRM_OK);
+  auto one = [&](int i) {
+    if (hipSetDevice(ctxs[i]->device) != hipSuccess) {
+      rcs[i] = CRM_ERR_DEVICE;
+      return;
+    }
+    crm_class_desc c = classes[i];
+    rcs[i] = ctxs[i]->eng->run(&c.dims, &c.col, dt_gl, &c.in, &c.state, &c.rad, &c.out, true, 0);
+  };
+  // one host thread per class: each drives its own stream, so the classes' kernels overlap on
+  // the device while every class keeps its per-substep host decisions
+  std::vector<std::thread> th;
+  for (int i = 1; i < n; i++) th.emplace_back(one, i);
+  one(0);
+  for (auto& t : th) t.join();
+  int first = CRM_OK;
+  for (int i = 0; i < n; i++) {
+    if (status) status[i] = rcs[i];
+    if (!first && rcs[i]) first = rcs[i];
+  }
+  return first;
+}
+
+int crm_hip_set_schedule(crm_hip_ctx* ctx, const int block[CRM_HIP_NKERNEL]) {
+  if (!ctx || !ctx->eng || !block) return CRM_ERR_ARG;
+  return ctx->eng->set_schedule(block);
+}
+
+int crm_hip_get_schedule(const crm_hip_ctx* ctx, int block[CRM_HIP_NKERNEL]) {
+  if (!ctx || !ctx->eng || !block) return CRM_ERR_ARG;
+  ctx->eng->get_schedule(block);
+  return CRM_OK;
+}
+
+int crm_hip_autotune(crm_hip_ctx* ctx, const crm_dims* dims, const crm_column_info* col, double dt_gl,
+                     const crm_input_view* in, crm_state_view* state, crm_rad_view* rad, crm_output_view* out,
+                     int block_out[CRM_HIP_NKERNEL]) {
+  if (!ctx || !ctx->eng) return CRM_ERR_ARG;
+  if (hipSetDevice(ctx->device) != hipSuccess) return CRM_ERR_DEVICE;
+  return ctx->eng->autotune(dims, col, dt_gl, in, state, rad, out, block_out);
+}
+
+int crm_hip_set_graphs(crm_hip_ctx* ctx, int on) {
+  if (!ctx || !ctx->eng) return CRM_ERR_ARG;
+  ctx->eng->set_graphs(on != 0);
+  return CRM_OK;
+}
+
+long long crm_hip_graph_launches(const crm_hip_ctx* ctx) {
+  if (!ctx || !ctx->eng) return -1;
+  return ctx->eng->graph_count();
 }
 
 int crm_hip_get_stats(const crm_hip_ctx* ctx, crm_hip_stats* stats) {

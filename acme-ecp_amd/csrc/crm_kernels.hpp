@@ -19,6 +19,12 @@ namespace CRM_NS {
 #endif
 #define CRM_LB __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRM_WAVES_PER_EU)))
 
+// Clears one word of the batch-global maxima (a kernel rather than a memset, so that it can
+// sit in a captured graph).
+__global__ void k_clear_u64(unsigned long long* p) {
+  if (threadIdx.x == 0) *p = 0ull;
+}
+
 // ------------------------------------------------------------------------------------------
 // Reference profiles from crm_input: crm_module.F90:320-375; grid factors sgs.F90:219-228;
 // sponge profile damping.F90:44-61; roughness crm_module.F90:583-590 (z0_est

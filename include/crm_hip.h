@@ -168,6 +168,47 @@ int crm_hip_run_device(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_
  * 872-876). Default: off. */
 int crm_hip_accel_init(crm_hip_ctx *ctx, int use_crm_accel, double crm_accel_factor, int crm_accel_uv);
 
+/* ---- Batches of uneven size classes, launch schedule, graphs -------------------------------
+ * These have no counterpart in the reference, whose crm() runs one compile-time size class per
+ * build (cam/crmdims.F90:13-22); they serve callers that advance several classes at once. */
+
+/* Kernel classes of one call, in launch order (CRM_K_* of crm_hip_debug.h). */
+#define CRM_HIP_NKERNEL 15
+
+/* One size class of crm_hip_run_many: the arguments of one crm_hip_run_device call. */
+typedef struct crm_class_desc {
+  crm_dims dims;
+  crm_column_info col;
+  crm_input_view in;
+  crm_state_view state;
+  crm_rad_view rad;
+  crm_output_view out;
+} crm_class_desc;
+
+/* Advance n size classes (uneven nx, nz, 2D and 3D) by dt_gl each, concurrently on the
+ * device: class i runs on context ctxs[i] (its own stream, buffers, schedule and graphs) with
+ * exactly the semantics of crm_hip_run_device(ctxs[i], &classes[i].dims, ...), including its
+ * own batch-global subcycle and sedimentation step counts. status[i] receives each class's
+ * code; the return value is the first nonzero one. Contexts must be distinct. */
+int crm_hip_run_many(int n, crm_hip_ctx *const *ctxs, const crm_class_desc *classes, double dt_gl,
+                     int *status);
+
+/* Workgroup size (64, 128, 192 or 256 threads) of each kernel class of a context. */
+int crm_hip_set_schedule(crm_hip_ctx *ctx, const int block[CRM_HIP_NKERNEL]);
+int crm_hip_get_schedule(const crm_hip_ctx *ctx, int block[CRM_HIP_NKERNEL]);
+
+/* Autotuner: runs the call once per candidate workgroup size (64, 128, 256) with every kernel
+ * class timed on the context's stream, keeps the fastest size per class for later calls of
+ * this context and writes it to block_out (may be NULL). The views are device pointers and are
+ * advanced by every trial call: pass scratch copies. */
+int crm_hip_autotune(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col, double dt_gl,
+                     const crm_input_view *in, crm_state_view *state, crm_rad_view *rad,
+                     crm_output_view *out, int block_out[CRM_HIP_NKERNEL]);
+
+/* Steady substeps (one subcycle, from the third substep on) replay captured HIP graphs of the
+ * whole substep; on by default. */
+int crm_hip_set_graphs(crm_hip_ctx *ctx, int on);
+
 int crm_hip_get_stats(const crm_hip_ctx *ctx, crm_hip_stats *stats);
 int crm_hip_finalize(crm_hip_ctx *ctx);
 const char *crm_hip_errstr(int code);

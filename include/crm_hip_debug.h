@@ -71,6 +71,9 @@ int crm_hip_kernel_time(const crm_hip_ctx *ctx, int kid, double *ms_total, long 
 
 const char *crm_hip_kernel_name(int kid);
 
+/* Number of substeps replayed from captured graphs since the context was created. */
+long long crm_hip_graph_launches(const crm_hip_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

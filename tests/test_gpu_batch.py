@@ -1,0 +1,131 @@
+"""Graph replay of steady substeps, launch schedules, the autotuner and concurrent batches of
+uneven size classes (crm_hip_run_many).  None of these may change a result: every check is
+bitwise equality with the plain eager call of the same class, plus one parity check of a
+batched class against the oracle."""
+import numpy as np
+import pytest
+
+import oracle_lib
+from crm_hip import KERNELS, CrmHip, DeviceCase, run_many
+from crm_inputs import make_case
+from parity import compare_case, convective_case
+
+pytestmark = pytest.mark.gpu
+
+CLASSES = [  # (ncrms, nx, ny, nz, dx, dt): the uneven-batch size classes, scaled down
+    (64, 16, 1, 28, 4000.0, 20.0),
+    (32, 32, 1, 28, 4000.0, 20.0),
+    (16, 64, 1, 58, 1000.0, 5.0),
+    (8, 8, 8, 28, 4000.0, 20.0),
+]
+
+
+def _case(cls, seed):
+    n, nx, ny, nz, dx, dt = cls
+    return convective_case(n, nx=nx, ny=ny, nz=nz, dx=dx, dt=dt, seed=seed)
+
+
+def _same(a, b):
+    for g in ("state", "rad", "out"):
+        for k, v in getattr(a, g).items():
+            assert np.array_equal(getattr(b, g)[k], v), (g, k)
+
+
+@pytest.mark.parametrize("cls", CLASSES, ids=lambda c: f"{c[0]}x{c[1]}x{c[2]}x{c[3]}")
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_graphs_replay_bitwise(cls, prec):
+    """Three consecutive calls (the graphs of the first are replayed by the later ones, and
+    each call replays its graphs many times) against the same calls run eagerly."""
+    case = _case(cls, seed=81)
+    eager, graphed = case.copy(), case.copy()
+    he, hg = CrmHip(0, prec), CrmHip(0, prec)
+    try:
+        he.set_graphs(False)
+        for _ in range(3):
+            he.run(eager, 10 * cls[5])
+            hg.run(graphed, 10 * cls[5])
+            _same(eager, graphed)
+        assert he.graph_launches() == 0
+        assert hg.graph_launches() > 0
+    finally:
+        he.close()
+        hg.close()
+
+
+def test_schedule_does_not_change_results():
+    case = _case(CLASSES[0], seed=82)
+    a, b = case.copy(), case.copy()
+    h = CrmHip(0)
+    try:
+        h.run(a, 100.0)
+        h.set_schedule({k: 64 for k in KERNELS})
+        assert set(h.schedule().values()) == {64}
+        h.run(b, 100.0)
+        _same(a, b)
+    finally:
+        h.close()
+
+
+def test_autotune_schedule():
+    """The tuner returns one of its candidate sizes for every class and leaves a context whose
+    results are those of the default schedule."""
+    cls = CLASSES[1]
+    case = _case(cls, seed=83)
+    h, ref = CrmHip(0), CrmHip(0)
+    try:
+        scratch = DeviceCase(case.copy())
+        sched = h.autotune(scratch, 3 * cls[5])
+        print(sched)
+        assert set(sched.values()) <= {64, 128, 256}
+        assert h.schedule() == sched
+        a, b = case.copy(), case.copy()
+        h.run(a, 100.0)
+        ref.run(b, 100.0)
+        _same(a, b)
+    finally:
+        h.close()
+        ref.close()
+
+
+def test_run_many_matches_separate_calls():
+    """Four uneven classes advanced concurrently, each equal to its own separate call."""
+    cases = [_case(c, seed=84 + j) for j, c in enumerate(CLASSES)]
+    alone = []
+    for c in cases:
+        h = CrmHip(0)
+        try:
+            a = c.copy()
+            h.run(a, 200.0)
+            alone.append(a)
+        finally:
+            h.close()
+    hips = [CrmHip(0) for _ in CLASSES]
+    try:
+        dcs = [DeviceCase(c.copy()) for c in cases]
+        status = run_many(hips, dcs, 200.0)
+        assert status == [0] * len(CLASSES)
+        for dc, a in zip(dcs, alone):
+            _same(dc.download(dc.case), a)
+    finally:
+        for h in hips:
+            h.close()
+
+
+def test_run_many_parity():
+    """Batched 2D classes against the oracle (FP64 bars: 1e-10 after one substep, 1e-8 after
+    several)."""
+    cls = [CLASSES[0], CLASSES[2]]
+    cases = [make_case(c[0], c[1], c[3], c[4], c[5], randomize=True, seed_base=90) for c in cls]
+    hips = [CrmHip(0) for _ in cls]
+    try:
+        dcs = [DeviceCase(c.copy()) for c in cases]
+        assert run_many(hips, dcs, 20.0) == [0, 0]
+        for dc, c in zip(dcs, cases):
+            want = c.copy()
+            assert oracle_lib.oracle().run(want, 20.0) == 0
+            errs = compare_case(dc.download(dc.case), want, dt_gl=20.0)
+            tol = 1e-10 if c.dt == 20.0 else 1e-8  # one substep, or four
+            assert max(errs.values()) <= tol, sorted(errs.items(), key=lambda kv: -kv[1])[:3]
+    finally:
+        for h in hips:
+            h.close()
