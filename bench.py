@@ -138,6 +138,52 @@ def measure(prec, case, args, torch):
                          "algorithmic_bytes_per_launch": bytes_launch}}
 
 
+# Uneven batch: one 1200 s GCM step of several CRM size classes advanced together
+# (crm_hip_run_many), each class with the workgroup schedule the autotuner chose for it.
+# (ncrms, nx, ny, nz, dx, dt).  The 256 x 128x58 class of this batch is not built (128-column
+# CRMs need more than one wave per instance) and is left out.
+UNEVEN = [(2048, 16, 1, 28, 4000.0, 20.0), (1024, 32, 1, 28, 4000.0, 20.0),
+          (1024, 64, 1, 58, 1000.0, 5.0), (128, 8, 8, 28, 4000.0, 20.0)]
+
+
+def measure_uneven(prec, args, torch):
+    from crm_hip import CrmHip, DeviceCase, run_many
+    dt_gl = WORKLOAD["dt_gl"]
+    cases = [make_case(n, nx, nz, dx, dt, ny=ny, randomize=True) for n, nx, ny, nz, dx, dt in UNEVEN]
+    hips = [CrmHip(0, prec) for _ in UNEVEN]
+    scheds = []
+    for h, c, cls in zip(hips, cases, UNEVEN):
+        scratch = DeviceCase(c.copy(), "cuda:0")
+        # tuning calls of 3 substeps on a scratch copy
+        sched = h.autotune(scratch, 3 * cls[5])
+        # classes with a fixed launch shape (one instance per workgroup for the pressure solve,
+        # one thread per instance or level for the set-up kernels) are not reported
+        scheds.append({k: v for k, v in sched.items()
+                       if k not in ("setup", "setperturb", "precip_init", "press")})
+        del scratch
+    dcs = [DeviceCase(c, "cuda:0") for c in cases]
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        run_many(hips, dcs, dt_gl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n in range(args.steps):
+        run_many(hips, dcs, dt_gl)
+        log(f"[uneven {prec}] step {n + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    colsteps = sum(n * nx * ny * int(dt_gl / dt) for n, nx, ny, nz, dx, dt in UNEVEN)
+    graphs = sum(h.graph_launches() for h in hips)
+    for h in hips:
+        h.close()
+    return {"value": colsteps * args.steps / sec, "unit": "column-steps/s",
+            "ms_per_step": 1e3 * sec / args.steps,
+            "classes": [f"{n}x{nx}x{ny}x{nz} dx {dx / 1000:g} km dt {dt:g} s" for n, nx, ny, nz, dx, dt in UNEVEN],
+            "schedule_threads_per_workgroup": scheds, "graph_substeps": graphs,
+            "note": "one 1200 s GCM step of every class per step, classes concurrent on their own "
+                    "streams; 128-column class not built"}
+
+
 def state_error(d64, d32, torch):
     """Normwise relative difference max|a-b|/max|a| of the mixed-precision state and
     domain-mean outputs against the FP64 ones, computed on the device."""
@@ -162,6 +208,8 @@ def main():
                     help="instances in the CPU baseline sample (0 skips it)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="skip the mixed-precision measurement")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-batch measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale for the HBM "
                          "byte counters)")
@@ -210,6 +258,10 @@ def main():
                          "reference profiles, statistics and outputs; error after the same "
                          f"{args.warmup + args.steps} GCM steps from the same initial state"}
 
+    uneven = None
+    if not args.no_uneven:
+        uneven = {p: measure_uneven(p, args, torch) for p in (("fp64",) if args.fp64_only else ("fp64", "mixed"))}
+
     cpu = None
     if args.cpu_ncrms > 0:
         log(f"CPU baseline on {args.cpu_ncrms} instances")
@@ -237,6 +289,7 @@ def main():
                            "bytes_per_column_substep": STATE_WORDS_PER_LEVEL * 8 * w["nz"]},
         "kernel_ms_per_step_warmup": r64["kernel_ms"],
         "mixed_precision": mixed,
+        "uneven_batch": uneven,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
