@@ -162,6 +162,14 @@ struct DevT {
   TermVel tv;
   Gam gam;
   double z0c1;                // 3.14159/2 - 3 ln 2 in REAL(4) (crmsurface.F90:125)
+  // microphysics network: inputs [point][9] and outputs [point][8] of this subcycle (null:
+  // plain microphysics everywhere)
+  const float *mlp_x, *mlp_y;
+  // (input, target) pairs of the plain microphysics, every samp_stride-th point
+  float *samp_x, *samp_t;
+  unsigned long long *samp_n;
+  long long samp_cap;
+  int samp_stride;
 };
 
 template <typename R>

@@ -82,6 +82,14 @@ struct Engine : CrmEngine {
   // keyed by the kernel arguments: the buffer rotation of the Adams-Bashforth scheme repeats
   // every 12 substeps, so a call reuses at most 12 graphs, and later calls reuse them too
   bool use_graphs = true;
+  // microphysics network and the recording of training pairs
+  const crm_mlp* mlp = nullptr;
+  float *mlp_x = nullptr, *mlp_y = nullptr;
+  size_t mlp_cap = 0;  // points
+  float *samp_x = nullptr, *samp_t = nullptr;
+  unsigned long long* samp_n = nullptr;
+  long long samp_cap = 0;
+  int samp_stride = 1;
   std::map<std::string, hipGraphExec_t> graphs;
   long long graph_launches = 0;
   void drop_graphs() {
@@ -119,6 +127,25 @@ struct Engine : CrmEngine {
     if (!on) drop_graphs();
   }
   long long graph_count() const override { return graph_launches; }
+  int set_mlp(const crm_mlp* m) override {
+    if (m && crm_mlp_ns::mlp_device(m) != device) return CRM_ERR_ARG;
+    mlp = m;
+    return CRM_OK;
+  }
+  int record_samples(float* X, float* T, long long cap, int stride) override {
+    if (X && (!T || cap < 0 || stride < 1)) return CRM_ERR_ARG;
+    if (!samp_n && hipMalloc(&samp_n, sizeof(unsigned long long)) != hipSuccess) return CRM_ERR_NOMEM;
+    samp_x = X;
+    samp_t = T;
+    samp_cap = cap;
+    samp_stride = stride;
+    return hipMemset(samp_n, 0, sizeof(unsigned long long)) == hipSuccess ? CRM_OK : CRM_ERR_DEVICE;
+  }
+  long long samples_count() const override {
+    unsigned long long n = 0;
+    if (!samp_n || hipMemcpy(&n, samp_n, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    return std::min((long long)n, samp_cap);
+  }
   int autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
                crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) override;
 };
@@ -457,6 +484,27 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     nstop = (int)(nstop / (1 + acc_f));
   }
   d.dt = dt;
+  // microphysics network / training-pair recording: per-point inputs (and outputs) of every
+  // subcycle
+  const size_t npts = (size_t)ncrms * nzm * nx;
+  const bool net_inputs = ctx->mlp != nullptr || ctx->samp_x != nullptr;
+  if (net_inputs && ctx->mlp_cap < npts) {
+    if (ctx->mlp_x) HIPCHECK(hipFree(ctx->mlp_x));
+    if (ctx->mlp_y) HIPCHECK(hipFree(ctx->mlp_y));
+    ctx->mlp_x = ctx->mlp_y = nullptr;
+    ctx->mlp_cap = 0;
+    if (hipMalloc(&ctx->mlp_x, npts * CRM_MLP_NIN * sizeof(float)) != hipSuccess ||
+        hipMalloc(&ctx->mlp_y, npts * 8 * sizeof(float)) != hipSuccess)
+      return CRM_ERR_NOMEM;
+    ctx->mlp_cap = npts;
+  }
+  d.mlp_x = net_inputs ? ctx->mlp_x : nullptr;
+  d.mlp_y = ctx->mlp ? ctx->mlp_y : nullptr;
+  d.samp_x = ctx->samp_x;
+  d.samp_t = ctx->samp_t;
+  d.samp_n = ctx->samp_n;
+  d.samp_cap = ctx->samp_cap;
+  d.samp_stride = ctx->samp_stride;
   double dt3[3] = {0.0, 0.0, 0.0};
   int na = 0, nb = 1, nc = 2;
   const dim3 gpress((unsigned)((ncrms + (64 / nx) - 1) / (64 / nx)));
@@ -503,7 +551,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     const bool last = (nstep == nstop);
     // Steady substeps (one subcycle, constant Adams-Bashforth coefficients from the third
     // substep on) replay a captured HIP graph of the whole substep, statistics included.
-    if (ctx->use_graphs && ncycle == 1 && nstep >= 3 && !acc && ctx->prof_mask == 0 && phase_stop == 0) {
+    if (ctx->use_graphs && ncycle == 1 && nstep >= 3 && !acc && !net_inputs && ctx->prof_mask == 0 &&
+        phase_stop == 0) {
       d.icycle = 1;
       d.dtn = dt;
       dt3[na] = d.dtn;
@@ -596,6 +645,9 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       if (last && phase_stop == 18) goto stopped;
       {
         const int accel = (acc && !ceased) ? 1 : 0;
+        if (net_inputs)
+          TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_mlp_inputs<NXV, R><<<dim3((unsigned)((npts + 255) / 256)), b256, 0, s>>>(d, ctx->mlp_x))));
+        if (ctx->mlp) TIMED(CRM_K_MICRO, crm_mlp_ns::mlp_infer(s, ctx->mlp, ctx->mlp_x, (long)npts, ctx->mlp_y));
         TIMED(CRM_K_MICRO, NX_DISPATCH(nx, (k_micro<NXV, R><<<L.g(CRM_K_MICRO), L.b(CRM_K_MICRO), 0, s>>>(d, accel))));
         if (accel && last && phase_stop == 19) goto stopped;  // before the acceleration
         if (accel)  // crm_module.F90:872-876
@@ -776,6 +828,9 @@ Engine<R>::~Engine() {
     (void)hipFree(kv.second.eign);
   }
   drop_graphs();
+  if (mlp_x) (void)hipFree(mlp_x);
+  if (mlp_y) (void)hipFree(mlp_y);
+  if (samp_n) (void)hipFree(samp_n);
   if (pool) (void)hipFree(pool);
   if (stage) (void)hipFree(stage);
   if (gmax) (void)hipFree(gmax);

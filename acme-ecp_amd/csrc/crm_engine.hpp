@@ -2,6 +2,7 @@
 // one per bulk-field precision, each compiled in its own translation unit.
 #pragma once
 #include "../../include/crm_hip.h"
+#include "../../include/crm_mlp.h"
 
 struct CrmEngine {
   virtual ~CrmEngine() {}
@@ -19,6 +20,9 @@ struct CrmEngine {
   virtual void get_schedule(int* block) const = 0;
   virtual void set_graphs(bool on) = 0;
   virtual long long graph_count() const = 0;
+  virtual int set_mlp(const crm_mlp* m) = 0;
+  virtual int record_samples(float* X, float* T, long long cap, int stride) = 0;
+  virtual long long samples_count() const = 0;
   virtual int autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
                        crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) = 0;
 };

@@ -153,6 +153,22 @@ int crm_hip_set_graphs(crm_hip_ctx* ctx, int on) {
   return CRM_OK;
 }
 
+int crm_hip_set_micro_mlp(crm_hip_ctx* ctx, crm_mlp* mlp) {
+  if (!ctx || !ctx->eng) return CRM_ERR_ARG;
+  return ctx->eng->set_mlp(mlp);
+}
+
+int crm_hip_record_micro_samples(crm_hip_ctx* ctx, float* X, float* T, long long cap, int stride) {
+  if (!ctx || !ctx->eng) return CRM_ERR_ARG;
+  if (hipSetDevice(ctx->device) != hipSuccess) return CRM_ERR_DEVICE;
+  return ctx->eng->record_samples(X, T, cap, stride);
+}
+
+long long crm_hip_micro_samples_count(const crm_hip_ctx* ctx) {
+  if (!ctx || !ctx->eng) return -1;
+  return ctx->eng->samples_count();
+}
+
 long long crm_hip_graph_launches(const crm_hip_ctx* ctx) {
   if (!ctx || !ctx->eng) return -1;
   return ctx->eng->graph_count();

@@ -7,6 +7,7 @@
 #pragma once
 #include "../../include/crm_hip.h"
 #include "crm_common.hpp"
+#include "crm_mlp_dev.hpp"
 #include <type_traits>
 
 namespace CRM_NS {
@@ -1931,6 +1932,37 @@ __global__ void k_precip_init(DevT<R> d) {
 // per level followed by the level means.  Also finds the cloud-ice level range for the next
 // ice fall (ice_fall.F90:34-46).
 // ------------------------------------------------------------------------------------------
+// Inputs of the microphysics network at the entry of micro_proc, per grid point: T, p, rho,
+// qv, qcl, qci, qpl, qpi, w, with the current total water and precipitation split by the
+// temperature ramps of the last diagnose (microphysics.F90:331-357) and T from the static
+// energy with that split (diagnose.F90).
+template <int NX, typename R>
+__global__ void k_mlp_inputs(DevT<R> d, float* X) {
+  const size_t np = (size_t)d.ncrms * d.nzm * NX;
+  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= np) return;
+  const int i = (int)(o % NX);
+  const size_t ck = o / NX;
+  const int k = (int)(ck % d.nzm), c = (int)(ck / d.nzm);
+  const double a_bg = A_BG(), a_pr = A_PR();
+  const double q = fmax(0.0, (double)d.qt[o]), qp = d.qp[o], qn = d.qn[o], tabs = d.tabs[o];
+  const double omn = fmax(0.0, fmin(1.0, (tabs - TBGMIN) * a_bg));
+  const double omp = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * a_pr));
+  const double qcl = qn * omn, qci = qn * (1. - omn), qpl = qp * omp, qpi = qp * (1. - omp);
+  const double T = (double)d.t[o] - prof(d, P_GAMAZ, c)[k] + FAC_COND_D * (qcl + qpl) + FAC_SUB_D * (qci + qpi);
+  const size_t ow = ((size_t)c * d.nz + k) * NX + i;
+  float* x = X + o * CRM_MLP_NIN;
+  x[0] = (float)T;
+  x[1] = (float)prof(d, P_PRES, c)[k];
+  x[2] = (float)prof(d, P_RHO, c)[k];
+  x[3] = (float)(q - qn);
+  x[4] = (float)qcl;
+  x[5] = (float)qci;
+  x[6] = (float)qpl;
+  x[7] = (float)qpi;
+  x[8] = (float)(0.5 * ((double)d.w[ow] + (double)d.w[ow + NX]));
+}
+
 // diagnose (diagnose.F90:8-177) at level k: tabs from t and the condensate split, the level
 // means, the cloud-ice level range of ice_fall; writes the microphysics fields and tabs.
 template <int NX, typename R, typename S_>
@@ -2033,6 +2065,7 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
     R t = t_pre;
     R q = fmax(R(0.0), qt_pre);
     R qp = qp_pre;
+    const R q_in = q, qp_in = qp;  // the entry state, for recorded training pairs
     const R u_k = u_pre, v_k = v_pre;
     if (k + 1 < nzm) {
       t_pre = d.t[o + NX];
@@ -2041,8 +2074,41 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
       u_pre = d.u[o + NX];
       if (y3) v_pre = d.v[o + NX];
     }
+    // the network's rates where it applies (inside its training envelope): condensate and
+    // precipitation stepped by the rates, clipped at zero, total water q + qp kept, the
+    // vapour taking the remainder (condensate scaled down should the vapour go negative); t
+    // (the static energy) is untouched, so the later diagnose conserves energy as the scheme
+    // does; T by its rate sets the temperature of the condensate split
+    const bool net = d.mlp_y != nullptr && d.mlp_y[o * 8 + 6] > 0.5f;
+    R tabs;
+    R src = R(0.0), evp = R(0.0);
+    R qn;
+    if (net) {
+      const float* y = d.mlp_y + o * 8;
+      const float* x = d.mlp_x + o * CRM_MLP_NIN;
+      const R qn_in = (R)x[4] + (R)x[5];
+      R qn_n = fmax(R(0.0), qn_in + ((R)y[1] + (R)y[2]) * dtn);
+      R qp_n = fmax(R(0.0), qp_in + ((R)y[3] + (R)y[4]) * dtn);
+      const R wtot = q + qp_in;
+      R qv_n = wtot - qn_n - qp_n;
+      if (qv_n < R(0.)) {
+        const R f = (qn_n + qp_n) > R(0.) ? fmax(R(0.0), wtot) / (qn_n + qp_n) : R(0.);
+        qn_n = qn_n * f;
+        qp_n = qp_n * f;
+        qv_n = R(0.);
+      }
+      const R dqp = qp_n - qp_in;
+      if (dqp > R(0.))
+        src = dqp;
+      else
+        evp = dqp;
+      tabs = (R)x[0] + (R)y[5] * dtn;
+      q = qv_n + qn_n;
+      qn = qn_n;
+      qp = qp_n;
+    } else {
     // cloud(): saturation adjustment with Newton iterations
-    R tabs = t - gamaz[k];
+    tabs = t - gamaz[k];
     R tabs1 = (tabs + fac1 * qp) / (R(1.) + fac2 * qp);
     R qsatt, om;
     if (tabs1 >= TBGMAX) {
@@ -2055,7 +2121,6 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
       om = an * tabs1 - bn;
       qsatt = om * qsatw(tabs1, pres) + (R(1.) - om) * qsati(tabs1, pres);
     }
-    R qn;
     if (q > qsatt) {
       int niter = 0;
       R dtabs = R(100.), dqsat = R(0.);
@@ -2103,7 +2168,6 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
     tabs = tabs1;
     qp = fmax(R(0.0), qp);
     // precip_proc()
-    R src = R(0.0), evp = R(0.0);
     if (qn + qp > R(0.)) {
       R omn = fmax(R(0.0), fmin(R(1.0), (tabs - TBGMIN) * a_bg));
       R omp = fmax(R(0.0), fmin(R(1.0), (tabs - TPRMIN) * a_pr));
@@ -2167,6 +2231,7 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
       qp = fmax(R(0.0), qp);
       q = q + (dq - qp);
     }
+    }  // plain microphysics
     R ssrc = s.sum(src), sevp = s.sum(evp);
     if (live && i == 0) {
       prof(d, P_QPSRC, c)[k] = ssrc;
@@ -2176,6 +2241,33 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
     // needs; k_accel then applies it and diagnoses)
     R qv, qcl, qci, qpl, qpi;
     micro_diag(q, qn, qp, tabs, a_bg, a_pr, qv, qcl, qci, qpl, qpi);
+    // (input, target) pairs of the plain scheme for training the network
+    if (d.samp_x && !net && live && o % (size_t)d.samp_stride == 0) {
+      const float* xin = d.mlp_x + o * CRM_MLP_NIN;
+      const unsigned long long slot = atomicAdd(d.samp_n, 1ull);
+      if ((long long)slot < d.samp_cap) {
+        float* xs = d.samp_x + slot * CRM_MLP_NIN;
+        float* ts = d.samp_t + slot * CRM_MLP_NOUT;
+        for (int f = 0; f < CRM_MLP_NIN; f++) xs[f] = xin[f];
+        // the entry split again in FP64 (as k_mlp_inputs forms it; the fields still hold
+        // the previous diagnose), so that the rates are differences of FP64 values
+        const double qn0 = d.qn[o], tb = d.tabs[o];
+        const double omn = fmax(0.0, fmin(1.0, (tb - TBGMIN) * (double)a_bg));
+        const double omp = fmax(0.0, fmin(1.0, (tb - TPRMIN) * (double)a_pr));
+        const double b_qv = (double)q_in - qn0, b_qcl = qn0 * omn, b_qci = qn0 * (1. - omn);
+        const double b_qpl = (double)qp_in * omp, b_qpi = (double)qp_in * (1. - omp);
+        const double b_T = (double)t - (double)gamaz[k] + FAC_COND_D * (b_qcl + b_qpl) + FAC_SUB_D * (b_qci + b_qpi);
+        const double idtn = 1.0 / (double)dtn;
+        const double tdiag = (double)t - (double)gamaz[k] + FAC_COND_D * ((double)qcl + (double)qpl) +
+                             FAC_SUB_D * ((double)qci + (double)qpi);
+        ts[0] = (float)(((double)qv - b_qv) * idtn);
+        ts[1] = (float)(((double)qcl - b_qcl) * idtn);
+        ts[2] = (float)(((double)qci - b_qci) * idtn);
+        ts[3] = (float)(((double)qpl - b_qpl) * idtn);
+        ts[4] = (float)(((double)qpi - b_qpi) * idtn);
+        ts[5] = (float)((tdiag - b_T) * idtn);
+      }
+    }
     if (accel) {
       accel_means<NX, R>(d, s, c, live, k, o, y3, t, qv, qcl, qci);
       if (live) {

@@ -20,13 +20,14 @@ def _declared():
     names = set()
     for h in HEADERS:
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
-        names |= set(re.findall(r"\b(crm_hip_\w+)\s*\(", text))
+        names |= set(re.findall(r"\b(crm_(?:hip|mlp)_\w+)\s*\(", text))
     return names
 
 
 def test_headers_declare_the_abi():
+    import crm_mlp
     names = _declared()
-    assert set(crm_hip.ABI_SYMBOLS) | set(crm_hip.DEBUG_SYMBOLS) == names
+    assert set(crm_hip.ABI_SYMBOLS) | set(crm_hip.DEBUG_SYMBOLS) | set(crm_mlp.SYMBOLS) == names
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,223 @@
+"""The microphysics network on the GPU: forward pass and gradients against the CPU reference
+(tests/mlp_ref.py, BF16 rounding emulated), training, the recording of training pairs by the
+CRM, and the network inside the CRM (exact water, non-negative condensate, envelope fallback,
+and what the trained network gains over none)."""
+import numpy as np
+import pytest
+
+import crm_mlp
+import mlp_ref
+from crm_hip import CrmHip
+from parity import budget_residuals, convective_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _rand_net(seed=3):
+    rng = np.random.default_rng(seed)
+    p = crm_mlp.init_params(seed).astype(np.float32)
+    p += rng.normal(0, 0.02, p.size).astype(np.float32)  # non-zero biases too
+    mean = rng.normal(0, 0.5, 9).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, 9).astype(np.float32)
+    oscale = rng.uniform(0.5, 2.0, 6).astype(np.float32)
+    return p, mean, scale, oscale
+
+
+def _normwise(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def test_forward_matches_reference():
+    torch = _torch()
+    p, mean, scale, oscale = _rand_net()
+    m = crm_mlp.CrmMlp(0)
+    try:
+        m.set_params(p)
+        m.set_norm(mean, scale, oscale)
+        X = np.random.default_rng(4).normal(0, 1.5, (1000, 9)).astype(np.float32)
+        Yg = m.forward(torch.from_numpy(X).cuda()).cpu().numpy()
+        Yr, _ = mlp_ref.forward(p, X, mean, scale, oscale, "bf16")
+        err = _normwise(Yg, Yr)
+        print("forward normwise error %.2e" % err)
+        assert err < 5e-3
+    finally:
+        m.close()
+
+
+def test_gradient_matches_reference():
+    """Gradient of the loss w.r.t. every parameter block against the CPU reference."""
+    torch = _torch()
+    p, mean, scale, oscale = _rand_net(5)
+    m = crm_mlp.CrmMlp(0)
+    try:
+        m.set_params(p)
+        m.set_norm(mean, scale, oscale)
+        rng = np.random.default_rng(6)
+        X = rng.normal(0, 1.5, (777, 9)).astype(np.float32)  # not a multiple of 16: padding
+        T = rng.normal(0, 1.0, (777, 6)).astype(np.float32)
+        g, L = m.grad(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda())
+        Lr, gr = mlp_ref.loss_grad(p, X, T, mean, scale, oscale, "bf16")
+        assert abs(L - Lr) <= 1e-3 * abs(Lr)
+        offs = np.cumsum([0, 9 * 128, 128, 128 * 128, 128, 128 * 128, 128, 128 * 6, 6])
+        for b in range(8):
+            e = _normwise(g[offs[b]:offs[b + 1]], gr[offs[b]:offs[b + 1]])
+            print("block", b, "normwise error %.2e" % e)
+            assert e < 2e-2, b
+    finally:
+        m.close()
+
+
+def test_training_reduces_loss():
+    torch = _torch()
+    rng = np.random.default_rng(8)
+    X = rng.uniform(-2, 2, (20000, 9)).astype(np.float32)
+    W = rng.normal(0, 1, (9, 6)).astype(np.float32)
+    T = (np.tanh(X @ W) + 0.1 * np.sin(3 * X[:, :6])).astype(np.float32)
+    m = crm_mlp.CrmMlp(0)
+    try:
+        m.set_params(crm_mlp.init_params(1))
+        m.set_norm(np.zeros(9), np.ones(9), np.ones(6))
+        hist = crm_mlp.train(m, torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda(), steps=300,
+                             batch=4096, lr=2e-3)
+        print("loss %.3e -> %.3e" % (hist[0], hist[-1]))
+        assert hist[-1] < 0.1 * hist[0]
+    finally:
+        m.close()
+
+
+def _record(hip, case, dt_gl, cap, stride):
+    X, T = crm_mlp.record(hip, cap, stride)
+    hip.run(case, dt_gl)
+    n = crm_mlp.stop_recording(hip)
+    return X[:n].contiguous(), T[:n].contiguous(), n
+
+
+def test_recorded_pairs_conserve_water():
+    """Targets recorded from the plain scheme: the five water rates sum to zero (micro_proc
+    moves water between species only)."""
+    hip = CrmHip(0)
+    try:
+        case = convective_case(8, seed=91)
+        X, T, n = _record(hip, case.copy(), 200.0, 200000, 7)
+        assert n > 1000
+        t = T.double().cpu().numpy()
+        x = X.cpu().numpy()
+        assert np.all(np.isfinite(t)) and np.all(np.isfinite(x))
+        s = t[:, :5].sum(axis=1)
+        mag = np.abs(t[:, :5]).sum(axis=1)
+        assert np.all(np.abs(s) <= 1e-5 * mag + 1e-14), np.abs(s).max()
+        assert np.abs(t[:, 1:5]).max() > 0  # the plume condenses and rains
+    finally:
+        hip.close()
+
+
+def _domain_means(c):
+    n = c.ncrms
+    return np.concatenate([c.out["sltend"][:n].ravel() / 1004.64, c.out["qltend"][:n].ravel() * 2.5e6 / 1004.64])
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """A network trained on pairs the CRM records over one GCM step of convective columns."""
+    torch = _torch()
+    hip = CrmHip(0)
+    m = crm_mlp.CrmMlp(0)
+    try:
+        case = convective_case(24, seed=92)
+        X, T, n = _record(hip, case.copy(), 1200.0, 2000000, 1)
+        m.set_norm(*crm_mlp.fit_norm(X, T))
+        m.set_params(crm_mlp.init_params(7))
+        hist = crm_mlp.train(m, X, T, steps=6000, batch=16384, lr=2e-3)
+        print("trained on %d pairs, loss %.3e -> %.3e" % (n, hist[0], hist[-1]))
+        yield m, hist
+    finally:
+        hip.close()
+
+
+def test_network_in_crm(trained):
+    """The network in the CRM over a GCM step: total water and static energy budgets close to
+    round-off, condensate stays non-negative, and the domain-mean heating and moistening lie
+    far closer to the plain scheme's than with an untrained network of zero output."""
+    m, hist = trained
+    assert np.mean(hist[-50:]) < 0.05 * hist[0]
+    case = convective_case(24, seed=93)
+    hip = CrmHip(0)
+    try:
+        plain = case.copy()
+        assert hip.debug_run(plain, 1200.0, 0) == 0
+        pw, ph = budget_residuals(case, plain, 1200.0, hip.debug_get)
+        crm_mlp.attach(hip, m)
+        # the network's step keeps the total water of every point: qt + qp before
+        # micro_proc (phase 18) and after it (phase 20) of the same substep
+        npt = 24 * (case.nz + 1) * case.nx
+        assert hip.debug_run(case.copy(), case.dt, 18) == 0
+        w18 = hip.debug_get("qt", npt) + hip.debug_get("qp", npt)
+        assert hip.debug_run(case.copy(), case.dt, 20) == 0
+        w20 = hip.debug_get("qt", npt) + hip.debug_get("qp", npt)
+        dw = np.abs(w20 - w18).max()
+        print("largest change of a point's total water in the network's step: %.1e" % dw)
+        assert dw <= 4 * np.finfo(np.float64).eps * np.abs(w18).max()
+        net = case.copy()
+        assert hip.debug_run(net, 1200.0, 0) == 0
+        rw, rh = budget_residuals(case, net, 1200.0, hip.debug_get)
+        print("GCM step budgets: plain water %.2e energy %.2e; network water %.2e energy %.2e" % (pw, ph, rw, rh))
+        # Over many substeps the static energy still closes to round-off.  Total water gains
+        # what the scheme's own clipping of negative total water in cloud() adds
+        # (MICRO_SAM1MOM/cloud.F90:42) in cells whose vapour the network's step exhausted
+        # (the fixer hands any deficit to the condensate, leaving zero vapour); stated, bounded
+        assert rh < 1e-12
+        assert rw < 1e-6
+        for f in ("qt", "qp", "qn"):
+            assert net.state[f][:24].min() >= 0.0, f
+        # untrained: zero weights, zero rates (condensate and precipitation frozen)
+        z = crm_mlp.CrmMlp(0)
+        try:
+            z.set_norm(*[m.norm[k] for k in ("mean", "scale", "oscale", "lo", "hi")])
+            z.set_params(np.zeros(crm_mlp.NPARAM, np.float32))
+            crm_mlp.attach(hip, z)
+            zero = case.copy()
+            hip.run(zero, 1200.0)
+        finally:
+            crm_mlp.attach(hip, None)
+            z.close()
+        ref = _domain_means(plain)
+        errs = {}
+        for name, c in (("trained", net), ("zero-output", zero)):
+            d = _domain_means(c) - ref
+            errs[name] = (np.abs(d).max() / np.abs(ref).max(), np.linalg.norm(d) / np.linalg.norm(ref))
+        print("domain-mean heating/moistening error vs plain (max, rms):", errs)
+        # measured 0.18-0.30 (max) and 0.13-0.26 (rms) for the trained network (training
+        # order varies with the atomic accumulation of gradients) against 0.97 / 0.75 without
+        assert errs["trained"][0] < 0.5 * errs["zero-output"][0]
+        assert errs["trained"][1] < 0.5 * errs["zero-output"][1]
+    finally:
+        hip.close()
+
+
+def test_envelope_fallback_is_the_plain_scheme(trained):
+    """With an empty training envelope every point falls back to the plain scheme: the call
+    is then bitwise the plain call."""
+    m, _ = trained
+    z = crm_mlp.CrmMlp(0)
+    hip = CrmHip(0)
+    try:
+        z.set_params(m.params())
+        nm = m.norm
+        z.set_norm(nm["mean"], nm["scale"], nm["oscale"], lo=np.full(9, 1e30), hi=np.full(9, -1e30))
+        case = convective_case(6, seed=94)
+        plain, fb = case.copy(), case.copy()
+        hip.run(plain, 200.0)
+        crm_mlp.attach(hip, z)
+        hip.run(fb, 200.0)
+        crm_mlp.attach(hip, None)
+        for g in ("state", "out", "rad"):
+            for k, v in getattr(plain, g).items():
+                assert np.array_equal(getattr(fb, g)[k], v), (g, k)
+    finally:
+        hip.close()
+        z.close()
