@@ -70,7 +70,7 @@ __global__ void k_inputs(const float* X, int n, int mp, const float* norm, bf16*
   const long m = idx / KP;
   const int c = (int)(idx % KP);
   float v = 0.f;
-  if (m < n && c < CRM_MLP_NIN) v = (X[m * CRM_MLP_NIN + c] - norm[NORM_MEAN + c]) / norm[NORM_SCALE + c];
+  if (m < n && c < CRM_MLP_NIN) v = (X[m * CRM_MLP_NIN + c] - norm[NORM_MEAN + c]) * norm[NORM_RSCALE + c];
   Xn[idx] = to_bf16(v);
 }
 
@@ -102,15 +102,6 @@ __global__ void k_colsum(const bf16* G, int mp, int ld, int ncol, float* out) {
   float sacc = 0.f;
   for (long m = part; m < mp; m += nparts) sacc += from_bf16(G[m * ld + c]);
   unsafeAtomicAdd(&out[c], sacc);
-}
-
-// De-normalised outputs: Y[n][6] = Yn[m][0..5] * oscale.
-__global__ void k_outputs(const float* Yn, int n, const float* norm, float* Y) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)n * CRM_MLP_NOUT) return;
-  const long m = idx / CRM_MLP_NOUT;
-  const int c = (int)(idx % CRM_MLP_NOUT);
-  Y[idx] = Yn[m * NP + c] * norm[NORM_OSCALE + c];
 }
 
 // Packed FP32 parameters -> padded BF16 weights and FP32 biases used by the GEMMs.
@@ -177,80 +168,216 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, float lr, f
   p[idx] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// Inference for the CRM: one workgroup (4 waves) per 64 grid points, the whole network in one
-// pass with the activations in LDS.  X[n][9] raw inputs; Y[n][8]: the 6 de-normalised rates,
-// then 1 where the inputs lie inside the training envelope (else 0), then 0.
-constexpr int HLD = NH + 8;  // LDS row stride of the activations (bf16)
+// Fused inference, the whole network in registers.  The layers run transposed, H^T = W^T A^T:
+// the weights are the A operand (rows = output features) and the points the B operand
+// (columns), so the D fragment of one layer -- lane (r, q) holds features 4q..4q+3 of point r
+// -- is the B fragment of the next one, and activations never leave the registers.  The
+// 128-wide layers use v_mfma_f32_16x16x32_bf16, whose 32 K slots a lane fills with eight
+// values; the K order inside a 32-feature block is a free choice as long as A and B agree,
+// and we take slot (q, i) = feature 4q + i (i < 4) or 16 + 4q + (i - 4) (i >= 4), which is
+// exactly the two D fragments of output blocks 2kb and 2kb + 1.  The weights are stored once
+// per network in that fragment order (k_frags: each lane's operand is 8 or 16 contiguous
+// bytes) and copied into LDS by every workgroup; a wave owns 64 points (4 blocks of 16) per
+// tile and reads each weight fragment once for the 4 blocks.  The grid is persistent: two
+// workgroups per CU (the weights take 74 KiB of LDS) loop over the tiles.
+constexpr int F_L1 = 0;                     // layer 1: 8 output blocks x 64 lanes x 4 (K = 16)
+constexpr int F_L2 = F_L1 + 8 * 64 * 4;     // layer 2: (8 output x 4 K blocks) x 64 lanes x 8
+constexpr int F_L3 = F_L2 + 32 * 64 * 8;    // layer 3: as layer 2
+constexpr int F_L4 = F_L3 + 32 * 64 * 8;    // layer 4: 4 K blocks x 64 lanes x 8 (16 outputs)
+constexpr int F_SIZE = F_L4 + 4 * 64 * 8;   // bf16 elements
+constexpr int B_SIZE = 3 * NH + NP;         // biases b1 b2 b3 b4
 
-__device__ __forceinline__ void infer_layer(const bf16 (*A)[HLD], int kdim, const bf16* W, const float* b,
-                                            bf16 (*H)[HLD], int wave, int lane) {
-  const int r = lane & 15, q = lane >> 4;
-  for (int tile = wave * 8; tile < wave * 8 + 8; tile++) {
-    const int rb = tile >> 3, cb = tile & 7;
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kk = 0; kk < kdim; kk += 16) {
-      s4 av, bv;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        av[j] = (short)A[16 * rb + r][kk + 4 * q + j];
-        bv[j] = (short)W[(kk + 4 * q + j) * NH + 16 * cb + r];
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) H[16 * rb + 4 * q + j][16 * cb + r] = to_bf16(fmaxf(acc[j] + b[16 * cb + r], 0.f));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));  // 8 bf16, element 0 in the low half
+typedef unsigned u2 __attribute__((ext_vector_type(2)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bh2 __attribute__((ext_vector_type(2)));
+typedef short sh2 __attribute__((ext_vector_type(2)));
+
+// Input feature (or K index inside a 32-block) of slot i of lane quarter q.
+__device__ __forceinline__ int slot32(int q, int i) { return i < 4 ? 4 * q + i : 16 + 4 * q + (i - 4); }
+
+// Weights in fragment order, from the packed FP32 parameters (one thread per element).
+__global__ void k_frags(const float* p, bf16* F) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= F_SIZE) return;
+  float w = 0.f;
+  if (idx < F_L2) {
+    const int ob = idx / 256, l = (idx / 4) % 64, i = idx % 4;
+    const int n = 16 * ob + l % 16, k = 4 * (l / 16) + i;
+    if (k < CRM_MLP_NIN) w = p[OFF_W1 + k * NH + n];
+  } else if (idx < F_L4) {
+    const int j = idx < F_L3 ? idx - F_L2 : idx - F_L3, off = idx < F_L3 ? OFF_W2 : OFF_W3;
+    const int fr = j / 512, ob = fr / 4, kb = fr % 4, l = (j / 8) % 64, i = j % 8;
+    const int n = 16 * ob + l % 16, k = 32 * kb + slot32(l / 16, i);
+    w = p[off + k * NH + n];
+  } else {
+    const int j = idx - F_L4, kb = j / 512, l = (j / 8) % 64, i = j % 8;
+    const int n = l % 16, k = 32 * kb + slot32(l / 16, i);
+    if (n < CRM_MLP_NOUT) w = p[OFF_W4 + k * CRM_MLP_NOUT + n];
   }
+  F[idx] = to_bf16(w);
 }
 
-__global__ void __launch_bounds__(256) k_infer(const float* X, long n, MlpView v, float* Y) {
-  __shared__ bf16 xs[64][HLD];
-  __shared__ bf16 ha[64][HLD];
-  __shared__ bf16 hb[64][HLD];
-  const long p0 = (long)blockIdx.x * 64;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const float* nm = v.norm;
-  if (t < 64) {
-    const long p = p0 + t;
-    bool in = p < n;
+__device__ __forceinline__ f4 mfma32(u4 a, u4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a), __builtin_bit_cast(b8, b), c, 0, 0, 0);
+}
+
+// relu of two values rounded to BF16 (nearest, ties to even: v_cvt_pk_bf16_f32), packed.  The
+// rounding keeps the sign, so relu after it equals relu before it, and on BF16 bit patterns
+// relu is the signed 16-bit max with 0 (negative values and -0 have the sign bit set).
+__device__ __forceinline__ unsigned relu_pk(float a, float b) {
+  const bh2 h = __builtin_convertvector((f2){a, b}, bh2);
+  return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(sh2, h), (sh2){0, 0}));
+}
+
+// D fragment of output block ob -> its half of the next layer's B fragment of block ob / 2.
+__device__ __forceinline__ void relu_store(const f4& acc, u4& dst, int half) {
+  dst[2 * half] = relu_pk(acc[0], acc[1]);
+  dst[2 * half + 1] = relu_pk(acc[2], acc[3]);
+}
+
+// A 128 -> 128 hidden layer for the wave's four point blocks.  The epilogue of output block
+// ob - 1 is written after the products of block ob, so it runs under them.
+__device__ __forceinline__ void hidden128(const bf16* fr, const float* bias, const u4 (&in)[4][4], u4 (&out)[4][4],
+                                          int lane) {
+  const int q = lane >> 4;
+  f4 acc[2][4];
 #pragma unroll
-    for (int c = 0; c < KP; c++) {
-      float f = 0.f;
-      if (c < CRM_MLP_NIN && p < n) {
-        const float x = X[p * CRM_MLP_NIN + c];
-        in = in && x >= nm[NORM_LO + c] && x <= nm[NORM_HI + c];
-        f = (x - nm[NORM_MEAN + c]) / nm[NORM_SCALE + c];
-      }
-      xs[t][c] = to_bf16(f);
+  for (int ob = 0; ob < 8; ob++) {
+    const f4 b = *(const f4*)&bias[16 * ob + 4 * q];
+    f4(&a)[4] = acc[ob & 1];
+#pragma unroll
+    for (int pb = 0; pb < 4; pb++) a[pb] = b;
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      const u4 w = *(const u4*)&fr[((ob * 4 + kb) * 64 + lane) * 8];
+#pragma unroll
+      for (int pb = 0; pb < 4; pb++) a[pb] = mfma32(w, in[pb][kb], a[pb]);
     }
-    if (p < n) {
-      Y[p * 8 + 6] = in ? 1.f : 0.f;
-      Y[p * 8 + 7] = 0.f;
+    if (ob > 0) {
+#pragma unroll
+      for (int pb = 0; pb < 4; pb++) relu_store(acc[(ob - 1) & 1][pb], out[pb][(ob - 1) >> 1], (ob - 1) & 1);
     }
   }
-  __syncthreads();
-  infer_layer(xs, KP, v.W1, v.b1, ha, wave, lane);
-  __syncthreads();
-  infer_layer(ha, NH, v.W2, v.b2, hb, wave, lane);
-  __syncthreads();
-  infer_layer(hb, NH, v.W3, v.b3, ha, wave, lane);
-  __syncthreads();
-  {
-    // output layer: wave w computes rows 16w..16w+15
-    const int r = lane & 15, q = lane >> 4, rb = wave;
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kk = 0; kk < NH; kk += 16) {
-      s4 av, bv;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        av[j] = (short)ha[16 * rb + r][kk + 4 * q + j];
-        bv[j] = (short)v.W4[(kk + 4 * q + j) * NP + r];
+  for (int pb = 0; pb < 4; pb++) relu_store(acc[1][pb], out[pb][3], 1);
+}
+
+// X[n][9] raw inputs.  MODE 0 (the CRM): Y[n][8] = the 6 de-normalised rates, then 1 where
+// the inputs lie inside the training envelope (else 0), then 0.  MODE 1: Y[n][6], the rates.
+template <int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_infer(const float* X, long n, MlpView v,
+                                                                                       float* Y) {
+  __shared__ __attribute__((aligned(16))) bf16 fl[F_SIZE];
+  __shared__ __attribute__((aligned(16))) float bl[B_SIZE];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int i = t; i < F_SIZE / 8; i += 256) ((uint4*)fl)[i] = ((const uint4*)v.F)[i];
+  for (int i = t; i < NH; i += 256) {
+    bl[i] = v.b1[i];
+    bl[NH + i] = v.b2[i];
+    bl[2 * NH + i] = v.b3[i];
+  }
+  if (t < NP) bl[3 * NH + t] = v.b4[t];
+  __syncthreads();
+
+  const int r = lane & 15, q = lane >> 4;
+  const float* nm = v.norm;
+  float mean[4], rs[4], lo[4], hi[4], osc[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int f = 4 * q + i;
+    const bool ok = f < CRM_MLP_NIN;
+    mean[i] = ok ? nm[NORM_MEAN + f] : 0.f;
+    rs[i] = ok ? nm[NORM_RSCALE + f] : 1.f;
+    lo[i] = ok ? nm[NORM_LO + f] : -INFINITY;
+    hi[i] = ok ? nm[NORM_HI + f] : INFINITY;
+    osc[i] = f < CRM_MLP_NOUT ? nm[NORM_OSCALE + f] : 0.f;
+  }
+  const long ntiles = (n + 63) / 64, stride = (long)gridDim.x * 4;
+  // raw inputs of the tile: every lane loads its 16 slots unconditionally from clamped (valid)
+  // addresses, so the loads issue together; slots past the 9 features or past n are masked
+  float xr[4][4];
+  auto load_tile = [&](long tile) {
+#pragma unroll
+    for (int pb = 0; pb < 4; pb++) {
+      const long pt = min(tile * 64 + 16 * pb + r, n - 1);
+#pragma unroll
+      for (int i = 0; i < 4; i++) xr[pb][i] = X[pt * CRM_MLP_NIN + min(4 * q + i, CRM_MLP_NIN - 1)];
+    }
+  };
+  long tile = (long)blockIdx.x * 4 + wave;
+  if (tile < ntiles) load_tile(tile);
+  for (; tile < ntiles; tile += stride) {
+    const long p0 = tile * 64;
+    // an offset the compiler cannot see through, so the weight reads stay inside the loop
+    // (hoisted out of it, the whole network would be held in registers and spill)
+    int fo = 0;
+    asm volatile("" : "+v"(fo));
+    const bf16* fw = fl + fo;
+    s4 x0[4];
+    int in[4];
+#pragma unroll
+    for (int pb = 0; pb < 4; pb++) {
+      const bool live = p0 + 16 * pb + r < n;
+      float xn[4];
+      in[pb] = 1;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const bool have = 4 * q + i < CRM_MLP_NIN && live;
+        const float xv = xr[pb][i];
+        if (have) in[pb] &= (xv >= lo[i] && xv <= hi[i]);
+        xn[i] = have ? (xv - mean[i]) * rs[i] : 0.f;
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+      const bh2 h0 = __builtin_convertvector((f2){xn[0], xn[1]}, bh2);
+      const bh2 h1 = __builtin_convertvector((f2){xn[2], xn[3]}, bh2);
+      x0[pb] = __builtin_bit_cast(s4, (u2){__builtin_bit_cast(unsigned, h0), __builtin_bit_cast(unsigned, h1)});
+    }
+    // the next tile's inputs load under this tile's products
+    if (tile + stride < ntiles) load_tile(tile + stride);
+    u4 ha[4][4], hb[4][4];
+    // layer 1 (K = 16: the 9 inputs, padded)
+#pragma unroll
+    for (int ob = 0; ob < 8; ob++) {
+      const f4 b = *(const f4*)&bl[16 * ob + 4 * q];
+      const s4 w = *(const s4*)&fw[F_L1 + (ob * 64 + lane) * 4];
+#pragma unroll
+      for (int pb = 0; pb < 4; pb++)
+        relu_store(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, x0[pb], b, 0, 0, 0), ha[pb][ob >> 1], ob & 1);
+    }
+    hidden128(fw + F_L2, bl + NH, ha, hb, lane);
+    hidden128(fw + F_L3, bl + 2 * NH, hb, ha, lane);
+    // layer 4: outputs 4q..4q+3 of point r (q < 2 hold the 6 rates and padding)
+    const f4 b4 = *(const f4*)&bl[3 * NH + 4 * q];
+    f4 acc[4] = {b4, b4, b4, b4};
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      const u4 w = *(const u4*)&fw[F_L4 + (kb * 64 + lane) * 8];
+#pragma unroll
+      for (int pb = 0; pb < 4; pb++) acc[pb] = mfma32(w, ha[pb][kb], acc[pb]);
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const long p = p0 + 16 * rb + 4 * q + j;
-      if (r < CRM_MLP_NOUT && p < n) Y[p * 8 + r] = (acc[j] + v.b4[r]) * nm[NORM_OSCALE + r];
+    for (int pb = 0; pb < 4; pb++) {
+      const long pt = p0 + 16 * pb + r;
+      if constexpr (MODE == 0) {
+        // the point's envelope flag: all four lane quarters agree
+        int a = in[pb] & __shfl_xor(in[pb], 16);
+        a &= __shfl_xor(a, 32);
+        if (q < 2 && pt < n) {
+          f4 o;
+          o[0] = acc[pb][0] * osc[0];
+          o[1] = acc[pb][1] * osc[1];
+          o[2] = q == 0 ? acc[pb][2] * osc[2] : (a ? 1.f : 0.f);
+          o[3] = q == 0 ? acc[pb][3] * osc[3] : 0.f;
+          *(f4*)&Y[pt * 8 + 4 * q] = o;
+        }
+      } else {
+        if (q < 2 && pt < n) {
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (4 * q + j < CRM_MLP_NOUT) Y[pt * CRM_MLP_NOUT + 4 * q + j] = acc[pb][j] * osc[j];
+        }
+      }
     }
   }
 }
@@ -264,6 +391,8 @@ struct crm_mlp {
   hipStream_t s = nullptr;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // packed FP32
   bf16 *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *W4 = nullptr;
+  bf16* F = nullptr;  // weights in the fragment order of the fused inference
+  int ncu = 0;        // compute units of the device (persistent inference grid)
   float *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *b4 = nullptr;
   float* gpad = nullptr;  // gW1 gW2 gW3 gW4 gb1 gb2 gb3 gb4, contiguous
   float* norm = nullptr;  // [NORM_SIZE]
@@ -276,6 +405,14 @@ struct crm_mlp {
 };
 
 namespace {
+// Persistent grid of the fused inference: two workgroups per CU, fewer for small batches.
+template <int MODE>
+void launch_infer(hipStream_t s, const crm_mlp* m, const float* X, long n, float* Y) {
+  const long tiles = (n + 63) / 64;
+  const long grid = std::min<long>((tiles + 3) / 4, 2L * m->ncu);
+  if (n > 0) k_infer<MODE><<<dim3((unsigned)grid), dim3(256), 0, s>>>(X, n, mlp_view(m), Y);
+}
+
 constexpr size_t GPAD = (size_t)KP * NH + 2 * NH * NH + NH * NP + 3 * NH + NP;
 
 #define MLPCHECK(x)                                   \
@@ -366,6 +503,7 @@ int backward(crm_mlp* m, const float* X, const float* T, int n) {
 void unpack(crm_mlp* m) {
   k_unpack<<<dim3((NH * NH + 255) / 256), dim3(256), 0, m->s>>>(m->p, m->W1, m->W2, m->W3, m->W4, m->b1, m->b2, m->b3,
                                                                  m->b4);
+  k_frags<<<dim3((F_SIZE + 255) / 256), dim3(256), 0, m->s>>>(m->p, m->F);
 }
 }  // namespace
 
@@ -388,6 +526,7 @@ int crm_mlp_create(crm_mlp** out, int device) {
             hipMalloc(&m->W2, NH * NH * sizeof(bf16)) == hipSuccess &&
             hipMalloc(&m->W3, NH * NH * sizeof(bf16)) == hipSuccess &&
             hipMalloc(&m->W4, NH * NP * sizeof(bf16)) == hipSuccess &&
+            hipMalloc(&m->F, F_SIZE * sizeof(bf16)) == hipSuccess &&
             hipMalloc(&m->b1, NH * sizeof(float)) == hipSuccess && hipMalloc(&m->b2, NH * sizeof(float)) == hipSuccess &&
             hipMalloc(&m->b3, NH * sizeof(float)) == hipSuccess && hipMalloc(&m->b4, NP * sizeof(float)) == hipSuccess &&
             hipMalloc(&m->gpad, GPAD * sizeof(float)) == hipSuccess &&
@@ -397,11 +536,16 @@ int crm_mlp_create(crm_mlp** out, int device) {
     crm_mlp_destroy(m);
     return CRM_ERR_NOMEM;
   }
+  if (hipDeviceGetAttribute(&m->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || m->ncu < 1) {
+    crm_mlp_destroy(m);
+    return CRM_ERR_DEVICE;
+  }
   (void)hipMemset(m->p, 0, CRM_MLP_NPARAM * sizeof(float));
   (void)hipMemset(m->m, 0, CRM_MLP_NPARAM * sizeof(float));
   (void)hipMemset(m->v, 0, CRM_MLP_NPARAM * sizeof(float));
   for (int c = 0; c < 16; c++) {
     m->host_norm[NORM_SCALE + c] = 1.f;
+    m->host_norm[NORM_RSCALE + c] = 1.f;
     m->host_norm[NORM_OSCALE + c] = 1.f;
     m->host_norm[NORM_LO + c] = -INFINITY;
     m->host_norm[NORM_HI + c] = INFINITY;
@@ -416,7 +560,7 @@ int crm_mlp_create(crm_mlp** out, int device) {
 int crm_mlp_destroy(crm_mlp* m) {
   if (!m) return CRM_ERR_ARG;
   (void)hipSetDevice(m->device);
-  void* bufs[] = {m->p, m->g, m->m, m->v, m->W1, m->W2, m->W3, m->W4, m->b1, m->b2, m->b3, m->b4, m->gpad,
+  void* bufs[] = {m->p, m->g, m->m, m->v, m->W1, m->W2, m->W3, m->W4, m->F, m->b1, m->b2, m->b3, m->b4, m->gpad,
                   m->norm, m->loss, m->Xn, m->H1, m->H2, m->H3, m->dY, m->dA, m->dB, m->Y};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -451,6 +595,7 @@ int crm_mlp_set_norm(crm_mlp* m, const float* mean, const float* scale, const fl
     if (scale) {
       if (!(scale[c] > 0.f)) return CRM_ERR_ARG;
       m->host_norm[NORM_SCALE + c] = scale[c];
+      m->host_norm[NORM_RSCALE + c] = 1.f / scale[c];
     }
     if (lo) m->host_norm[NORM_LO + c] = lo[c];
     if (hi) m->host_norm[NORM_HI + c] = hi[c];
@@ -467,18 +612,7 @@ int crm_mlp_set_norm(crm_mlp* m, const float* mean, const float* scale, const fl
 
 int crm_mlp_forward(crm_mlp* m, const float* X, int n, float* Y) {
   if (!m || !X || !Y || n < 0) return CRM_ERR_ARG;
-  const int chunk = 1 << 20;
-  for (int r0 = 0; r0 < n; r0 += chunk) {
-    const int nr = std::min(chunk, n - r0);
-    const int mp = (nr + 15) / 16 * 16;
-    int rc = ensure_rows(m, mp);
-    if (rc) return rc;
-    k_inputs<<<dim3((unsigned)(((long)mp * KP + 255) / 256)), dim3(256), 0, m->s>>>(X + (long)r0 * CRM_MLP_NIN, nr, mp,
-                                                                                  m->norm, m->Xn);
-    forward_rows(m, mp);
-    k_outputs<<<dim3((unsigned)(((long)nr * CRM_MLP_NOUT + 255) / 256)), dim3(256), 0, m->s>>>(
-        m->Y, nr, m->norm, Y + (long)r0 * CRM_MLP_NOUT);
-  }
+  launch_infer<1>(m->s, m, X, n, Y);
   MLPCHECK(hipGetLastError());
   MLPCHECK(hipStreamSynchronize(m->s));
   return CRM_OK;
@@ -518,6 +652,7 @@ MlpView mlp_view(const crm_mlp* m) {
   v.W2 = m->W2;
   v.W3 = m->W3;
   v.W4 = m->W4;
+  v.F = m->F;
   v.b1 = m->b1;
   v.b2 = m->b2;
   v.b3 = m->b3;
@@ -526,7 +661,5 @@ MlpView mlp_view(const crm_mlp* m) {
   return v;
 }
 int mlp_device(const crm_mlp* m) { return m->device; }
-void mlp_infer(hipStream_t s, const crm_mlp* m, const float* X, long n, float* Y) {
-  if (n > 0) k_infer<<<dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s>>>(X, n, mlp_view(m), Y);
-}
+void mlp_infer(hipStream_t s, const crm_mlp* m, const float* X, long n, float* Y) { launch_infer<0>(s, m, X, n, Y); }
 }  // namespace crm_mlp_ns

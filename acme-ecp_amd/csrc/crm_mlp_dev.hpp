@@ -26,8 +26,8 @@ constexpr int OFF_W4 = OFF_B3 + NH;
 constexpr int OFF_B4 = OFF_W4 + NH * CRM_MLP_NOUT;
 static_assert(OFF_B4 + CRM_MLP_NOUT == CRM_MLP_NPARAM, "packed parameter count");
 
-// normalisation block [NORM_SIZE]: mean, scale, oscale, lo, hi (16 entries each)
-enum { NORM_MEAN = 0, NORM_SCALE = 16, NORM_OSCALE = 32, NORM_LO = 48, NORM_HI = 64, NORM_SIZE = 80 };
+// normalisation block [NORM_SIZE]: mean, scale, oscale, lo, hi, 1 / scale (16 entries each)
+enum { NORM_MEAN = 0, NORM_SCALE = 16, NORM_OSCALE = 32, NORM_LO = 48, NORM_HI = 64, NORM_RSCALE = 80, NORM_SIZE = 96 };
 
 __device__ __forceinline__ bf16 to_bf16(float x) {  // round to nearest, ties to even
   unsigned u = __float_as_uint(x);
@@ -37,9 +37,11 @@ __device__ __forceinline__ bf16 to_bf16(float x) {  // round to nearest, ties to
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float((unsigned)h << 16); }
 
 // Device-side view of a network: padded BF16 weights W1[16][128], W2, W3 [128][128],
-// W4[128][16], FP32 biases (b4 padded to 16), and the normalisation block.
+// W4[128][16], the same weights in the fragment order of the fused inference (F), FP32 biases
+// (b4 padded to 16), the normalisation block.
 struct MlpView {
   const bf16 *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *W4 = nullptr;
+  const bf16* F = nullptr;
   const float *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *b4 = nullptr;
   const float* norm = nullptr;
 };

@@ -184,6 +184,62 @@ def measure_uneven(prec, args, torch):
                     "streams; 128-column class not built"}
 
 
+def measure_learned(case, args, torch):
+    """The learned microphysics inside the full workload (FP64 engine): the network trained on
+    pairs the CRM records from one GCM step of 512 instances of the same soundings, then the
+    full-size call timed with the network in place of micro_proc, and its domain-mean heating
+    and moistening compared with the plain scheme from the same initial state."""
+    import crm_mlp
+    from crm_hip import CrmHip, DeviceCase
+    w = WORKLOAD
+    hip = CrmHip(0)
+    m = crm_mlp.CrmMlp(0)
+    rec = make_case(512, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True, seed_base=4096)
+    t0 = time.perf_counter()
+    X, T = crm_mlp.record(hip, 4_000_000, 97)
+    hip.run(rec, w["dt_gl"])
+    n = crm_mlp.stop_recording(hip)
+    X, T = X[:n].contiguous(), T[:n].contiguous()
+    m.set_norm(*crm_mlp.fit_norm(X, T))
+    m.set_params(crm_mlp.init_params(7))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist = crm_mlp.train(m, X, T, steps=6000, batch=16384, lr=2e-3)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    log(f"[learned] {n} pairs recorded in {t1 - t0:.1f} s, 6000 Adam steps in {t2 - t1:.1f} s, "
+        f"loss {hist[0]:.3e} -> {np.mean(hist[-50:]):.3e}")
+    del X, T
+    nstop = int(w["dt_gl"] / w["dt"])
+    # plain and learned from the same state, one GCM step each
+    d_plain = DeviceCase(case, "cuda:0")
+    hip.run_device(d_plain, w["dt_gl"])
+    crm_mlp.attach(hip, m)
+    d_net = DeviceCase(case, "cuda:0")
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    hip.run_device(d_net, w["dt_gl"])
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t3
+    errs = {}
+    for name in ("sltend", "qltend"):
+        a, b = d_net.arrays["out"][name], d_plain.arrays["out"][name]
+        errs[name] = {"max": (a - b).abs().max().item() / b.abs().max().item(),
+                      "rms": ((a - b).norm() / b.norm()).item()}
+    crm_mlp.attach(hip, None)
+    hip.close()
+    m.close()
+    return {"value": case.ncrms * w["nx"] * nstop / sec, "unit": "column-steps/s",
+            "ms_per_step": 1e3 * sec, "dtype": "f64 state, bf16 network with f32 accumulation",
+            "training": {"pairs": n, "adam_steps": 6000, "batch": 16384,
+                         "loss_first": hist[0], "loss_last50": float(np.mean(hist[-50:])),
+                         "record_s": t1 - t0, "train_s": t2 - t1},
+            "domain_mean_error_vs_plain": errs,
+            "note": "network 9-128-128-128-6 in place of micro_proc's adjustment and processes "
+                    "inside its training envelope (plain scheme outside), water kept exactly by "
+                    "the fixer; one GCM step from the bench state"}
+
+
 def state_error(d64, d32, torch):
     """Normwise relative difference max|a-b|/max|a| of the mixed-precision state and
     domain-mean outputs against the FP64 ones, computed on the device."""
@@ -210,6 +266,8 @@ def main():
                     help="skip the mixed-precision measurement")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch measurement")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-microphysics measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale for the HBM "
                          "byte counters)")
@@ -258,6 +316,10 @@ def main():
                          "reference profiles, statistics and outputs; error after the same "
                          f"{args.warmup + args.steps} GCM steps from the same initial state"}
 
+    learned = None
+    if not args.no_learned:
+        learned = measure_learned(case, args, torch)
+
     uneven = None
     if not args.no_uneven:
         uneven = {p: measure_uneven(p, args, torch) for p in (("fp64",) if args.fp64_only else ("fp64", "mixed"))}
@@ -290,6 +352,7 @@ def main():
         "kernel_ms_per_step_warmup": r64["kernel_ms"],
         "mixed_precision": mixed,
         "uneven_batch": uneven,
+        "learned_microphysics": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -7,7 +7,8 @@
  *   inputs  T, p, rho, qv, qcl, qci, qpl, qpi, w           (K, hPa, kg/m3, kg/kg x5, m/s)
  *   outputs d/dt of qv, qcl, qci, qpl, qpi, and of T        (1/s, K/s)
  * BF16 weights and activations on the matrix cores with FP32 accumulation; FP32 master
- * weights and Adam moments for training.  Inputs are normalised as (x - mean) / scale and
+ * weights and Adam moments for training.  Inputs are normalised as (x - mean) * (1 / scale)
+ * (the reciprocal rounded to FP32 once) and
  * outputs de-normalised by oscale (crm_mlp_set_norm); lo/hi bound the training envelope.
  *
  * All array arguments of the compute entry points are device pointers.

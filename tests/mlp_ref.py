@@ -37,7 +37,8 @@ def forward(p, X, mean, scale, oscale, rounding="bf16"):
     """Raw inputs X[n,9] -> (de-normalised outputs [n,6], cache for backward)."""
     dt = np.float32 if rounding == "bf16" else np.float64
     W1, b1, W2, b2, W3, b3, W4, b4 = [np.asarray(a, dtype=dt) for a in unpack(p)]
-    Xn = _r(((np.asarray(X, dt) - mean) / scale).astype(dt), rounding)
+    rs = (dt(1) / np.asarray(scale, dt)).astype(dt)  # the reciprocal, rounded once
+    Xn = _r(((np.asarray(X, dt) - np.asarray(mean, dt)) * rs).astype(dt), rounding)
     Ws = [_r(W, rounding) for W in (W1, W2, W3, W4)]
     H1 = _r(np.maximum(Xn @ Ws[0] + b1, 0), rounding)
     H2 = _r(np.maximum(H1 @ Ws[1] + b2, 0), rounding)
