@@ -332,6 +332,65 @@ struct Seg {
   __device__ __forceinline__ int imin(int v) const {
     return reduce(v, NX, [](int a, int b) { return ::min(a, b); });
   }
+  // Segment sums of N values at once; afterwards v[j] holds the sum of value j in the lanes
+  // where owns(j) is true (one lane per segment).  For NX == 64 the reduction is transposed:
+  // v_permlane32_swap on two different values gives each half of the wave the other half's
+  // share of one of them, so one swap and one add halve two sums at once; then
+  // v_permlane16_swap does the same for rows, and a row butterfly finishes.  Value j ends in
+  // every lane of row ROW[j % 4] of slot j / 4 (lanes 0-31 took the first value of each pair
+  // at the first step, rows 0 and 2 the first of each pair at the second).
+  template <int N>
+  __device__ __forceinline__ void multi_sum(double (&v)[N]) const {
+#ifndef CRM_SEG_BPERMUTE
+    if constexpr (NX == 64 && N > 2) {
+      constexpr int P = (N + 1) / 2, Q = (P + 1) / 2;
+      double a[2 * Q];
+#pragma unroll
+      for (int p = 0; p < P; p++) {
+        double x = v[2 * p], y = 2 * p + 1 < N ? v[2 * p + 1] : 0.0;
+        swap32(x, y);
+        a[p] = x + y;
+      }
+      if (P & 1) a[P] = 0.0;
+#pragma unroll
+      for (int t = 0; t < Q; t++) {
+        double x = a[2 * t], y = a[2 * t + 1];
+        swap16(x, y);
+        const double r = reduce(x + y, 16, [](double e, double f) { return e + f; });
+#pragma unroll
+        for (int j = 4 * t; j < 4 * t + 4 && j < N; j++) v[j] = r;
+      }
+      return;
+    }
+#endif
+#pragma unroll
+    for (int j = 0; j < N; j++) v[j] = sum(v[j]);
+  }
+  __device__ __forceinline__ bool owns(int j) const {
+#ifndef CRM_SEG_BPERMUTE
+    if constexpr (NX == 64) {
+      constexpr int ROW[4] = {0, 2, 1, 3};
+      return lane == 16 * ROW[j & 3];
+    }
+#endif
+    return i == 0;
+  }
+#ifndef CRM_SEG_BPERMUTE
+  // (a, b) -> (a with its upper half replaced by b's lower half, b with its lower half
+  // replaced by a's upper half); swap16 does the same for odd/even 16-lane rows
+  __device__ __forceinline__ static void swap32(double& a, double& b) {
+    auto h = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    auto l = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    a = __hiloint2double((int)h[0], (int)l[0]);
+    b = __hiloint2double((int)h[1], (int)l[1]);
+  }
+  __device__ __forceinline__ static void swap16(double& a, double& b) {
+    auto h = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    auto l = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    a = __hiloint2double((int)h[0], (int)l[0]);
+    b = __hiloint2double((int)h[1], (int)l[1]);
+  }
+#endif
 };
 
 // A 3D CRM of 8 x 8 columns on one wave: lane = 8 j + i.  i here is the column index within

@@ -2463,9 +2463,8 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       else if (X.qpl + X.qpi > R4(1.0e-4))
         mdi = rhow[k] * wk;
     }
-    double s_top = s.sum(top_hit), s_cld = s.sum(cld), s_mcup = s.sum(mcup), s_mcdn = s.sum(mcdn);
-    double s_mcuup = s.sum(mcuup), s_mcudn = s.sum(mcudn), s_liq = s.sum(qcl), s_ice = s.sum(qci);
-    double s_mui = s.sum(mui), s_mdi = s.sum(mdi);
+    double sv[10] = {top_hit, cld, mcup, mcdn, mcuup, mcudn, qcl, qci, mui, mdi};
+    s.multi_sum(sv);
     // radiation-column sums (crm_module.F90:979-1015)
     double r_t = tabs, r_qv = fmax(0.0, cur.qv), r_qc = qcl, r_qi = qci, r_cld = (qcl + qci > 0) ? 1.0 : 0.0;
     if (g > 1) {
@@ -2476,18 +2475,17 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       r_cld = s.gsum(r_cld, g);
     }
     if (live) {
-      if (i == 0) {
-        accum(&prof(d, P_CLDTOP, c)[nzm - 1 - k], s_top);  // the reference's index mix (crm_module.F90:918-925)
-        accum(&prof(d, P_CLD, c)[k], s_cld);
-        accum(&prof(d, P_MCUP, c)[k], s_mcup);
-        accum(&prof(d, P_MCDN, c)[k], s_mcdn);
-        accum(&prof(d, P_MCUUP, c)[k], s_mcuup);
-        accum(&prof(d, P_MCUDN, c)[k], s_mcudn);
-        accum(&prof(d, P_GLIQ, c)[k], s_liq);
-        accum(&prof(d, P_GICE, c)[k], s_ice);
-        accum(&prof(d, P_MUI, c)[k], s_mui);
-        accum(&prof(d, P_MDI, c)[k], s_mdi);
-      }
+      // the reference's index mix for the cloud top (crm_module.F90:918-925)
+      if (s.owns(0)) accum(&prof(d, P_CLDTOP, c)[nzm - 1 - k], sv[0]);
+      if (s.owns(1)) accum(&prof(d, P_CLD, c)[k], sv[1]);
+      if (s.owns(2)) accum(&prof(d, P_MCUP, c)[k], sv[2]);
+      if (s.owns(3)) accum(&prof(d, P_MCDN, c)[k], sv[3]);
+      if (s.owns(4)) accum(&prof(d, P_MCUUP, c)[k], sv[4]);
+      if (s.owns(5)) accum(&prof(d, P_MCUDN, c)[k], sv[5]);
+      if (s.owns(6)) accum(&prof(d, P_GLIQ, c)[k], sv[6]);
+      if (s.owns(7)) accum(&prof(d, P_GICE, c)[k], sv[7]);
+      if (s.owns(8)) accum(&prof(d, P_MUI, c)[k], sv[8]);
+      if (s.owns(9)) accum(&prof(d, P_MDI, c)[k], sv[9]);
       if (rad_owner) {
         d.rad[0 * rad_blk + rad_base] = ra[0] + r_t;
         d.rad[1 * rad_blk + rad_base] = ra[1] + r_qv;
@@ -2500,13 +2498,13 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
     up = cur;
     cur = nxt;
   }
-  double a = s.sum(cwp > thr ? ctt : 0.0), b = s.sum(cwph > thr ? cht : 0.0);
-  double m = s.sum(cwpm > thr ? cmt : 0.0), l = s.sum(cwpl > thr ? clt : 0.0);
-  if (live && i == 0) {
-    accum(&inst(d, I_CLTOT, c), a);
-    accum(&inst(d, I_CLHGH, c), b);
-    accum(&inst(d, I_CLMED, c), m);
-    accum(&inst(d, I_CLLOW, c), l);
+  double cv[4] = {cwp > thr ? ctt : 0.0, cwph > thr ? cht : 0.0, cwpm > thr ? cmt : 0.0, cwpl > thr ? clt : 0.0};
+  s.multi_sum(cv);
+  if (live) {
+    if (s.owns(0)) accum(&inst(d, I_CLTOT, c), cv[0]);
+    if (s.owns(1)) accum(&inst(d, I_CLHGH, c), cv[1]);
+    if (s.owns(2)) accum(&inst(d, I_CLMED, c), cv[2]);
+    if (s.owns(3)) accum(&inst(d, I_CLLOW, c), cv[3]);
   }
 }
 
