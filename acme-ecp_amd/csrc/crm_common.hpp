@@ -105,6 +105,8 @@ enum Prof {
   P_Z = 0, P_ZI, P_PRES, P_PRESI, P_ADZ, P_ADZW, P_RHO, P_RHOW, P_BET, P_GAMAZ,
   P_GRDFX, P_TAU,                                          // reference state (per call)
   P_IRHO, P_IADZ, P_IRHOADZ, P_IRHOWADZ, P_DDADZ, P_PRATIO,  // derived, per call
+  P_SRDZ, P_SRDX, P_SRDZW, P_SRDXW, P_SBETDZ, P_STKMAX,    // SGS level factors, per call
+  P_DRHOI,                                                 // scalar diffusion, per call
   P_T0, P_Q0, P_QV0, P_TABS0, P_U0, P_QN0, P_QP0,          // domain means (diagnose)
   P_V0,                                                    // (3D only)
   P_TTEND, P_QTEND, P_UTEND,                               // large-scale tendencies
