@@ -50,7 +50,8 @@ STATE_WORDS_PER_LEVEL = 16
 
 # HBM bytes per launch measured with rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE), each
 # scaled by what the same counter reads for a device copy of known size; written next to the
-# command that produced it.  Lines: "<kernel class> <fetch bytes> <write bytes>".
+# command that produced it.  Lines: "<kernel class> <fetch bytes> <write bytes>", the FP32
+# (mixed-precision) engine's kernels prefixed "mixed.".
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_4096x64x58.txt")
 
 
@@ -134,7 +135,8 @@ def measure(prec, case, args, torch):
             "kernel_ms": {k: v[0] / max(1, args.warmup) for k, v in warm.items()},
             "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "traffic": None, "avg_launch_ms": avg_ms, "launches": klaunch,
+                         "traffic": measured_traffic(dominant if prec == "fp64" else "mixed." + dominant),
+                         "avg_launch_ms": avg_ms, "launches": klaunch,
                          "algorithmic_bytes_per_launch": bytes_launch}}
 
 
@@ -260,7 +262,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--ncrms", type=int, default=WORKLOAD["ncrms"])
-    ap.add_argument("--cpu-ncrms", type=int, default=32,
+    ap.add_argument("--cpu-ncrms", type=int, default=64,
                     help="instances in the CPU baseline sample (0 skips it)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="skip the mixed-precision measurement")
@@ -346,7 +348,7 @@ def main():
                                f"step ({nstop} substeps) per call, inputs resident in HBM",
                    "ncrms": ncrms, "nx": w["nx"], "nz": w["nz"], "substeps": nstop,
                    "precision": "fp64", "max_ncycle": r64["max_ncycle"]},
-        "roofline": dict(r64["roofline"], traffic=measured_traffic(r64["roofline"]["kernel"])),
+        "roofline": r64["roofline"],
         "whole_step_hbm": {"achieved": whole, "frac": whole / HBM_PEAK_GBS, "unit": "GB/s",
                            "bytes_per_column_substep": STATE_WORDS_PER_LEVEL * 8 * w["nz"]},
         "kernel_ms_per_step_warmup": r64["kernel_ms"],
