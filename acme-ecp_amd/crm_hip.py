@@ -176,7 +176,7 @@ class CrmHip:
         return rc
 
     def accel_init(self, use_crm_accel, crm_accel_factor=0.0, crm_accel_uv=False):
-        """Mean-state acceleration for later calls (crm_accel_init, accelerate_crm.F90:16-42)."""
+        """Mean-state acceleration for later calls (crm_accel_init, accelerate_crm.F90:30-38,45-75)."""
         rc = self.lib.crm_hip_accel_init(self.ctx, int(bool(use_crm_accel)), float(crm_accel_factor),
                                          int(bool(crm_accel_uv)))
         if rc:

@@ -474,7 +474,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   if (!(dt_gl >= dm->dt) || !(dt_gl / dm->dt < 1.0e6)) return CRM_ERR_ARG;
   int nstop = (int)(dt_gl / dm->dt);
   const double dt = dt_gl / nstop;
-  // mean-state acceleration: crm_accel_nstop (accelerate_crm.F90:44-62, crm_module.F90:718-720)
+  // mean-state acceleration: crm_accel_nstop (accelerate_crm.F90:78-106, crm_module.F90:718-720)
   const bool acc = ctx->acc_use != 0;
   const double acc_f = ctx->acc_factor;
   bool ceased = false;

@@ -100,7 +100,7 @@ module crm_hip_bindings
       type(crm_output_view), intent(inout) :: out
     end function
 
-    ! crm_accel_init replacement (accelerate_crm.F90:16-42)
+    ! crm_accel_init replacement (accelerate_crm.F90:30-38,45-75)
     integer(c_int) function crm_hip_accel_init(ctx, use_crm_accel, crm_accel_factor, crm_accel_uv) &
         bind(C, name="crm_hip_accel_init")
       import :: c_int, c_ptr, c_double

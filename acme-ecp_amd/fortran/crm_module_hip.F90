@@ -14,7 +14,7 @@
 ! and the CRM type modules); crm_hip_bindings.F90 holds the ISO_C_BINDING layer and builds
 ! on its own.  Not covered: CLUBB_CRM, MAML, ECPP statistics (crm_ecpp_output is accepted
 ! and left untouched).  Mean-state acceleration takes its settings from the physics namelist
-! as crm_accel_init does (accelerate_crm.F90:16-42).
+! as crm_accel_init does (accelerate_crm.F90:30-38,45-75).
 module crm_module
   use iso_c_binding
   use shr_kind_mod,          only: r8 => shr_kind_r8

@@ -45,7 +45,7 @@ enum {
   CRM_ERR_DEVICE = 6,     /* HIP runtime error                                              */
   CRM_ERR_NOMEM = 7,      /* device allocation failed                                       */
   CRM_ERR_UNSUPPORTED = 8, /* configuration not provided by this build                      */
-  CRM_ERR_ACCEL = 9       /* 1 + crm_accel_factor does not divide nstop (accelerate_crm.F90:51-58) */
+  CRM_ERR_ACCEL = 9       /* 1 + crm_accel_factor does not divide nstop (accelerate_crm.F90:78-106) */
 };
 
 /* Arithmetic of the bulk stencils. The pressure solve (DFT + Thomas sweep), the hydrostatic
@@ -161,7 +161,7 @@ int crm_hip_run_device(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_
                        crm_rad_view *rad, crm_output_view *out);
 
 /* Mean-state acceleration settings of a context, the replacement of crm_accel_init
- * (accelerate_crm.F90:16-42, which reads them from the physics namelist): with
+ * (accelerate_crm.F90:30-38,45-75, which reads them from the physics namelist): with
  * use_crm_accel != 0 every later call integrates nstop / (1 + crm_accel_factor) substeps and
  * amplifies the change of the horizontal-mean t and qt (and u, v when crm_accel_uv != 0) of
  * each subcycle by crm_accel_factor (accelerate_crm.F90:109-366, crm_module.F90:718-720,

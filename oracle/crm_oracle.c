@@ -1957,7 +1957,7 @@ static void diagnose(void) {
 }
 
 /* Mean-state acceleration, accelerate_crm.F90 (Jones, Bretherton and Pritchard 2015).
- * Settings of crm_accel_init (accelerate_crm.F90:16-42): use_crm_accel, crm_accel_factor
+ * Settings of crm_accel_init (accelerate_crm.F90:30-38,45-75): use_crm_accel, crm_accel_factor
  * (1 + factor = 'a'), crm_accel_uv. */
 static struct {
   int use;
@@ -2252,7 +2252,7 @@ int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, cons
   S.nstop = (int)(dt_gl / S.dt);
   S.dt = dt_gl / S.nstop;
   double icrm_run_time = 1. / dt_gl;
-  /* crm_accel_nstop (accelerate_crm.F90:44-62), crm_module.F90:718-720 */
+  /* crm_accel_nstop (accelerate_crm.F90:78-106), crm_module.F90:718-720 */
   int accel_cease = 0;
   if (ACC.use) {
     if (S.nstop % (int)(1 + ACC.factor) != 0) {

@@ -75,7 +75,7 @@ def oracle():
 
 def oracle_accel_init(use_crm_accel, crm_accel_factor=0.0, crm_accel_uv=False):
     """Mean-state acceleration settings of the C restatement (process-wide, like the
-    reference's module variables set by crm_accel_init, accelerate_crm.F90:16-42)."""
+    reference's module variables set by crm_accel_init, accelerate_crm.F90:30-38,45-75)."""
     lib = C.CDLL(ORACLE_SO)
     lib.oracle_accel_init.argtypes = [C.c_int, C.c_double, C.c_int]
     lib.oracle_accel_init(int(bool(use_crm_accel)), float(crm_accel_factor), int(bool(crm_accel_uv)))
