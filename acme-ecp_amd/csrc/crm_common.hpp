@@ -334,6 +334,9 @@ struct Seg {
   __device__ __forceinline__ int imin(int v) const {
     return reduce(v, NX, [](int a, int b) { return ::min(a, b); });
   }
+  // true when p holds in some lane of the wave (a superset of the segment: every segment of
+  // the wave takes the branch it guards, so control flow stays uniform)
+  __device__ __forceinline__ bool any(bool p) const { return __any(p); }
   // Segment sums of N values at once; afterwards v[j] holds the sum of value j in the lanes
   // where owns(j) is true (one lane per segment).  For NX == 64 the reduction is transposed:
   // v_permlane32_swap on two different values gives each half of the wave the other half's
@@ -420,7 +423,102 @@ struct Seg3 {
   __device__ __forceinline__ T max(T v) const { return all.max(v); }
   __device__ __forceinline__ int imax(int v) const { return all.imax(v); }
   __device__ __forceinline__ int imin(int v) const { return all.imin(v); }
+  __device__ __forceinline__ bool any(bool p) const { return __any(p); }
 };
+
+// A 2D CRM of 128 columns on the two waves of a 128-thread workgroup, one instance per
+// workgroup: column i = 64 * (wave & 1) + lane.
+// - A rotation moves the wave's own 64 lanes with DPP and fetches the one column at the seam
+//   from the other wave through LDS.
+// - A reduction combines the two waves' butterflies through LDS, wave 0's part first, so both
+//   waves hold the bitwise same result.
+// Every exchange is one workgroup barrier, so both waves must make every member call in the
+// same order; the kernels' control flow around these calls is uniform per instance.  Two LDS
+// slots alternate: a slot is written again only after the barrier of the call in between.
+struct Seg128 {
+  static constexpr bool Y3 = false;
+  static constexpr int SLOTS = 16;  // values per wave and exchange
+  Seg<64> w;
+  int lane, i, half;
+  mutable int slot = 0;
+  __device__ __forceinline__ Seg128() {
+    lane = threadIdx.x & 63;
+    half = (threadIdx.x >> 6) & 1;
+    i = 64 * half + lane;
+  }
+  // this call's exchange area: [2 waves][SLOTS] doubles
+  __device__ __forceinline__ double* xbuf() const {
+    __shared__ double buf[2 * 2 * SLOTS];
+    double* p = buf + slot * 2 * SLOTS;
+    slot ^= 1;
+    return p;
+  }
+  // value held by column (i + d) mod 128, d = +1 or -1
+  template <typename T>
+  __device__ __forceinline__ T rot(T v, int d) const {
+    T r = w.rot(v, d);
+    double* p = xbuf();
+    if (lane == (d > 0 ? 0 : 63)) p[half] = (double)v;
+    __syncthreads();
+    if (lane == (d > 0 ? 63 : 0)) r = (T)p[half ^ 1];
+    return r;
+  }
+  template <typename T, typename Op>
+  __device__ __forceinline__ T combine(T v, Op op) const {
+    double* p = xbuf();
+    if (lane == 0) p[half] = (double)v;
+    __syncthreads();
+    return op((T)p[0], (T)p[1]);
+  }
+  template <typename T>
+  __device__ __forceinline__ T sum(T v) const {
+    return combine(w.sum(v), [](T a, T b) { return a + b; });
+  }
+  // sum over aligned groups of g lanes (g a power of two dividing 128)
+  template <typename T>
+  __device__ __forceinline__ T gsum(T v, int g) const {
+    return g <= 64 ? w.gsum(v, g) : sum(v);
+  }
+  template <typename T>
+  __device__ __forceinline__ T max(T v) const {
+    return combine(w.max(v), [](T a, T b) { return fmax(a, b); });
+  }
+  __device__ __forceinline__ int imax(int v) const {
+    return combine(w.imax(v), [](int a, int b) { return ::max(a, b); });
+  }
+  __device__ __forceinline__ int imin(int v) const {
+    return combine(w.imin(v), [](int a, int b) { return ::min(a, b); });
+  }
+  // true when p holds in some lane of either wave: both waves take the branch it guards
+  __device__ __forceinline__ bool any(bool p) const {
+    return combine((int)__any(p), [](int a, int b) { return a | b; }) != 0;
+  }
+  template <int N>
+  __device__ __forceinline__ void multi_sum(double (&v)[N]) const {
+    static_assert(N <= SLOTS, "too many values for one exchange");
+    w.multi_sum(v);
+    double* p = xbuf();
+#pragma unroll
+    for (int j = 0; j < N; j++)
+      if (w.owns(j)) p[half * SLOTS + j] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < N; j++) v[j] = p[j] + p[SLOTS + j];
+  }
+  __device__ __forceinline__ bool owns(int) const { return i == 0; }
+};
+
+// The lane geometry of a 2D CRM of NX columns.
+template <int NX>
+struct SegSel {
+  using type = Seg<NX>;
+};
+template <>
+struct SegSel<128> {
+  using type = Seg128;
+};
+template <int NX>
+using SegFor = typename SegSel<NX>::type;
 
 // Instance of this thread's segment; false when past ncrms (the segment then idles but still
 // joins cross-lane operations of its wave).

@@ -198,7 +198,7 @@ static void prof_harvest(Engine<R>* ctx) {
     }                                                                               \
   } while (0)
 
-static bool supported_nx(int nx) { return nx == 8 || nx == 16 || nx == 32 || nx == 64; }
+static bool supported_nx(int nx) { return nx == 8 || nx == 16 || nx == 32 || nx == 64 || nx == 128; }
 static bool supported_nz(int nz) { return nz == 28 || nz == 58; }
 
 // REAL(4) Gamma of the reference (gammafff.c:13 via precip_init.F90:24-45)
@@ -324,17 +324,22 @@ static int get_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform& o
 // ---- kernel dispatch over the compile-time column count ------------------------------------
 template <int NX, typename R, typename G>
 static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, const DevT<R>& d, int na, int nb, int nc) {
-  if (NX == 64) {  // matrix-core transforms; one instance per block
+  if constexpr (NX == 64) {  // matrix-core transforms; one instance per block
     if (nzm == 28)
       k_press64<28, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
     else
       k_press64<58, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
-    return;
+  } else if constexpr (NX == 128) {  // one instance per block of 4 waves
+    if (nzm == 28)
+      k_press128<28, R><<<dim3((unsigned)d.ncrms), b, (size_t)28 * 128 * sizeof(double), s>>>(d, na, nb, nc);
+    else
+      k_press128<58, R><<<dim3((unsigned)d.ncrms), b, (size_t)58 * 128 * sizeof(double), s>>>(d, na, nb, nc);
+  } else {
+    if (nzm == 28)
+      k_press<NX, 28, R><<<g, b, sh, s>>>(d, na, nb, nc);
+    else
+      k_press<NX, 58, R><<<g, b, sh, s>>>(d, na, nb, nc);
   }
-  if (nzm == 28)
-    k_press<NX, 28, R><<<g, b, sh, s>>>(d, na, nb, nc);
-  else
-    k_press<NX, 58, R><<<g, b, sh, s>>>(d, na, nb, nc);
 }
 
 struct Launcher {
@@ -342,9 +347,12 @@ struct Launcher {
   hipStream_t s;
   const int* block;  // threads per workgroup of each kernel class
   dim3 grid_inst(int b) const { return dim3((unsigned)((ncrms + b - 1) / b)); }
+  // threads per workgroup of segment-kernel class kid: 128-column instances span two waves
+  // whose exchanges are workgroup barriers, so they run one instance per workgroup
+  int bs(int kid) const { return nx == 128 ? 128 : block[kid]; }
   // grid and workgroup of segment-kernel class kid
-  dim3 g(int kid) const { return dim3((unsigned)(((size_t)ncrms * nx + block[kid] - 1) / block[kid])); }
-  dim3 b(int kid) const { return dim3((unsigned)block[kid]); }
+  dim3 g(int kid) const { return dim3((unsigned)(((size_t)ncrms * nx + bs(kid) - 1) / bs(kid))); }
+  dim3 b(int kid) const { return dim3((unsigned)bs(kid)); }
 };
 
 // NXV: columns per instance; GEO: the lane geometry (a 2D row of NXV columns, or the 8 x 8
@@ -369,6 +377,11 @@ struct Launcher {
       case 32: {                    \
         constexpr int NXV = 32;     \
         using GEO = Seg<32>;        \
+        CALL;                       \
+      } break;                      \
+      case 128: {                   \
+        constexpr int NXV = 128;    \
+        using GEO = Seg128;         \
         CALL;                       \
       } break;                      \
       default: {                    \
@@ -507,7 +520,9 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   d.samp_stride = ctx->samp_stride;
   double dt3[3] = {0.0, 0.0, 0.0};
   int na = 0, nb = 1, nc = 2;
-  const dim3 gpress((unsigned)((ncrms + (64 / nx) - 1) / (64 / nx)));
+  // k_press: the 64/nx instances of one wave per block (k_press64, k_press128: one per block)
+  const int press_per_block = nx < 64 ? 64 / nx : 1;
+  const dim3 gpress((unsigned)((ncrms + press_per_block - 1) / press_per_block));
   const size_t press_lds = (size_t)nzm * 64 * sizeof(double);
   unsigned long long err_flag = 0;
   HIPCHECK(hipEventRecord(ctx->ev0, s));

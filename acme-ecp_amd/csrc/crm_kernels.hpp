@@ -184,7 +184,7 @@ __device__ __forceinline__ creal A_GR() { return 1. / (TGRMAX - TGRMIN); }
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_init(DevT<R> d, crm_input_view in, crm_state_view st, crm_output_view out, double dt_gl) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, ld = d.ld, plev = d.plev, i = s.i;
@@ -362,7 +362,7 @@ __global__ void k_setperturb(DevT<R> d, const int* gcol) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_kurant(DevT<R> d) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
@@ -414,7 +414,7 @@ __global__ void CRM_LB k_kurant(DevT<R> d) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
@@ -508,10 +508,10 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
     // large-scale forcing with level-wise refill of negative vapour
     t = t + ttend[k] * dtn;
     qt = qt + qtend[k] * dtn;
-    // the level sums only matter when some column went negative; the vote is over the whole
-    // wave, so every lane of a segment takes the same branch
+    // the level sums only matter when some column went negative; the vote covers every lane
+    // of the instance (and of the wave), so all of them take the same branch
     const bool neg = qt < R(0.);
-    if (__any(neg)) {
+    if (s.any(neg)) {
       R nneg = s.sum(neg ? R(1.0) : R(0.0));
       R qneg = s.sum(neg ? qt : R(0.0));
       R qpoz = s.sum(neg ? R(0.0) : qt);
@@ -607,7 +607,7 @@ __device__ __forceinline__ R shear3(const G& s, int k, const R* srdz, const R* s
 // Smagorinsky eddy diffusivity: shear_prod2D.F90:7-96 (3D: shear_prod3D.F90:7-140) and
 // tke_full.F90:11-317 (dosmagor).
 // ------------------------------------------------------------------------------------------
-template <int NX, typename R, typename G = Seg<NX>>
+template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_sgs(DevT<R> d) {
   G s;
   int c;
@@ -772,7 +772,7 @@ __global__ void CRM_LB k_sgs(DevT<R> d) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __device__ __forceinline__ void mom2(const DevT<R>& d, int na) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
@@ -1098,7 +1098,7 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
   }
 }
 
-template <int NX, typename R, typename G = Seg<NX>>
+template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_mom(DevT<R> d, int na) {
   if constexpr (G::Y3)
     mom3<R>(d, na);
@@ -1121,7 +1121,7 @@ __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
   const int cg = blockIdx.x * (64 / NX) + lane / NX;
   const bool live = cg < d.ncrms;
   const int c = live ? cg : d.ncrms - 1;
-  Seg<NX> s;
+  SegFor<NX> s;
   const int nz = NZM + 1;
   const double dz = inst(d, I_DZ, c);
   const double* adz = prof(d, P_ADZ, c);
@@ -1239,6 +1239,133 @@ __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
     }
   }
   if (wave == 0 && live) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Pressure for 128-column CRMs: k_press's algorithm and operation order with one instance per
+// block of 4 waves, as two groups of 128 columns to which the levels are dealt alternately.
+// The slab [NZM][128] lives in LDS.  Column neighbours are read from memory (right-hand side)
+// or from the slab (gradient), so no cross-wave exchange is needed; the Thomas sweep runs on
+// group 0.
+// ------------------------------------------------------------------------------------------
+template <int NZM, typename R>
+__global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
+  constexpr int NX = 128;
+  extern __shared__ double slab[];  // [NZM][128]
+  const int i = threadIdx.x & (NX - 1), grp = threadIdx.x >> 7;
+  const int c = blockIdx.x;  // one block per instance
+  const int ip = (i + 1) & (NX - 1), im = (i - 1) & (NX - 1);
+  const int nz = NZM + 1;
+  const double dz = inst(d, I_DZ, c);
+  const double* adz = prof(d, P_ADZ, c);
+  const double* adzw = prof(d, P_ADZW, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  auto P = [&](int k, int col) -> double& { return slab[(size_t)k * NX + col]; };
+  const double rdx = 1. / d.dx, btat = d.bt / d.at, ctat = d.ct / d.at;
+  const double dta = 1. / d.dt3na / d.at;
+  const R *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
+  const R *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
+  // 1. right-hand side
+  for (int k = grp; k < NZM; k += 2) {
+    const size_t row = ((size_t)c * NZM + k) * NX, o = row + i, op = row + ip;
+    const size_t ow = ((size_t)c * nz + k) * NX + i, own = ow + NX;
+    double rdz = 1. / (adz[k] * dz);
+    double rup = rhow[k + 1] / rho[k] * rdz;
+    double rdn = rhow[k] / rho[k] * rdz;
+    double uu = d.u[o], uap = ua[o], ubp = ub[o], ucp = uc[o];
+    double pv = (rdx * ((double)d.u[op] - uu) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
+                (rdx * ((double)ua[op] - uap) + (wa[own] * rup - wa[ow] * rdn)) +
+                btat * (rdx * ((double)ub[op] - ubp) + (wb[own] * rup - wb[ow] * rdn)) +
+                ctat * (rdx * ((double)uc[op] - ucp) + (wc[own] * rup - wc[ow] * rdn));
+    P(k, i) = pv * rho[k];
+  }
+  __syncthreads();
+  // 2. forward transform (every thread reads its whole row before any entry is overwritten)
+  for (int k0 = 0; k0 < NZM; k0 += 2) {
+    const int k = k0 + grp;
+    double acc = 0.0;
+    if (k < NZM) {
+#pragma unroll 8
+      for (int q = 0; q < NX; q++) acc += d.Ft[q * NX + i] * P(k, q);
+    }
+    __syncthreads();
+    if (k < NZM) P(k, i) = acc;
+  }
+  __syncthreads();
+  // 3. Thomas sweep per slot (group 0)
+  if (grp == 0) {
+    const double eig = d.eign[i];
+    double alfa[NZM];
+    auto acoef = [&](int k) { return rhow[k] / (adz[k] * adzw[k] * dz * dz); };
+    auto ccoef = [&](int k) { return rhow[k + 1] / (adz[k] * adzw[k + 1] * dz * dz); };
+    double b;
+    if (i == 0)
+      b = 1.0 / (eig * rho[0] - acoef(0) - ccoef(0));
+    else
+      b = 1.0 / (eig * rho[0] - ccoef(0));
+    alfa[0] = -ccoef(0) * b;
+    double beta = P(0, i) * b;
+    P(0, i) = beta;
+#pragma unroll
+    for (int k = 1; k < NZM - 1; k++) {
+      double ak = acoef(k), ck = ccoef(k);
+      double e = 1.0 / (eig * rho[k] - ak - ck + ak * alfa[k - 1]);
+      alfa[k] = -ck * e;
+      beta = (P(k, i) - ak * beta) * e;
+      P(k, i) = beta;
+    }
+    {
+      int k = NZM - 1;
+      double ak = acoef(k);
+      double top = (P(k, i) - ak * beta) / (eig * rho[k] - ak + ak * alfa[k - 1]);
+      P(k, i) = top;
+#pragma unroll
+      for (int kk = NZM - 2; kk >= 0; kk--) {
+        top = alfa[kk] * top + P(kk, i);
+        P(kk, i) = top;
+      }
+    }
+  }
+  __syncthreads();
+  // 4. inverse transform
+  for (int k0 = 0; k0 < NZM; k0 += 2) {
+    const int k = k0 + grp;
+    double acc = 0.0;
+    if (k < NZM) {
+#pragma unroll 8
+      for (int q = 0; q < NX; q++) acc += d.Gt[q * NX + i] * P(k, q);
+    }
+    __syncthreads();
+    if (k < NZM) P(k, i) = acc;
+  }
+  __syncthreads();
+  // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities
+  const double dtdx = d.dtn / d.dx, dtdz = d.dtn / dz, dt3 = d.dt3na;
+  R* uan = d.dudt[na];
+  R* wan = d.dwdt[na];
+  R* unc = d.dudt[nc];
+  R* wnc = d.dwdt[nc];
+  for (int k = grp; k < NZM; k += 2) {
+    size_t o = ((size_t)c * NZM + k) * NX + i;
+    size_t ow = ((size_t)c * nz + k) * NX + i;
+    double pk = P(k, i);
+    double pim = P(k, im);
+    double pkb = P(k > 0 ? k - 1 : 0, i);
+    double rdz = 1. / (dz * adzw[k]);
+    double du_a = ua[o] - (pk - pim) * rdx;
+    double dw_a = wa[ow] - (pk - pkb) * rdz;
+    double uu = d.u[o], ww = d.w[ow];
+    double un = uu + dt3 * (d.at * du_a + d.bt * ub[o] + d.ct * uc[o]);
+    double wn = ww + dt3 * (d.at * dw_a + d.bt * wb[ow] + d.ct * wc[ow]);
+    uan[o] = du_a;
+    wan[ow] = dw_a;
+    unc[o] = un;
+    wnc[ow] = wn;
+    d.u[o] = 0.5 * (uu + un) * (rho[k] * dtdx);
+    d.w[ow] = 0.5 * (ww + wn) * (rhow[k] * dtdz);
+  }
+  if (grp == 0) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1697,7 +1824,7 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
 }
 
 // advect_all_scalars.F90:7-94: t, then the two microphysics fields.
-template <int NX, typename R, typename G = Seg<NX>>
+template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_mpdata(DevT<R> d) {
   G s;
   int c;
@@ -1714,7 +1841,7 @@ __global__ void CRM_LB k_mpdata(DevT<R> d) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_precip(DevT<R> d) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, i = s.i;
@@ -1909,7 +2036,7 @@ __device__ void diffuse_one(const DevT<R>& d, const G& s, int c, bool live, R* f
   if (flux && live && i == 0) flux[0] = R(0.0);
 }
 
-template <int NX, typename R, typename G = Seg<NX>>
+template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_diffuse(DevT<R> d) {
   G s;
   int c;
@@ -2074,7 +2201,7 @@ __device__ __forceinline__ void accel_means(const DevT<R>& d, const S_& s, int c
 
 template <int NX, typename R>
 __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, i = s.i;
@@ -2339,7 +2466,7 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_accel(DevT<R> d, double factor, int uv) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, i = s.i;
@@ -2401,7 +2528,7 @@ __global__ void CRM_LB k_accel(DevT<R> d, double factor, int uv) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_stats(DevT<R> d) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
@@ -2550,7 +2677,7 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
 template <int NX, typename R>
 __global__ void CRM_LB k_final(DevT<R> d, crm_input_view in, crm_state_view st, crm_rad_view rad, crm_output_view out,
                         double dt_gl, int nstop) {
-  Seg<NX> s;
+  SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, ld = d.ld, plev = d.plev, i = s.i;

@@ -62,7 +62,8 @@ def test_leading_dimension(hip):
 def test_context_grows_and_shrinks(hip):
     """One context serves calls of different batch sizes and size classes in turn."""
     for n, nx, nz, dx, dt in [(2, 32, 28, 4000.0, 20.0), (70, 32, 28, 4000.0, 20.0),
-                              (3, 64, 58, 1000.0, 5.0), (2, 32, 28, 4000.0, 20.0)]:
+                              (3, 64, 58, 1000.0, 5.0), (2, 128, 58, 1000.0, 5.0),
+                              (2, 32, 28, 4000.0, 20.0)]:
         case = convective_case(n, nx=nx, nz=nz, dx=dx, dt=dt, seed=n)
         want, got = case.copy(), case.copy()
         assert oracle_lib.oracle().run(want, dt) == 0
@@ -84,6 +85,7 @@ def test_cfl_violation_is_reported(hip):
     (dict(nx_rad=3), CRM_ERR_NXRAD),          # crm_module.F90:686-692
     (dict(dt=40.0), CRM_ERR_DAMPING),         # damping.F90:39-42
     (dict(nx=48), CRM_ERR_UNSUPPORTED),       # column count without a compiled kernel
+    (dict(nx=256), CRM_ERR_UNSUPPORTED),
     (dict(nz=40), CRM_ERR_UNSUPPORTED),       # level count without a compiled kernel
     (dict(nx=16, ny=16), CRM_ERR_UNSUPPORTED),  # 3D is compiled for 8 x 8 columns only
     (dict(nx=8, ny=8, nx_rad=2, ny_rad=2), CRM_ERR_UNSUPPORTED),  # 3D radiation groups

@@ -17,6 +17,7 @@ CLASSES = [  # (ncrms, nx, ny, nz, dx, dt): the uneven-batch size classes, scale
     (32, 32, 1, 28, 4000.0, 20.0),
     (16, 64, 1, 58, 1000.0, 5.0),
     (8, 8, 8, 28, 4000.0, 20.0),
+    (3, 128, 1, 58, 1000.0, 5.0),
 ]
 
 

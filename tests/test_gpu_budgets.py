@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "convective_32x28": (lambda: convective_case(3, seed=2), 1200.0),
     "convective_64x58": (lambda: convective_case(2, nx=64, nz=58, dx=1000.0, dt=5.0, seed=3), 200.0),
+    "convective_128x58": (lambda: convective_case(2, nx=128, nz=58, dx=1000.0, dt=5.0, seed=4), 200.0),
     "quiet_32x28": (lambda: make_case(3, 32, 28, 4000.0, 20.0, randomize=True), 1200.0),
     "batch_866x32x28": (lambda: make_case(866, 32, 28, 4000.0, 20.0, randomize=True), 1200.0),
 }

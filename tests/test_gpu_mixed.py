@@ -28,7 +28,7 @@ def hip64():
 
 
 SHAPES = [(4, 32, 28, 4000.0, 20.0), (3, 64, 58, 1000.0, 5.0), (5, 16, 28, 4000.0, 20.0),
-          (9, 8, 28, 4000.0, 20.0)]
+          (9, 8, 28, 4000.0, 20.0), (2, 128, 58, 1000.0, 5.0)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}x{s[2]}")

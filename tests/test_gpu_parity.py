@@ -24,6 +24,7 @@ SHAPES = [  # (ncrms, nx, nz, dx, dt): every compiled column count, both level c
     (3, 64, 58, 1000.0, 5.0),
     (5, 16, 28, 4000.0, 20.0),
     (9, 8, 28, 4000.0, 20.0),
+    (2, 128, 58, 1000.0, 5.0),   # two waves per instance
 ]
 PHASES = [7, 9, 11, 13, 15, 17, 18, 20]
 
@@ -127,15 +128,19 @@ def test_batch_global_subcycling(hip):
     assert max(errs.values()) <= TOL_N, _worst(errs)
 
 
-def test_radiation_groups(hip):
+@pytest.mark.parametrize("shape", [(32, 28, 4000.0, 20.0, 8), (128, 58, 1000.0, 5.0, 1),
+                                   (128, 58, 1000.0, 5.0, 4)], ids=lambda s: f"{s[0]}x{s[1]}_rad{s[4]}")
+def test_radiation_groups(hip, shape):
     """crm_nx_rad < crm_nx: radiation inputs and outputs on column groups
-    (crm_module.F90:766-777, 979-1015, 1086-1107)."""
-    case = convective_case(3, seed=13, nx_rad=8)
+    (crm_module.F90:766-777, 979-1015, 1086-1107); at 128 columns a group spans both waves of
+    an instance (nx_rad 1) or stays inside one (nx_rad 4)."""
+    nx, nz, dx, dt, nx_rad = shape
+    case = convective_case(3, nx=nx, nz=nz, dx=dx, dt=dt, seed=13, nx_rad=nx_rad)
     case.rad["qrad"][...] = np.random.default_rng(1).uniform(-2e-5, 2e-5, case.rad["qrad"].shape)
     want, got = case.copy(), case.copy()
-    assert oracle_lib.oracle().run(want, 60.0) == 0
-    hip.run(got, 60.0)
-    errs = compare_case(got, want, dt_gl=60.0)
+    assert oracle_lib.oracle().run(want, 3 * dt) == 0
+    hip.run(got, 3 * dt)
+    errs = compare_case(got, want, dt_gl=3 * dt)
     assert max(errs.values()) <= TOL_N, _worst(errs)
 
 

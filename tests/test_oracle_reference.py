@@ -77,6 +77,13 @@ def test_bench_class_bitexact():
     _assert_cases_equal(cr, co)
 
 
+def test_128_column_class_bitexact():
+    """The widest 2D class (two waves per instance on the GPU)."""
+    case = convective_case(2, nx=128, nz=58, dx=1000.0, dt=5.0, seed=3)
+    _, _, cr, co = _run_both(case, 50.0)
+    _assert_cases_equal(cr, co)
+
+
 @pytest.mark.parametrize("n", [8, 16, 32, 64, 128])
 @pytest.mark.parametrize("isign", [-1, 1])
 def test_fft991_bitexact(n, isign):
