@@ -127,6 +127,7 @@ struct Engine : CrmEngine {
     if (!on) drop_graphs();
   }
   long long graph_count() const override { return graph_launches; }
+  hipStream_t get_stream() const override { return stream; }
   int set_mlp(const crm_mlp* m) override {
     if (m && crm_mlp_ns::mlp_device(m) != device) return CRM_ERR_ARG;
     mlp = m;

@@ -1,6 +1,8 @@
 // crm_engine.hpp -- the interface between the C ABI (crm_hip.hip) and the device engines,
 // one per bulk-field precision, each compiled in its own translation unit.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include "../../include/crm_hip.h"
 #include "../../include/crm_mlp.h"
 
@@ -25,7 +27,12 @@ struct CrmEngine {
   virtual long long samples_count() const = 0;
   virtual int autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
                        crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) = 0;
+  virtual hipStream_t get_stream() const = 0;
 };
+
+// The device and stream of a context (for the GCM-side kernels of crm_glue.hip).
+int crm_ctx_device(const crm_hip_ctx* ctx);
+hipStream_t crm_ctx_stream(const crm_hip_ctx* ctx);
 
 namespace crm {
 CrmEngine* make_engine_fp64(int device, int* rc);

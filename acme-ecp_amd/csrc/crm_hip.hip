@@ -54,6 +54,9 @@ struct crm_hip_ctx {
   CrmEngine* eng = nullptr;
 };
 
+int crm_ctx_device(const crm_hip_ctx* ctx) { return ctx->device; }
+hipStream_t crm_ctx_stream(const crm_hip_ctx* ctx) { return ctx->eng->get_stream(); }
+
 extern "C" {
 
 int crm_hip_init(crm_hip_ctx** out, const crm_hip_config* cfg) {
