@@ -160,7 +160,7 @@ struct DevT {
   const double *Ft, *Gt;      // transform operators [NX][NX]
   const double *eign;         // [NX] horizontal eigenvalues per slot
   unsigned long long *gmax;   // [0] kurant cfl bits, [1] precipitation cfl bits, [2] error flag,
-                              // [3] mean-state acceleration cease flag
+                              // [3] mean-state acceleration cease flag, then the kurant slots
   TermVel tv;
   Gam gam;
   double z0c1;                // 3.14159/2 - 3 ln 2 in REAL(4) (crmsurface.F90:125)
@@ -173,6 +173,14 @@ struct DevT {
   long long samp_cap;
   int samp_stride;
 };
+
+// The kurant maximum and the precipitation Courant maximum are each gathered in KURANT_SLOTS
+// words, one 128-byte line apart, after the four flag words of gmax: instance c updates slot
+// c % KURANT_SLOTS, so thousands of instances finishing together do not queue on one address;
+// the reader takes the maximum of the slots (the host for kurant, k_precip for the other).
+constexpr int GMAX_FLAGS = 4, KURANT_SLOTS = 32, KURANT_STRIDE = 16;
+constexpr int GMAX_PCFL = GMAX_FLAGS + KURANT_SLOTS * KURANT_STRIDE;  // precipitation slots
+constexpr int GMAX_WORDS = GMAX_PCFL + KURANT_SLOTS * KURANT_STRIDE;
 
 template <typename R>
 __device__ __forceinline__ double* prof(const DevT<R>& d, int p, int c) {

@@ -549,14 +549,17 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     }
     nstep++;
     d.nstep = nstep;
-    unsigned long long bits[4] = {0, 0, 0, 0};
-    HIPCHECK(hipMemsetAsync(ctx->gmax, 0, sizeof(unsigned long long), s));
+    unsigned long long bits[GMAX_PCFL];
+    HIPCHECK(hipMemsetAsync(ctx->gmax + GMAX_FLAGS, 0, (GMAX_PCFL - GMAX_FLAGS) * sizeof(unsigned long long), s));
     TIMED(CRM_K_KURANT, NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.g(CRM_K_KURANT), L.b(CRM_K_KURANT), 0, s>>>(d))));
     HIPCHECK(hipMemcpyAsync(bits, ctx->gmax, sizeof(bits), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     cease(bits[3], nstep - 1);
+    // non-negative doubles order as their bit patterns
+    unsigned long long cbits = 0;
+    for (int j = 0; j < KURANT_SLOTS; j++) cbits = std::max(cbits, bits[GMAX_FLAGS + j * KURANT_STRIDE]);
     double cfl;
-    std::memcpy(&cfl, &bits[0], sizeof(cfl));
+    std::memcpy(&cfl, &cbits, sizeof(cfl));
     // ncycle = max(1, ceiling(cfl / 0.7)) > 16 is an abort in the reference (kurant.F90:53-75);
     // the test is made before the conversion so that huge or non-finite values cannot overflow
     const double q = cfl / (double)0.7f;
@@ -582,7 +585,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
         NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na)));
         NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
-        k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + 1);
+        k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL);
         NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d)));
         NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d)));
         std::swap(d.u, d.dudt[nc]);
@@ -649,7 +652,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       if (last && phase_stop == 11) goto stopped;
       TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
       if (last && phase_stop == 13) goto stopped;
-      k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + 1);
+      k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL);
       TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d))));
       TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d))));
       if (last && phase_stop == 15) goto stopped;
@@ -1003,7 +1006,7 @@ static CrmEngine* make_engine(int device, int* rc) {
   e->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess ||
       hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
-      hipMalloc(&e->gmax, 4 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&e->gmax, GMAX_WORDS * sizeof(unsigned long long)) != hipSuccess) {
     delete e;
     *rc = CRM_ERR_DEVICE;
     return nullptr;

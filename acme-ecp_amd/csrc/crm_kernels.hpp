@@ -20,10 +20,10 @@ namespace CRM_NS {
 #endif
 #define CRM_LB __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRM_WAVES_PER_EU)))
 
-// Clears one word of the batch-global maxima (a kernel rather than a memset, so that it can
-// sit in a captured graph).
+// Clears the precipitation Courant slots of the batch-global maxima (a kernel rather than a
+// memset, so that it can sit in a captured graph); launched with KURANT_SLOTS threads.
 __global__ void k_clear_u64(unsigned long long* p) {
-  if (threadIdx.x == 0) *p = 0ull;
+  if (threadIdx.x < KURANT_SLOTS) p[threadIdx.x * KURANT_STRIDE] = 0ull;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -373,38 +373,27 @@ __global__ void CRM_LB k_kurant(DevT<R> d) {
   const double dxf = sqrt((1. / d.dx) * (1. / d.dx) + (y3 ? (1. / d.dx) * (1. / d.dx) : 0.0));
   // Every term is a monotone function of a level maximum over the columns, and rounding is
   // monotone, so the maximum over columns is taken once, of the per-column maximum over
-  // levels: the same value as per-level column maxima (kurant.F90:30-60).
-  double cfl = 0.0, wm_prev = 0.0, uhm_prev = 0.0;
+  // levels: the same value as per-level column maxima (kurant.F90:30-60).  A maximum is exact
+  // in any order, so the levels are independent iterations (the w of level k+1 is read again
+  // rather than carried) and several levels' loads are in flight together.
+  double cfl = 0.0;
   auto W = [&](int k) { return (double)d.w[((size_t)c * nz + k) * NX + i]; };
   auto U = [&](int k) { return (double)d.u[((size_t)c * nzm + k) * NX + i]; };
   auto V = [&](int k) { return y3 ? (double)d.v[((size_t)c * nzm + k) * NX + i] : 0.0; };
   auto T = [&](int k) { return (double)d.tkh[((size_t)c * nzm + k) * NX + i]; };
-  double w_n = W(0), u_n = U(0), v_n = V(0), t_n = T(0);  // read one level ahead
-  for (int k = 0; k <= nzm; k++) {
-    double wm = 0.0, uhm = 0.0;
-    if (k < nzm) {
-      const double ww = w_n, uu = u_n, vv = v_n, tkh = t_n;
-      if (k + 1 < nzm) {
-        w_n = W(k + 1);
-        u_n = U(k + 1);
-        v_n = V(k + 1);
-        t_n = T(k + 1);
-      }
-      wm = fabs(ww);
-      uhm = sqrt(uu * uu + vv * vv);
-      double dzz = dz * adzw[k];
-      double sgs = fmax(fmax(0.5 * tkh * 1.0 * d.dt / (dzz * dzz), 0.5 * tkh * grdf[k] * d.dt / (d.dx * d.dx)), 0.0);
-      cfl = fmax(cfl, sgs);
-    }
-    if (k >= 1) {
-      int kk = k - 1;
-      cfl = fmax(cfl, fmax(uhm_prev * d.dt * dxf, fmax(wm_prev, wm) * d.dt / (dz * adzw[kk])));
-    }
-    wm_prev = wm;
-    uhm_prev = uhm;
+#pragma unroll 4
+  for (int k = 0; k < nzm; k++) {
+    const double uu = U(k), vv = V(k), tkh = T(k);
+    // the lid's w counts as zero (the reference's loop stops below it)
+    const double wm = fabs(W(k)), wm_up = k + 1 < nzm ? fabs(W(k + 1)) : 0.0;
+    const double uhm = sqrt(uu * uu + vv * vv);
+    const double dzz = dz * adzw[k];
+    const double sgs = fmax(fmax(0.5 * tkh * 1.0 * d.dt / (dzz * dzz), 0.5 * tkh * grdf[k] * d.dt / (d.dx * d.dx)), 0.0);
+    cfl = fmax(cfl, sgs);
+    cfl = fmax(cfl, fmax(uhm * d.dt * dxf, fmax(wm, wm_up) * d.dt / (dz * adzw[k])));
   }
   cfl = s.max(cfl);
-  if (live && i == 0) atomic_max_nonneg(&d.gmax[0], cfl);
+  if (live && i == 0) atomic_max_nonneg(&d.gmax[GMAX_FLAGS + (c % KURANT_SLOTS) * KURANT_STRIDE], cfl);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1819,7 +1808,7 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
   for (; K <= nzm + 4; K++) step(K, std::false_type{});
   if (want_cfl) {
     pcfl = s.max(pcfl);
-    if (live && i == 0) atomic_max_nonneg(&d.gmax[1], pcfl);
+    if (live && i == 0) atomic_max_nonneg(&d.gmax[GMAX_PCFL + (c % KURANT_SLOTS) * KURANT_STRIDE], pcfl);
   }
 }
 
@@ -1855,7 +1844,10 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
   const R* irhoadz_p = profr(d, P_IRHOADZ, c);
   const R eps = R4(1.e-10);
   const R a_pr = A_PR();
-  R prec_cfl = __longlong_as_double((long long)d.gmax[1]);
+  // non-negative doubles order as their bit patterns
+  unsigned long long pbits = 0;
+  for (int j = 0; j < KURANT_SLOTS; j++) pbits = ::max(pbits, d.gmax[GMAX_PCFL + j * KURANT_STRIDE]);
+  R prec_cfl = __longlong_as_double((long long)pbits);
   int nprec = 1;
   if (!(prec_cfl <= NPREC_MAX * R4(0.9))) {
     // non-finite or runaway precipitation Courant number: one sweep, and the host reports
