@@ -31,7 +31,7 @@ ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_acc
 # include/crm_hip_debug.h
 DEBUG_SYMBOLS = ["crm_hip_debug_run", "crm_hip_debug_get", "crm_hip_fft991_operators",
                  "crm_hip_profile_kernels", "crm_hip_kernel_time", "crm_hip_kernel_name",
-                 "crm_hip_graph_launches"]
+                 "crm_hip_graph_launches", "crm_hip_debug_stop_call_at"]
 # Kernel classes of one call (CRM_K_* of include/crm_hip_debug.h), in launch order.
 KERNELS = ["setup", "init", "setperturb", "kurant", "precip_init", "pre", "sgs", "mom", "press",
            "mpdata", "precip", "diffuse", "micro", "stats", "final"]
@@ -92,6 +92,7 @@ def load():
     lib.crm_hip_get_schedule.argtypes = [P, IA]
     lib.crm_hip_autotune.argtypes = views + [IA]
     lib.crm_hip_set_graphs.argtypes = [P, C.c_int]
+    lib.crm_hip_debug_stop_call_at.argtypes = [P, C.c_int]
     lib.crm_hip_graph_launches.argtypes = [P]
     lib.crm_hip_graph_launches.restype = C.c_longlong
     lib.crm_hip_run_many.argtypes = [C.c_int, C.POINTER(P), C.POINTER(CrmClassDesc), C.c_double,
@@ -206,11 +207,23 @@ class CrmHip:
             raise CrmError(rc, "crm_hip_autotune")
         return dict(zip(KERNELS, list(a)))
 
-    def set_graphs(self, on):
-        self.lib.crm_hip_set_graphs(self.ctx, int(bool(on)))
+    def set_graphs(self, mode):
+        """0 (or False): plain launches; 1: graphs per steady substep; 2 (or True, the
+        default): one graph per call with the subcycle decision on the device."""
+        if isinstance(mode, bool):
+            mode = 2 if mode else 0
+        rc = self.lib.crm_hip_set_graphs(self.ctx, int(mode))
+        if rc:
+            raise CrmError(rc, "crm_hip_set_graphs")
 
     def graph_launches(self):
         return int(self.lib.crm_hip_graph_launches(self.ctx))
+
+    def debug_stop_call_at(self, nstep):
+        """Test hook: captured calls stop at substep nstep (0: off) and resume on the host."""
+        rc = self.lib.crm_hip_debug_stop_call_at(self.ctx, int(nstep))
+        if rc:
+            raise CrmError(rc, "crm_hip_debug_stop_call_at")
 
     def debug_run(self, case, dt_gl, phase_stop):
         return self._call(self.lib.crm_hip_debug_run, case.dims(), case.column_info(), dt_gl,

@@ -159,8 +159,10 @@ struct DevT {
                               // nx_rad counts the nx_rad*ny_rad radiation columns)
   const double *Ft, *Gt;      // transform operators [NX][NX]
   const double *eign;         // [NX] horizontal eigenvalues per slot
-  unsigned long long *gmax;   // [0] kurant cfl bits, [1] precipitation cfl bits, [2] error flag,
-                              // [3] mean-state acceleration cease flag, then the kurant slots
+  unsigned long long *gmax;   // [0] stop flag of a captured call, [1] unused, [2] error flag,
+                              // [3] mean-state acceleration cease flag, then the kurant and
+                              // precipitation Courant slots
+  const unsigned long long *stop;  // &gmax[0] inside a captured call, else null
   TermVel tv;
   Gam gam;
   double z0c1;                // 3.14159/2 - 3 ln 2 in REAL(4) (crmsurface.F90:125)
@@ -179,6 +181,7 @@ struct DevT {
 // c % KURANT_SLOTS, so thousands of instances finishing together do not queue on one address;
 // the reader takes the maximum of the slots (the host for kurant, k_precip for the other).
 constexpr int GMAX_FLAGS = 4, KURANT_SLOTS = 32, KURANT_STRIDE = 16;
+constexpr int GMAX_STOP = 0;  // flag word: the substep that stopped a captured call
 constexpr int GMAX_PCFL = GMAX_FLAGS + KURANT_SLOTS * KURANT_STRIDE;  // precipitation slots
 constexpr int GMAX_WORDS = GMAX_PCFL + KURANT_SLOTS * KURANT_STRIDE;
 

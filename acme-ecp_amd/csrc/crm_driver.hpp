@@ -81,7 +81,8 @@ struct Engine : CrmEngine {
   // HIP graphs of the steady substep (one subcycle, constant Adams-Bashforth coefficients),
   // keyed by the kernel arguments: the buffer rotation of the Adams-Bashforth scheme repeats
   // every 12 substeps, so a call reuses at most 12 graphs, and later calls reuse them too
-  bool use_graphs = true;
+  int use_graphs = CRM_GRAPHS_CALL;
+  int debug_stop = 0;  // test hook: substep at which a captured call stops (0: none)
   // microphysics network and the recording of training pairs
   const crm_mlp* mlp = nullptr;
   float *mlp_x = nullptr, *mlp_y = nullptr;
@@ -122,10 +123,11 @@ struct Engine : CrmEngine {
   void get_schedule(int* blk) const override {
     for (int k = 0; k < CRM_K_COUNT; k++) blk[k] = block[k];
   }
-  void set_graphs(bool on) override {
-    use_graphs = on;
-    if (!on) drop_graphs();
+  void set_graphs(int mode) override {
+    use_graphs = mode;
+    drop_graphs();
   }
+  void set_debug_stop(int nstep) override { debug_stop = nstep; }
   long long graph_count() const override { return graph_launches; }
   hipStream_t get_stream() const override { return stream; }
   int set_mlp(const crm_mlp* m) override {
@@ -440,6 +442,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   const bool y3 = dm->ny > 1;
   const int nx = dm->nx * dm->ny, nzm = dm->nz, ncrms = dm->ncrms;  // nx: columns per instance
   const int nxr = dm->nx_rad * dm->ny_rad;
+  // set-up (allocations, null-stream clears and uploads) excludes captures of other threads
+  std::unique_lock<std::mutex> setup_lock(crm_device_mutex());
   rc = ensure_buffers(ctx, ncrms, nx, nzm, nxr, y3);
   if (rc) return rc;
   Transform tf;
@@ -536,7 +540,106 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       nstop = (int)(nstop + (nstop - nstep_flagged + 1) * acc_f);
     }
   };
+  setup_lock.unlock();
   int nstep = 0;
+  d.stop = nullptr;
+  // Whole-call graph: the call's substeps captured once with one subcycle each and replayed;
+  // k_decide stops the replay at the first substep that needs more subcycles (every later
+  // kernel returns at entry), and the loop below resumes that substep from the host's record
+  // of its bookkeeping before it (slot rotation, buffer swaps, Adams-Bashforth history).
+  if (ctx->use_graphs == CRM_GRAPHS_CALL && !acc && !net_inputs && ctx->prof_mask == 0 && phase_stop == 0 &&
+      nstop <= 2048) {
+    struct Snap {
+      int na, nb, nc;
+      DevT<R> d;
+      double dt3[3];
+    };
+    std::vector<Snap> snap((size_t)nstop + 1);
+    const double* qrad = rad->qrad;
+    d.stop = ctx->gmax + GMAX_STOP;
+    DevT<R> dk;  // the key: the call's starting arguments
+    std::memcpy(&dk, &d, sizeof(d));
+    std::string key = "call";
+    key.append((const char*)&dk, sizeof(dk));
+    const int ints[6] = {na, nb, nc, nx, nstop, ctx->debug_stop};
+    key.append((const char*)ints, sizeof(ints));
+    key.append((const char*)dt3, sizeof(dt3));
+    key.append((const char*)&qrad, sizeof(qrad));
+    key.append((const char*)ctx->block, sizeof(ctx->block));
+    // the call's sequence; with launch false only the host bookkeeping is replayed
+    auto sequence = [&](bool launch) {
+      for (int n = 1; n <= nstop; n++) {
+        snap[n] = Snap{na, nb, nc, d, {dt3[0], dt3[1], dt3[2]}};
+        d.nstep = n;
+        d.icycle = 1;
+        d.dtn = dt;
+        dt3[na] = d.dtn;
+        d.dtfactor = 1.0;
+        abcoefs(n, dt3, na, nb, nc, d.at, d.bt, d.ct);
+        d.dt3na = dt3[na];
+        if (launch) {
+          k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_FLAGS, d.stop);
+          NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.g(CRM_K_KURANT), L.b(CRM_K_KURANT), 0, s>>>(d)));
+          k_decide<<<1, 64, 0, s>>>(ctx->gmax, n, ctx->debug_stop);
+          k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d);
+          NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad)));
+          NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
+          NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na)));
+          NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
+          k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL, d.stop);
+          NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d)));
+          NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d)));
+        }
+        std::swap(d.u, d.dudt[nc]);
+        std::swap(d.w, d.dwdt[nc]);
+        if (y3) std::swap(d.v, d.dvdt[nc]);
+        if (launch) {
+          NX_DISPATCH(nx, (k_diffuse<NXV, R, GEO><<<L.g(CRM_K_DIFFUSE), L.b(CRM_K_DIFFUSE), 0, s>>>(d)));
+          NX_DISPATCH(nx, (k_micro<NXV, R><<<L.g(CRM_K_MICRO), L.b(CRM_K_MICRO), 0, s>>>(d, 0)));
+        }
+        int nn = na;
+        na = nc;
+        nc = nb;
+        nb = nn;
+        if (launch) NX_DISPATCH(nx, (k_stats<NXV, R><<<L.g(CRM_K_STATS), L.b(CRM_K_STATS), 0, s>>>(d)));
+      }
+    };
+    HIPCHECK(hipMemsetAsync(ctx->gmax + GMAX_STOP, 0, sizeof(unsigned long long), s));
+    hipGraphExec_t exec = nullptr;
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+      std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
+      hipGraph_t graph = nullptr;
+      HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      sequence(true);
+      HIPCHECK(hipStreamEndCapture(s, &graph));
+      HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      ctx->graphs[key] = exec;
+    } else {
+      exec = it->second;
+      sequence(false);
+    }
+    HIPCHECK(hipGraphLaunch(exec, s));
+    ctx->graph_launches += nstop;
+    unsigned long long stopped_at = 0;
+    HIPCHECK(hipMemcpyAsync(&stopped_at, ctx->gmax + GMAX_STOP, sizeof(stopped_at), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (stopped_at == 0) {
+      nstep = nstop;
+    } else {  // resume substep stopped_at as it was before it
+      const Snap& sn = snap[stopped_at];
+      na = sn.na;
+      nb = sn.nb;
+      nc = sn.nc;
+      d = sn.d;
+      for (int j = 0; j < 3; j++) dt3[j] = sn.dt3[j];
+      nstep = (int)stopped_at - 1;
+      ctx->graph_launches -= nstop - nstep;
+    }
+    if (nstep >= 1) ctx->stats.ncycle_max = std::max(ctx->stats.ncycle_max, 1);
+    d.stop = nullptr;
+  }
   while (true) {
     if (nstep >= nstop) {
       if (acc && !ceased) {
@@ -570,7 +673,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     const bool last = (nstep == nstop);
     // Steady substeps (one subcycle, constant Adams-Bashforth coefficients from the third
     // substep on) replay a captured HIP graph of the whole substep, statistics included.
-    if (ctx->use_graphs && ncycle == 1 && nstep >= 3 && !acc && !net_inputs && ctx->prof_mask == 0 &&
+    if (ctx->use_graphs != CRM_GRAPHS_OFF && ncycle == 1 && nstep >= 3 && !acc && !net_inputs && ctx->prof_mask == 0 &&
         phase_stop == 0) {
       d.icycle = 1;
       d.dtn = dt;
@@ -585,7 +688,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
         NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na)));
         NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
-        k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL);
+        k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL, d.stop);
         NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d)));
         NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d)));
         std::swap(d.u, d.dudt[nc]);
@@ -611,6 +714,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       auto it = ctx->graphs.find(key);
       hipGraphExec_t exec = nullptr;
       if (it == ctx->graphs.end()) {
+        std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
         hipGraph_t graph = nullptr;
         HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         body();
@@ -652,7 +756,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       if (last && phase_stop == 11) goto stopped;
       TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
       if (last && phase_stop == 13) goto stopped;
-      k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL);
+      k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL, d.stop);
       TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d))));
       TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d))));
       if (last && phase_stop == 15) goto stopped;

@@ -20,7 +20,8 @@ struct CrmEngine {
   virtual int accel_init(int use, double factor, int uv) = 0;
   virtual int set_schedule(const int* block) = 0;
   virtual void get_schedule(int* block) const = 0;
-  virtual void set_graphs(bool on) = 0;
+  virtual void set_graphs(int mode) = 0;
+  virtual void set_debug_stop(int nstep) = 0;
   virtual long long graph_count() const = 0;
   virtual int set_mlp(const crm_mlp* m) = 0;
   virtual int record_samples(float* X, float* T, long long cap, int stride) = 0;
@@ -29,6 +30,13 @@ struct CrmEngine {
                        crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) = 0;
   virtual hipStream_t get_stream() const = 0;
 };
+
+// Serialises stream captures with the library's null-stream work (buffer set-up, operator
+// uploads) across threads: the engine streams are blocking streams, so null-stream work of one
+// context would join, and invalidate, another context's capture in progress
+// (crm_hip_run_many runs classes on concurrent threads).
+#include <mutex>
+std::mutex& crm_device_mutex();
 
 // The device and stream of a context (for the GCM-side kernels of crm_glue.hip).
 int crm_ctx_device(const crm_hip_ctx* ctx);

@@ -54,6 +54,11 @@ struct crm_hip_ctx {
   CrmEngine* eng = nullptr;
 };
 
+std::mutex& crm_device_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 int crm_ctx_device(const crm_hip_ctx* ctx) { return ctx->device; }
 hipStream_t crm_ctx_stream(const crm_hip_ctx* ctx) { return ctx->eng->get_stream(); }
 
@@ -152,7 +157,8 @@ int crm_hip_autotune(crm_hip_ctx* ctx, const crm_dims* dims, const crm_column_in
 
 int crm_hip_set_graphs(crm_hip_ctx* ctx, int on) {
   if (!ctx || !ctx->eng) return CRM_ERR_ARG;
-  ctx->eng->set_graphs(on != 0);
+  if (on < CRM_GRAPHS_OFF || on > CRM_GRAPHS_CALL) return CRM_ERR_ARG;
+  ctx->eng->set_graphs(on);
   return CRM_OK;
 }
 
@@ -175,6 +181,12 @@ long long crm_hip_micro_samples_count(const crm_hip_ctx* ctx) {
 long long crm_hip_graph_launches(const crm_hip_ctx* ctx) {
   if (!ctx || !ctx->eng) return -1;
   return ctx->eng->graph_count();
+}
+
+int crm_hip_debug_stop_call_at(crm_hip_ctx* ctx, int nstep) {
+  if (!ctx || !ctx->eng || nstep < 0) return CRM_ERR_ARG;
+  ctx->eng->set_debug_stop(nstep);
+  return CRM_OK;
 }
 
 int crm_hip_get_stats(const crm_hip_ctx* ctx, crm_hip_stats* stats) {

@@ -20,10 +20,30 @@ namespace CRM_NS {
 #endif
 #define CRM_LB __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CRM_WAVES_PER_EU)))
 
-// Clears the precipitation Courant slots of the batch-global maxima (a kernel rather than a
-// memset, so that it can sit in a captured graph); launched with KURANT_SLOTS threads.
-__global__ void k_clear_u64(unsigned long long* p) {
+// A captured call runs every substep with one subcycle; k_decide sets gmax[GMAX_STOP] to the
+// first substep that needs more, and every kernel of the call then returns at entry (the flag
+// is the same for every thread, so no workgroup barrier is split); the host resumes that
+// substep eagerly.  d.stop is null outside such a call.
+#define CRM_STOP_CHECK(d) \
+  if ((d).stop && *(const volatile unsigned long long*)(d).stop) return
+
+// Clears KURANT_SLOTS slots of the batch-global maxima (a kernel rather than a memset, so that
+// it can sit in a captured graph); launched with at least KURANT_SLOTS threads.
+__global__ void k_clear_u64(unsigned long long* p, const unsigned long long* stop) {
+  if (stop && *(const volatile unsigned long long*)stop) return;
   if (threadIdx.x < KURANT_SLOTS) p[threadIdx.x * KURANT_STRIDE] = 0ull;
+}
+
+// The subcycle decision of kurant.F90:53-75 on the device, for a captured call: one subcycle
+// when cfl / 0.7 <= 1; anything else (more subcycles, the reference's abort, a non-finite
+// value) stops the captured call at this substep for the host to take over; so does
+// nstep == stop_at (a test hook, 0 when off).
+__global__ void k_decide(unsigned long long* gmax, int nstep, int stop_at) {
+  if (threadIdx.x != 0 || gmax[GMAX_STOP]) return;
+  unsigned long long cbits = 0;
+  for (int j = 0; j < KURANT_SLOTS; j++) cbits = ::max(cbits, gmax[GMAX_FLAGS + j * KURANT_STRIDE]);
+  const double q = __longlong_as_double((long long)cbits) / (double)0.7f;
+  if (!(q <= 1.0) || nstep == stop_at) gmax[GMAX_STOP] = (unsigned long long)nstep;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -362,6 +382,7 @@ __global__ void k_setperturb(DevT<R> d, const int* gcol) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_kurant(DevT<R> d) {
+  CRM_STOP_CHECK(d);
   SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -403,6 +424,7 @@ __global__ void CRM_LB k_kurant(DevT<R> d) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
+  CRM_STOP_CHECK(d);
   SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -598,6 +620,7 @@ __device__ __forceinline__ R shear3(const G& s, int k, const R* srdz, const R* s
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_sgs(DevT<R> d) {
+  CRM_STOP_CHECK(d);
   G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -1089,6 +1112,7 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
 
 template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_mom(DevT<R> d, int na) {
+  CRM_STOP_CHECK(d);
   if constexpr (G::Y3)
     mom3<R>(d, na);
   else
@@ -1104,6 +1128,7 @@ __global__ void CRM_LB k_mom(DevT<R> d, int na) {
 // ------------------------------------------------------------------------------------------
 template <int NX, int NZM, typename R>
 __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
+  CRM_STOP_CHECK(d);
   extern __shared__ double slab[];  // [64/NX instances][NZM][NX] == [NZM][64] per lane column
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane & (NX - 1);
@@ -1239,6 +1264,7 @@ __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
 // ------------------------------------------------------------------------------------------
 template <int NZM, typename R>
 __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
+  CRM_STOP_CHECK(d);
   constexpr int NX = 128;
   extern __shared__ double slab[];  // [NZM][128]
   const int i = threadIdx.x & (NX - 1), grp = threadIdx.x >> 7;
@@ -1374,6 +1400,7 @@ constexpr int PLD = 66;  // slab row stride in doubles: conflict-free A-fragment
 
 template <int NZM, typename R, typename G = Seg<64>>
 __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
+  CRM_STOP_CHECK(d);
   constexpr int NX = 64, NK = 64;
   extern __shared__ double lds[];
   double* P = lds;              // [NK][PLD] slab: rhs, coefficients, pressure
@@ -1815,6 +1842,7 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
 // advect_all_scalars.F90:7-94: t, then the two microphysics fields.
 template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_mpdata(DevT<R> d) {
+  CRM_STOP_CHECK(d);
   G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -1830,6 +1858,7 @@ __global__ void CRM_LB k_mpdata(DevT<R> d) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_precip(DevT<R> d) {
+  CRM_STOP_CHECK(d);
   SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -2030,6 +2059,7 @@ __device__ void diffuse_one(const DevT<R>& d, const G& s, int c, bool live, R* f
 
 template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_diffuse(DevT<R> d) {
+  CRM_STOP_CHECK(d);
   G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -2043,6 +2073,7 @@ __global__ void CRM_LB k_diffuse(DevT<R> d) {
 // ------------------------------------------------------------------------------------------
 template <typename R>
 __global__ void k_precip_init(DevT<R> d) {
+  CRM_STOP_CHECK(d);
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= d.ncrms * d.nzm) return;
   int c = idx / d.nzm, k = idx - c * d.nzm;
@@ -2193,6 +2224,7 @@ __device__ __forceinline__ void accel_means(const DevT<R>& d, const S_& s, int c
 
 template <int NX, typename R>
 __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
+  CRM_STOP_CHECK(d);
   SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
@@ -2520,6 +2552,7 @@ __global__ void CRM_LB k_accel(DevT<R> d, double factor, int uv) {
 // ------------------------------------------------------------------------------------------
 template <int NX, typename R>
 __global__ void CRM_LB k_stats(DevT<R> d) {
+  CRM_STOP_CHECK(d);
   SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);

@@ -205,9 +205,14 @@ int crm_hip_autotune(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_in
                      const crm_input_view *in, crm_state_view *state, crm_rad_view *rad,
                      crm_output_view *out, int block_out[CRM_HIP_NKERNEL]);
 
-/* Steady substeps (one subcycle, from the third substep on) replay captured HIP graphs of the
- * whole substep; on by default. */
-int crm_hip_set_graphs(crm_hip_ctx *ctx, int on);
+/* HIP graphs.  CRM_GRAPHS_SUBSTEP: steady substeps (one subcycle, from the third substep on)
+ * replay a captured graph of the whole substep, the subcycle count decided on the host before
+ * each.  CRM_GRAPHS_CALL (the default): a call replays one captured graph of all its substeps
+ * with one subcycle each; the subcycle count is decided on the device, and a substep that needs
+ * more stops the replay there and continues as with CRM_GRAPHS_SUBSTEP.  CRM_GRAPHS_OFF: plain
+ * launches.  Results are bitwise the same in every mode. */
+enum { CRM_GRAPHS_OFF = 0, CRM_GRAPHS_SUBSTEP = 1, CRM_GRAPHS_CALL = 2 };
+int crm_hip_set_graphs(crm_hip_ctx *ctx, int mode);
 
 int crm_hip_get_stats(const crm_hip_ctx *ctx, crm_hip_stats *stats);
 int crm_hip_finalize(crm_hip_ctx *ctx);

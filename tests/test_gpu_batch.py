@@ -34,7 +34,8 @@ def _same(a, b):
 
 @pytest.mark.parametrize("cls", CLASSES, ids=lambda c: f"{c[0]}x{c[1]}x{c[2]}x{c[3]}")
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_graphs_replay_bitwise(cls, prec):
+@pytest.mark.parametrize("mode", [1, 2], ids=["substep", "call"])
+def test_graphs_replay_bitwise(cls, prec, mode):
     """Three consecutive calls (the graphs of the first are replayed by the later ones, and
     each call replays its graphs many times) against the same calls run eagerly."""
     case = _case(cls, seed=81)
@@ -42,6 +43,7 @@ def test_graphs_replay_bitwise(cls, prec):
     he, hg = CrmHip(0, prec), CrmHip(0, prec)
     try:
         he.set_graphs(False)
+        hg.set_graphs(mode)
         for _ in range(3):
             he.run(eager, 10 * cls[5])
             hg.run(graphed, 10 * cls[5])
@@ -130,3 +132,24 @@ def test_run_many_parity():
     finally:
         for h in hips:
             h.close()
+
+
+
+@pytest.mark.parametrize("stop_at", [1, 2, 3, 7, 10])
+def test_call_graph_resumes_mid_call(stop_at):
+    """A captured call stopped at any substep (the stop of a substep that needs more subcycles,
+    forced here by the test hook) resumes on the host from its record of the state before that
+    substep: the result is bitwise the eager call's."""
+    case = _case(CLASSES[1], seed=85)
+    eager, graphed = case.copy(), case.copy()
+    he, hg = CrmHip(0), CrmHip(0)
+    try:
+        he.set_graphs(False)
+        hg.debug_stop_call_at(stop_at)
+        for _ in range(2):
+            he.run(eager, 10 * CLASSES[1][5])
+            hg.run(graphed, 10 * CLASSES[1][5])
+            _same(eager, graphed)
+    finally:
+        he.close()
+        hg.close()

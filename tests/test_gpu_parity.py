@@ -91,6 +91,19 @@ def test_reference_fixture_parity(hip, path):
     assert max(errs.values()) <= tol, _worst(errs)
 
 
+def test_single_instance_gcm_step(hip):
+    """The reference's own CPU-runnable configuration: one 32x28 CRM, dx 4 km, dt 20 s, a full
+    1200 s GCM step (60 substeps) on a tropical sounding, GCM and CRM levels as in the
+    benchmark inputs."""
+    case = make_case(1, 32, 28, 4000.0, 20.0)
+    want, got = case.copy(), case.copy()
+    assert oracle_lib.oracle().run(want, 1200.0) == 0
+    hip.run(got, 1200.0)
+    errs = compare_case(got, want, dt_gl=1200.0)
+    print(_worst(errs))
+    assert max(errs.values()) <= TOL_N, _worst(errs)
+
+
 def test_866_instances_one_substep(hip):
     """866 instances of the 32x28 CRM with per-instance soundings, one substep."""
     case = make_case(866, 32, 28, 4000.0, 20.0, randomize=True)
