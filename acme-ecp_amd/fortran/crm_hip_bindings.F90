@@ -52,6 +52,16 @@ module crm_hip_bindings
     type(c_ptr) :: precflux, qt_ls, qt_trans, qp_trans, qp_fall, qp_src, qp_evp, t_ls
   end type
 
+  ! GCM-side glue (include/crm_glue.h): the physics_state members crm_physics_tend reads (q as
+  ! its vapour, cloud-liquid and cloud-ice slices) and the cam_in_t members
+  type, bind(C), public :: crm_gcm_state_view
+    type(c_ptr) :: zm, zi, t, qv, qcl, qci, ps, pmid, pint, pdel, phis, u, v
+  end type
+
+  type, bind(C), public :: crm_gcm_surface_view
+    type(c_ptr) :: ocnfrac, wsx, wsy, shf, lhf
+  end type
+
   type, bind(C), public :: crm_hip_config
     integer(c_int) :: device, precision, max_ncrms
   end type
@@ -64,6 +74,7 @@ module crm_hip_bindings
 
   public :: crm_hip_init, crm_hip_run, crm_hip_run_device, crm_hip_accel_init, crm_hip_get_stats, &
             crm_hip_finalize, crm_hip_errstr, crm_hip_message
+  public :: crm_hip_gcm_input, crm_hip_gcm_flux_bypass, crm_hip_gcm_tend, crm_hip_check_energy_chng
 
   interface
     integer(c_int) function crm_hip_init(ctx, cfg) bind(C, name="crm_hip_init")
@@ -123,6 +134,46 @@ module crm_hip_bindings
     type(c_ptr) function crm_hip_errstr(code) bind(C, name="crm_hip_errstr")
       import :: c_int, c_ptr
       integer(c_int), value :: code
+    end function
+
+    ! GCM-side glue on device arrays (crm_physics.F90:1005-1010,1050-1125; :1652-1703;
+    ! :1158-1169,1203-1209,1310-1330; check_energy.F90:358-561); pass c_loc of device buffers
+    integer(c_int) function crm_hip_gcm_input(ctx, ncol, pcols, pver, crm_nz, nrad, state, srf, crm_angle, &
+                                              inp, rad) bind(C, name="crm_hip_gcm_input")
+      import :: c_int, c_ptr, crm_gcm_state_view, crm_gcm_surface_view, crm_input_view, crm_rad_view
+      type(c_ptr), value :: ctx, crm_angle
+      integer(c_int), value :: ncol, pcols, pver, crm_nz, nrad
+      type(crm_gcm_state_view), intent(in) :: state
+      type(crm_gcm_surface_view), intent(in) :: srf
+      type(crm_input_view), intent(in) :: inp
+      type(crm_rad_view), intent(in) :: rad
+    end function
+
+    integer(c_int) function crm_hip_gcm_flux_bypass(ctx, ncol, pcols, pver, pdel, shf, cflx_vap, ptend_s, &
+                                                    ptend_qv) bind(C, name="crm_hip_gcm_flux_bypass")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, pdel, shf, cflx_vap, ptend_s, ptend_qv
+      integer(c_int), value :: ncol, pcols, pver
+    end function
+
+    integer(c_int) function crm_hip_gcm_tend(ctx, ncol, pcols, pver, crm_nz, nrad, pdel, out, rad, qrs, qrl, &
+                                             ptend_s, ptend_qv, ptend_qcl, ptend_qci, sp_rad_flux) &
+        bind(C, name="crm_hip_gcm_tend")
+      import :: c_int, c_ptr, crm_output_view, crm_rad_view
+      type(c_ptr), value :: ctx, pdel, qrs, qrl, ptend_s, ptend_qv, ptend_qcl, ptend_qci, sp_rad_flux
+      integer(c_int), value :: ncol, pcols, pver, crm_nz, nrad
+      type(crm_output_view), intent(inout) :: out
+      type(crm_rad_view), intent(in) :: rad
+    end function
+
+    integer(c_int) function crm_hip_check_energy_chng(ctx, ncol, pcols, pver, state, ztodt, flx_vap, flx_cnd, &
+                                                      flx_ice, flx_sen, te_cur, tw_cur, te_tnd, tw_tnd, &
+                                                      te_rer, tw_rer) bind(C, name="crm_hip_check_energy_chng")
+      import :: c_int, c_ptr, c_double, crm_gcm_state_view
+      type(c_ptr), value :: ctx, flx_vap, flx_cnd, flx_ice, flx_sen, te_cur, tw_cur, te_tnd, tw_tnd, te_rer, tw_rer
+      integer(c_int), value :: ncol, pcols, pver
+      type(crm_gcm_state_view), intent(in) :: state
+      real(c_double), value :: ztodt
     end function
   end interface
 

@@ -28,12 +28,16 @@ program probe
   type(crm_output_view) :: vo
   type(crm_hip_config) :: cfg
   type(crm_hip_stats) :: st
+  type(crm_gcm_state_view) :: gs
+  type(crm_gcm_surface_view) :: gf
   integer(c_int) :: rc
-  print '(A,8I6)', 'sizes', c_sizeof(d), c_sizeof(col), c_sizeof(vi), c_sizeof(vs), &
-        c_sizeof(vr), c_sizeof(vo), c_sizeof(cfg), c_sizeof(st)
+  print '(A,10I6)', 'sizes', c_sizeof(d), c_sizeof(col), c_sizeof(vi), c_sizeof(vs), &
+        c_sizeof(vr), c_sizeof(vo), c_sizeof(cfg), c_sizeof(st), c_sizeof(gs), c_sizeof(gf)
   print '(A,A)', 'msg2 ', crm_hip_message(CRM_ERR_CFL)
   rc = crm_hip_run(c_null_ptr, d, col, 1200.0_c_double, vi, vs, vr, vo)
   print '(A,I3)', 'run ', rc
+  rc = crm_hip_gcm_input(c_null_ptr, 1, 1, 30, 28, 1, gs, gf, c_null_ptr, vi, vr)
+  print '(A,I3)', 'glue ', rc
 end program
 """
 
@@ -57,9 +61,11 @@ def probe(tmp_path_factory):
 def test_struct_sizes_match_c(probe):
     class Cfg(C.Structure):
         _fields_ = [("a", C.c_int), ("b", C.c_int), ("c", C.c_int)]
+    import crm_glue
     want = [C.sizeof(CrmDims), C.sizeof(CrmColumnInfo), C.sizeof(CrmInputView),
             C.sizeof(CrmStateView), C.sizeof(CrmRadView), C.sizeof(CrmOutputView),
-            C.sizeof(Cfg), C.sizeof(crm_hip.CrmHipStats)]
+            C.sizeof(Cfg), C.sizeof(crm_hip.CrmHipStats), C.sizeof(crm_glue.GcmStateView),
+            C.sizeof(crm_glue.GcmSurfaceView)]
     assert [int(v) for v in probe["sizes"].split()] == want
 
 
@@ -69,3 +75,4 @@ def test_status_text_through_fortran(probe):
 
 def test_null_context_rejected(probe):
     assert int(probe["run"]) == crm_hip.CRM_ERR_ARG
+    assert int(probe["glue"]) == crm_hip.CRM_ERR_ARG
