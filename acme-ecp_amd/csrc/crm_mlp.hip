@@ -44,10 +44,18 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16* A, long sam, long sak,
   const bf16* b = B + (tn * 16 + r) * sbn;
   for (int k = k0; k < k1; k += 16) {
     s4 av, bv;
+    // K contiguous (forward activations, transposed weights): one 8-byte load per operand
+    if (sak == 1) {
+      av = *(const s4*)&a[k + 4 * q];
+    } else {
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      av[j] = (short)a[(long)(k + 4 * q + j) * sak];
-      bv[j] = (short)b[(long)(k + 4 * q + j) * sbk];
+      for (int j = 0; j < 4; j++) av[j] = (short)a[(long)(k + 4 * q + j) * sak];
+    }
+    if (sbk == 1) {
+      bv = *(const s4*)&b[k + 4 * q];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[j] = (short)b[(long)(k + 4 * q + j) * sbk];
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
   }
@@ -75,33 +83,46 @@ __global__ void k_inputs(const float* X, int n, int mp, const float* norm, bf16*
 }
 
 // Loss of the normalised outputs and its gradient: dY = 2 (Y - T / oscale) / (6 n) in BF16.
+// Grid-stride over the elements; one atomic per workgroup (a few hundred in all, so they do not
+// queue on the one address).
 __global__ void k_loss(const float* Y, const float* T, int n, int mp, const float* norm, bf16* dY, float* loss) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float red[4];
   float e2 = 0.f;
-  if (idx < (long)mp * NP) {
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < (long)mp * NP;
+       idx += (long)gridDim.x * blockDim.x) {
     const long m = idx / NP;
     const int c = (int)(idx % NP);
     float g = 0.f;
     if (m < n && c < CRM_MLP_NOUT) {
       const float d = Y[idx] - T[m * CRM_MLP_NOUT + c] / norm[NORM_OSCALE + c];
-      e2 = d * d;
+      e2 += d * d;
       g = 2.f * d / (6.f * (float)n);
     }
     dY[idx] = to_bf16(g);
   }
-  // one atomic per wave
   for (int o = 32; o >= 1; o >>= 1) e2 += __shfl_xor(e2, o, 64);
-  if ((threadIdx.x & 63) == 0 && e2 != 0.f) unsafeAtomicAdd(loss, e2 / (6.f * (float)n));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float b = red[0] + red[1] + red[2] + red[3];
+    if (b != 0.f) unsafeAtomicAdd(loss, b / (6.f * (float)n));
+  }
 }
 
-// Column sums of a BF16 [mp][ld] matrix over its rows, accumulated into out[ncol] (FP32).
+// Column sums of a BF16 [mp][ld] matrix over its rows, accumulated into out[ncol] (FP32).  A
+// workgroup (256 threads) covers 64 columns with 4 row groups that meet in LDS; blockIdx.y
+// splits the rows.
 __global__ void k_colsum(const bf16* G, int mp, int ld, int ncol, float* out) {
-  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int part = blockIdx.y, nparts = gridDim.y;
-  if (c >= ncol) return;
+  __shared__ float red[4][64];
+  const int cl = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
+  const long step = (long)gridDim.y * 4;
   float sacc = 0.f;
-  for (long m = part; m < mp; m += nparts) sacc += from_bf16(G[m * ld + c]);
-  unsafeAtomicAdd(&out[c], sacc);
+  if (c < ncol)
+    for (long m = (long)blockIdx.y * 4 + grp; m < mp; m += step) sacc += from_bf16(G[m * ld + c]);
+  red[grp][cl] = sacc;
+  __syncthreads();
+  if (grp == 0 && c < ncol) unsafeAtomicAdd(&out[c], (red[0][cl] + red[1][cl]) + (red[2][cl] + red[3][cl]));
 }
 
 // Packed FP32 parameters -> padded BF16 weights and FP32 biases used by the GEMMs.
@@ -469,7 +490,8 @@ int backward(crm_mlp* m, const float* X, const float* T, int n) {
   forward_rows(m, mp);
   MLPCHECK(hipMemsetAsync(m->loss, 0, sizeof(float), s));
   MLPCHECK(hipMemsetAsync(m->gpad, 0, GPAD * sizeof(float), s));
-  k_loss<<<dim3((unsigned)(((long)mp * NP + 255) / 256)), dim3(256), 0, s>>>(m->Y, T, n, mp, m->norm, m->dY, m->loss);
+  k_loss<<<dim3((unsigned)std::min<long>(((long)mp * NP + 255) / 256, 256)), dim3(256), 0, s>>>(m->Y, T, n, mp, m->norm,
+                                                                                               m->dY, m->loss);
   float* gW1 = m->gpad;
   float* gW2 = gW1 + KP * NH;
   float* gW3 = gW2 + NH * NH;
@@ -478,23 +500,23 @@ int backward(crm_mlp* m, const float* X, const float* T, int n) {
   float* gb2 = gb1 + NH;
   float* gb3 = gb2 + NH;
   float* gb4 = gb3 + NH;
-  const int ks = std::max(1, mp / 2048);  // split the batch reduction of the weight gradients
+  const int ks = std::max(1, mp / 256);  // split the batch reduction of the weight gradients
   const dim3 cs(2, 64);
   // layer 4
   gemm<EPI_ATOMIC_F32>(s, m->H3, 1, NH, m->dY, NP, 1, NH, NP, mp, nullptr, nullptr, 0, gW4, nullptr, NP, ks);
-  k_colsum<<<dim3(1, 64), dim3(64), 0, s>>>(m->dY, mp, NP, NP, gb4);
+  k_colsum<<<dim3(1, 64), dim3(256), 0, s>>>(m->dY, mp, NP, NP, gb4);
   gemm<EPI_MASK_BF16>(s, m->dY, NP, 1, m->W4, 1, NP, mp, NH, NP, nullptr, m->H3, NH, nullptr, m->dA, NH);
   // layer 3
   gemm<EPI_ATOMIC_F32>(s, m->H2, 1, NH, m->dA, NH, 1, NH, NH, mp, nullptr, nullptr, 0, gW3, nullptr, NH, ks);
-  k_colsum<<<cs, dim3(64), 0, s>>>(m->dA, mp, NH, NH, gb3);
+  k_colsum<<<cs, dim3(256), 0, s>>>(m->dA, mp, NH, NH, gb3);
   gemm<EPI_MASK_BF16>(s, m->dA, NH, 1, m->W3, 1, NH, mp, NH, NH, nullptr, m->H2, NH, nullptr, m->dB, NH);
   // layer 2
   gemm<EPI_ATOMIC_F32>(s, m->H1, 1, NH, m->dB, NH, 1, NH, NH, mp, nullptr, nullptr, 0, gW2, nullptr, NH, ks);
-  k_colsum<<<cs, dim3(64), 0, s>>>(m->dB, mp, NH, NH, gb2);
+  k_colsum<<<cs, dim3(256), 0, s>>>(m->dB, mp, NH, NH, gb2);
   gemm<EPI_MASK_BF16>(s, m->dB, NH, 1, m->W2, 1, NH, mp, NH, NH, nullptr, m->H1, NH, nullptr, m->dA, NH);
   // layer 1
   gemm<EPI_ATOMIC_F32>(s, m->Xn, 1, KP, m->dA, NH, 1, KP, NH, mp, nullptr, nullptr, 0, gW1, nullptr, NH, ks);
-  k_colsum<<<cs, dim3(64), 0, s>>>(m->dA, mp, NH, NH, gb1);
+  k_colsum<<<cs, dim3(256), 0, s>>>(m->dA, mp, NH, NH, gb1);
   k_pack_grad<<<dim3((CRM_MLP_NPARAM + 255) / 256), dim3(256), 0, s>>>(gW1, gW2, gW3, gW4, gb1, gb2, gb3, gb4, m->g);
   MLPCHECK(hipGetLastError());
   return CRM_OK;
