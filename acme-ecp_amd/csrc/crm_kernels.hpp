@@ -1565,25 +1565,30 @@ __device__ __forceinline__ T ppos(T y) { return fmax((T)0.0, y); }
 template <typename T>
 __device__ __forceinline__ T pneg(T y) { return -fmin((T)0.0, y); }
 
-// terminal velocity of precipitation, microphysics.F90:453-486
+// terminal velocity of precipitation, microphysics.F90:453-486.  The three regimes (all rain,
+// no rain, mixed) share their powers: each of the rain, graupel and snow powers is taken once,
+// by the lanes whose regime uses it, so a wave whose lanes straddle the regimes evaluates at
+// most three powers instead of six.  The arguments are the reference's in every regime (with
+// omp == 0, q - omp*q is q exactly), and a power a regime does not use multiplies a zero
+// weight there, so each lane's velocity is the one its own regime's formula gives.
 template <typename T>
 __device__ __forceinline__ T term_vel(const TermVel& tv, T q, T rho, T tabs) {
   T v = 0;
   if (q > QP_THRESHOLD) {
-    T omp = fmax(T(0.0), fmin(T(1.0), (tabs - TPRMIN) * (T)A_PR()));
-    if (omp == T(1.)) {
-      v = (T)tv.vrain * fpow<T>(rho * q, (T)tv.crain);
-    } else if (omp == T(0.)) {
-      T omg = fmax(T(0.0), fmin(T(1.0), (tabs - TGRMIN) * (T)A_GR()));
-      T qgg = omg * q, qss = q - qgg;
-      v = (omg * (T)tv.vgrau * fpow<T>(rho * qgg, (T)tv.cgrau) + (T(1.) - omg) * (T)tv.vsnow * fpow<T>(rho * qss, (T)tv.csnow));
-    } else {
-      T omg = fmax(T(0.0), fmin(T(1.0), (tabs - TGRMIN) * (T)A_GR()));
-      T qrr = omp * q, qss = q - qrr, qgg = omg * qss;
-      qss = qss - qgg;
-      v = (omp * (T)tv.vrain * fpow<T>(rho * qrr, (T)tv.crain) +
-           (T(1.) - omp) * (omg * (T)tv.vgrau * fpow<T>(rho * qgg, (T)tv.cgrau) + (T(1.) - omg) * (T)tv.vsnow * fpow<T>(rho * qss, (T)tv.csnow)));
-    }
+    const T omp = fmax(T(0.0), fmin(T(1.0), (tabs - TPRMIN) * (T)A_PR()));
+    const T omg = fmax(T(0.0), fmin(T(1.0), (tabs - TGRMIN) * (T)A_GR()));
+    const T qrr = omp * q, qsg = q - qrr, qgg = omg * qsg, qss = qsg - qgg;
+    T pr = 0, pg = 0, ps = 0;
+    if (omp > T(0.)) pr = fpow_fall<T>(rho * qrr, (T)tv.crain);
+    if (omp < T(1.) && omg > T(0.)) pg = fpow_fall<T>(rho * qgg, (T)tv.cgrau);
+    if (omp < T(1.) && omg < T(1.)) ps = fpow_fall<T>(rho * qss, (T)tv.csnow);
+    const T vsg = omg * (T)tv.vgrau * pg + (T(1.) - omg) * (T)tv.vsnow * ps;
+    if (omp == T(1.))
+      v = (T)tv.vrain * pr;
+    else if (omp == T(0.))
+      v = vsg;
+    else
+      v = omp * (T)tv.vrain * pr + (T(1.) - omp) * vsg;
   }
   return v;
 }
