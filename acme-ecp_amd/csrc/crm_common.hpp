@@ -195,6 +195,20 @@ template <typename R>
 __device__ __forceinline__ R* profr(const DevT<R>& d, int p, int c) {
   return d.profr + ((size_t)p * d.ncrms + c) * d.nz;
 }
+// A profile that the kernel only reads, as a constant-address-space pointer: with a
+// wave-uniform instance its loads become scalar loads, which the hardware counts apart from
+// the vector loads, so waiting for a profile value does not also wait for the field loads
+// issued ahead of it (vector loads are counted in issue order).
+template <typename R>
+using cprof = const __attribute__((address_space(4))) R*;
+template <typename R>
+__device__ __forceinline__ cprof<R> profc(const DevT<R>& d, int p, int c) {
+  return (cprof<R>)(d.profr + ((size_t)p * d.ncrms + c) * d.nz);
+}
+template <typename R>
+__device__ __forceinline__ cprof<double> profcd(const DevT<R>& d, int p, int c) {
+  return (cprof<double>)(d.prof + ((size_t)p * d.ncrms + c) * d.nz);
+}
 template <typename R>
 __device__ __forceinline__ double& inst(const DevT<R>& d, int p, int c) {
   return d.inst[(size_t)p * d.ncrms + c];
@@ -534,10 +548,14 @@ template <int NX>
 using SegFor = typename SegSel<NX>::type;
 
 // Instance of this thread's segment; false when past ncrms (the segment then idles but still
-// joins cross-lane operations of its wave).
-template <int NX>
+// joins cross-lane operations of its wave).  A segment of 64 or more lanes fills whole waves,
+// so its instance is wave-uniform; UNIFORM tells the compiler so, which turns the per-instance
+// profile loads into scalar loads.  A kernel whose scalar registers would then spill (k_stats
+// measured slower with it) passes false.
+template <int NX, bool UNIFORM = true>
 __device__ __forceinline__ bool seg_instance(int ncrms, int& c) {
   int g = (blockIdx.x * blockDim.x + threadIdx.x) / NX;
+  if constexpr (NX >= 64 && UNIFORM) g = __builtin_amdgcn_readfirstlane(g);
   c = g < ncrms ? g : ncrms - 1;
   return g < ncrms;
 }

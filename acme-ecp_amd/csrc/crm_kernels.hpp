@@ -388,8 +388,8 @@ __global__ void CRM_LB k_kurant(DevT<R> d) {
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
   const double dz = inst(d, I_DZ, c);
-  const double* adzw = prof(d, P_ADZW, c);
-  const double* grdf = prof(d, P_GRDFX, c);
+  const cprof<double> adzw = profcd(d, P_ADZW, c);
+  const cprof<double> grdf = profcd(d, P_GRDFX, c);
   const bool y3 = d.ny > 1;  // 3D: dy == dx
   const double dxf = sqrt((1. / d.dx) * (1. / d.dx) + (y3 ? (1. / d.dx) * (1. / d.dx) : 0.0));
   // Every term is a monotone function of a level maximum over the columns, and rounding is
@@ -430,18 +430,18 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
   const R dtn = d.dtn, dz = inst(d, I_DZ, c);
-  const R* adz = profr(d, P_ADZ, c);
-  const R* rho = profr(d, P_RHO, c);
-  const R* bet = profr(d, P_BET, c);
-  const R* tabs0 = profr(d, P_TABS0, c);
-  const R* qv0 = profr(d, P_QV0, c);
-  const R* qn0 = profr(d, P_QN0, c);
-  const R* qp0 = profr(d, P_QP0, c);
-  const R* ttend = profr(d, P_TTEND, c);
-  const R* qtend = profr(d, P_QTEND, c);
-  const R* utend = profr(d, P_UTEND, c);
-  const R* vtend = profr(d, P_VTEND, c);
-  const R* tau = profr(d, P_TAU, c);
+  const cprof<R> adz = profc(d, P_ADZ, c);
+  const cprof<R> rho = profc(d, P_RHO, c);
+  const cprof<R> bet = profc(d, P_BET, c);
+  const cprof<R> tabs0 = profc(d, P_TABS0, c);
+  const cprof<R> qv0 = profc(d, P_QV0, c);
+  const cprof<R> qn0 = profc(d, P_QN0, c);
+  const cprof<R> qp0 = profc(d, P_QP0, c);
+  const cprof<R> ttend = profc(d, P_TTEND, c);
+  const cprof<R> qtend = profc(d, P_QTEND, c);
+  const cprof<R> utend = profc(d, P_UTEND, c);
+  const cprof<R> vtend = profc(d, P_VTEND, c);
+  const cprof<R> tau = profc(d, P_TAU, c);
   const bool y3 = d.ny > 1;
   const int n_damp = (int)inst(d, I_NDAMP, c);
   const int kmin = (int)inst(d, I_KMIN, c), kmax = (int)inst(d, I_KMAX, c);
@@ -578,8 +578,8 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
 // read at the four corners of the cell.
 // ------------------------------------------------------------------------------------------
 template <typename R, typename G>
-__device__ __forceinline__ R shear3(const G& s, int k, const R* srdz, const R* srdx, const R* srdzw, const R* srdxw,
-                                    const R* u0, const R* v0, R um, R u, R up, R vm, R v, R vp, R w0, R w1) {
+__device__ __forceinline__ R shear3(const G& s, int k, cprof<R> srdz, cprof<R> srdx, cprof<R> srdzw, cprof<R> srdxw,
+                                    cprof<R> u0, cprof<R> v0, R um, R u, R up, R vm, R v, R vp, R w0, R w1) {
   const int kc = k + 1, kb = k - 1;
   const R rdz = srdz[k], rdx = srdx[k];
   const R rdzw_up = srdzw[kc], rdx_up = srdxw[kc];
@@ -626,17 +626,17 @@ __global__ void CRM_LB k_sgs(DevT<R> d) {
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
   const R dz = inst(d, I_DZ, c), dx = d.dx;
-  const R* adz = profr(d, P_ADZ, c);
-  const R* adzw = profr(d, P_ADZW, c);
-  const R* bet = profr(d, P_BET, c);
-  const R* presi = profr(d, P_PRESI, c);
-  const R* u0 = profr(d, P_U0, c);
-  const R* srdz = profr(d, P_SRDZ, c);
-  const R* srdx = profr(d, P_SRDX, c);
-  const R* srdzw = profr(d, P_SRDZW, c);
-  const R* srdxw = profr(d, P_SRDXW, c);
-  const R* sbetdz = profr(d, P_SBETDZ, c);
-  const R* stkmax = profr(d, P_STKMAX, c);
+  const cprof<R> adz = profc(d, P_ADZ, c);
+  const cprof<R> adzw = profc(d, P_ADZW, c);
+  const cprof<R> bet = profc(d, P_BET, c);
+  const cprof<R> presi = profc(d, P_PRESI, c);
+  const cprof<R> u0 = profc(d, P_U0, c);
+  const cprof<R> srdz = profc(d, P_SRDZ, c);
+  const cprof<R> srdx = profc(d, P_SRDX, c);
+  const cprof<R> srdzw = profc(d, P_SRDZW, c);
+  const cprof<R> srdxw = profc(d, P_SRDXW, c);
+  const cprof<R> sbetdz = profc(d, P_SBETDZ, c);
+  const cprof<R> stkmax = profc(d, P_STKMAX, c);
   const R Cs = R4(0.15), Ck = R4(0.1), Pr = R(1.);
   const R Ce = Ck * Ck * Ck / (Cs * Cs * Cs * Cs);
   auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
@@ -644,7 +644,7 @@ __global__ void CRM_LB k_sgs(DevT<R> d) {
   // window: u (and v in 3D) at k-1,k,k+1 and w at k,k+1
   R um = 0, u = F(d.u, 0), up = F(d.u, 1);
   R vm = 0, v = 0, vp = 0;
-  const R* v0 = profr(d, P_V0, c);
+  const cprof<R> v0 = profc(d, P_V0, c);
   if constexpr (G::Y3) {
     v = F(d.v, 0);
     vp = F(d.v, 1);
@@ -789,11 +789,11 @@ __device__ __forceinline__ void mom2(const DevT<R>& d, int na) {
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
   const R dz = inst(d, I_DZ, c), dx = d.dx;
-  const R* adz = profr(d, P_ADZ, c);
-  const R* adzw = profr(d, P_ADZW, c);
-  const R* rho = profr(d, P_RHO, c);
-  const R* rhow = profr(d, P_RHOW, c);
-  const R* grdf = profr(d, P_GRDFX, c);
+  const cprof<R> adz = profc(d, P_ADZ, c);
+  const cprof<R> adzw = profc(d, P_ADZW, c);
+  const cprof<R> rho = profc(d, P_RHO, c);
+  const cprof<R> rhow = profc(d, P_RHOW, c);
+  const cprof<R> grdf = profc(d, P_GRDFX, c);
   R* dudt = d.dudt[na];
   R* dwdt = d.dwdt[na];
   const R dx25 = R(0.25) / dx, dz25 = R(1.) / (R(4.) * dz);
@@ -928,11 +928,11 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
   const R dz = inst(d, I_DZ, c), dx = d.dx;
-  const R* adz = profr(d, P_ADZ, c);
-  const R* adzw = profr(d, P_ADZW, c);
-  const R* rho = profr(d, P_RHO, c);
-  const R* rhow = profr(d, P_RHOW, c);
-  const R* grdf = profr(d, P_GRDFX, c);  // grdf_y == grdf_x
+  const cprof<R> adz = profc(d, P_ADZ, c);
+  const cprof<R> adzw = profc(d, P_ADZW, c);
+  const cprof<R> rho = profc(d, P_RHO, c);
+  const cprof<R> rhow = profc(d, P_RHOW, c);
+  const cprof<R> grdf = profc(d, P_GRDFX, c);  // grdf_y == grdf_x
   R* dudt = d.dudt[na];
   R* dvdt = d.dvdt[na];
   R* dwdt = d.dwdt[na];
@@ -1604,13 +1604,13 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
                            double* flux, bool want_cfl) {
   constexpr bool Y3 = G::Y3;  // 3D: advect_scalar3D.F90:8-349 (dy == dx)
   const int nzm = d.nzm, nz = d.nz, i = s.i;
-  const R* adz = profr(d, P_ADZ, c);
-  const R* rho = profr(d, P_RHO, c);
-  const R* irho = profr(d, P_IRHO, c);
-  const R* iadz = profr(d, P_IADZ, c);
-  const R* irhowadz = profr(d, P_IRHOWADZ, c);
-  const R* ddadz = profr(d, P_DDADZ, c);
-  const R* pratio = profr(d, P_PRATIO, c);
+  const cprof<R> adz = profc(d, P_ADZ, c);
+  const cprof<R> rho = profc(d, P_RHO, c);
+  const cprof<R> irho = profc(d, P_IRHO, c);
+  const cprof<R> iadz = profc(d, P_IADZ, c);
+  const cprof<R> irhowadz = profc(d, P_IRHOWADZ, c);
+  const cprof<R> ddadz = profc(d, P_DDADZ, c);
+  const cprof<R> pratio = profc(d, P_PRATIO, c);
   const R eps = R4(1.e-10);
   auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
   auto Wv = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
@@ -1869,13 +1869,13 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, i = s.i;
   const R dz = inst(d, I_DZ, c), dtn = d.dtn;
-  const R* adz = profr(d, P_ADZ, c);
-  const R* rho = profr(d, P_RHO, c);
-  const R* rhow = profr(d, P_RHOW, c);
+  const cprof<R> adz = profc(d, P_ADZ, c);
+  const cprof<R> rho = profc(d, P_RHO, c);
+  const cprof<R> rhow = profc(d, P_RHOW, c);
   double* qpfall = prof(d, P_QPFALL, c);
   double* precflux = prof(d, P_PRECFLUX, c);
-  const R* pratio = profr(d, P_PRATIO, c);
-  const R* irhoadz_p = profr(d, P_IRHOADZ, c);
+  const cprof<R> pratio = profc(d, P_PRATIO, c);
+  const cprof<R> irhoadz_p = profc(d, P_IRHOADZ, c);
   const R eps = R4(1.e-10);
   const R a_pr = A_PR();
   // non-negative doubles order as their bit patterns
@@ -2011,13 +2011,13 @@ __device__ void diffuse_one(const DevT<R>& d, const G& s, int c, bool live, R* f
                             double* flux) {
   const int nzm = d.nzm, i = s.i;
   const R dz = inst(d, I_DZ, c), dtn = d.dtn;
-  const R* adz = profr(d, P_ADZ, c);
-  const R* adzw = profr(d, P_ADZW, c);
-  const R* rho = profr(d, P_RHO, c);
-  const R* rhow = profr(d, P_RHOW, c);
-  const R* grdf = profr(d, P_GRDFX, c);
-  const R* irhoadz = profr(d, P_IRHOADZ, c);
-  const R* drhoi = profr(d, P_DRHOI, c);
+  const cprof<R> adz = profc(d, P_ADZ, c);
+  const cprof<R> adzw = profc(d, P_ADZW, c);
+  const cprof<R> rho = profc(d, P_RHO, c);
+  const cprof<R> rhow = profc(d, P_RHOW, c);
+  const cprof<R> grdf = profc(d, P_GRDFX, c);
+  const cprof<R> irhoadz = profc(d, P_IRHOADZ, c);
+  const cprof<R> drhoi = profc(d, P_DRHOI, c);
   const R rdx2 = (R)(R(1.) / (d.dx * d.dx)), rdz2 = (R)1. / (dz * dz);
   auto Q = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
   R f_k = Q(f, 0), f_n = Q(f, 1), tkh_k = Q(d.tkh, 0), tkh_n = Q(d.tkh, 1);
@@ -2161,7 +2161,7 @@ __device__ __forceinline__ void diagnose_level(const DevT<R>& d, const S_& s, in
                                                bool y3, R t, R q, R qp, R qn, R qv, R qcl, R qci, R qpl, R qpi,
                                                R u, R v, int& kmin, int& kmax) {
   const R factor = R(1.) / (R)NX;
-  const R* gamaz = profr(d, P_GAMAZ, c);
+  const cprof<R> gamaz = profc(d, P_GAMAZ, c);
   R tabs = t - gamaz[k] + FAC_COND * (qcl + qpl) + FAC_SUB * (qci + qpi);
   R u0 = s.sum(u) * factor;
   R v0 = y3 ? s.sum(v) * factor : R(0.0);
@@ -2236,8 +2236,8 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
   const int nzm = d.nzm, i = s.i;
   const bool y3 = d.ny > 1;
   const R dtn = d.dtn, factor = R(1.) / (R)NX;
-  const R* gamaz = profr(d, P_GAMAZ, c);
-  const R* presp = profr(d, P_PRES, c);
+  const cprof<R> gamaz = profc(d, P_GAMAZ, c);
+  const cprof<R> presp = profc(d, P_PRES, c);
   const R an = R(1.) / (TBGMAX - TBGMIN), bn = TBGMIN * an;
   const R ap = R(1.) / (TPRMAX - TPRMIN), bp = TPRMIN * ap;
   const R fac1 = FAC_COND + (1 + bp) * FAC_FUS, fac2 = FAC_FUS * ap;
@@ -2560,7 +2560,7 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
   CRM_STOP_CHECK(d);
   SegFor<NX> s;
   int c;
-  bool live = seg_instance<NX>(d.ncrms, c);
+  bool live = seg_instance<NX, false>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
   const double dz = inst(d, I_DZ, c);
   const double* rho = prof(d, P_RHO, c);
@@ -2713,10 +2713,10 @@ __global__ void CRM_LB k_final(DevT<R> d, crm_input_view in, crm_state_view st, 
   const int nzm = d.nzm, nz = d.nz, ld = d.ld, plev = d.plev, i = s.i;
   const double factor_xy = 1. / (double)NX, factor_xyt = factor_xy / (double)nstop;
   const double icrm_run_time = 1. / dt_gl, dz = inst(d, I_DZ, c);
-  const double* rho = prof(d, P_RHO, c);
-  const double* rhow = prof(d, P_RHOW, c);
-  const double* u0 = prof(d, P_U0, c);
-  const double* v0 = prof(d, P_V0, c);
+  const cprof<double> rho = profcd(d, P_RHO, c);
+  const cprof<double> rhow = profcd(d, P_RHOW, c);
+  const cprof<double> u0 = profcd(d, P_U0, c);
+  const cprof<double> v0 = profcd(d, P_V0, c);
   const bool y3 = d.ny > 1;
   const int ptop0 = plev - nzm;  // 0-based GCM index of the top CRM level
   auto IN = [&](const double* p, int l) { return p[(size_t)c + (size_t)ld * l]; };
@@ -2831,8 +2831,8 @@ __global__ void CRM_LB k_final(DevT<R> d, crm_input_view in, crm_state_view st, 
     }
     // mass fluxes: interface l (0..plev) is w level plev-l
     double jt = plev * 1.0, mx = 1.0;
-    const double* mui = prof(d, P_MUI, c);
-    const double* mdi = prof(d, P_MDI, c);
+    const cprof<double> mui = profcd(d, P_MUI, c);
+    const cprof<double> mdi = profcd(d, P_MDI, c);
     auto MU = [&](int lw) { int kw = plev - lw; return (kw >= 0 && kw <= nzm - 1) ? mui[kw] : 0.0; };
     auto MD = [&](int lw) { int kw = plev - lw; return (kw >= 0 && kw <= nzm - 1) ? mdi[kw] : 0.0; };
     for (int l = 0; l < plev; l++) {
