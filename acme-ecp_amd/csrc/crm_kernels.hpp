@@ -1475,9 +1475,22 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   transform(d.Ft);
   // 3. Thomas sweep per slot (wave 0, lane = slot), factors in LDS
   if (wave == 0) {
+    static_assert(NZM <= 64, "one level per lane");
     const double eig = d.eign[i];
-    auto acoef = [&](int k) { return rhow[k] / (adz[k] * adzw[k] * dz * dz); };
-    auto ccoef = [&](int k) { return rhow[k + 1] / (adz[k] * adzw[k + 1] * dz * dz); };
+    // the level coefficients do not depend on the recurrence: lane k computes level k's,
+    // and the sweep broadcasts them (the sweep then waits on one division per level, not
+    // three plus the profile loads)
+    const int kl = i < NZM ? i : NZM - 1;
+    const double ac_l = rhow[kl] / (adz[kl] * adzw[kl] * dz * dz);
+    const double cc_l = rhow[kl + 1] / (adz[kl] * adzw[kl + 1] * dz * dz);
+    const double rho_l = rho[kl];
+    auto lane_of = [](double v, int k) {
+      const long long b = __double_as_longlong(v);
+      const int lo = __builtin_amdgcn_readlane((int)b, k), hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+      return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    };
+    auto acoef = [&](int k) { return lane_of(ac_l, k); };
+    auto ccoef = [&](int k) { return lane_of(cc_l, k); };
     double b;
     if (i == 0)
       b = 1.0 / (eig * rho[0] - acoef(0) - ccoef(0));
@@ -1489,7 +1502,7 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
     P[i] = beta;
     for (int k = 1; k < NZM - 1; k++) {
       double ak = acoef(k), ck = ccoef(k);
-      double e = 1.0 / (eig * rho[k] - ak - ck + ak * alfa);
+      double e = 1.0 / (eig * lane_of(rho_l, k) - ak - ck + ak * alfa);
       alfa = -ck * e;
       ALF[k * 64 + i] = alfa;
       beta = (P[k * PLD + i] - ak * beta) * e;
@@ -2571,10 +2584,11 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
   const int nxr = d.nx_rad, g = NX / nxr;
   auto Q = [&](const R* p, int k) -> double { return p[((size_t)c * nzm + k) * NX + i]; };
   auto Wv = [&](int k) -> double { return d.w[((size_t)c * nz + k) * NX + i]; };
-  // the fields of one level; the sweep runs down from the top and reads level k-1 while it
-  // works on level k
+  // the fields and profiles of one level; the sweep runs down from the top and reads level
+  // k-1 while it works on level k (the profiles too: vector loads complete in issue order, so
+  // a profile read issued after the prefetch would wait for it)
   struct Lvl {
-    double qcl, qci, qpl, qpi, tabs, qv, w;
+    double qcl, qci, qpl, qpi, tabs, qv, w, rho, rhow, adz, pres;
   };
   auto load = [&](int k, Lvl& L) {
     L.qcl = Q(d.qcl, k);
@@ -2584,6 +2598,10 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
     L.tabs = Q(d.tabs, k);
     L.qv = Q(d.qv, k);
     L.w = Wv(k);
+    L.rho = rho[k];
+    L.rhow = rhow[k];
+    L.adz = adz[k];
+    L.pres = pres[k];
   };
   double cwp = 0, cwph = 0, cwpm = 0, cwpl = 0, clt = 0, cmt = 0, cht = 0, ctt = 0;
   bool flag_top = true;
@@ -2603,7 +2621,7 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
     const double qcl = cur.qcl, qci = cur.qci, qpl = cur.qpl, qpi = cur.qpi, tabs = cur.tabs;
     const double wk = cur.w;
     // column water paths from the top (the reference's level nz-k)
-    double tmp1 = rho[k] * adz[k] * dz * (qcl + qci);
+    double tmp1 = cur.rho * cur.adz * dz * (qcl + qci);
     cwp += tmp1;
     ctt = fmax(1.0, ctt);
     double top_hit = 0.0;
@@ -2611,10 +2629,10 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       top_hit = 1.0;
       flag_top = false;
     }
-    if (pres[k] >= 700.) {
+    if (cur.pres >= 700.) {
       cwpl += tmp1;
       clt = fmax(1.0, clt);
-    } else if (pres[k] < 400.) {
+    } else if (cur.pres < 400.) {
       cwph += tmp1;
       cht = fmax(1.0, cht);
     } else {
@@ -2622,9 +2640,9 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       cmt = fmax(1.0, cmt);
     }
     // level statistics (the reference's level k)
-    double mass = rho[k] * adz[k] * dz;
+    double mass = cur.rho * cur.adz * dz;
     double ws = w_up + wk;
-    double rw = rho[k] * 0.5 * ws;
+    double rw = cur.rho * 0.5 * ws;
     double cld = 0, mcup = 0, mcdn = 0, mcuup = 0, mcudn = 0;
     if (mass * (qcl + qci) > thr) {
       cld = 1.0;
@@ -2644,17 +2662,15 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
     double mui = 0, mdi = 0;
     if (wk > 0.) {
       const Lvl& X = k >= 1 ? nxt : cur;
-      const int kx = k >= 1 ? k - 1 : 0;
-      double qs = qsatw(X.tabs, pres[kx]);
-      if (X.qcl + X.qci > fmin(R4(1.e-5), 0.01 * qs)) mui = rhow[k] * wk;
+      double qs = qsatw(X.tabs, X.pres);
+      if (X.qcl + X.qci > fmin(R4(1.e-5), 0.01 * qs)) mui = cur.rhow * wk;
     } else if (wk < 0.) {
       const Lvl& X = k + 1 < nzm ? up : cur;
-      const int kx = k + 1 < nzm ? k + 1 : nzm - 1;
-      double qs = qsatw(X.tabs, pres[kx]);
+      double qs = qsatw(X.tabs, X.pres);
       if (X.qcl + X.qci > fmin(R4(1.e-5), 0.01 * qs))
-        mdi = rhow[k] * wk;
+        mdi = cur.rhow * wk;
       else if (X.qpl + X.qpi > R4(1.0e-4))
-        mdi = rhow[k] * wk;
+        mdi = cur.rhow * wk;
     }
     double sv[10] = {top_hit, cld, mcup, mcdn, mcuup, mcudn, qcl, qci, mui, mdi};
     s.multi_sum(sv);
