@@ -1660,11 +1660,15 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
     nu1 = nu2;
     nw1 = nw2;
     nv1 = nv2;
-    if (K + 2 < nzm) {
-      nf2 = F(f, K + 2);
-      nu2 = F(d.u, K + 2);
-      nw2 = Wv(K + 2);
-      if constexpr (Y3) nv2 = F(d.v, K + 2);
+    // unconditional, clamped at the top level: past it the ring already holds that level's
+    // values, so the reload changes nothing, and a load that always issues can write the ring
+    // register directly (a conditional one is preceded by a copy that waits for earlier loads)
+    {
+      const int kn = ::min(K + 2, nzm - 1);
+      nf2 = F(f, kn);
+      nu2 = F(d.u, kn);
+      nw2 = Wv(kn);
+      if constexpr (Y3) nv2 = F(d.v, kn);
     }
     R tq_new = R(0.0);
     if (want_cfl && (ST || (K >= 3 && K - 3 < nzm))) tq_new = F(d.tabs, K - 3);
@@ -1848,6 +1852,13 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
     // FP64 issues every instruction it keeps (measured 27% faster peeled); the FP32 sweep
     // was measured faster as one loop
     for (; K < 5 && K <= nzm + 4; K++) step(K, std::false_type{});
+    // two levels per trip: fewer of the prefetched values are copied along the ring before
+    // their use (a copy waits for its load); with the unconditional prefetch, measured 8%
+    // faster at 64 x 58
+    for (; K + 1 < nzm; K += 2) {
+      step(K, std::true_type{});
+      step(K + 1, std::true_type{});
+    }
     for (; K < nzm; K++) step(K, std::true_type{});
   }
   for (; K <= nzm + 4; K++) step(K, std::false_type{});
