@@ -1854,10 +1854,13 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
     for (; K < 5 && K <= nzm + 4; K++) step(K, std::false_type{});
     // two levels per trip: fewer of the prefetched values are copied along the ring before
     // their use (a copy waits for its load); with the unconditional prefetch, measured 8%
-    // faster at 64 x 58
-    for (; K + 1 < nzm; K += 2) {
-      step(K, std::true_type{});
-      step(K + 1, std::true_type{});
+    // faster at 64 x 58.  Only for the 2D 64-column segment: the other shapes already hold
+    // more per lane and spilled more with it (the uneven classes measured slower)
+    if constexpr (NX == 64 && !Y3) {
+      for (; K + 1 < nzm; K += 2) {
+        step(K, std::true_type{});
+        step(K + 1, std::true_type{});
+      }
     }
     for (; K < nzm; K++) step(K, std::true_type{});
   }
