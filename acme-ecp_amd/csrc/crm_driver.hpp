@@ -87,6 +87,9 @@ struct Engine : CrmEngine {
   const crm_mlp* mlp = nullptr;
   float *mlp_x = nullptr, *mlp_y = nullptr;
   size_t mlp_cap = 0;  // points
+  // the call's radiative heating in the sweep layout [instance][level][radiation column]
+  double* qrad_t = nullptr;
+  size_t qrad_cap = 0;  // elements
   float *samp_x = nullptr, *samp_t = nullptr;
   unsigned long long* samp_n = nullptr;
   long long samp_cap = 0;
@@ -486,6 +489,20 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     const size_t np = (size_t)NPROF * ncrms * (nzm + 1);
     k_prof_mirror<R><<<dim3((unsigned)((np + 255) / 256)), b256, 0, s>>>(d.prof, d.profr, np);
   }
+  // crm_rad%qrad is constant over the call: one transpose from the caller's (pcols, nx_rad,
+  // ny_rad, nz) order, so every substep's k_pre reads it contiguously
+  {
+    const size_t nq = (size_t)ncrms * nzm * d.nx_rad;
+    if (ctx->qrad_cap < nq) {
+      if (ctx->qrad_t) HIPCHECK(hipFree(ctx->qrad_t));
+      ctx->qrad_t = nullptr;
+      ctx->qrad_cap = 0;
+      if (hipMalloc(&ctx->qrad_t, nq * sizeof(double)) != hipSuccess) return CRM_ERR_NOMEM;
+      ctx->qrad_cap = nq;
+    }
+    k_qrad_order<<<dim3((unsigned)((nq + 255) / 256)), b256, 0, s>>>(rad->qrad, ctx->qrad_t, ncrms, d.ld, nzm,
+                                                                     d.nx_rad);
+  }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemsetAsync(ctx->gmax + 2, 0, 2 * sizeof(unsigned long long), s));
 
@@ -555,7 +572,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       double dt3[3];
     };
     std::vector<Snap> snap((size_t)nstop + 1);
-    const double* qrad = rad->qrad;
+    const double* qrad = ctx->qrad_t;
     d.stop = ctx->gmax + GMAX_STOP;
     DevT<R> dk;  // the key: the call's starting arguments
     std::memcpy(&dk, &d, sizeof(d));
@@ -681,7 +698,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       d.dtfactor = 1.0;
       abcoefs(nstep, dt3, na, nb, nc, d.at, d.bt, d.ct);
       d.dt3na = dt3[na];
-      const double* qrad = rad->qrad;
+      const double* qrad = ctx->qrad_t;
       auto body = [&]() {
         k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d);
         NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad)));
@@ -747,7 +764,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       if (icyc == 1)
         TIMED(CRM_K_PRECIP_INIT,
               (k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d)));
-      const double* qrad = rad->qrad;
+      const double* qrad = ctx->qrad_t;
       TIMED(CRM_K_PRE, NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad))));
       if (last && phase_stop == 7) goto stopped;
       TIMED(CRM_K_SGS, NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d))));
@@ -952,6 +969,7 @@ Engine<R>::~Engine() {
   }
   drop_graphs();
   if (mlp_x) (void)hipFree(mlp_x);
+  if (qrad_t) (void)hipFree(qrad_t);
   if (mlp_y) (void)hipFree(mlp_y);
   if (samp_n) (void)hipFree(samp_n);
   if (pool) (void)hipFree(pool);

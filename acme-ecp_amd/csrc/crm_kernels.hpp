@@ -175,6 +175,19 @@ __global__ void k_setup(DevT<R> d, crm_input_view in) {
 }
 
 // FP32 copies of the level profiles (mixed-precision form only).
+// crm_rad%qrad from the caller's (pcols, nx_rad, ny_rad, nz) order (crm_rad_module.F90:17-34)
+// into [instance][level][radiation column]; consecutive threads read consecutive instances
+__global__ void k_qrad_order(const double* __restrict__ qrad, double* __restrict__ out, int ncrms, int ld, int nzm,
+                             int nxr) {
+  const size_t n = (size_t)ncrms * nzm * nxr;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int c = (int)(t % ncrms);
+  const size_t rk = t / ncrms;  // ir + nxr * k
+  const int ir = (int)(rk % nxr), k = (int)(rk / nxr);
+  out[((size_t)c * nzm + k) * nxr + ir] = qrad[(size_t)c + (size_t)ld * rk];
+}
+
 template <typename R>
 __global__ void k_prof_mirror(const double* src, R* dst, size_t n) {
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -487,7 +500,7 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
     L.t = d.t[o];
     L.qt = d.qt[o];
     L.v = y3 ? d.v[o] : R(0.0);
-    L.qr = qrad[(size_t)c + (size_t)d.ld * ((size_t)ir + (size_t)nxr * k)];
+    L.qr = qrad[((size_t)c * nzm + k) * nxr + ir];  // [instance][level][radiation column]
   };
   Lvl nxt;
   load(0, nxt);
