@@ -157,8 +157,9 @@ struct DevT {
   R *profr;                   // the same profiles in R (== prof when R is double)
   double *inst;               // NINST blocks of [ncrms]
   R *coef;                    // NCOEF blocks of [ncrms][nz]
-  double *rad;                // 5 blocks of [ncrms][nzm][nx_rad]: t, qv, qc, qi, cld (in 3D
-                              // nx_rad counts the nx_rad*ny_rad radiation columns)
+  R *rad;                     // 5 blocks of [ncrms][nzm][nx_rad]: t, qv, qc, qi, cld (in 3D
+                              // nx_rad counts the nx_rad*ny_rad radiation columns); sums in
+                              // R: FP64, or FP32 in the mixed engine
   const double *Ft, *Gt;      // transform operators [NX][NX]
   const double *eign;         // [NX] horizontal eigenvalues per slot
   unsigned long long *gmax;   // [0] stop flag of a captured call, [1] unused, [2] error flag,

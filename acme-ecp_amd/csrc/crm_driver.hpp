@@ -253,7 +253,7 @@ static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr, b
                           mirror ? (size_t)NPROF * ncap * nz * sizeof(R) : 0,
                           (size_t)NINST * ncap * sizeof(double),
                           (size_t)NCOEF * ncap * nz * sizeof(R),
-                          5 * (size_t)ncap * nzm * nxr * sizeof(double),   // radiation sums
+                          5 * (size_t)ncap * nzm * nxr * sizeof(R),        // radiation sums
                           y3 ? NSCAL3 * scal * sizeof(R) : 0};             // v, dvdt x3
   size_t bytes = 0;
   for (size_t b : sizes) bytes += al(b);
@@ -297,7 +297,7 @@ static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr, b
   }
   d.inst = (double*)take(sizes[5]);
   d.coef = (R*)take(sizes[6]);
-  d.rad = (double*)take(sizes[7]);
+  d.rad = (R*)take(sizes[7]);
   R* vf = (R*)take(sizes[8]);
   d.v = y3 ? vf : nullptr;
   for (int q = 0; q < 3; q++) d.dvdt[q] = y3 ? vf + (size_t)(q + 1) * scal : nullptr;

@@ -332,7 +332,7 @@ __global__ void CRM_LB k_init(DevT<R> d, crm_input_view in, crm_state_view st, c
   for (int p = P_UWLE; p < NPROF; p++)
     for (int k = i; k < nz; k += NX) prof(d, p, c)[k] = 0.0;
   for (int q = 0; q < 5; q++)
-    for (int k = i; k < nzm * d.nx_rad; k += NX) d.rad[((size_t)q * d.ncrms + c) * nzm * d.nx_rad + k] = 0.0;
+    for (int k = i; k < nzm * d.nx_rad; k += NX) d.rad[((size_t)q * d.ncrms + c) * nzm * d.nx_rad + k] = R(0.0);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2642,7 +2642,9 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
   for (int k = nzm - 1; k >= 0; k--) {
     if (k >= 1) load(k - 1, nxt);
     const size_t rad_base = (size_t)c * nzm * nxr + (size_t)k * nxr + i / g;
-    double ra[5] = {0, 0, 0, 0, 0};
+    // the sums are R: in the mixed engine FP32, which halves the bytes of this kernel's
+    // largest traffic (each addition rounds once, from the FP64 sum of the two terms)
+    R ra[5] = {0, 0, 0, 0, 0};
     if (rad_owner)
       for (int q = 0; q < 5; q++) ra[q] = d.rad[q * rad_blk + rad_base];
     const double qcl = cur.qcl, qci = cur.qci, qpl = cur.qpl, qpi = cur.qpi, tabs = cur.tabs;
@@ -2723,11 +2725,11 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       if (s.owns(8)) accum(&prof(d, P_MUI, c)[k], sv[8]);
       if (s.owns(9)) accum(&prof(d, P_MDI, c)[k], sv[9]);
       if (rad_owner) {
-        d.rad[0 * rad_blk + rad_base] = ra[0] + r_t;
-        d.rad[1 * rad_blk + rad_base] = ra[1] + r_qv;
-        d.rad[2 * rad_blk + rad_base] = ra[2] + r_qc;
-        d.rad[3 * rad_blk + rad_base] = ra[3] + r_qi;
-        d.rad[4 * rad_blk + rad_base] = ra[4] + r_cld;
+        d.rad[0 * rad_blk + rad_base] = (R)((double)ra[0] + r_t);
+        d.rad[1 * rad_blk + rad_base] = (R)((double)ra[1] + r_qv);
+        d.rad[2 * rad_blk + rad_base] = (R)((double)ra[2] + r_qc);
+        d.rad[3 * rad_blk + rad_base] = (R)((double)ra[3] + r_qi);
+        d.rad[4 * rad_blk + rad_base] = (R)((double)ra[4] + r_cld);
       }
     }
     w_up = wk;
@@ -2936,7 +2938,7 @@ __global__ void CRM_LB k_final(DevT<R> d, crm_input_view in, crm_state_view st, 
       int k = idx / nxr, ir = idx - k * nxr;
       size_t src = (size_t)c * nzm * nxr + idx;
       size_t dst = (size_t)c + (size_t)ld * ((size_t)ir + (size_t)nxr * k);
-      for (int q = 0; q < 5; q++) outs[q][dst] = d.rad[q * blk + src] * f;
+      for (int q = 0; q < 5; q++) outs[q][dst] = (double)d.rad[q * blk + src] * f;
     }
   }
 }
