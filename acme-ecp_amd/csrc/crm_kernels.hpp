@@ -2046,71 +2046,90 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
 // SGS diffusion of t, qt, qp: diffuse_scalar.F90:8-72 and diffuse_scalar2D.F90:5-141 (3D:
 // diffuse_scalar3D.F90:6-218, dy == dx; zero surface and top fluxes in the CRM).
 // ------------------------------------------------------------------------------------------
-template <int NX, typename R, typename G>
-__device__ void diffuse_one(const DevT<R>& d, const G& s, int c, bool live, R* f, double* fdiff,
-                            double* flux) {
+// The three fields diffuse independently and share the eddy diffusivity, so one sweep carries
+// all three: tkh and the level profiles are read once per level instead of once per field, and
+// the three fields' chains interleave.  Each field's arithmetic is the one-field sweep's, so
+// the results are the same bits.
+template <int NX, typename R, typename G = SegFor<NX>>
+__global__ void CRM_LB k_diffuse(DevT<R> d) {
+  CRM_STOP_CHECK(d);
+  G s;
+  int c;
+  const bool live = seg_instance<NX>(d.ncrms, c);
+  constexpr int NF = 3;
+  R* const fs[NF] = {d.t, d.qt, d.qp};
+  double* const fdiffs[NF] = {nullptr, prof(d, P_MKDIFF1, c), prof(d, P_MKDIFF2, c)};
+  double* const fluxes[NF] = {nullptr, prof(d, P_MKWSB1, c), prof(d, P_MKWSB2, c)};
   const int nzm = d.nzm, i = s.i;
   const R dz = inst(d, I_DZ, c), dtn = d.dtn;
-  const cprof<R> adz = profc(d, P_ADZ, c);
   const cprof<R> adzw = profc(d, P_ADZW, c);
-  const cprof<R> rho = profc(d, P_RHO, c);
   const cprof<R> rhow = profc(d, P_RHOW, c);
   const cprof<R> grdf = profc(d, P_GRDFX, c);
   const cprof<R> irhoadz = profc(d, P_IRHOADZ, c);
   const cprof<R> drhoi = profc(d, P_DRHOI, c);
   const R rdx2 = (R)(R(1.) / (d.dx * d.dx)), rdz2 = (R)1. / (dz * dz);
   auto Q = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
-  R f_k = Q(f, 0), f_n = Q(f, 1), tkh_k = Q(d.tkh, 0), tkh_n = Q(d.tkh, 1);
-  R flx_prev = R(0.0) * (R(1.) / dz) * rhow[0];
+  R f_k[NF], f_n[NF], flx_prev[NF];
+#pragma unroll
+  for (int q = 0; q < NF; q++) {
+    f_k[q] = Q(fs[q], 0);
+    f_n[q] = Q(fs[q], 1);
+    flx_prev[q] = R(0.0) * (R(1.) / dz) * rhow[0];
+  }
+  R tkh_k = Q(d.tkh, 0), tkh_n = Q(d.tkh, 1);
   for (int k = 0; k < nzm; k++) {
-    R f_nn = (k + 2 < nzm) ? Q(f, k + 2) : R(0.0);
-    R tkh_nn = (k + 2 < nzm) ? Q(d.tkh, k + 2) : R(0.0);
-    R rdx5 = R(0.5) * rdx2 * grdf[k];
-    R tkx = rdx5 * (tkh_k + s.rot(tkh_k, 1));
-    R flxh = -tkx * (s.rot(f_k, 1) - f_k);
-    R dfh = R(0.) - (flxh - s.rot(flxh, -1));
-    if constexpr (G::Y3) {
-      R tky = rdx5 * (tkh_k + s.roty(tkh_k, 1));
-      R flxy = -tky * (s.roty(f_k, 1) - f_k);
-      dfh = dfh - (flxy - s.roty(flxy, -1));
+    R f_nn[NF];
+#pragma unroll
+    for (int q = 0; q < NF; q++) f_nn[q] = (k + 2 < nzm) ? Q(fs[q], k + 2) : R(0.0);
+    const R tkh_nn = (k + 2 < nzm) ? Q(d.tkh, k + 2) : R(0.0);
+    const R rdx5 = R(0.5) * rdx2 * grdf[k];
+    const R tkx = rdx5 * (tkh_k + s.rot(tkh_k, 1));
+    R tky = R(0.0);
+    if constexpr (G::Y3) tky = rdx5 * (tkh_k + s.roty(tkh_k, 1));
+    const bool inner = k <= nzm - 2;
+    R tkz = R(0.0), rhoi_z = R(0.0);
+    if (inner) {
+      rhoi_z = drhoi[k];
+      const R rdz5 = R(0.5) * rdz2 * R(1.0);
+      tkz = rdz5 * (tkh_k + tkh_n);
     }
-    R flxv;
-    if (k <= nzm - 2) {
-      const R rhoi = drhoi[k];
-      R rdz5 = R(0.5) * rdz2 * R(1.0);
-      R tkz = rdz5 * (tkh_k + tkh_n);
-      flxv = -tkz * (f_n - f_k) * rhoi;
-      R sv = s.sum(flxv);
-      if (flux && live && i == 0) flux[k + 1] = sv;
-    } else {
-      flxv = R(0.0) * (R(1.) / dz) * (R(1.) / adzw[nzm]) * rhow[nzm];
+    const R rhoi = irhoadz[k];
+#pragma unroll
+    for (int q = 0; q < NF; q++) {
+      const R fk = f_k[q];
+      const R flxh = -tkx * (s.rot(fk, 1) - fk);
+      R dfh = R(0.) - (flxh - s.rot(flxh, -1));
+      if constexpr (G::Y3) {
+        const R flxy = -tky * (s.roty(fk, 1) - fk);
+        dfh = dfh - (flxy - s.roty(flxy, -1));
+      }
+      R flxv;
+      if (inner) {
+        flxv = -tkz * (f_n[q] - fk) * rhoi_z;
+        if (fluxes[q]) {
+          const R sv = s.sum(flxv);
+          if (live && i == 0) fluxes[q][k + 1] = sv;
+        }
+      } else {
+        flxv = R(0.0) * (R(1.) / dz) * (R(1.) / adzw[nzm]) * rhow[nzm];
+      }
+      const R dfdt = dtn * (dfh - (flxv - flx_prev[q]) * rhoi);
+      const R fnew = fk + dfdt;
+      if (live) fs[q][((size_t)c * nzm + k) * NX + i] = fnew;
+      if (fdiffs[q]) {
+        const R sd = s.sum(fnew - fk);
+        if (live && i == 0) fdiffs[q][k] = sd;
+      }
+      flx_prev[q] = flxv;
+      f_k[q] = f_n[q];
+      f_n[q] = f_nn[q];
     }
-    R rhoi = irhoadz[k];
-    R dfdt = dtn * (dfh - (flxv - flx_prev) * rhoi);
-    R fnew = f_k + dfdt;
-    R sd = s.sum(fnew - f_k);
-    if (live) {
-      f[((size_t)c * nzm + k) * NX + i] = fnew;
-      if (fdiff && i == 0) fdiff[k] = sd;
-    }
-    flx_prev = flxv;
-    f_k = f_n;
-    f_n = f_nn;
     tkh_k = tkh_n;
     tkh_n = tkh_nn;
   }
-  if (flux && live && i == 0) flux[0] = R(0.0);
-}
-
-template <int NX, typename R, typename G = SegFor<NX>>
-__global__ void CRM_LB k_diffuse(DevT<R> d) {
-  CRM_STOP_CHECK(d);
-  G s;
-  int c;
-  bool live = seg_instance<NX>(d.ncrms, c);
-  diffuse_one<NX, R, G>(d, s, c, live, d.t, nullptr, nullptr);
-  diffuse_one<NX, R, G>(d, s, c, live, d.qt, prof(d, P_MKDIFF1, c), prof(d, P_MKWSB1, c));
-  diffuse_one<NX, R, G>(d, s, c, live, d.qp, prof(d, P_MKDIFF2, c), prof(d, P_MKWSB2, c));
+#pragma unroll
+  for (int q = 0; q < NF; q++)
+    if (fluxes[q] && live && i == 0) fluxes[q][0] = R(0.0);
 }
 
 // ------------------------------------------------------------------------------------------
