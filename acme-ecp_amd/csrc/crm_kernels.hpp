@@ -1861,9 +1861,10 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
     ww3_a = ww3_new;
   };
   int K = 0;
-  if constexpr (std::is_same<R, double>::value) {
-    // FP64 issues every instruction it keeps (measured 27% faster peeled); the FP32 sweep
-    // was measured faster as one loop
+  if constexpr (std::is_same<R, double>::value || (NX == 64 && !Y3)) {
+    // FP64 issues every instruction it keeps (measured 27% faster peeled).  The FP32 sweep
+    // was measured faster as one loop until the prefetch became unconditional; since then the
+    // 2D 64-column FP32 sweep is peeled too (18% faster), the other FP32 shapes stay one loop
     for (; K < 5 && K <= nzm + 4; K++) step(K, std::false_type{});
     // two levels per trip: fewer of the prefetched values are copied along the ring before
     // their use (a copy waits for its load); with the unconditional prefetch, measured 8%
