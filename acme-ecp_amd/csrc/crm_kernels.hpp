@@ -1861,23 +1861,22 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
     ww3_a = ww3_new;
   };
   int K = 0;
-  if constexpr (std::is_same<R, double>::value || (NX == 64 && !Y3)) {
-    // FP64 issues every instruction it keeps (measured 27% faster peeled).  The FP32 sweep
-    // was measured faster as one loop until the prefetch became unconditional; since then the
-    // 2D 64-column FP32 sweep is peeled too (18% faster), the other FP32 shapes stay one loop
-    for (; K < 5 && K <= nzm + 4; K++) step(K, std::false_type{});
-    // two levels per trip: fewer of the prefetched values are copied along the ring before
-    // their use (a copy waits for its load); with the unconditional prefetch, measured 8%
-    // faster at 64 x 58.  Only for the 2D 64-column segment: the other shapes already hold
-    // more per lane and spilled more with it (the uneven classes measured slower)
-    if constexpr (NX == 64 && !Y3) {
-      for (; K + 1 < nzm; K += 2) {
-        step(K, std::true_type{});
-        step(K + 1, std::true_type{});
-      }
+  // Peeled into fill, steady state and drain.  FP64 issues every instruction it keeps
+  // (measured 27% faster peeled).  The FP32 sweep was measured faster as one loop until the
+  // prefetch became unconditional; peeled since (2D 64-column 18% faster, the uneven classes
+  // 4%).
+  for (; K < 5 && K <= nzm + 4; K++) step(K, std::false_type{});
+  // two levels per trip: fewer of the prefetched values are copied along the ring before
+  // their use (a copy waits for its load); with the unconditional prefetch, measured 8%
+  // faster at 64 x 58.  Only for the 2D 64-column segment: the other shapes already hold
+  // more per lane and spilled more with it (the uneven classes measured slower)
+  if constexpr (NX == 64 && !Y3) {
+    for (; K + 1 < nzm; K += 2) {
+      step(K, std::true_type{});
+      step(K + 1, std::true_type{});
     }
-    for (; K < nzm; K++) step(K, std::true_type{});
   }
+  for (; K < nzm; K++) step(K, std::true_type{});
   for (; K <= nzm + 4; K++) step(K, std::false_type{});
   if (want_cfl) {
     pcfl = s.max(pcfl);
