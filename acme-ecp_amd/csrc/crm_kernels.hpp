@@ -1950,7 +1950,10 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
     R www_a = 0;
     R mx_a = 0, mn_a = 0;
     R fz_a = 0;
-    for (int K = 0; K <= nzm + 2; K++) {
+    // one level step; ST: steady state (4 <= K < nzm), where every stage is active and every
+    // boundary select is the interior one, so the guards fold away (as in MPDATA)
+    auto step = [&](const int K, auto steady) {
+      constexpr bool ST = decltype(steady)::value;
       const R cq = nq1, ca = na1;
       nq1 = nq2;
       na1 = na2;
@@ -1958,9 +1961,9 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
         nq2 = Q(d.qp, K + 2);
         na2 = Q(d.tabs, K + 2);
       }
-      const R tt_new = (K >= 1 && K - 1 < nzm) ? Q(d.t, K - 1) : R(0.0);
+      const R tt_new = (ST || (K >= 1 && K - 1 < nzm)) ? Q(d.t, K - 1) : R(0.0);
       // a: level K, upwind flux
-      if (K < nzm) {
+      if (ST || K < nzm) {
         tq_0 = cq;
         ta_0 = ca;
         R v = pratio[K] * term_vel(d.tv, tq_0, rho[K], ta_0);
@@ -1972,35 +1975,35 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
       // b: level K-1, upwind update, first bounds, antidiffusive flux
       const int B = K - 1;
       R tq1_new = 0, mx0_new = 0, mn0_new = 0, www_new = 0;
-      if (B >= 0 && B < nzm) {
+      if (ST || (B >= 0 && B < nzm)) {
         R irhoadz = irhoadz_p[B];
         tq1_new = tq_1 - (fz0_0 - fz0_1) * irhoadz;
-        R tkb = (B >= 1) ? tq_2 : tq_1, tkc = (B + 1 < nzm) ? tq_0 : tq_1;
+        R tkb = (ST || B >= 1) ? tq_2 : tq_1, tkc = (ST || B + 1 < nzm) ? tq_0 : tq_1;
         mx0_new = fmax(fmax(tkb, tkc), tq_1);
         mn0_new = fmin(fmin(tkb, tkc), tq_1);
-        R t1b = (B >= 1) ? tq1_a : tq1_new, wpb = (B >= 1) ? wp_2 : wp_1;
+        R t1b = (ST || B >= 1) ? tq1_a : tq1_new, wpb = (ST || B >= 1) ? wp_2 : wp_1;
         www_new = R(0.5) * (R(1.) + wp_1 * irhoadz) * (t1b * wpb - tq1_new * wp_1);
       }
       // d: level K-2, limiter and corrected flux
       const int D = K - 2;
       R mx_new = 0, mn_new = 0, fz_new = 0;
-      if (D >= 0 && D < nzm) {
-        R t1b = (D >= 1) ? tq1_b : tq1_a, t1c = (D + 1 < nzm) ? tq1_new : tq1_a;
+      if (ST || (D >= 0 && D < nzm)) {
+        R t1b = (ST || D >= 1) ? tq1_b : tq1_a, t1c = (ST || D + 1 < nzm) ? tq1_new : tq1_a;
         R mx1 = fmax(fmax(fmax(t1b, t1c), tq1_a), mx0_a);
         R mn1 = fmin(fmin(fmin(t1b, t1c), tq1_a), mn0_a);
-        R wwc = (D + 1 < nzm) ? www_new : www_a;
+        R wwc = (ST || D + 1 < nzm) ? www_new : www_a;
         R ra = rho[D] * adz[D];
         mx_new = ra * (mx1 - tq1_a) / (pneg(wwc) + ppos(www_a) + eps);
         mn_new = ra * (tq1_a - mn1) / (ppos(wwc) + pneg(www_a) + eps);
-        R mxb = (D >= 1) ? mx_a : mx_new, mnb = (D >= 1) ? mn_a : mn_new;
+        R mxb = (ST || D >= 1) ? mx_a : mx_new, mnb = (ST || D >= 1) ? mn_a : mn_new;
         fz_new = fz0_2 + ppos(www_a) * fmin(fmin(R(1.0), mx_new), mnb) - pneg(www_a) * fmin(fmin(R(1.0), mxb), mn_new);
       }
       // f: level K-3, update of qp and t, statistics
       const int Fl = K - 3;
-      if (Fl >= 0 && Fl < nzm) {
+      if (ST || (Fl >= 0 && Fl < nzm)) {
         R irhoadz = irhoadz_p[Fl];
-        R fzF = fz_a, fzF1 = (Fl + 1 < nzm) ? fz_new : R(0.0);
-        R lf = lfac_of(ta_3), lf1 = (Fl + 1 < nzm) ? lfac_of(ta_2) : R(0.0);
+        R fzF = fz_a, fzF1 = (ST || Fl + 1 < nzm) ? fz_new : R(0.0);
+        R lf = lfac_of(ta_3), lf1 = (ST || Fl + 1 < nzm) ? lfac_of(ta_2) : R(0.0);
         R dq = -(fzF1 - fzF) * irhoadz;
         R lat_heat = -(lf1 * fzF1 - lf * fzF) * irhoadz;
         R sfall = s.sum(dq * R(1.));
@@ -2012,7 +2015,7 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
             accum(&qpfall[Fl], sfall);
             accum(&precflux[Fl], -sflux);
           }
-          if (Fl == 0) {
+          if (!ST && Fl == 0) {
             d.precsfc[(size_t)c * NX + i] -= fzF * R(1.);
             d.precssfc[(size_t)c * NX + i] -= fzF * (R(1.) - omega0) * R(1.);
           }
@@ -2038,7 +2041,11 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
       mx_a = mx_new;
       mn_a = mn_new;
       fz_a = fz_new;
-    }
+    };
+    int K = 0;
+    for (; K < 4 && K <= nzm + 2; K++) step(K, std::false_type{});
+    for (; K < nzm; K++) step(K, std::true_type{});
+    for (; K <= nzm + 2; K++) step(K, std::false_type{});
   }
 }
 
