@@ -2082,6 +2082,350 @@ static void setperturb(int c, int iseed) {
 #define IN2(p, c, k) ((p)[(size_t)(c) + (size_t)ld * (size_t)(k)])
 #define G4(p, c, i, k) ((p)[(size_t)(c) + (size_t)ld * ((size_t)(i) + (size_t)nx * (size_t)(k))])
 
+/* ---- per-substep statistics and end-of-call outputs ------------------------------------
+ * The statistics of crm_module.F90:896-1073 (cloud fractions and tops, water paths, mass
+ * fluxes, radiation-column sums, interface mass fluxes) and the outputs of
+ * crm_module.F90:1078-1659.  They are separate functions so that oracle_stats_probe can run
+ * them on a constructed state. */
+typedef struct {
+  int ncrms, ld, plev, nx, nzm, nz, nxr, ptop; /* ptop 1-based as in the reference */
+  double factor_xy, icrm_run_time;
+  const crm_input_view *in;
+  crm_rad_view *rad;
+  crm_output_view *out;
+  double *mui, *mdi; /* interface mass fluxes [c][plev + 2] */
+} stats_args;
+
+/* A planted mistake in the statistics, selected by oracle_mutate (0: none).  The tests use it
+ * to show that their closed forms notice each kind of slip; no other code sets it. */
+static int MUT = 0;
+enum {
+  MUT_NONE = 0,
+  MUT_EUDU,      /* entrainment / detrainment branches swapped                   */
+  MUT_SPLIT,     /* 700 / 400 hPa bounds of the low / high cloud layers swapped  */
+  MUT_CLD_XY,    /* cloud fraction averaged over columns only, not substeps      */
+  MUT_MC_WMIN,   /* cloudy updraft / downdraft threshold wmin instead of 2 wmin  */
+  MUT_JTMX,      /* jt / mx taken as the last / first flux level                 */
+  MUT_WP_LEVEL,  /* water paths scaled by the pdel of the level below           */
+  MUT_CLDTOP,    /* every level below the cloud top counted as a top             */
+  MUT_TKEZ,      /* resolved kinetic energy without the factor 1/2               */
+  MUT_FLUX_DT,   /* resolved scalar flux converted with dt instead of dtn        */
+  MUT_MU_LEVEL,  /* updraft cloud test at the level above the interface          */
+  MUT_RAD_QV,    /* radiation vapour sum without the clip at zero                */
+  MUT_PREC_SPLIT /* convective / stratiform split at 10 mm/h instead of 10 mm/day */
+};
+void oracle_mutate(int m) { MUT = m; }
+
+static void stats_step(const stats_args *a) {
+  const int ncrms = a->ncrms, ld = a->ld, plev = a->plev, nx = a->nx, nzm = a->nzm, nz = a->nz, nxr = a->nxr;
+  const crm_input_view *in = a->in;
+  crm_rad_view *rad = a->rad;
+  crm_output_view *out = a->out;
+  double *mui = a->mui, *mdi = a->mdi;
+  const double wmin = 2.;
+  const double cwp_threshold = R4(0.001);
+  const double p_low = MUT == MUT_SPLIT ? 400. : 700., p_high = MUT == MUT_SPLIT ? 700. : 400.;
+  const double wcut = MUT == MUT_MC_WMIN ? wmin : 2 * wmin;
+  (void)in;
+
+  for (int c = 0; c < ncrms; c++)
+    for (int i = 1; i <= nx; i++) {
+      double cwp = 0, cwph = 0, cwpm = 0, cwpl = 0, clt = 0, cmt = 0, cht = 0, ctt = 0;
+      int flag_top = 1;
+      for (int k = 1; k <= nzm; k++) {
+        int l = plev - k; /* 0-based */
+        int kk = nz - k;
+        double tmp1 = PR(S.rho, c, kk) * PR(S.adz, c, kk) * S.dz[c] * (FLD(S.qcl, c, i, kk) + FLD(S.qci, c, i, kk));
+        cwp += tmp1;
+        ctt = MAX(1.0, ctt);
+        if (cwp > cwp_threshold && flag_top) {
+          IN2(out->cldtop, c, l) += 1;
+          flag_top = MUT == MUT_CLDTOP;
+        }
+        if (PR(S.pres, c, kk) >= p_low) {
+          cwpl += tmp1;
+          clt = MAX(1.0, clt);
+        } else if (PR(S.pres, c, kk) < p_high) {
+          cwph += tmp1;
+          cht = MAX(1.0, cht);
+        } else {
+          cwpm += tmp1;
+          cmt = MAX(1.0, cmt);
+        }
+        tmp1 = PR(S.rho, c, k) * PR(S.adz, c, k) * S.dz[c];
+        double ws = FLD(S.w, c, i, k + 1) + FLD(S.w, c, i, k);
+        double rw = PR(S.rho, c, k) * 0.5 * ws;
+        if (tmp1 * (FLD(S.qcl, c, i, k) + FLD(S.qci, c, i, k)) > cwp_threshold) {
+          IN2(out->cld, c, l) += 1.0;
+          if (ws > wcut) {
+            IN2(out->mcup, c, l) += rw * 1.0;
+            IN2(out->mcuup, c, l) += rw * (1.0 - 1.0);
+          }
+          if (ws < -wcut) {
+            IN2(out->mcdn, c, l) += rw * 1.0;
+            IN2(out->mcudn, c, l) += rw * (1. - 1.0);
+          }
+        } else {
+          if (ws > 2 * wmin) IN2(out->mcuup, c, l) += rw;
+          if (ws < -2 * wmin) IN2(out->mcudn, c, l) += rw;
+        }
+        IN2(out->gliqwp, c, l) += FLD(S.qcl, c, i, k);
+        IN2(out->gicewp, c, l) += FLD(S.qci, c, i, k);
+      }
+      if (cwp > cwp_threshold) out->cltot[c] += ctt;
+      if (cwph > cwp_threshold) out->clhgh[c] += cht;
+      if (cwpm > cwp_threshold) out->clmed[c] += cmt;
+      if (cwpl > cwp_threshold) out->cllow[c] += clt;
+    }
+  for (int c = 0; c < ncrms; c++)
+    for (int k = 1; k <= nzm; k++)
+      for (int i = 1; i <= nx; i++) {
+        int ir = (i - 1) / (nx / nxr);
+        size_t o = (size_t)c + (size_t)ld * ((size_t)ir + (size_t)nxr * (k - 1));
+        rad->temperature[o] += FLD(S.tabs, c, i, k);
+        rad->qv[o] += MUT == MUT_RAD_QV ? FLD(S.qv, c, i, k) : MAX(0.0, FLD(S.qv, c, i, k));
+        rad->qc[o] += FLD(S.qcl, c, i, k);
+        rad->qi[o] += FLD(S.qci, c, i, k);
+        if (FLD(S.qcl, c, i, k) + FLD(S.qci, c, i, k) > 0) rad->cld[o] += 1.0;
+      }
+  for (int c = 0; c < ncrms; c++)
+    for (int i = 1; i <= nx; i++)
+      for (int k = 1; k <= nzm + 1; k++) {
+        int l = plev + 1 - k; /* 0-based index of plev+1-k+1 */
+        double wk = FLD(S.w, c, i, k);
+        if (wk > 0.) {
+          int kx = MUT == MUT_MU_LEVEL ? MIN(k, nzm) : MAX(1, k - 1);
+          double qsat = qsatw(FLD(S.tabs, c, i, kx), PR(S.pres, c, kx));
+          if (FLD(S.qcl, c, i, kx) + FLD(S.qci, c, i, kx) > MIN(R4(1.e-5), 0.01 * qsat))
+            mui[(size_t)c * (plev + 2) + l] += PR(S.rhow, c, k) * wk;
+        } else if (wk < 0.) {
+          int kx = MIN(k + 1, nzm);
+          double qsat = qsatw(FLD(S.tabs, c, i, kx), PR(S.pres, c, kx));
+          if (FLD(S.qcl, c, i, kx) + FLD(S.qci, c, i, kx) > MIN(R4(1.e-5), 0.01 * qsat))
+            mdi[(size_t)c * (plev + 2) + l] += PR(S.rhow, c, k) * wk;
+          else if (FLD(S.qpl, c, i, kx) + FLD(S.qpi, c, i, kx) > R4(1.0e-4))
+            mdi[(size_t)c * (plev + 2) + l] += PR(S.rhow, c, k) * wk;
+        }
+      }
+}
+
+static void stats_final(const stats_args *a, crm_state_view *st) {
+  const int ncrms = a->ncrms, ld = a->ld, plev = a->plev, nx = a->nx, nzm = a->nzm, nxr = a->nxr, ptop = a->ptop;
+  const double factor_xy = a->factor_xy, icrm_run_time = a->icrm_run_time;
+  const crm_input_view *in = a->in;
+  crm_rad_view *rad = a->rad;
+  crm_output_view *out = a->out;
+  double *mui = a->mui, *mdi = a->mdi;
+  {
+    double factor_xyt = factor_xy / (double)S.nstop;
+    double tmp1 = ((double)nxr / (double)nx) * 1.0 / (double)S.nstop;
+    for (int c = 0; c < ncrms; c++)
+      for (int k = 0; k < nzm; k++)
+        for (int ir = 0; ir < nxr; ir++) {
+          size_t o = (size_t)c + (size_t)ld * ((size_t)ir + (size_t)nxr * k);
+          rad->temperature[o] *= tmp1;
+          rad->qv[o] *= tmp1;
+          rad->qc[o] *= tmp1;
+          rad->qi[o] *= tmp1;
+          rad->cld[o] *= tmp1;
+        }
+    double *tln = (double *)calloc(plev, sizeof(double)), *qln = (double *)calloc(plev, sizeof(double));
+    double *qccln = (double *)calloc(plev, sizeof(double)), *qiiln = (double *)calloc(plev, sizeof(double));
+    for (int c = 0; c < ncrms; c++) {
+      for (int k = 0; k < plev; k++) {
+        if (k < ptop - 1) {
+          tln[k] = IN2(in->tl, c, k);
+          qln[k] = IN2(in->ql, c, k);
+          qccln[k] = IN2(in->qccl, c, k);
+          qiiln[k] = IN2(in->qiil, c, k);
+        } else {
+          tln[k] = qln[k] = qccln[k] = qiiln[k] = 0.;
+        }
+      }
+      double cp_ = 0., cps = 0.;
+      for (int k = 1; k <= nzm; k++)
+        for (int i = 1; i <= nx; i++) {
+          int l = plev - k;
+          double pd = IN2(in->pdel, c, l);
+          cp_ += (FLD(S.qpl, c, i, k) + FLD(S.qpi, c, i, k)) * pd;
+          cps += FLD(S.qpi, c, i, k) * pd;
+          tln[l] += FLD(S.tabs, c, i, k);
+          qln[l] += FLD(S.qv, c, i, k);
+          qccln[l] += FLD(S.qcl, c, i, k);
+          qiiln[l] += FLD(S.qci, c, i, k);
+        }
+      for (int k = ptop - 1; k < plev; k++) {
+        tln[k] *= factor_xy;
+        qln[k] *= factor_xy;
+        qccln[k] *= factor_xy;
+        qiiln[k] *= factor_xy;
+      }
+      for (int k = 0; k < plev; k++) {
+        IN2(out->sltend, c, k) = CP * (tln[k] - IN2(in->tl, c, k)) * icrm_run_time;
+        IN2(out->qltend, c, k) = (qln[k] - IN2(in->ql, c, k)) * icrm_run_time;
+        IN2(out->qcltend, c, k) = (qccln[k] - IN2(in->qccl, c, k)) * icrm_run_time;
+        IN2(out->qiltend, c, k) = (qiiln[k] - IN2(in->qiil, c, k)) * icrm_run_time;
+      }
+      out->prectend[c] = (cp_ - out->prectend[c]) / GGR * factor_xy * icrm_run_time;
+      out->precstend[c] = (cps - out->precstend[c]) / GGR * factor_xy * icrm_run_time;
+      for (int k = ptop - 1; k <= ptop; k++) {
+        IN2(out->sltend, c, k) = 0.;
+        IN2(out->qltend, c, k) = 0.;
+        IN2(out->qcltend, c, k) = 0.;
+        IN2(out->qiltend, c, k) = 0.;
+      }
+    }
+    free(tln);
+    free(qln);
+    free(qccln);
+    free(qiiln);
+    for (int c = 0; st && c < ncrms; c++)
+      for (int k = 1; k <= nzm; k++)
+        for (int i = 1; i <= nx; i++) {
+          G4(st->u_wind, c, i - 1, k - 1) = FLD(S.u, c, i, k);
+          G4(st->v_wind, c, i - 1, k - 1) = 0.0;
+          G4(st->w_wind, c, i - 1, k - 1) = FLD(S.w, c, i, k);
+          G4(st->temperature, c, i - 1, k - 1) = FLD(S.tabs, c, i, k);
+          G4(st->qt, c, i - 1, k - 1) = FLD(S.qt, c, i, k);
+          G4(st->qp, c, i - 1, k - 1) = FLD(S.qp, c, i, k);
+          G4(st->qn, c, i - 1, k - 1) = FLD(S.qn, c, i, k);
+          G4(out->tk, c, i - 1, k - 1) = FLD(S.tk, c, i, k);
+          G4(out->tkh, c, i - 1, k - 1) = FLD(S.tkh, c, i, k);
+          G4(out->qcl, c, i - 1, k - 1) = FLD(S.qcl, c, i, k);
+          G4(out->qci, c, i - 1, k - 1) = FLD(S.qci, c, i, k);
+          G4(out->qpl, c, i - 1, k - 1) = FLD(S.qpl, c, i, k);
+          G4(out->qpi, c, i - 1, k - 1) = FLD(S.qpi, c, i, k);
+        }
+    for (int c = 0; c < ncrms; c++) {
+      out->z0m[c] = S.z0[c];
+      out->taux[c] = S.taux0[c] / (double)S.nstop;
+      out->tauy[c] = S.tauy0[c] / (double)S.nstop;
+    }
+    for (int c = 0; c < ncrms; c++)
+      for (int k = 1; k <= nzm; k++)
+        for (int i = 1; i <= nx; i++) {
+          int l = plev - k;
+          IN2(out->qc_mean, c, l) += FLD(S.qcl, c, i, k);
+          IN2(out->qi_mean, c, l) += FLD(S.qci, c, i, k);
+          IN2(out->qr_mean, c, l) += FLD(S.qpl, c, i, k);
+          double omg = MAX(0.0, MIN(1.0, (FLD(S.tabs, c, i, k) - TGRMIN) * S.a_gr));
+          IN2(out->qg_mean, c, l) += FLD(S.qpi, c, i, k) * omg;
+          IN2(out->qs_mean, c, l) += FLD(S.qpi, c, i, k) * (1. - omg);
+        }
+    for (int c = 0; c < ncrms; c++)
+      for (int k = 0; k < plev; k++) {
+        IN2(out->cld, c, k) = MIN(1.0, IN2(out->cld, c, k) * (MUT == MUT_CLD_XY ? factor_xy : factor_xyt));
+        IN2(out->cldtop, c, k) = MIN(1.0, IN2(out->cldtop, c, k) * factor_xyt);
+        const int kp = MUT == MUT_WP_LEVEL ? MIN(k + 1, plev - 1) : k;
+        IN2(out->gicewp, c, k) = IN2(out->gicewp, c, k) * IN2(in->pdel, c, kp) * 1000. / GGR * factor_xyt;
+        IN2(out->gliqwp, c, k) = IN2(out->gliqwp, c, k) * IN2(in->pdel, c, kp) * 1000. / GGR * factor_xyt;
+        IN2(out->mcup, c, k) *= factor_xyt;
+        IN2(out->mcdn, c, k) *= factor_xyt;
+        IN2(out->mcuup, c, k) *= factor_xyt;
+        IN2(out->mcudn, c, k) *= factor_xyt;
+        IN2(out->mctot, c, k) = IN2(out->mcup, c, k) + IN2(out->mcdn, c, k) + IN2(out->mcuup, c, k) + IN2(out->mcudn, c, k);
+        IN2(out->qc_mean, c, k) *= factor_xy;
+        IN2(out->qi_mean, c, k) *= factor_xy;
+        IN2(out->qs_mean, c, k) *= factor_xy;
+        IN2(out->qg_mean, c, k) *= factor_xy;
+        IN2(out->qr_mean, c, k) *= factor_xy;
+      }
+    for (int c = 0; c < ncrms; c++) {
+      double precc = 0, precl = 0, precsc = 0, precsl = 0;
+      for (int i = 1; i <= nx; i++) {
+        double ps = FLD(S.precsfc, c, i, 1) * S.dz[c] / S.dt / (double)S.nstop;
+        double pss = FLD(S.precssfc, c, i, 1) * S.dz[c] / S.dt / (double)S.nstop;
+        if (ps > 10. / (MUT == MUT_PREC_SPLIT ? 3600. : 86400.)) {
+          precc += ps;
+          precsc += pss;
+        } else {
+          precl += ps;
+          precsl += pss;
+        }
+        out->prec_crm[(size_t)c + (size_t)ld * (i - 1)] = ps / 1000.;
+      }
+      out->precc[c] = precc * factor_xy / 1000.;
+      out->precl[c] = precl * factor_xy / 1000.;
+      out->precsc[c] = precsc * factor_xy / 1000.;
+      out->precsl[c] = precsl * factor_xy / 1000.;
+      out->cltot[c] *= factor_xyt;
+      out->clhgh[c] *= factor_xyt;
+      out->clmed[c] *= factor_xyt;
+      out->cllow[c] *= factor_xyt;
+      out->jt_crm[c] = plev * 1.0;
+      out->mx_crm[c] = 1.0;
+    }
+    for (int c = 0; c < ncrms; c++)
+      for (int k = 0; k < plev; k++) {
+        double *mu = &mui[(size_t)c * (plev + 2)], *md = &mdi[(size_t)c * (plev + 2)];
+        double pd = IN2(in->pdel, c, k);
+        double mu_ = 0.5 * (mu[k] + mu[k + 1]) * GGR / 100.;
+        double md_ = 0.5 * (md[k] + md[k + 1]) * GGR / 100.;
+        IN2(out->mu_crm, c, k) = mu_;
+        IN2(out->md_crm, c, k) = md_;
+        IN2(out->eu_crm, c, k) = 0.;
+        if (MUT == MUT_EUDU ? mu[k] - mu[k + 1] < 0 : mu[k] - mu[k + 1] > 0)
+          IN2(out->eu_crm, c, k) = (mu[k] - mu[k + 1]) * GGR / pd;
+        else
+          IN2(out->du_crm, c, k) = -1.0 * (mu[k] - mu[k + 1]) * GGR / pd;
+        if (md[k + 1] - md[k] < 0) IN2(out->ed_crm, c, k) = (md[k] - md[k + 1]) * GGR / pd;
+        if (fabs(mu_) > R4(1.0e-15) || fabs(md_) > R4(1.0e-15)) {
+          double kk = k + 1;
+          if (MUT == MUT_JTMX) {
+            out->jt_crm[c] = MAX(kk, out->jt_crm[c]);
+            out->mx_crm[c] = MIN(kk, out->mx_crm[c]);
+          } else {
+            out->jt_crm[c] = MIN(kk, out->jt_crm[c]);
+            out->mx_crm[c] = MAX(kk, out->mx_crm[c]);
+          }
+        }
+      }
+    for (int c = 0; c < ncrms; c++)
+      for (int k = 1; k <= nzm; k++) {
+        double u2z = 0, w2z = 0, tke = 0, tkz = 0;
+        for (int i = 1; i <= nx; i++) {
+          double du = FLD(S.u, c, i, k) - PR(S.u0, c, k);
+          u2z += du * du;
+          w2z += 0.5 * (FLD(S.w, c, i, k + 1) * FLD(S.w, c, i, k + 1) + FLD(S.w, c, i, k) * FLD(S.w, c, i, k));
+          tke += FLD(S.tke, c, i, k);
+          tkz += FLD(S.tk, c, i, k);
+        }
+        double t1 = S.dz[c] / PR(S.rhow, c, k);
+        double t2 = t1 / (MUT == MUT_FLUX_DT ? S.dt : S.dtn);
+        int l = plev - k;
+        double mkwsb1 = PR(S.mkwsb[0], c, k) * t1 * PR(S.rhow, c, k) * factor_xy / S.nstop;
+        double mkwsb2 = PR(S.mkwsb[1], c, k) * t1 * PR(S.rhow, c, k) * factor_xy / S.nstop;
+        double mkwle1 = PR(S.mkwle[0], c, k) * t2 * PR(S.rhow, c, k) * factor_xy / S.nstop;
+        double mkwle2 = PR(S.mkwle[1], c, k) * t2 * PR(S.rhow, c, k) * factor_xy / S.nstop;
+        double mkadv1 = PR(S.mkadv[0], c, k) * factor_xy * icrm_run_time;
+        double mkadv2 = PR(S.mkadv[1], c, k) * factor_xy * icrm_run_time;
+        double mkdiff1 = PR(S.mkdiff[0], c, k) * factor_xy * icrm_run_time;
+        double mkdiff2 = PR(S.mkdiff[1], c, k) * factor_xy * icrm_run_time;
+        double qpsrc = PR(S.qpsrc, c, k) * factor_xy * icrm_run_time;
+        double qpevp = PR(S.qpevp, c, k) * factor_xy * icrm_run_time;
+        double qpfall = PR(S.qpfall, c, k) * factor_xy * icrm_run_time;
+        double precflux = PR(S.precflux, c, k) * factor_xy * S.dz[c] / S.dt / S.nstop;
+        IN2(out->flux_u, c, l) = (PR(S.uwle, c, k) + PR(S.uwsb, c, k)) * t1 * factor_xy / S.nstop;
+        IN2(out->flux_v, c, l) = 0.0 * t1 * factor_xy / S.nstop;
+        IN2(out->flux_qt, c, l) = mkwle1 + mkwsb1;
+        IN2(out->fluxsgs_qt, c, l) = mkwsb1;
+        IN2(out->flux_qp, c, l) = mkwle2 + mkwsb2;
+        IN2(out->qt_trans, c, l) = mkadv1 + mkdiff1;
+        IN2(out->qp_trans, c, l) = mkadv2 + mkdiff2;
+        IN2(out->tkesgsz, c, l) = PR(S.rho, c, k) * tke * factor_xy;
+        IN2(out->tkez, c, l) = PR(S.rho, c, k) * (MUT == MUT_TKEZ ? 1.0 : 0.5) * (u2z + 0.0 + w2z) * factor_xy + IN2(out->tkesgsz, c, l);
+        IN2(out->tkz, c, l) = tkz * factor_xy;
+        IN2(out->precflux, c, l) = precflux / 1000.;
+        IN2(out->qp_fall, c, l) = qpfall;
+        IN2(out->qp_evp, c, l) = qpevp;
+        IN2(out->qp_src, c, l) = qpsrc;
+        IN2(out->qt_ls, c, l) = PR(S.qtend, c, k);
+        IN2(out->t_ls, c, l) = PR(S.ttend, c, k);
+      }
+    for (int c = 0; c < ncrms; c++) out->timing_factor[c] = out->timing_factor[c] / S.nstop;
+  }
+}
+
 int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, const crm_input_view *in,
                crm_state_view *st, crm_rad_view *rad, crm_output_view *out, int phase_stop) {
   int ncrms = d->ncrms, ld = d->ld, plev = d->plev, nx = d->nx, nzm = d->nz;
@@ -2102,8 +2446,6 @@ int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, cons
   S.dt3[0] = S.dt3[1] = S.dt3[2] = 0.0;
   S.nstep = 0;
   const double umax = 0.5 * d->dx / d->dt;
-  const double wmin = 2.;
-  const double cwp_threshold = R4(0.001);
   int nxr = d->nx_rad, ptop = plev - nzm + 1; /* 1-based as in the reference */
   double idt_gl = 1. / dt_gl, factor_xy = 1. / (double)(nx);
   int status = CRM_OK;
@@ -2241,6 +2583,7 @@ int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, cons
       for (int c = 0; c < ncrms; c++) IN2(plev_outs[n], c, k) = 0.;
   double *mui = (double *)calloc((size_t)ncrms * (plev + 2), sizeof(double));
   double *mdi = (double *)calloc((size_t)ncrms * (plev + 2), sizeof(double));
+  const stats_args sa = {ncrms, ld, plev, nx, nzm, nz, nxr, ptop, factor_xy, 1. / dt_gl, in, rad, out, mui, mdi};
 
   if (precip_init() != CRM_OK) {
     status = CRM_ERR_UNSUPPORTED;
@@ -2251,7 +2594,6 @@ int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, cons
 
   S.nstop = (int)(dt_gl / S.dt);
   S.dt = dt_gl / S.nstop;
-  double icrm_run_time = 1. / dt_gl;
   /* crm_accel_nstop (accelerate_crm.F90:78-106), crm_module.F90:718-720 */
   int accel_cease = 0;
   if (ACC.use) {
@@ -2365,290 +2707,11 @@ int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, cons
 #undef STOP_AT
 
     /* per-step statistics, crm_module.F90:896-1073 */
-    for (int c = 0; c < ncrms; c++)
-      for (int i = 1; i <= nx; i++) {
-        double cwp = 0, cwph = 0, cwpm = 0, cwpl = 0, clt = 0, cmt = 0, cht = 0, ctt = 0;
-        int flag_top = 1;
-        for (int k = 1; k <= nzm; k++) {
-          int l = plev - k; /* 0-based */
-          int kk = nz - k;
-          double tmp1 = PR(S.rho, c, kk) * PR(S.adz, c, kk) * S.dz[c] * (FLD(S.qcl, c, i, kk) + FLD(S.qci, c, i, kk));
-          cwp += tmp1;
-          ctt = MAX(1.0, ctt);
-          if (cwp > cwp_threshold && flag_top) {
-            IN2(out->cldtop, c, l) += 1;
-            flag_top = 0;
-          }
-          if (PR(S.pres, c, kk) >= 700.) {
-            cwpl += tmp1;
-            clt = MAX(1.0, clt);
-          } else if (PR(S.pres, c, kk) < 400.) {
-            cwph += tmp1;
-            cht = MAX(1.0, cht);
-          } else {
-            cwpm += tmp1;
-            cmt = MAX(1.0, cmt);
-          }
-          tmp1 = PR(S.rho, c, k) * PR(S.adz, c, k) * S.dz[c];
-          double ws = FLD(S.w, c, i, k + 1) + FLD(S.w, c, i, k);
-          double rw = PR(S.rho, c, k) * 0.5 * ws;
-          if (tmp1 * (FLD(S.qcl, c, i, k) + FLD(S.qci, c, i, k)) > cwp_threshold) {
-            IN2(out->cld, c, l) += 1.0;
-            if (ws > 2 * wmin) {
-              IN2(out->mcup, c, l) += rw * 1.0;
-              IN2(out->mcuup, c, l) += rw * (1.0 - 1.0);
-            }
-            if (ws < -2 * wmin) {
-              IN2(out->mcdn, c, l) += rw * 1.0;
-              IN2(out->mcudn, c, l) += rw * (1. - 1.0);
-            }
-          } else {
-            if (ws > 2 * wmin) IN2(out->mcuup, c, l) += rw;
-            if (ws < -2 * wmin) IN2(out->mcudn, c, l) += rw;
-          }
-          IN2(out->gliqwp, c, l) += FLD(S.qcl, c, i, k);
-          IN2(out->gicewp, c, l) += FLD(S.qci, c, i, k);
-        }
-        if (cwp > cwp_threshold) out->cltot[c] += ctt;
-        if (cwph > cwp_threshold) out->clhgh[c] += cht;
-        if (cwpm > cwp_threshold) out->clmed[c] += cmt;
-        if (cwpl > cwp_threshold) out->cllow[c] += clt;
-      }
-    for (int c = 0; c < ncrms; c++)
-      for (int k = 1; k <= nzm; k++)
-        for (int i = 1; i <= nx; i++) {
-          int ir = (i - 1) / (nx / nxr);
-          size_t o = (size_t)c + (size_t)ld * ((size_t)ir + (size_t)nxr * (k - 1));
-          rad->temperature[o] += FLD(S.tabs, c, i, k);
-          rad->qv[o] += MAX(0.0, FLD(S.qv, c, i, k));
-          rad->qc[o] += FLD(S.qcl, c, i, k);
-          rad->qi[o] += FLD(S.qci, c, i, k);
-          if (FLD(S.qcl, c, i, k) + FLD(S.qci, c, i, k) > 0) rad->cld[o] += 1.0;
-        }
-    for (int c = 0; c < ncrms; c++)
-      for (int i = 1; i <= nx; i++)
-        for (int k = 1; k <= nzm + 1; k++) {
-          int l = plev + 1 - k; /* 0-based index of plev+1-k+1 */
-          double wk = FLD(S.w, c, i, k);
-          if (wk > 0.) {
-            int kx = MAX(1, k - 1);
-            double qsat = qsatw(FLD(S.tabs, c, i, kx), PR(S.pres, c, kx));
-            if (FLD(S.qcl, c, i, kx) + FLD(S.qci, c, i, kx) > MIN(R4(1.e-5), 0.01 * qsat))
-              mui[(size_t)c * (plev + 2) + l] += PR(S.rhow, c, k) * wk;
-          } else if (wk < 0.) {
-            int kx = MIN(k + 1, nzm);
-            double qsat = qsatw(FLD(S.tabs, c, i, kx), PR(S.pres, c, kx));
-            if (FLD(S.qcl, c, i, kx) + FLD(S.qci, c, i, kx) > MIN(R4(1.e-5), 0.01 * qsat))
-              mdi[(size_t)c * (plev + 2) + l] += PR(S.rhow, c, k) * wk;
-            else if (FLD(S.qpl, c, i, kx) + FLD(S.qpi, c, i, kx) > R4(1.0e-4))
-              mdi[(size_t)c * (plev + 2) + l] += PR(S.rhow, c, k) * wk;
-          }
-        }
+    stats_step(&sa);
   }
 
   /* end-of-call outputs, crm_module.F90:1078-1659 */
-  {
-    double factor_xyt = factor_xy / (double)S.nstop;
-    double tmp1 = ((double)nxr / (double)nx) * 1.0 / (double)S.nstop;
-    for (int c = 0; c < ncrms; c++)
-      for (int k = 0; k < nzm; k++)
-        for (int ir = 0; ir < nxr; ir++) {
-          size_t o = (size_t)c + (size_t)ld * ((size_t)ir + (size_t)nxr * k);
-          rad->temperature[o] *= tmp1;
-          rad->qv[o] *= tmp1;
-          rad->qc[o] *= tmp1;
-          rad->qi[o] *= tmp1;
-          rad->cld[o] *= tmp1;
-        }
-    double *tln = (double *)calloc(plev, sizeof(double)), *qln = (double *)calloc(plev, sizeof(double));
-    double *qccln = (double *)calloc(plev, sizeof(double)), *qiiln = (double *)calloc(plev, sizeof(double));
-    for (int c = 0; c < ncrms; c++) {
-      for (int k = 0; k < plev; k++) {
-        if (k < ptop - 1) {
-          tln[k] = IN2(in->tl, c, k);
-          qln[k] = IN2(in->ql, c, k);
-          qccln[k] = IN2(in->qccl, c, k);
-          qiiln[k] = IN2(in->qiil, c, k);
-        } else {
-          tln[k] = qln[k] = qccln[k] = qiiln[k] = 0.;
-        }
-      }
-      double cp_ = 0., cps = 0.;
-      for (int k = 1; k <= nzm; k++)
-        for (int i = 1; i <= nx; i++) {
-          int l = plev - k;
-          double pd = IN2(in->pdel, c, l);
-          cp_ += (FLD(S.qpl, c, i, k) + FLD(S.qpi, c, i, k)) * pd;
-          cps += FLD(S.qpi, c, i, k) * pd;
-          tln[l] += FLD(S.tabs, c, i, k);
-          qln[l] += FLD(S.qv, c, i, k);
-          qccln[l] += FLD(S.qcl, c, i, k);
-          qiiln[l] += FLD(S.qci, c, i, k);
-        }
-      for (int k = ptop - 1; k < plev; k++) {
-        tln[k] *= factor_xy;
-        qln[k] *= factor_xy;
-        qccln[k] *= factor_xy;
-        qiiln[k] *= factor_xy;
-      }
-      for (int k = 0; k < plev; k++) {
-        IN2(out->sltend, c, k) = CP * (tln[k] - IN2(in->tl, c, k)) * icrm_run_time;
-        IN2(out->qltend, c, k) = (qln[k] - IN2(in->ql, c, k)) * icrm_run_time;
-        IN2(out->qcltend, c, k) = (qccln[k] - IN2(in->qccl, c, k)) * icrm_run_time;
-        IN2(out->qiltend, c, k) = (qiiln[k] - IN2(in->qiil, c, k)) * icrm_run_time;
-      }
-      out->prectend[c] = (cp_ - out->prectend[c]) / GGR * factor_xy * icrm_run_time;
-      out->precstend[c] = (cps - out->precstend[c]) / GGR * factor_xy * icrm_run_time;
-      for (int k = ptop - 1; k <= ptop; k++) {
-        IN2(out->sltend, c, k) = 0.;
-        IN2(out->qltend, c, k) = 0.;
-        IN2(out->qcltend, c, k) = 0.;
-        IN2(out->qiltend, c, k) = 0.;
-      }
-    }
-    free(tln);
-    free(qln);
-    free(qccln);
-    free(qiiln);
-    for (int c = 0; c < ncrms; c++)
-      for (int k = 1; k <= nzm; k++)
-        for (int i = 1; i <= nx; i++) {
-          G4(st->u_wind, c, i - 1, k - 1) = FLD(S.u, c, i, k);
-          G4(st->v_wind, c, i - 1, k - 1) = 0.0;
-          G4(st->w_wind, c, i - 1, k - 1) = FLD(S.w, c, i, k);
-          G4(st->temperature, c, i - 1, k - 1) = FLD(S.tabs, c, i, k);
-          G4(st->qt, c, i - 1, k - 1) = FLD(S.qt, c, i, k);
-          G4(st->qp, c, i - 1, k - 1) = FLD(S.qp, c, i, k);
-          G4(st->qn, c, i - 1, k - 1) = FLD(S.qn, c, i, k);
-          G4(out->tk, c, i - 1, k - 1) = FLD(S.tk, c, i, k);
-          G4(out->tkh, c, i - 1, k - 1) = FLD(S.tkh, c, i, k);
-          G4(out->qcl, c, i - 1, k - 1) = FLD(S.qcl, c, i, k);
-          G4(out->qci, c, i - 1, k - 1) = FLD(S.qci, c, i, k);
-          G4(out->qpl, c, i - 1, k - 1) = FLD(S.qpl, c, i, k);
-          G4(out->qpi, c, i - 1, k - 1) = FLD(S.qpi, c, i, k);
-        }
-    for (int c = 0; c < ncrms; c++) {
-      out->z0m[c] = S.z0[c];
-      out->taux[c] = S.taux0[c] / (double)S.nstop;
-      out->tauy[c] = S.tauy0[c] / (double)S.nstop;
-    }
-    for (int c = 0; c < ncrms; c++)
-      for (int k = 1; k <= nzm; k++)
-        for (int i = 1; i <= nx; i++) {
-          int l = plev - k;
-          IN2(out->qc_mean, c, l) += FLD(S.qcl, c, i, k);
-          IN2(out->qi_mean, c, l) += FLD(S.qci, c, i, k);
-          IN2(out->qr_mean, c, l) += FLD(S.qpl, c, i, k);
-          double omg = MAX(0.0, MIN(1.0, (FLD(S.tabs, c, i, k) - TGRMIN) * S.a_gr));
-          IN2(out->qg_mean, c, l) += FLD(S.qpi, c, i, k) * omg;
-          IN2(out->qs_mean, c, l) += FLD(S.qpi, c, i, k) * (1. - omg);
-        }
-    for (int c = 0; c < ncrms; c++)
-      for (int k = 0; k < plev; k++) {
-        IN2(out->cld, c, k) = MIN(1.0, IN2(out->cld, c, k) * factor_xyt);
-        IN2(out->cldtop, c, k) = MIN(1.0, IN2(out->cldtop, c, k) * factor_xyt);
-        IN2(out->gicewp, c, k) = IN2(out->gicewp, c, k) * IN2(in->pdel, c, k) * 1000. / GGR * factor_xyt;
-        IN2(out->gliqwp, c, k) = IN2(out->gliqwp, c, k) * IN2(in->pdel, c, k) * 1000. / GGR * factor_xyt;
-        IN2(out->mcup, c, k) *= factor_xyt;
-        IN2(out->mcdn, c, k) *= factor_xyt;
-        IN2(out->mcuup, c, k) *= factor_xyt;
-        IN2(out->mcudn, c, k) *= factor_xyt;
-        IN2(out->mctot, c, k) = IN2(out->mcup, c, k) + IN2(out->mcdn, c, k) + IN2(out->mcuup, c, k) + IN2(out->mcudn, c, k);
-        IN2(out->qc_mean, c, k) *= factor_xy;
-        IN2(out->qi_mean, c, k) *= factor_xy;
-        IN2(out->qs_mean, c, k) *= factor_xy;
-        IN2(out->qg_mean, c, k) *= factor_xy;
-        IN2(out->qr_mean, c, k) *= factor_xy;
-      }
-    for (int c = 0; c < ncrms; c++) {
-      double precc = 0, precl = 0, precsc = 0, precsl = 0;
-      for (int i = 1; i <= nx; i++) {
-        double ps = FLD(S.precsfc, c, i, 1) * S.dz[c] / S.dt / (double)S.nstop;
-        double pss = FLD(S.precssfc, c, i, 1) * S.dz[c] / S.dt / (double)S.nstop;
-        if (ps > 10. / 86400.) {
-          precc += ps;
-          precsc += pss;
-        } else {
-          precl += ps;
-          precsl += pss;
-        }
-        out->prec_crm[(size_t)c + (size_t)ld * (i - 1)] = ps / 1000.;
-      }
-      out->precc[c] = precc * factor_xy / 1000.;
-      out->precl[c] = precl * factor_xy / 1000.;
-      out->precsc[c] = precsc * factor_xy / 1000.;
-      out->precsl[c] = precsl * factor_xy / 1000.;
-      out->cltot[c] *= factor_xyt;
-      out->clhgh[c] *= factor_xyt;
-      out->clmed[c] *= factor_xyt;
-      out->cllow[c] *= factor_xyt;
-      out->jt_crm[c] = plev * 1.0;
-      out->mx_crm[c] = 1.0;
-    }
-    for (int c = 0; c < ncrms; c++)
-      for (int k = 0; k < plev; k++) {
-        double *mu = &mui[(size_t)c * (plev + 2)], *md = &mdi[(size_t)c * (plev + 2)];
-        double pd = IN2(in->pdel, c, k);
-        double mu_ = 0.5 * (mu[k] + mu[k + 1]) * GGR / 100.;
-        double md_ = 0.5 * (md[k] + md[k + 1]) * GGR / 100.;
-        IN2(out->mu_crm, c, k) = mu_;
-        IN2(out->md_crm, c, k) = md_;
-        IN2(out->eu_crm, c, k) = 0.;
-        if (mu[k] - mu[k + 1] > 0)
-          IN2(out->eu_crm, c, k) = (mu[k] - mu[k + 1]) * GGR / pd;
-        else
-          IN2(out->du_crm, c, k) = -1.0 * (mu[k] - mu[k + 1]) * GGR / pd;
-        if (md[k + 1] - md[k] < 0) IN2(out->ed_crm, c, k) = (md[k] - md[k + 1]) * GGR / pd;
-        if (fabs(mu_) > R4(1.0e-15) || fabs(md_) > R4(1.0e-15)) {
-          double kk = k + 1;
-          out->jt_crm[c] = MIN(kk, out->jt_crm[c]);
-          out->mx_crm[c] = MAX(kk, out->mx_crm[c]);
-        }
-      }
-    for (int c = 0; c < ncrms; c++)
-      for (int k = 1; k <= nzm; k++) {
-        double u2z = 0, w2z = 0, tke = 0, tkz = 0;
-        for (int i = 1; i <= nx; i++) {
-          double du = FLD(S.u, c, i, k) - PR(S.u0, c, k);
-          u2z += du * du;
-          w2z += 0.5 * (FLD(S.w, c, i, k + 1) * FLD(S.w, c, i, k + 1) + FLD(S.w, c, i, k) * FLD(S.w, c, i, k));
-          tke += FLD(S.tke, c, i, k);
-          tkz += FLD(S.tk, c, i, k);
-        }
-        double t1 = S.dz[c] / PR(S.rhow, c, k);
-        double t2 = t1 / S.dtn;
-        int l = plev - k;
-        double mkwsb1 = PR(S.mkwsb[0], c, k) * t1 * PR(S.rhow, c, k) * factor_xy / S.nstop;
-        double mkwsb2 = PR(S.mkwsb[1], c, k) * t1 * PR(S.rhow, c, k) * factor_xy / S.nstop;
-        double mkwle1 = PR(S.mkwle[0], c, k) * t2 * PR(S.rhow, c, k) * factor_xy / S.nstop;
-        double mkwle2 = PR(S.mkwle[1], c, k) * t2 * PR(S.rhow, c, k) * factor_xy / S.nstop;
-        double mkadv1 = PR(S.mkadv[0], c, k) * factor_xy * icrm_run_time;
-        double mkadv2 = PR(S.mkadv[1], c, k) * factor_xy * icrm_run_time;
-        double mkdiff1 = PR(S.mkdiff[0], c, k) * factor_xy * icrm_run_time;
-        double mkdiff2 = PR(S.mkdiff[1], c, k) * factor_xy * icrm_run_time;
-        double qpsrc = PR(S.qpsrc, c, k) * factor_xy * icrm_run_time;
-        double qpevp = PR(S.qpevp, c, k) * factor_xy * icrm_run_time;
-        double qpfall = PR(S.qpfall, c, k) * factor_xy * icrm_run_time;
-        double precflux = PR(S.precflux, c, k) * factor_xy * S.dz[c] / S.dt / S.nstop;
-        IN2(out->flux_u, c, l) = (PR(S.uwle, c, k) + PR(S.uwsb, c, k)) * t1 * factor_xy / S.nstop;
-        IN2(out->flux_v, c, l) = 0.0 * t1 * factor_xy / S.nstop;
-        IN2(out->flux_qt, c, l) = mkwle1 + mkwsb1;
-        IN2(out->fluxsgs_qt, c, l) = mkwsb1;
-        IN2(out->flux_qp, c, l) = mkwle2 + mkwsb2;
-        IN2(out->qt_trans, c, l) = mkadv1 + mkdiff1;
-        IN2(out->qp_trans, c, l) = mkadv2 + mkdiff2;
-        IN2(out->tkesgsz, c, l) = PR(S.rho, c, k) * tke * factor_xy;
-        IN2(out->tkez, c, l) = PR(S.rho, c, k) * 0.5 * (u2z + 0.0 + w2z) * factor_xy + IN2(out->tkesgsz, c, l);
-        IN2(out->tkz, c, l) = tkz * factor_xy;
-        IN2(out->precflux, c, l) = precflux / 1000.;
-        IN2(out->qp_fall, c, l) = qpfall;
-        IN2(out->qp_evp, c, l) = qpevp;
-        IN2(out->qp_src, c, l) = qpsrc;
-        IN2(out->qt_ls, c, l) = PR(S.qtend, c, k);
-        IN2(out->t_ls, c, l) = PR(S.ttend, c, k);
-      }
-    for (int c = 0; c < ncrms; c++) out->timing_factor[c] = out->timing_factor[c] / S.nstop;
-  }
+  stats_final(&sa, st);
 cleanup_keep:
   free(colprec);
   free(colprecs);
@@ -2748,3 +2811,93 @@ int oracle_get(const char *name, double *buf) {
 }
 
 void oracle_release(void) { state_free(); }
+
+/* Overwrite an internal field of the live state, in the layout of oracle_get: grid fields
+ * [c][k][i] (interior points), profiles [c][k].  Covers the fields the statistics read.
+ * Returns the number of values read from buf, or -1 for an unknown name or no live state. */
+int oracle_set(const char *name, const double *buf) {
+  if (!S.live) return -1;
+  size_t m = 0;
+  int nx = S.nx;
+  fld *f = NULL;
+  double *p = NULL;
+  int nk = S.nzm;
+  if (!strcmp(name, "u")) f = &S.u;
+  else if (!strcmp(name, "w")) { f = &S.w; nk = S.nz; }
+  else if (!strcmp(name, "tabs")) f = &S.tabs;
+  else if (!strcmp(name, "qv")) f = &S.qv;
+  else if (!strcmp(name, "qcl")) f = &S.qcl;
+  else if (!strcmp(name, "qci")) f = &S.qci;
+  else if (!strcmp(name, "qpl")) f = &S.qpl;
+  else if (!strcmp(name, "qpi")) f = &S.qpi;
+  else if (!strcmp(name, "tke")) f = &S.tke;
+  else if (!strcmp(name, "tk")) f = &S.tk;
+  else if (!strcmp(name, "precsfc")) { f = &S.precsfc; nk = 1; }
+  else if (!strcmp(name, "precssfc")) { f = &S.precssfc; nk = 1; }
+  else if (!strcmp(name, "u0")) p = S.u0;
+  else if (!strcmp(name, "uwle")) { p = S.uwle; nk = S.nz; }
+  else if (!strcmp(name, "uwsb")) { p = S.uwsb; nk = S.nz; }
+  else if (!strcmp(name, "mkwle1")) { p = S.mkwle[0]; nk = S.nz; }
+  else if (!strcmp(name, "mkwle2")) { p = S.mkwle[1]; nk = S.nz; }
+  else if (!strcmp(name, "mkwsb1")) { p = S.mkwsb[0]; nk = S.nz; }
+  else if (!strcmp(name, "mkwsb2")) { p = S.mkwsb[1]; nk = S.nz; }
+  else if (!strcmp(name, "mkadv1")) { p = S.mkadv[0]; nk = S.nz; }
+  else if (!strcmp(name, "mkadv2")) { p = S.mkadv[1]; nk = S.nz; }
+  else if (!strcmp(name, "mkdiff1")) { p = S.mkdiff[0]; nk = S.nz; }
+  else if (!strcmp(name, "mkdiff2")) { p = S.mkdiff[1]; nk = S.nz; }
+  else if (!strcmp(name, "qpsrc")) { p = S.qpsrc; nk = S.nz; }
+  else if (!strcmp(name, "qpevp")) { p = S.qpevp; nk = S.nz; }
+  else if (!strcmp(name, "qpfall")) { p = S.qpfall; nk = S.nz; }
+  else if (!strcmp(name, "precflux")) { p = S.precflux; nk = S.nz; }
+  else
+    return -1;
+  for (int c = 0; c < S.ncrms; c++)
+    for (int k = 1; k <= nk; k++) {
+      if (f)
+        for (int i = 1; i <= nx; i++) FLD(*f, c, i, k) = buf[m++];
+      else
+        PR(p, c, k) = buf[m++];
+    }
+  return (int)m;
+}
+
+/* Run the per-substep statistics (crm_module.F90:896-1073) nsteps times on the live state,
+ * as a previous oracle_crm call set it up (profiles from `in`) and oracle_set adjusted it,
+ * then the end-of-call outputs (crm_module.F90:1078-1659) into rad and out; the state views
+ * are not written.  The substep length is dt_gl / nsteps and dtn is the length of the last
+ * subcycle.  As in a call, the plev-level outputs and the radiation sums start from zero, and
+ * cltot..cllow and prectend/precstend start from the values the caller passes. */
+int oracle_stats_probe(const crm_dims *d, double dt_gl, double dtn, int nsteps, const crm_input_view *in,
+                       crm_rad_view *rad, crm_output_view *out) {
+  if (!S.live || !d || d->ncrms != S.ncrms || d->nx != S.nx || d->nz != S.nzm || nsteps < 1 ||
+      d->ny_rad != 1 || d->nx_rad < 1 || d->nx % d->nx_rad)
+    return CRM_ERR_ARG;
+  const int ncrms = d->ncrms, ld = d->ld, plev = d->plev, nzm = d->nz, nxr = d->nx_rad;
+  for (int c = 0; c < ncrms; c++)
+    for (int k = 0; k < nzm; k++)
+      for (int ir = 0; ir < nxr; ir++) {
+        size_t o = (size_t)c + (size_t)ld * ((size_t)ir + (size_t)nxr * k);
+        rad->temperature[o] = rad->qv[o] = rad->qc[o] = rad->qi[o] = rad->cld[o] = 0.0;
+      }
+  double *plev_outs[] = {out->cld, out->cldtop, out->gicewp, out->gliqwp, out->mctot, out->mcup, out->mcdn,
+                         out->mcuup, out->mcudn, out->qc_mean, out->qi_mean, out->qs_mean, out->qg_mean,
+                         out->qr_mean, out->mu_crm, out->md_crm, out->eu_crm, out->du_crm, out->ed_crm,
+                         out->flux_qt, out->flux_u, out->flux_v, out->fluxsgs_qt, out->tkez, out->tkesgsz,
+                         out->tkz, out->flux_qp, out->precflux, out->qt_trans, out->qp_trans, out->qp_fall,
+                         out->qp_evp, out->qp_src, out->qt_ls, out->t_ls};
+  for (size_t n = 0; n < sizeof(plev_outs) / sizeof(plev_outs[0]); n++)
+    for (int k = 0; k < plev; k++)
+      for (int c = 0; c < ncrms; c++) IN2(plev_outs[n], c, k) = 0.;
+  double *mui = (double *)calloc((size_t)ncrms * (plev + 2), sizeof(double));
+  double *mdi = (double *)calloc((size_t)ncrms * (plev + 2), sizeof(double));
+  S.nstop = nsteps;
+  S.dt = dt_gl / nsteps;
+  S.dtn = dtn;
+  const stats_args sa = {ncrms, ld, plev, d->nx, nzm, nzm + 1, nxr, plev - nzm + 1, 1. / (double)d->nx,
+                         1. / dt_gl, in, rad, out, mui, mdi};
+  for (int n = 0; n < nsteps; n++) stats_step(&sa);
+  stats_final(&sa, NULL);
+  free(mui);
+  free(mdi);
+  return CRM_OK;
+}

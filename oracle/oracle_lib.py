@@ -79,3 +79,50 @@ def oracle_accel_init(use_crm_accel, crm_accel_factor=0.0, crm_accel_uv=False):
     lib = C.CDLL(ORACLE_SO)
     lib.oracle_accel_init.argtypes = [C.c_int, C.c_double, C.c_int]
     lib.oracle_accel_init(int(bool(use_crm_accel)), float(crm_accel_factor), int(bool(crm_accel_uv)))
+
+
+class StatsProbe:
+    """The oracle's per-substep statistics and end-of-call outputs (crm_module.F90:896-1659)
+    run on a constructed state: set_up() runs the set-up of a call on `case` (reference
+    profiles from its crm_input), set() overwrites internal fields, run() evaluates the
+    statistics into case.rad / case.out.  mutate(m) plants one of the oracle's numbered
+    mistakes (0 removes it), so that a test can show its checks notice it."""
+
+    def __init__(self):
+        self.lib = C.CDLL(ORACLE_SO)
+        self.lib.oracle_set.restype = C.c_int
+        self.lib.oracle_set.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
+        self.lib.oracle_stats_probe.restype = C.c_int
+        self.lib.oracle_mutate.argtypes = [C.c_int]
+        self.runner = oracle()
+
+    def set_up(self, case):
+        work = case.copy()
+        rc = self.runner.run(work, case.dt, phase_stop=20)
+        if rc != 0:
+            raise RuntimeError(f"oracle set-up failed: {rc}")
+
+    def get(self, name, size):
+        return self.runner.field(name, size)
+
+    def set(self, name, values):
+        buf = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        n = self.lib.oracle_set(name.encode(), buf.ctypes.data_as(C.POINTER(C.c_double)))
+        if n != buf.size:
+            raise ValueError(f"oracle_set({name}): {n} values taken, {buf.size} given")
+
+    def run(self, case, dt_gl, dtn, nsteps):
+        dims = case.dims()
+        vi, _, vr, vo = case.views()
+        return self.lib.oracle_stats_probe(C.byref(dims), C.c_double(dt_gl), C.c_double(dtn),
+                                           C.c_int(nsteps), C.byref(vi), C.byref(vr), C.byref(vo))
+
+    def mutate(self, m):
+        self.lib.oracle_mutate(int(m))
+
+
+# Planted mistakes of oracle_mutate (the enum in crm_oracle.c).
+MUTATIONS = {"eu_du_swapped": 1, "cloud_layers_700_400_swapped": 2, "cld_not_time_mean": 3,
+             "mc_threshold_wmin": 4, "jt_mx_swapped": 5, "water_path_pdel_level": 6,
+             "cldtop_every_level": 7, "tkez_no_half": 8, "scalar_flux_dt_not_dtn": 9,
+             "updraft_cloud_level_above": 10, "rad_qv_unclipped": 11, "prec_split_per_hour": 12}
