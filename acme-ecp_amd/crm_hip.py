@@ -31,7 +31,8 @@ ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_acc
 # include/crm_hip_debug.h
 DEBUG_SYMBOLS = ["crm_hip_debug_run", "crm_hip_debug_get", "crm_hip_fft991_operators",
                  "crm_hip_profile_kernels", "crm_hip_kernel_time", "crm_hip_kernel_name",
-                 "crm_hip_graph_launches", "crm_hip_debug_stop_call_at"]
+                 "crm_hip_graph_launches", "crm_hip_graph_cache_size",
+                 "crm_hip_debug_stop_call_at"]
 # Kernel classes of one call (CRM_K_* of include/crm_hip_debug.h), in launch order.
 KERNELS = ["setup", "init", "setperturb", "kurant", "precip_init", "pre", "sgs", "mom", "press",
            "mpdata", "precip", "diffuse", "micro", "stats", "final"]
@@ -95,6 +96,7 @@ def load():
     lib.crm_hip_debug_stop_call_at.argtypes = [P, C.c_int]
     lib.crm_hip_graph_launches.argtypes = [P]
     lib.crm_hip_graph_launches.restype = C.c_longlong
+    lib.crm_hip_graph_cache_size.argtypes = [P]
     lib.crm_hip_run_many.argtypes = [C.c_int, C.POINTER(P), C.POINTER(CrmClassDesc), C.c_double,
                                      C.POINTER(C.c_int)]
     lib.crm_hip_finalize.argtypes = [P]
@@ -218,6 +220,9 @@ class CrmHip:
 
     def graph_launches(self):
         return int(self.lib.crm_hip_graph_launches(self.ctx))
+
+    def graph_cache_size(self):
+        return int(self.lib.crm_hip_graph_cache_size(self.ctx))
 
     def debug_stop_call_at(self, nstep):
         """Test hook: captured calls stop at substep nstep (0: off) and resume on the host."""

@@ -146,7 +146,7 @@ struct DevT {
   int ncrms, nzm, nz, plev, ld, nx_rad;
   int ny;                     // 1 for 2D; 8 for the 3D 8 x 8 CRM (column index 8 j + i)
   double dx, dt, dtn, dt3na, at, bt, ct, dtfactor;
-  int nstep, icycle, nprec_max_slot;
+  int nstep, icycle;
   // prognostic and diagnostic fields [c][k][i]
   R *u, *w, *t, *qt, *qp, *qn, *tabs, *qv, *qcl, *qci, *qpl, *qpi;
   R *tk, *tkh, *tke;
@@ -162,7 +162,8 @@ struct DevT {
                               // R: FP64, or FP32 in the mixed engine
   const double *Ft, *Gt;      // transform operators [NX][NX]
   const double *eign;         // [NX] horizontal eigenvalues per slot
-  unsigned long long *gmax;   // [0] stop flag of a captured call, [1] unused, [2] error flag,
+  unsigned long long *gmax;   // [0] stop flag of a captured call, [1] largest sedimentation
+                              // sub-step count nprec of the call, [2] error flag,
                               // [3] mean-state acceleration cease flag, then the kurant and
                               // precipitation Courant slots
   const unsigned long long *stop;  // &gmax[0] inside a captured call, else null
@@ -185,6 +186,7 @@ struct DevT {
 // the reader takes the maximum of the slots (the host for kurant, k_precip for the other).
 constexpr int GMAX_FLAGS = 4, KURANT_SLOTS = 32, KURANT_STRIDE = 16;
 constexpr int GMAX_STOP = 0;  // flag word: the substep that stopped a captured call
+constexpr int GMAX_NPREC = 1;  // the call's largest nprec (microphysics.F90:602-662)
 constexpr int GMAX_PCFL = GMAX_FLAGS + KURANT_SLOTS * KURANT_STRIDE;  // precipitation slots
 constexpr int GMAX_WORDS = GMAX_PCFL + KURANT_SLOTS * KURANT_STRIDE;
 

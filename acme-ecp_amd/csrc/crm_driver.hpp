@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <list>
 #include <map>
 #include <string>
 #include <type_traits>
@@ -95,11 +96,42 @@ struct Engine : CrmEngine {
   long long samp_cap = 0;
   int samp_stride = 1;
   std::map<std::string, hipGraphExec_t> graphs;
+  // The cache keeps at most GRAPH_CACHE_MAX graphs and evicts the least recently used: a
+  // caller that passes other buffers on every call (per-chunk arrays through
+  // crm_hip_run_device) would otherwise keep one whole-call graph per buffer set.  A steady
+  // substep rotates through 12 argument sets, so a call needs at most 13.
+  static constexpr size_t GRAPH_CACHE_MAX = 32;
+  std::list<std::string> graph_lru;  // most recently used first
   long long graph_launches = 0;
   void drop_graphs() {
+    if (!graphs.empty() && stream) (void)hipStreamSynchronize(stream);
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     graphs.clear();
+    graph_lru.clear();
   }
+  hipGraphExec_t graph_get(const std::string& key) {
+    auto it = graphs.find(key);
+    if (it == graphs.end()) return nullptr;
+    graph_lru.remove(key);
+    graph_lru.push_front(key);
+    return it->second;
+  }
+  void graph_put(const std::string& key, hipGraphExec_t exec) {
+    if (graphs.size() >= GRAPH_CACHE_MAX) {
+      (void)hipStreamSynchronize(stream);  // no evicted graph is still running
+      while (graphs.size() >= GRAPH_CACHE_MAX && !graph_lru.empty()) {
+        auto old = graphs.find(graph_lru.back());
+        if (old != graphs.end()) {
+          (void)hipGraphExecDestroy(old->second);
+          graphs.erase(old);
+        }
+        graph_lru.pop_back();
+      }
+    }
+    graphs[key] = exec;
+    graph_lru.push_front(key);
+  }
+  int graph_cache_size() const override { return (int)graphs.size(); }
 
   ~Engine() override;
   int run(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
@@ -504,7 +536,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
                                                                      d.nx_rad);
   }
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipMemsetAsync(ctx->gmax + 2, 0, 2 * sizeof(unsigned long long), s));
+  HIPCHECK(hipMemsetAsync(ctx->gmax + GMAX_NPREC, 0, 3 * sizeof(unsigned long long), s));
 
   if (!(dt_gl >= dm->dt) || !(dt_gl / dm->dt < 1.0e6)) return CRM_ERR_ARG;
   int nstop = (int)(dt_gl / dm->dt);
@@ -547,6 +579,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   const dim3 gpress((unsigned)((ncrms + press_per_block - 1) / press_per_block));
   const size_t press_lds = (size_t)nzm * 64 * sizeof(double);
   unsigned long long err_flag = 0;
+  unsigned long long tail[2] = {0, 0};  // read back after the call: the largest nprec, the error flag
   HIPCHECK(hipEventRecord(ctx->ev0, s));
   // A mean t change above 5 K flagged by the acceleration during substep n stops it for the
   // rest of the call and lengthens it (accelerate_crm.F90:255-278); the flag is read with the
@@ -618,13 +651,14 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         na = nc;
         nc = nb;
         nb = nn;
-        if (launch) NX_DISPATCH(nx, (k_stats<NXV, R><<<L.g(CRM_K_STATS), L.b(CRM_K_STATS), 0, s>>>(d)));
+        if (launch) {
+          NX_DISPATCH(nx, (k_stats<NXV, R><<<L.g(CRM_K_STATS), L.b(CRM_K_STATS), 0, s>>>(d)));
+        }
       }
     };
     HIPCHECK(hipMemsetAsync(ctx->gmax + GMAX_STOP, 0, sizeof(unsigned long long), s));
-    hipGraphExec_t exec = nullptr;
-    auto it = ctx->graphs.find(key);
-    if (it == ctx->graphs.end()) {
+    hipGraphExec_t exec = ctx->graph_get(key);
+    if (!exec) {
       std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
       hipGraph_t graph = nullptr;
       HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -632,9 +666,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       HIPCHECK(hipStreamEndCapture(s, &graph));
       HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
       (void)hipGraphDestroy(graph);
-      ctx->graphs[key] = exec;
+      ctx->graph_put(key, exec);
     } else {
-      exec = it->second;
       sequence(false);
     }
     HIPCHECK(hipGraphLaunch(exec, s));
@@ -728,9 +761,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       key.append((const char*)ints, sizeof(ints));
       key.append((const char*)&qrad, sizeof(qrad));
       key.append((const char*)ctx->block, sizeof(ctx->block));
-      auto it = ctx->graphs.find(key);
-      hipGraphExec_t exec = nullptr;
-      if (it == ctx->graphs.end()) {
+      hipGraphExec_t exec = ctx->graph_get(key);
+      if (!exec) {
         std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
         hipGraph_t graph = nullptr;
         HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -738,9 +770,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         HIPCHECK(hipStreamEndCapture(s, &graph));
         HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
-        ctx->graphs[key] = exec;
+        ctx->graph_put(key, exec);
       } else {
-        exec = it->second;
         // the host side of the substep: buffer swap of uvw and the index rotation
         std::swap(d.u, d.dudt[nc]);
         std::swap(d.w, d.dwdt[nc]);
@@ -807,8 +838,10 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   TIMED(CRM_K_FINAL, NX_DISPATCH(nx, (k_final<NXV, R><<<L.g(CRM_K_FINAL), L.b(CRM_K_FINAL), 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop))));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(ctx->ev1, s));
-  HIPCHECK(hipMemcpyAsync(&err_flag, ctx->gmax + 2, sizeof(err_flag), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(tail, ctx->gmax + GMAX_NPREC, sizeof(tail), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
+  err_flag = tail[1];
+  ctx->stats.nprec_max = (int)tail[0];
   {
     float ms = 0.f;
     HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

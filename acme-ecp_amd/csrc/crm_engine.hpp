@@ -23,6 +23,7 @@ struct CrmEngine {
   virtual void set_graphs(int mode) = 0;
   virtual void set_debug_stop(int nstep) = 0;
   virtual long long graph_count() const = 0;
+  virtual int graph_cache_size() const = 0;
   virtual int set_mlp(const crm_mlp* m) = 0;
   virtual int record_samples(float* X, float* T, long long cap, int stride) = 0;
   virtual long long samples_count() const = 0;

@@ -183,6 +183,11 @@ long long crm_hip_graph_launches(const crm_hip_ctx* ctx) {
   return ctx->eng->graph_count();
 }
 
+int crm_hip_graph_cache_size(const crm_hip_ctx* ctx) {
+  if (!ctx || !ctx->eng) return -1;
+  return ctx->eng->graph_cache_size();
+}
+
 int crm_hip_debug_stop_call_at(crm_hip_ctx* ctx, int nstep) {
   if (!ctx || !ctx->eng || nstep < 0) return CRM_ERR_ARG;
   ctx->eng->set_debug_stop(nstep);

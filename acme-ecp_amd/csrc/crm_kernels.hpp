@@ -1930,6 +1930,10 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
   } else if (prec_cfl > R4(0.9)) {
     nprec = (int)ceil(prec_cfl / R4(0.9));
   }
+  // the call's largest count, for crm_hip_stats: nprec is batch-global, so one lane of one
+  // instance keeps it (launches are stream-ordered)
+  if (live && c == 0 && i == 0 && (unsigned long long)nprec > d.gmax[GMAX_NPREC])
+    d.gmax[GMAX_NPREC] = (unsigned long long)nprec;
   const R rnprec = (R)nprec;
   auto Q = [&](R* p, int k) -> R& { return p[((size_t)c * nzm + k) * NX + i]; };
   auto lfac_of = [&](R tabs) {

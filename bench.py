@@ -27,7 +27,9 @@ import numpy as np  # noqa: E402
 
 from crm_inputs import make_case  # noqa: E402
 
-WORKLOAD = dict(ncrms=4096, nx=64, nz=58, dx=1000.0, dt=5.0, dt_gl=1200.0)
+# The operational SP1V1 CRM (cam/cime_config/config_component.xml:100,106): 64 x 1 columns,
+# 58 levels, dx 1 km, dt 5 s, radiation on 16 column groups of 4.
+WORKLOAD = dict(ncrms=4096, nx=64, nz=58, dx=1000.0, dt=5.0, dt_gl=1200.0, nx_rad=16)
 HBM_PEAK_GBS = 8000.0
 # Compulsory HBM words per grid cell of one launch of each fused kernel: every field the
 # kernel reads counted once, every field it writes counted once (2D, FP64 words).
@@ -70,74 +72,130 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(ncrms_sample):
-    """The reference CRM (compiled from the reference sources, oracle/_ref) on one core over a
-    bounded sample of the workload; falls back to the C restatement if it is not built."""
+def _cpu_worker(first, count, tag, barrier, results):
+    """One CPU-baseline process: instances first..first+count-1 of the workload through the
+    reference CRM build `tag` (oracle/_ref), timed from a barrier that every process passes
+    together.  The reference keeps its state in module variables, so each process is one
+    independent crm() caller, as each MPI rank of the GCM is."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_lib
     w = WORKLOAD
-    case = make_case(ncrms_sample, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True)
+    case = make_case(count, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True, seed_base=first,
+                     nx_rad=w["nx_rad"])
     kind = "reference"
     try:
-        runner = oracle_lib.reference(case)
+        path = oracle_lib.ref_so_path(case)
+        if tag:
+            path = path[:-3] + tag + ".so"
+        runner = oracle_lib._Runner(path, "ref_crm", "ref_get")
     except (FileNotFoundError, OSError):
+        if tag:
+            results.put((first, None, "missing", 0.0))
+            barrier.wait()
+            return
         runner, kind = oracle_lib.oracle(), "port"
+    barrier.wait()
     t0 = time.perf_counter()
     rc = runner.run(case, w["dt_gl"])
     sec = time.perf_counter() - t0
-    if rc != 0:
-        raise RuntimeError(f"CPU baseline returned status {rc}")
+    results.put((first, rc, kind, sec))
+
+
+def cpu_baseline(per_proc, procs, tag=""):
+    """The reference CRM (compiled from the reference sources, oracle/_ref) on `procs` host
+    cores at once, each process advancing its own `per_proc` instances of the workload by one
+    GCM step; the wall time is the slowest process's.  Falls back to the C restatement where
+    the reference build is missing.  Processes are started fresh (spawn): none of them touches
+    the GPU."""
+    import multiprocessing as mp
+    w = WORKLOAD
+    ctx = mp.get_context("spawn")
+    barrier = ctx.Barrier(procs)
+    results = ctx.Queue()
+    ps = [ctx.Process(target=_cpu_worker, args=(p * per_proc, per_proc, tag, barrier, results))
+          for p in range(procs)]
+    for p in ps:
+        p.start()
+    got = [results.get() for _ in ps]
+    for p in ps:
+        p.join()
+    if any(r[2] == "missing" for r in got):
+        return None
+    if any(r[1] != 0 for r in got):
+        raise RuntimeError(f"CPU baseline returned status {[r[1] for r in got]}")
+    sec = max(r[3] for r in got)
+    kind = got[0][2]
     nstop = int(w["dt_gl"] / w["dt"])
-    colsteps = ncrms_sample * w["nx"] * nstop
-    return {"value": colsteps / sec, "unit": "column-steps/s", "cores": 1, "kind": kind,
-            "sample": f"{ncrms_sample} of the 4096 instances, full 1200 s GCM step "
-                      f"({nstop} substeps), one process, {sec:.1f} s"}
+    colsteps = procs * per_proc * w["nx"] * nstop
+    return {"value": colsteps / sec, "unit": "column-steps/s", "cores": procs, "kind": kind,
+            "sample": f"{procs * per_proc} of the {w['ncrms']} instances in {procs} processes "
+                      f"of {per_proc} (one per core), full 1200 s GCM step ({nstop} substeps); "
+                      f"slowest process {sec:.1f} s, fastest {min(r[3] for r in got):.1f} s",
+            "build": "flang -O3 -march=x86-64-v3" if tag else "flang -O2"}
 
 
 def measure(prec, case, args, torch):
-    """Warmup and timed crm() calls of one precision on a fresh device copy of case."""
+    """Warmup and timed crm() calls of one precision on a fresh device copy of case, then a
+    separate pass with every kernel class timed by HIP events for the roofline."""
     from crm_hip import KERNELS, CrmHip, DeviceCase
     w = WORKLOAD
     nstop = int(w["dt_gl"] / w["dt"])
     dcase = DeviceCase(case, "cuda:0")
     torch.cuda.synchronize()
     hip = CrmHip(0, prec)
-    # warmup: the first call also sizes the device pool; time every kernel class to find the
-    # dominant one
-    hip.profile_kernels(KERNELS)
-    for n in range(args.warmup):
+    # warmup on the path a caller gets (the default: one captured HIP graph per call, no
+    # events); the first call also sizes the device pool and captures the graph
+    for n in range(max(1, args.warmup)):
         hip.run_device(dcase, w["dt_gl"])
-        log(f"[{prec}] warmup step {n + 1}/{args.warmup} done")
-    warm = hip.kernel_times()
-    dominant = max((k for k in warm if k in KERNEL_WORDS), key=lambda k: warm[k][0])
-    log(f"[{prec}] warmup kernel time [ms]: " +
-        ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in warm.items()))
-    # timed region: only the dominant kernel carries events
-    hip.profile_kernels([dominant])
+        log(f"[{prec}] warmup step {n + 1}/{max(1, args.warmup)} done")
     torch.cuda.synchronize()
+    g0 = hip.graph_launches()
     t0 = time.perf_counter()
-    ncyc = []
+    ncyc, nprec = [], []
     for n in range(args.steps):
         hip.run_device(dcase, w["dt_gl"])
-        ncyc.append(hip.stats().ncycle_max)
+        st = hip.stats()
+        ncyc.append(st.ncycle_max)
+        nprec.append(st.nprec_max)
         log(f"[{prec}] step {n + 1}/{args.steps} done")
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
-    kms, klaunch = hip.kernel_times()[dominant]
+    graph_substeps = hip.graph_launches() - g0
+    # roofline pass: plain launches with a HIP-event pair around every kernel on the library's
+    # own stream (graphs are off while events are on); the state keeps advancing
+    hip.profile_kernels(KERNELS)
+    for n in range(args.profile_steps):
+        hip.run_device(dcase, w["dt_gl"])
+        log(f"[{prec}] roofline pass step {n + 1}/{args.profile_steps} done")
+    times = hip.kernel_times()
+    hip.profile_kernels([])
     hip.close()
+    dominant = max((k for k in times if k in KERNEL_WORDS), key=lambda k: times[k][0])
+    log(f"[{prec}] kernel time over {args.profile_steps} steps [ms/launches]: " +
+        ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in times.items()))
+    kms, klaunch = times[dominant]
     value = case.ncrms * w["nx"] * nstop * args.steps / sec
     word = 8 if prec == "fp64" else 4
     bytes_launch = KERNEL_WORDS[dominant] * word * case.ncrms * w["nx"] * w["nz"]
     avg_ms = kms / klaunch
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    per_kernel = {}
+    for k, (ms, n) in times.items():
+        e = {"ms_per_step": ms / args.profile_steps, "avg_launch_us": 1e3 * ms / n}
+        if k in KERNEL_WORDS:
+            b = KERNEL_WORDS[k] * word * case.ncrms * w["nx"] * w["nz"]
+            e["hbm_frac_algorithmic"] = b / (ms / n * 1e-3) / 1e9 / HBM_PEAK_GBS
+        per_kernel[k] = e
     return {"value": value, "ms_per_step": 1e3 * sec / args.steps, "max_ncycle": max(ncyc),
-            "dcase": dcase,
-            "kernel_ms": {k: v[0] / max(1, args.warmup) for k, v in warm.items()},
+            "max_nprec": max(nprec), "graph_substeps_timed": graph_substeps,
+            "dcase": dcase, "kernels": per_kernel,
             "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "traffic": measured_traffic(dominant if prec == "fp64" else "mixed." + dominant),
                          "avg_launch_ms": avg_ms, "launches": klaunch,
-                         "algorithmic_bytes_per_launch": bytes_launch}}
+                         "algorithmic_bytes_per_launch": bytes_launch,
+                         "timing": f"HIP events on the library stream, separate pass of "
+                                   f"{args.profile_steps} GCM steps after the timed ones"}}
 
 
 # Uneven batch: one 1200 s GCM step of several CRM size classes advanced together
@@ -262,8 +320,13 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--ncrms", type=int, default=WORKLOAD["ncrms"])
-    ap.add_argument("--cpu-ncrms", type=int, default=64,
-                    help="instances in the CPU baseline sample (0 skips it)")
+    ap.add_argument("--profile-steps", type=int, default=2,
+                    help="GCM steps of the separate pass that times every kernel with HIP events")
+    ap.add_argument("--cpu-per-proc", type=int, default=16,
+                    help="instances per process in the CPU baseline sample (0 skips it)")
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="CPU baseline processes (default: the cores this process may use, "
+                         "at most 16)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="skip the mixed-precision measurement")
     ap.add_argument("--no-uneven", action="store_true",
@@ -291,13 +354,23 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     w = WORKLOAD
     ncrms = args.ncrms
     nstop = int(w["dt_gl"] / w["dt"])
-    log(f"building inputs: {ncrms} x {w['nx']}x{w['nz']}")
-    case = make_case(ncrms, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True)
+    # The CPU baseline first, before this process touches the GPU.
+    cpu = cpu_native = None
+    if args.cpu_per_proc > 0:
+        procs = args.cpu_procs or min(16, len(os.sched_getaffinity(0)))
+        log(f"CPU baseline: {procs} processes of {args.cpu_per_proc} instances")
+        cpu = cpu_baseline(args.cpu_per_proc, procs)
+        cpu_native = cpu_baseline(args.cpu_per_proc, procs, tag="_native")
+        if cpu and cpu_native:
+            cpu["o3_x86_64_v3"] = {k: cpu_native[k] for k in ("value", "sample", "build")}
+
+    import torch
+
+    log(f"building inputs: {ncrms} x {w['nx']}x{w['nz']}, {w['nx_rad']} radiation columns")
+    case = make_case(ncrms, w["nx"], w["nz"], w["dx"], w["dt"], randomize=True, nx_rad=w["nx_rad"])
 
     res = {}
     for prec in ("fp64", "mixed"):
@@ -313,23 +386,19 @@ def main():
         mixed = {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
                  "speedup_vs_fp64": r32["value"] / value, "roofline": r32["roofline"],
                  "error_vs_fp64": state_error(r64["dcase"], r32["dcase"], torch),
-                 "kernel_ms_per_step_warmup": r32["kernel_ms"],
+                 "kernels": r32["kernels"],
                  "note": "FP32 bulk fields, advection, SGS and microphysics; FP64 pressure solve, "
                          "reference profiles, statistics and outputs; error after the same "
-                         f"{args.warmup + args.steps} GCM steps from the same initial state"}
+                         f"{max(1, args.warmup) + args.steps + args.profile_steps} GCM steps "
+                         "from the same initial state"}
 
     learned = None
     if not args.no_learned:
-        learned = measure_learned(case, args, torch)
+        learned = measure_learned(r64["dcase"], args, torch)
 
     uneven = None
     if not args.no_uneven:
         uneven = {p: measure_uneven(p, args, torch) for p in (("fp64",) if args.fp64_only else ("fp64", "mixed"))}
-
-    cpu = None
-    if args.cpu_ncrms > 0:
-        log(f"CPU baseline on {args.cpu_ncrms} instances")
-        cpu = cpu_baseline(args.cpu_ncrms)
 
     line = {
         "metric": "CRM column-steps/s on 1 MI355X, FP64 vs mixed-prec at stated error; HBM %peak",
@@ -344,14 +413,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: seeded tropical soundings per instance",
-        "config": {"workload": f"{ncrms} 2D CRMs of 64x58, dx 1 km, dt 5 s, one 1200 s GCM "
-                               f"step ({nstop} substeps) per call, inputs resident in HBM",
-                   "ncrms": ncrms, "nx": w["nx"], "nz": w["nz"], "substeps": nstop,
-                   "precision": "fp64", "max_ncycle": r64["max_ncycle"]},
+        "config": {"workload": f"{ncrms} 2D CRMs of 64x58 (SP1V1: radiation on {w['nx_rad']} "
+                               f"column groups), dx 1 km, dt 5 s, one 1200 s GCM step "
+                               f"({nstop} substeps) per call, inputs resident in HBM, default "
+                               f"path (one HIP graph per call)",
+                   "ncrms": ncrms, "nx": w["nx"], "nz": w["nz"], "nx_rad": w["nx_rad"],
+                   "substeps": nstop, "precision": "fp64", "max_ncycle": r64["max_ncycle"],
+                   "max_nprec": r64["max_nprec"],
+                   "graph_substeps_timed": r64["graph_substeps_timed"]},
         "roofline": r64["roofline"],
         "whole_step_hbm": {"achieved": whole, "frac": whole / HBM_PEAK_GBS, "unit": "GB/s",
                            "bytes_per_column_substep": STATE_WORDS_PER_LEVEL * 8 * w["nz"]},
-        "kernel_ms_per_step_warmup": r64["kernel_ms"],
+        "kernels": r64["kernels"],
         "mixed_precision": mixed,
         "uneven_batch": uneven,
         "learned_microphysics": learned,

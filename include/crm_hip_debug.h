@@ -74,6 +74,9 @@ const char *crm_hip_kernel_name(int kid);
 /* Number of substeps replayed from captured graphs since the context was created. */
 long long crm_hip_graph_launches(const crm_hip_ctx *ctx);
 
+/* Number of captured graphs the context holds (at most 32; least recently used first out). */
+int crm_hip_graph_cache_size(const crm_hip_ctx *ctx);
+
 /* Test hook: a captured call (CRM_GRAPHS_CALL) stops at substep nstep (1-based) as if it needed
  * more subcycles, and the host resumes it there; 0 turns the hook off. */
 int crm_hip_debug_stop_call_at(crm_hip_ctx *ctx, int nstep);

@@ -1482,6 +1482,9 @@ static double term_vel_qp(const termv_t *tv, double qploc, double rho, double ta
   return term_vel;
 }
 
+/* largest sedimentation sub-step count of the current call (reported as "nprec_max") */
+static int oracle_nprec_max = 1;
+
 static void precip_fall(void) {
   int nzm = S.nzm, nz = S.nz, nx = S.nx, ncrms = S.ncrms;
   termv_t tv;
@@ -1534,6 +1537,7 @@ static void precip_fall(void) {
   int nprec = 1;
   if (prec_cfl > R4(0.9)) {
     nprec = (int)ceil(prec_cfl / R4(0.9));
+    if (nprec > oracle_nprec_max) oracle_nprec_max = nprec;
     for (int c = 0; c < ncrms; c++)
       for (int i = 1; i <= nx; i++)
         for (int k = 1; k <= nzm; k++) A3(wp, c, i, k) = A3(wp, c, i, k) / (double)nprec;
@@ -2435,6 +2439,7 @@ int oracle_crm(const crm_dims *d, const crm_column_info *col, double dt_gl, cons
   if (60. < 2 * d->dt) return CRM_ERR_DAMPING;
   state_free();
   state_alloc(ncrms, nx, nzm);
+  oracle_nprec_max = 1;
   int nz = nzm + 1;
   S.plev = plev;
   S.dx = d->dx;
@@ -2745,7 +2750,8 @@ int oracle_get(const char *name, double *buf) {
     for (int c = 0; c < S.ncrms; c++)                                     \
       for (int k = 1; k <= (nk); k++) buf[m++] = PR(a, c, k);             \
   } while (0)
-  if (!strcmp(name, "u")) GRID(S.u, S.nzm);
+  if (!strcmp(name, "nprec_max")) buf[m++] = oracle_nprec_max;
+  else if (!strcmp(name, "u")) GRID(S.u, S.nzm);
   else if (!strcmp(name, "w")) GRID(S.w, S.nz);
   else if (!strcmp(name, "t")) GRID(S.t, S.nzm);
   else if (!strcmp(name, "tabs")) GRID(S.tabs, S.nzm);

@@ -3,7 +3,8 @@
 # the harness in this directory, into oracle/_ref/libcrmref_<class>.so.
 #
 # Usage: build_ref.sh NX NY NZ DX DT [NX_RAD NY_RAD]
-# Env:   CRM_REF_DIR (default /root/reference), FLANG (default ROCm flang), OPT (default -O2)
+# Env:   CRM_REF_DIR (default /root/reference), FLANG (default ROCm flang), OPT (default -O2),
+#        TAG (appended to the library name, for builds with other OPT flags)
 #
 # Test infrastructure only: the library is the oracle's reference leg and the CPU baseline.
 # Nothing from the reference tree is copied; objects go to oracle/_ref/<class>/ only.
@@ -15,13 +16,14 @@ OUT=$(cd "$HERE/.." && pwd)/_ref
 R=${CRM_REF_DIR:-/root/reference}
 FL=${FLANG:-/opt/rocm/lib/llvm/bin/flang}
 OPT=${OPT:--O2}
+TAG=${TAG:-}
 if [ ! -d "$R/components/cam/src/physics/crm" ]; then
   echo "reference tree not found at $R" >&2; exit 3
 fi
 if [ "$NY" -gt 1 ]; then Y3=1; else Y3=0; fi
 CLASS=${NX}x${NY}x${NZ}_dx${DX}_dt${DT}
 if [ "$NXR" != "$NX" ] || [ "$NYR" != "$NY" ]; then CLASS=${CLASS}_r${NXR}x${NYR}; fi
-B=$OUT/$CLASS
+B=$OUT/$CLASS$TAG
 mkdir -p "$B"
 C=$R/components/cam/src/physics/crm
 CAM=$R/components/cam/src/physics/cam
@@ -54,5 +56,5 @@ for f in $FILES; do
 done
 "$FL" $DEFS -w $OPT -fPIC -c "$HERE/crm_ref_harness.F90" -o crm_ref_harness.o
 gcc -O2 -fPIC -c "$C/gammafff.c" -o gammafff.o
-"$FL" -shared -o "$OUT/libcrmref_${CLASS}.so" $OBJS crm_ref_harness.o gammafff.o
-echo "$OUT/libcrmref_${CLASS}.so"
+"$FL" -shared -o "$OUT/libcrmref_${CLASS}${TAG}.so" $OBJS crm_ref_harness.o gammafff.o
+echo "$OUT/libcrmref_${CLASS}${TAG}.so"

@@ -124,3 +124,23 @@ def test_context_usable_after_errors(hip):
     assert oracle_lib.oracle().run(want, 20.0) == 0
     hip.run(got, 20.0)
     assert max(compare_case(got, want, dt_gl=20.0).values()) <= 1e-10
+
+
+@pytest.mark.parametrize("graphs", [0, 2])
+def test_stats_report_step_counts(hip, graphs):
+    """crm_hip_stats after a call: nstop and the largest sedimentation sub-step count nprec of
+    the call, as the oracle counts it (microphysics.F90:602-662), on the captured and the
+    plain-launch path."""
+    case = convective_case(6, seed=3)
+    o = oracle_lib.oracle()
+    assert o.run(case.copy(), 200.0) == 0
+    want = int(o.field("nprec_max", 1)[0])
+    hip.set_graphs(graphs)
+    try:
+        hip.run(case.copy(), 200.0)
+    finally:
+        hip.set_graphs(2)
+    st = hip.stats()
+    assert st.nstop == 10
+    assert st.ncycle_max >= 1
+    assert want > 1 and st.nprec_max == want

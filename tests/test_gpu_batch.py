@@ -153,3 +153,28 @@ def test_call_graph_resumes_mid_call(stop_at):
     finally:
         he.close()
         hg.close()
+
+
+def test_graph_cache_is_bounded():
+    """Every batch size a context sees gets its own captured call graph (the batch size is one
+    of the kernel arguments); the context keeps at most 32 of them, the least recently used
+    evicted, and results stay bitwise those of plain launches."""
+    n0 = 40
+    base = convective_case(n0, seed=86)
+    hg, he = CrmHip(0), CrmHip(0)
+    try:
+        he.set_graphs(False)
+        for j, n in enumerate(range(n0, 0, -1)):  # largest first: the buffers are sized once
+            case = base.copy()
+            case.ncrms = n
+            want = case.copy()
+            hg.run(case, 40.0)
+            assert hg.graph_cache_size() <= 32
+            if j % 13 == 0 or n == 1:
+                he.run(want, 40.0)
+                _same(want, case)
+        assert hg.graph_cache_size() == 32
+        assert hg.graph_launches() >= 2 * n0
+    finally:
+        hg.close()
+        he.close()

@@ -142,7 +142,8 @@ def test_batch_global_subcycling(hip):
 
 
 @pytest.mark.parametrize("shape", [(32, 28, 4000.0, 20.0, 8), (128, 58, 1000.0, 5.0, 1),
-                                   (128, 58, 1000.0, 5.0, 4)], ids=lambda s: f"{s[0]}x{s[1]}_rad{s[4]}")
+                                   (128, 58, 1000.0, 5.0, 4), (64, 58, 1000.0, 5.0, 16)],
+                         ids=lambda s: f"{s[0]}x{s[1]}_rad{s[4]}")
 def test_radiation_groups(hip, shape):
     """crm_nx_rad < crm_nx: radiation inputs and outputs on column groups
     (crm_module.F90:766-777, 979-1015, 1086-1107); at 128 columns a group spans both waves of
@@ -154,6 +155,19 @@ def test_radiation_groups(hip, shape):
     assert oracle_lib.oracle().run(want, 3 * dt) == 0
     hip.run(got, 3 * dt)
     errs = compare_case(got, want, dt_gl=3 * dt)
+    assert max(errs.values()) <= TOL_N, _worst(errs)
+
+
+def test_operational_sp1v1_against_reference(hip):
+    """The operational SP1V1 CRM (64 x 58, dx 1 km, dt 5 s, radiation on 16 column groups;
+    cam/cime_config/config_component.xml:100,106) against the reference routines compiled for
+    exactly that class, over 3 substeps with radiative heating."""
+    case = convective_case(5, nx=64, nz=58, dx=1000.0, dt=5.0, seed=17, nx_rad=16)
+    case.rad["qrad"][...] = np.random.default_rng(2).uniform(-2e-5, 2e-5, case.rad["qrad"].shape)
+    want, got = case.copy(), case.copy()
+    assert oracle_lib.reference(want).run(want, 15.0) == 0
+    hip.run(got, 15.0)
+    errs = compare_case(got, want, dt_gl=15.0)
     assert max(errs.values()) <= TOL_N, _worst(errs)
 
 
