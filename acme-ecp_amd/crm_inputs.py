@@ -156,13 +156,15 @@ def add_convection(c, seed=1):
         i0 = (3 * n) % nx
         cols = [(i0 + d) % nx for d in range(max(3, nx // 6))]
         klo, khi = max(1, nz // 12), max(3, nz // 2)
-        for i in cols:
-            for k in range(klo, khi):
-                qs = float(qsatw(t[i, k], p_mb[k]))
-                c.state["qn"][n, i, 0, k] = 1.5e-3 * (1.0 + 0.3 * rng.random())
-                c.state["qt"][n, i, 0, k] = qs * 1.01 + c.state["qn"][n, i, 0, k]
-                c.state["qp"][n, i, 0, k] = 2.0e-3 * rng.random()
-                c.state["w_wind"][n, i, 0, k] = 3.0 + 2.0 * rng.random()
+        # three draws per (column, level) in that order: condensate, precipitation, updraft
+        r = rng.random((len(cols), khi - klo, 3))
+        for j, i in enumerate(cols):
+            qs = qsatw(t[i, klo:khi], p_mb[klo:khi])
+            qn = 1.5e-3 * (1.0 + 0.3 * r[j, :, 0])
+            c.state["qn"][n, i, 0, klo:khi] = qn
+            c.state["qt"][n, i, 0, klo:khi] = qs * 1.01 + qn
+            c.state["qp"][n, i, 0, klo:khi] = 2.0e-3 * r[j, :, 1]
+            c.state["w_wind"][n, i, 0, klo:khi] = 3.0 + 2.0 * r[j, :, 2]
         # ice cloud aloft (T below 253 K) and falling snow
         kice = [k for k in range(nz) if t[0, k] < 245.0]
         for k in kice[: max(1, len(kice) // 3)]:

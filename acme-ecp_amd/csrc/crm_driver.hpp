@@ -362,7 +362,12 @@ static int get_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform& o
 // ---- kernel dispatch over the compile-time column count ------------------------------------
 template <int NX, typename R, typename G>
 static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, const DevT<R>& d, int na, int nb, int nc) {
-  if constexpr (NX == 64) {  // matrix-core transforms; one instance per block
+  if constexpr (NX == 64 && !G::Y3) {  // matrix-core transforms; one instance per 8-wave block
+    if (nzm == 28)
+      k_press64r<28, R><<<dim3((unsigned)d.ncrms), dim3(512), press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+    else
+      k_press64r<58, R><<<dim3((unsigned)d.ncrms), dim3(512), press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+  } else if constexpr (NX == 64) {  // 3D 8 x 8: one instance per 4-wave block
     if (nzm == 28)
       k_press64<28, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
     else

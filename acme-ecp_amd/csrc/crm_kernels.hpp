@@ -1571,6 +1571,173 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   if (wave == 0) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
 }
 
+// k_press64 for 2D CRMs of 64 columns on 8 waves (one instance per 512-thread workgroup):
+// wave w owns the slab rows k = w + 8 j.  The right-hand side pass keeps, per row, what the
+// gradient pass needs (u, w, their tendencies dudt(na), dwdt(na), and the b/c terms of the
+// Adams-Bashforth step) in registers, so the velocities and the Adams-Bashforth history are
+// read from memory once per launch instead of twice.  Eight waves of one workgroup fill a CU
+// as two four-wave workgroups of k_press64 do.
+template <int NZM, typename R>
+__global__ void __launch_bounds__(512) k_press64r(DevT<R> d, int na, int nb, int nc) {
+  CRM_STOP_CHECK(d);
+  constexpr int NX = 64, NK = 64, NW = 8;
+  constexpr int NJ = (NZM + NW - 1) / NW;  // rows of the level range per wave
+  extern __shared__ double lds[];
+  double* P = lds;              // [NK][PLD] slab: rhs, coefficients, pressure
+  double* ALF = lds + NK * PLD; // [NZM][64] Thomas factors
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x;  // grid is exactly ncrms blocks
+  const int i = lane;
+  Seg<64> s;
+  const int nz = NZM + 1;
+  const double dz = inst(d, I_DZ, c);
+  const double* adz = prof(d, P_ADZ, c);
+  const double* adzw = prof(d, P_ADZW, c);
+  const double* rho = prof(d, P_RHO, c);
+  const double* rhow = prof(d, P_RHOW, c);
+  const double rdx = 1. / d.dx, btat = d.bt / d.at, ctat = d.ct / d.at;
+  const double dta = 1. / d.dt3na / d.at;
+  const R *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
+  const R *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
+  const int r16 = lane & 15, q4 = lane >> 4;
+  // the gradient pass's inputs of each owned level
+  double cu[NJ], cua[NJ], cub[NJ], cw[NJ], cwa[NJ], cwb[NJ];
+  // 1. right-hand side (rows >= NZM of the slab are zero)
+#pragma unroll
+  for (int j = 0; j < NK / NW; j++) {
+    const int k = wave + NW * j;
+    double pv = 0.0;
+    if (k < NZM) {
+      size_t o = ((size_t)c * NZM + k) * NX + i;
+      size_t ow = ((size_t)c * nz + k) * NX + i;
+      size_t own = ow + NX;
+      double rdz = 1. / (adz[k] * dz);
+      double rup = rhow[k + 1] / rho[k] * rdz;
+      double rdn = rhow[k] / rho[k] * rdz;
+      double uu = d.u[o], uap = ua[o], ubp = ub[o], ucp = uc[o];
+      double ww = d.w[ow], wap = wa[ow], wbp = wb[ow], wcp = wc[ow];
+      pv = (rdx * (s.rot(uu, 1) - uu) + (d.w[own] * rup - ww * rdn)) * dta +
+           (rdx * (s.rot(uap, 1) - uap) + (wa[own] * rup - wap * rdn)) +
+           btat * (rdx * (s.rot(ubp, 1) - ubp) + (wb[own] * rup - wbp * rdn)) +
+           ctat * (rdx * (s.rot(ucp, 1) - ucp) + (wc[own] * rup - wcp * rdn));
+      pv = pv * rho[k];
+      if (j < NJ) {
+        cu[j] = uu;
+        cua[j] = uap;
+        cub[j] = d.bt * ubp + d.ct * ucp;
+        cw[j] = ww;
+        cwa[j] = wap;
+        cwb[j] = d.bt * wbp + d.ct * wcp;
+      }
+    }
+    P[k * PLD + i] = pv;
+  }
+  // one 64x64x64 transform of the slab in place: P <- P * M (M row-major 64x64); wave w makes
+  // output columns 16 (w & 3) .. +15 of row tiles 2 (w >> 2) and 2 (w >> 2) + 1
+  auto transform = [&](const double* M) {
+    const int cb = wave & 3, rb = (wave >> 2) * 2;
+    double bf[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) bf[j] = M[(4 * j + q4) * NX + 16 * cb + r16];
+    v4d acc[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) acc[r] = v4d{0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        double a = P[(16 * (rb + r) + r16) * PLD + 4 * j + q4];
+        acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bf[j], acc[r], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) P[(16 * (rb + r) + 4 * jj + q4) * PLD + 16 * cb + r16] = acc[r][jj];
+    __syncthreads();
+  };
+  // 2. forward transform
+  transform(d.Ft);
+  // 3. Thomas sweep per slot (wave 0, lane = slot), factors in LDS; as in k_press64
+  if (wave == 0) {
+    static_assert(NZM <= 64, "one level per lane");
+    const double eig = d.eign[i];
+    const int kl = i < NZM ? i : NZM - 1;
+    const double ac_l = rhow[kl] / (adz[kl] * adzw[kl] * dz * dz);
+    const double cc_l = rhow[kl + 1] / (adz[kl] * adzw[kl + 1] * dz * dz);
+    const double rho_l = rho[kl];
+    auto lane_of = [](double v, int k) {
+      const long long b = __double_as_longlong(v);
+      const int lo = __builtin_amdgcn_readlane((int)b, k), hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+      return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    };
+    auto acoef = [&](int k) { return lane_of(ac_l, k); };
+    auto ccoef = [&](int k) { return lane_of(cc_l, k); };
+    double b;
+    if (i == 0)
+      b = 1.0 / (eig * rho[0] - acoef(0) - ccoef(0));
+    else
+      b = 1.0 / (eig * rho[0] - ccoef(0));
+    double alfa = -ccoef(0) * b;
+    ALF[i] = alfa;
+    double beta = P[i] * b;
+    P[i] = beta;
+    for (int k = 1; k < NZM - 1; k++) {
+      double ak = acoef(k), ck = ccoef(k);
+      double e = 1.0 / (eig * lane_of(rho_l, k) - ak - ck + ak * alfa);
+      alfa = -ck * e;
+      ALF[k * 64 + i] = alfa;
+      beta = (P[k * PLD + i] - ak * beta) * e;
+      P[k * PLD + i] = beta;
+    }
+    {
+      int k = NZM - 1;
+      double ak = acoef(k);
+      double top = (P[k * PLD + i] - ak * beta) / (eig * rho[k] - ak + ak * alfa);
+      P[k * PLD + i] = top;
+      for (int kk = NZM - 2; kk >= 0; kk--) {
+        top = ALF[kk * 64 + i] * top + P[kk * PLD + i];
+        P[kk * PLD + i] = top;
+      }
+    }
+  }
+  // 4. inverse transform (its leading barrier also ends the Thomas sweep)
+  transform(d.Gt);
+  // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities, from the
+  // registers of pass 1
+  const double dtdx = d.dtn / d.dx, dtdz = d.dtn / dz, dt3 = d.dt3na;
+  R* uan = d.dudt[na];
+  R* wan = d.dwdt[na];
+  R* unc = d.dudt[nc];
+  R* wnc = d.dwdt[nc];
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const int k = wave + NW * j;
+    if (k < NZM) {
+      size_t o = ((size_t)c * NZM + k) * NX + i;
+      size_t ow = ((size_t)c * nz + k) * NX + i;
+      double pk = P[k * PLD + i];
+      double pim = s.rot(pk, -1);
+      double pkb = P[(k > 0 ? k - 1 : 0) * PLD + i];
+      double rdz = 1. / (dz * adzw[k]);
+      double du_a = cua[j] - (pk - pim) * rdx;
+      double dw_a = cwa[j] - (pk - pkb) * rdz;
+      double uu = cu[j], ww = cw[j];
+      double un = uu + dt3 * (d.at * du_a + cub[j]);
+      double wn = ww + dt3 * (d.at * dw_a + cwb[j]);
+      uan[o] = du_a;
+      wan[ow] = dw_a;
+      unc[o] = un;
+      wnc[ow] = wn;
+      d.u[o] = 0.5 * (uu + un) * (rho[k] * dtdx);
+      d.w[ow] = 0.5 * (ww + wn) * (rhow[k] * dtdz);
+    }
+  }
+  if (wave == 0) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
+}
+
 template <int NZM>
 constexpr size_t press64_lds_bytes() {
   return (size_t)(64 * PLD + NZM * 64) * sizeof(double);

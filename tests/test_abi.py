@@ -20,15 +20,17 @@ def _declared():
     names = set()
     for h in HEADERS:
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
-        names |= set(re.findall(r"\b(crm_(?:hip|mlp)_\w+)\s*\(", text))
+        names |= set(re.findall(r"\b(crm_(?:hip|mlp|hist)_\w+)\s*\(", text))
     return names
 
 
 def test_headers_declare_the_abi():
     import crm_glue
+    import crm_history
     import crm_mlp
     names = _declared()
-    bound = set(crm_hip.ABI_SYMBOLS) | set(crm_hip.DEBUG_SYMBOLS) | set(crm_mlp.SYMBOLS) | set(crm_glue.SYMBOLS)
+    bound = (set(crm_hip.ABI_SYMBOLS) | set(crm_hip.DEBUG_SYMBOLS) | set(crm_mlp.SYMBOLS) | set(crm_glue.SYMBOLS)
+             | set(crm_history.SYMBOLS))
     assert bound == names
 
 

@@ -121,40 +121,68 @@ def _domain_means(c):
     return np.concatenate([c.out["sltend"][:n].ravel() / 1004.64, c.out["qltend"][:n].ravel() * 2.5e6 / 1004.64])
 
 
-@pytest.fixture(scope="module")
-def trained():
+# (nx, nz, dx, dt) of the two shapes the network is trained and judged in: the 32 x 28 test
+# CRM and the 64 x 58 CRM of the learned-microphysics workload
+SHAPES = {"32x28": (32, 28, 4000.0, 20.0), "64x58": (64, 58, 1000.0, 5.0)}
+NTRAIN = {"32x28": 24, "64x58": 16}     # instances recorded
+NPAIRS = {"32x28": 2000000, "64x58": 4000000}
+NSTEPS = {"32x28": 6000, "64x58": 12000}  # Adam steps of 16384 pairs
+
+
+def _train(shape, seed):
     """A network trained on pairs the CRM records over one GCM step of convective columns."""
-    torch = _torch()
+    nx, nz, dx, dt = SHAPES[shape]
     hip = CrmHip(0)
     m = crm_mlp.CrmMlp(0)
     try:
-        case = convective_case(24, seed=92)
-        X, T, n = _record(hip, case.copy(), 1200.0, 2000000, 1)
+        case = convective_case(NTRAIN[shape], nx=nx, nz=nz, dx=dx, dt=dt, seed=seed)
+        npts = NTRAIN[shape] * nx * nz * int(1200.0 / dt)
+        stride = max(1, -(-npts // NPAIRS[shape]))
+        X, T, n = _record(hip, case.copy(), 1200.0, NPAIRS[shape], stride)
         m.set_norm(*crm_mlp.fit_norm(X, T))
         m.set_params(crm_mlp.init_params(7))
-        hist = crm_mlp.train(m, X, T, steps=6000, batch=16384, lr=2e-3)
-        print("trained on %d pairs, loss %.3e -> %.3e" % (n, hist[0], hist[-1]))
-        yield m, hist
+        hist = crm_mlp.train(m, X, T, steps=NSTEPS[shape], batch=16384, lr=2e-3)
+        print("%s: trained on %d pairs, loss %.3e -> %.3e" % (shape, n, hist[0], hist[-1]))
     finally:
         hip.close()
+    return m, hist
 
 
-def test_network_in_crm(trained):
+@pytest.fixture(scope="module")
+def trained():
+    m, hist = _train("32x28", 92)
+    yield m, hist
+    m.close()
+
+
+@pytest.fixture(scope="module")
+def trained64():
+    m, hist = _train("64x58", 95)
+    yield m, hist
+    m.close()
+
+
+def _check_network_in_crm(m, hist, shape, ncrms, seed):
     """The network in the CRM over a GCM step: total water and static energy budgets close to
-    round-off, condensate stays non-negative, and the domain-mean heating and moistening lie
-    far closer to the plain scheme's than with an untrained network of zero output."""
-    m, hist = trained
+    round-off, pointwise water kept exactly, condensate stays non-negative, and the domain-mean
+    heating and moistening lie far closer to the plain scheme's than with an untrained network
+    of zero output."""
+    nx, nz, dx, dt = SHAPES[shape]
     assert np.mean(hist[-50:]) < 0.05 * hist[0]
-    case = convective_case(24, seed=93)
+    case = convective_case(ncrms, nx=nx, nz=nz, dx=dx, dt=dt, seed=seed)
     hip = CrmHip(0)
     try:
         plain = case.copy()
         assert hip.debug_run(plain, 1200.0, 0) == 0
+        # the state convects: precipitation is produced and reaches the surface
+        assert np.abs(plain.out["qp_src"][:ncrms]).max() > 0
+        assert (plain.out["precc"] + plain.out["precl"])[:ncrms].max() > 0
+        assert plain.out["cltot"][:ncrms].max() > 0
         pw, ph = budget_residuals(case, plain, 1200.0, hip.debug_get)
         crm_mlp.attach(hip, m)
         # the network's step keeps the total water of every point: qt + qp before
         # micro_proc (phase 18) and after it (phase 20) of the same substep
-        npt = 24 * (case.nz + 1) * case.nx
+        npt = ncrms * (case.nz + 1) * case.nx
         assert hip.debug_run(case.copy(), case.dt, 18) == 0
         w18 = hip.debug_get("qt", npt) + hip.debug_get("qp", npt)
         assert hip.debug_run(case.copy(), case.dt, 20) == 0
@@ -171,11 +199,13 @@ def test_network_in_crm(trained):
         # (MICRO_SAM1MOM/cloud.F90:42) in cells whose vapour the network's step exhausted
         # (the fixer hands any deficit to the condensate, leaving zero vapour).  How many cells
         # that is depends on the trained network (training order follows the atomic
-        # accumulation of gradients): measured 1e-7 to 4.4e-6 of the column water; bounded
+        # accumulation of gradients): measured 1e-7 to 4.4e-6 of the column water at 32 x 28.
+        # Where the plain scheme itself loses water to that clipping (the 64 x 58 plume:
+        # 6e-4), the network may not lose more than twice as much.
         assert rh < 1e-12
-        assert rw < 2e-5
+        assert rw < max(2e-5, 2.0 * pw)
         for f in ("qt", "qp", "qn"):
-            assert net.state[f][:24].min() >= 0.0, f
+            assert net.state[f][:ncrms].min() >= 0.0, f
         # untrained: zero weights, zero rates (condensate and precipitation frozen)
         z = crm_mlp.CrmMlp(0)
         try:
@@ -192,30 +222,43 @@ def test_network_in_crm(trained):
         for name, c in (("trained", net), ("zero-output", zero)):
             d = _domain_means(c) - ref
             errs[name] = (np.abs(d).max() / np.abs(ref).max(), np.linalg.norm(d) / np.linalg.norm(ref))
-        print("domain-mean heating/moistening error vs plain (max, rms):", errs)
-        # measured 0.18-0.30 (max) and 0.13-0.26 (rms) for the trained network (training
-        # order varies with the atomic accumulation of gradients) against 0.97 / 0.75 without
+        print(shape, "domain-mean heating/moistening error vs plain (max, rms):", errs)
+        # 32 x 28: measured 0.18-0.30 (max) and 0.13-0.26 (rms) for the trained network
+        # (training order varies with the atomic accumulation of gradients) against 0.97 / 0.75
+        # without
         assert errs["trained"][0] < 0.5 * errs["zero-output"][0]
         assert errs["trained"][1] < 0.5 * errs["zero-output"][1]
     finally:
         hip.close()
 
 
-def test_envelope_fallback_is_the_plain_scheme(trained):
+def test_network_in_crm(trained):
+    m, hist = trained
+    _check_network_in_crm(m, hist, "32x28", 24, 93)
+
+
+def test_network_in_64x58_crm(trained64):
+    """The workload's shape: the 64-column input gather and the network branch of the
+    64-column microphysics kernel."""
+    m, hist = trained64
+    _check_network_in_crm(m, hist, "64x58", 8, 96)
+
+
+def _check_fallback(m, shape, ncrms, seed, dt_gl):
     """With an empty training envelope every point falls back to the plain scheme: the call
     is then bitwise the plain call."""
-    m, _ = trained
+    nx, nz, dx, dt = SHAPES[shape]
     z = crm_mlp.CrmMlp(0)
     hip = CrmHip(0)
     try:
         z.set_params(m.params())
         nm = m.norm
         z.set_norm(nm["mean"], nm["scale"], nm["oscale"], lo=np.full(9, 1e30), hi=np.full(9, -1e30))
-        case = convective_case(6, seed=94)
+        case = convective_case(ncrms, nx=nx, nz=nz, dx=dx, dt=dt, seed=seed)
         plain, fb = case.copy(), case.copy()
-        hip.run(plain, 200.0)
+        hip.run(plain, dt_gl)
         crm_mlp.attach(hip, z)
-        hip.run(fb, 200.0)
+        hip.run(fb, dt_gl)
         crm_mlp.attach(hip, None)
         for g in ("state", "out", "rad"):
             for k, v in getattr(plain, g).items():
@@ -223,3 +266,11 @@ def test_envelope_fallback_is_the_plain_scheme(trained):
     finally:
         hip.close()
         z.close()
+
+
+def test_envelope_fallback_is_the_plain_scheme(trained):
+    _check_fallback(trained[0], "32x28", 6, 94, 200.0)
+
+
+def test_envelope_fallback_64x58(trained64):
+    _check_fallback(trained64[0], "64x58", 5, 97, 50.0)
