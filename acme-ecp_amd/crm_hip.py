@@ -21,10 +21,13 @@ LIB_PATH = os.environ.get("CRM_HIP_LIB") or os.path.join(HERE, "lib", "libcrm_hi
 
 CRM_OK, CRM_ERR_ARG, CRM_ERR_CFL, CRM_ERR_NXRAD, CRM_ERR_DAMPING = 0, 1, 2, 3, 4
 CRM_ERR_NAN, CRM_ERR_DEVICE, CRM_ERR_NOMEM, CRM_ERR_UNSUPPORTED, CRM_ERR_ACCEL = 5, 6, 7, 8, 9
+CRM_ERR_SCRATCH = 10
+NOALLOC_SCRATCH_BYTES = 256 << 20
 PRECISION = {"fp64": 0, "mixed": 1}
 
 # Every function include/crm_hip.h declares.
-ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_accel_init",
+ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_run_device_noalloc",
+               "crm_hip_accel_init",
                "crm_hip_run_many", "crm_hip_set_schedule", "crm_hip_get_schedule",
                "crm_hip_autotune", "crm_hip_set_graphs", "crm_hip_get_stats", "crm_hip_finalize",
                "crm_hip_errstr"]
@@ -84,6 +87,7 @@ def load():
              C.POINTER(CrmOutputView)]
     lib.crm_hip_run.argtypes = views
     lib.crm_hip_run_device.argtypes = views
+    lib.crm_hip_run_device_noalloc.argtypes = views
     lib.crm_hip_debug_run.argtypes = views + [C.c_int]
     lib.crm_hip_debug_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
     lib.crm_hip_get_stats.argtypes = [P, C.POINTER(CrmHipStats)]
@@ -176,6 +180,15 @@ class CrmHip:
                         dcase.case.column_info(), dt_gl, dcase.views())
         if rc and check:
             raise CrmError(rc, "crm_hip_run_device")
+        return rc
+
+    def run_device_noalloc(self, dcase, dt_gl, check=True):
+        """crm() on a DeviceCase through the entry point that allocates nothing (every device
+        buffer of the call in the library's fixed per-GPU array)."""
+        rc = self._call(self.lib.crm_hip_run_device_noalloc, dcase.case.dims(),
+                        dcase.case.column_info(), dt_gl, dcase.views())
+        if rc and check:
+            raise CrmError(rc, "crm_hip_run_device_noalloc")
         return rc
 
     def accel_init(self, use_crm_accel, crm_accel_factor=0.0, crm_accel_uv=False):

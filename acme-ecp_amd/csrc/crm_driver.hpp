@@ -38,6 +38,9 @@ namespace {
 struct Transform {
   double *Ft = nullptr, *Gt = nullptr, *eign = nullptr;
 };
+struct HostTransform {
+  std::vector<double> Ft, Gt, eign;
+};
 
 enum { NSCAL = 17 };  // u t qt qp qn tabs qv qcl qci qpl qpi tk tkh tke dudt x3
 enum { NSCAL3 = 4 };  // 3D only: v dvdt x3
@@ -59,6 +62,9 @@ struct Engine : CrmEngine {
   int* gcol_dev = nullptr;
   int gcol_cap = 0;
   std::map<std::pair<int, double>, Transform> transforms;
+  // host copies of the operators, uploaded into the caller's memory by calls that allocate
+  // nothing (crm_hip_run_device_noalloc)
+  std::map<std::pair<int, double>, HostTransform> host_transforms;
   // staging of host views
   double* stage = nullptr;
   size_t stage_bytes = 0;
@@ -186,6 +192,16 @@ struct Engine : CrmEngine {
   }
   int autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
                crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) override;
+  int run_noalloc(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+                  crm_state_view* state, crm_rad_view* rad, crm_output_view* out, char* scratch,
+                  size_t scratch_bytes) override;
+};
+
+// Device memory of a call that allocates nothing: every buffer of the call is carved from
+// this caller-supplied region instead of the context's pool.
+struct CallScratch {
+  char* base;
+  size_t bytes;
 };
 
 // Start timing one launch of kernel class kid; returns the pair index or -1.
@@ -268,48 +284,46 @@ static void host_constants(DevT<R>& d) {
   d.z0c1 = (double)(3.14159f / 2.f - 3.f * std::log(two));
 }
 
+static size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Byte sizes of the blocks of the field pool for ncap instances (each block 256-byte aligned).
 template <typename R>
-static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr, bool y3) {
-  if (ncrms <= ctx->cap_ncrms && nx == ctx->cap_nx && nzm == ctx->cap_nzm && nxr == ctx->cap_nxr &&
-      y3 == ctx->cap_y3)
-    return CRM_OK;
-  int ncap = std::max(ncrms, nx == ctx->cap_nx && nzm == ctx->cap_nzm ? ctx->cap_ncrms : 0);
-  int nz = nzm + 1;
+static void pool_sizes(int ncap, int nx, int nzm, int nxr, bool y3, size_t sizes[9]) {
+  const int nz = nzm + 1;
   const size_t scal = (size_t)ncap * nzm * nx, wl = (size_t)ncap * nz * nx;
   constexpr bool mirror = !std::is_same<R, double>::value;
-  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  const size_t sizes[] = {NSCAL * scal * sizeof(R),                        // bulk scalar fields
-                          4 * wl * sizeof(R),                              // w and dwdt x3
-                          2 * (size_t)ncap * nx * sizeof(double),          // precsfc, precssfc
-                          (size_t)NPROF * ncap * nz * sizeof(double),      // profiles
-                          mirror ? (size_t)NPROF * ncap * nz * sizeof(R) : 0,
-                          (size_t)NINST * ncap * sizeof(double),
-                          (size_t)NCOEF * ncap * nz * sizeof(R),
-                          5 * (size_t)ncap * nzm * nxr * sizeof(R),        // radiation sums
-                          y3 ? NSCAL3 * scal * sizeof(R) : 0};             // v, dvdt x3
-  size_t bytes = 0;
-  for (size_t b : sizes) bytes += al(b);
-  ctx->drop_graphs();  // their kernel arguments name the old buffers
-  if (ctx->pool) HIPCHECK(hipFree(ctx->pool));
-  ctx->pool = nullptr;
-  if (hipMalloc(&ctx->pool, bytes) != hipSuccess) {
-    ctx->cap_ncrms = 0;
-    return CRM_ERR_NOMEM;
-  }
-  HIPCHECK(hipMemset(ctx->pool, 0, bytes));
-  ctx->pool_bytes = bytes;
-  ctx->cap_ncrms = ncap;
-  ctx->cap_nx = nx;
-  ctx->cap_nzm = nzm;
-  ctx->cap_nxr = nxr;
-  ctx->cap_y3 = y3;
-  char* p = ctx->pool;
+  sizes[0] = NSCAL * scal * sizeof(R);                        // bulk scalar fields
+  sizes[1] = 4 * wl * sizeof(R);                              // w and dwdt x3
+  sizes[2] = 2 * (size_t)ncap * nx * sizeof(double);          // precsfc, precssfc
+  sizes[3] = (size_t)NPROF * ncap * nz * sizeof(double);      // profiles
+  sizes[4] = mirror ? (size_t)NPROF * ncap * nz * sizeof(R) : 0;
+  sizes[5] = (size_t)NINST * ncap * sizeof(double);
+  sizes[6] = (size_t)NCOEF * ncap * nz * sizeof(R);
+  sizes[7] = 5 * (size_t)ncap * nzm * nxr * sizeof(R);        // radiation sums
+  sizes[8] = y3 ? NSCAL3 * scal * sizeof(R) : 0;              // v, dvdt x3
+}
+
+template <typename R>
+static size_t pool_bytes_for(int ncap, int nx, int nzm, int nxr, bool y3) {
+  size_t sizes[9], bytes = 0;
+  pool_sizes<R>(ncap, nx, nzm, nxr, y3, sizes);
+  for (size_t b : sizes) bytes += al256(b);
+  return bytes;
+}
+
+// Point the field pointers of d into the pool at p (laid out by pool_sizes).
+template <typename R>
+static void carve_pool(DevT<R>& d, char* p, int ncap, int nx, int nzm, int nxr, bool y3) {
+  size_t sizes[9];
+  pool_sizes<R>(ncap, nx, nzm, nxr, y3, sizes);
+  constexpr bool mirror = !std::is_same<R, double>::value;
+  const int nz = nzm + 1;
+  const size_t scal = (size_t)ncap * nzm * nx, wl = (size_t)ncap * nz * nx;
   auto take = [&](size_t b) {
     char* q = p;
-    p += al(b);
+    p += al256(b);
     return q;
   };
-  DevT<R>& d = ctx->dev;
   R* f = (R*)take(sizes[0]);
   R** sc[NSCAL] = {&d.u, &d.t, &d.qt, &d.qp, &d.qn, &d.tabs, &d.qv, &d.qcl, &d.qci,
                    &d.qpl, &d.qpi, &d.tk, &d.tkh, &d.tke, &d.dudt[0], &d.dudt[1], &d.dudt[2]};
@@ -333,6 +347,30 @@ static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr, b
   R* vf = (R*)take(sizes[8]);
   d.v = y3 ? vf : nullptr;
   for (int q = 0; q < 3; q++) d.dvdt[q] = y3 ? vf + (size_t)(q + 1) * scal : nullptr;
+}
+
+template <typename R>
+static int ensure_buffers(Engine<R>* ctx, int ncrms, int nx, int nzm, int nxr, bool y3) {
+  if (ncrms <= ctx->cap_ncrms && nx == ctx->cap_nx && nzm == ctx->cap_nzm && nxr == ctx->cap_nxr &&
+      y3 == ctx->cap_y3)
+    return CRM_OK;
+  int ncap = std::max(ncrms, nx == ctx->cap_nx && nzm == ctx->cap_nzm ? ctx->cap_ncrms : 0);
+  const size_t bytes = pool_bytes_for<R>(ncap, nx, nzm, nxr, y3);
+  ctx->drop_graphs();  // their kernel arguments name the old buffers
+  if (ctx->pool) HIPCHECK(hipFree(ctx->pool));
+  ctx->pool = nullptr;
+  if (hipMalloc(&ctx->pool, bytes) != hipSuccess) {
+    ctx->cap_ncrms = 0;
+    return CRM_ERR_NOMEM;
+  }
+  HIPCHECK(hipMemset(ctx->pool, 0, bytes));
+  ctx->pool_bytes = bytes;
+  ctx->cap_ncrms = ncap;
+  ctx->cap_nx = nx;
+  ctx->cap_nzm = nzm;
+  ctx->cap_nxr = nxr;
+  ctx->cap_y3 = y3;
+  carve_pool(ctx->dev, ctx->pool, ncap, nx, nzm, nxr, y3);
   return CRM_OK;
 }
 
@@ -356,6 +394,53 @@ static int get_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform& o
   HIPCHECK(hipMemcpy(t.eign, eign.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   ctx->transforms[key] = t;
   out = t;
+  return CRM_OK;
+}
+
+// Offsets of a call's buffers inside a CallScratch region: the field pool first (as
+// pool_sizes lays it out for exactly ncrms instances), then the transform operators, the
+// column ids, the call's radiative heating in the sweep layout and the batch-global maxima.
+struct ScratchPlan {
+  size_t pool, ft, gt, eign, gcol, qrad, gmax, total;
+};
+template <typename R>
+static ScratchPlan scratch_plan(int ncrms, int nxa, int nya, int nzm, int nxr) {
+  const int nx = nxa * nya;
+  ScratchPlan p{};
+  size_t o = pool_bytes_for<R>(ncrms, nx, nzm, nxr, nya > 1);
+  p.pool = o;
+  const size_t n = (size_t)nx;
+  p.ft = o;
+  o += al256(sizeof(double) * n * n);
+  p.gt = o;
+  o += al256(sizeof(double) * n * n);
+  p.eign = o;
+  o += al256(sizeof(double) * n);
+  p.gcol = o;
+  o += al256(sizeof(int) * (size_t)ncrms);
+  p.qrad = o;
+  o += al256(sizeof(double) * (size_t)ncrms * nzm * nxr);
+  p.gmax = o;
+  o += al256(sizeof(unsigned long long) * GMAX_WORDS);
+  p.total = o;
+  return p;
+}
+
+// The transform operators of (nx, ny, dx) copied into caller memory on stream s, from a host
+// copy the context keeps (built on the first call of a shape).
+template <typename R>
+static int upload_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform dst, hipStream_t s) {
+  auto key = std::make_pair(nx * 1000 + ny, dx);
+  auto it = ctx->host_transforms.find(key);
+  if (it == ctx->host_transforms.end()) {
+    HostTransform h;
+    crm::pressure_transform_host(nx, ny, dx, h.Ft, h.Gt, h.eign);
+    it = ctx->host_transforms.emplace(key, std::move(h)).first;
+  }
+  const size_t n = (size_t)nx * ny;
+  HIPCHECK(hipMemcpyAsync(dst.Ft, it->second.Ft.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(dst.Gt, it->second.Gt.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(dst.eign, it->second.eign.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
   return CRM_OK;
 }
 
@@ -476,20 +561,52 @@ static int validate(const crm_dims* dm) {
 template <typename R>
 static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* col, double dt_gl,
                     const crm_input_view* in, crm_state_view* st, crm_rad_view* rad, crm_output_view* out,
-                    int phase_stop) {
+                    int phase_stop, const CallScratch* sc = nullptr) {
   int rc = validate(dm);
   if (rc) return rc;
   const bool y3 = dm->ny > 1;
   const int nx = dm->nx * dm->ny, nzm = dm->nz, ncrms = dm->ncrms;  // nx: columns per instance
   const int nxr = dm->nx_rad * dm->ny_rad;
   // set-up (allocations, null-stream clears and uploads) excludes captures of other threads
+  hipStream_t s = ctx->stream;
   std::unique_lock<std::mutex> setup_lock(crm_device_mutex());
-  rc = ensure_buffers(ctx, ncrms, nx, nzm, nxr, y3);
-  if (rc) return rc;
   Transform tf;
-  rc = get_transform(ctx, dm->nx, dm->ny, dm->dx, tf);
-  if (rc) return rc;
-  DevT<R>& d = ctx->dev;
+  DevT<R> dscr{};  // the kernel arguments of a call in caller memory (sc)
+  unsigned long long* gmax = ctx->gmax;
+  int* gcol_dev = nullptr;
+  double* qrad_buf = nullptr;
+  // a call in caller memory runs plain launches without events: no graph or event is created
+  struct MaskRestore {
+    unsigned& m;
+    unsigned saved;
+    ~MaskRestore() { m = saved; }
+  } mask_restore{ctx->prof_mask, ctx->prof_mask};
+  if (sc) {
+    if (ctx->mlp || ctx->samp_x) return CRM_ERR_UNSUPPORTED;
+    const ScratchPlan pl = scratch_plan<R>(ncrms, dm->nx, dm->ny, nzm, nxr);
+    if (pl.total > sc->bytes) return CRM_ERR_SCRATCH;
+    ctx->prof_mask = 0;
+    char* b = sc->base;
+    carve_pool(dscr, b, ncrms, nx, nzm, nxr, y3);
+    tf.Ft = (double*)(b + pl.ft);
+    tf.Gt = (double*)(b + pl.gt);
+    tf.eign = (double*)(b + pl.eign);
+    gcol_dev = (int*)(b + pl.gcol);
+    qrad_buf = (double*)(b + pl.qrad);
+    gmax = (unsigned long long*)(b + pl.gmax);
+    // the region keeps the last call's contents: the pool and the maxima start from zero, as
+    // in a freshly allocated context
+    k_zero_u64<<<dim3(2048), dim3(256), 0, s>>>((unsigned long long*)b, pl.pool / 8);
+    k_zero_u64<<<dim3(1), dim3(256), 0, s>>>(gmax, GMAX_WORDS);
+    rc = upload_transform(ctx, dm->nx, dm->ny, dm->dx, tf, s);
+    if (rc) return rc;
+  } else {
+    rc = ensure_buffers(ctx, ncrms, nx, nzm, nxr, y3);
+    if (rc) return rc;
+    rc = get_transform(ctx, dm->nx, dm->ny, dm->dx, tf);
+    if (rc) return rc;
+  }
+  DevT<R>& d = sc ? dscr : ctx->dev;
   d.ncrms = ncrms;
   d.nzm = nzm;
   d.nz = nzm + 1;
@@ -502,9 +619,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   d.Ft = tf.Ft;
   d.Gt = tf.Gt;
   d.eign = tf.eign;
-  d.gmax = ctx->gmax;
+  d.gmax = gmax;
   host_constants(d);
-  hipStream_t s = ctx->stream;
   Launcher L{nx, nzm, ncrms, s, ctx->block};
   const dim3 b256(256);
   ctx->last_ok = false;
@@ -513,13 +629,16 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   TIMED(CRM_K_SETUP, (k_setup<R><<<L.grid_inst(64), dim3(64), 0, s>>>(d, *in)));
   TIMED(CRM_K_INIT, NX_DISPATCH(nx, (k_init<NXV, R><<<L.g(CRM_K_INIT), L.b(CRM_K_INIT), 0, s>>>(d, *in, *st, *out, dt_gl))));
   if (col && col->igstep <= 1) {
-    if (ctx->gcol_cap < ncrms) {
-      if (ctx->gcol_dev) HIPCHECK(hipFree(ctx->gcol_dev));
-      HIPCHECK(hipMalloc(&ctx->gcol_dev, sizeof(int) * ncrms));
-      ctx->gcol_cap = ncrms;
+    if (!sc) {
+      if (ctx->gcol_cap < ncrms) {
+        if (ctx->gcol_dev) HIPCHECK(hipFree(ctx->gcol_dev));
+        HIPCHECK(hipMalloc(&ctx->gcol_dev, sizeof(int) * ncrms));
+        ctx->gcol_cap = ncrms;
+      }
+      gcol_dev = ctx->gcol_dev;
     }
-    HIPCHECK(hipMemcpyAsync(ctx->gcol_dev, col->gcol, sizeof(int) * ncrms, hipMemcpyHostToDevice, s));
-    TIMED(CRM_K_SETPERTURB, NX_DISPATCH(nx, (k_setperturb<NXV, R><<<L.grid_inst(64), dim3(64), 0, s>>>(d, ctx->gcol_dev))));
+    HIPCHECK(hipMemcpyAsync(gcol_dev, col->gcol, sizeof(int) * ncrms, hipMemcpyHostToDevice, s));
+    TIMED(CRM_K_SETPERTURB, NX_DISPATCH(nx, (k_setperturb<NXV, R><<<L.grid_inst(64), dim3(64), 0, s>>>(d, gcol_dev))));
   }
   if constexpr (!std::is_same<R, double>::value) {
     // FP32 copies of the level profiles for the bulk kernels
@@ -530,18 +649,21 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   // ny_rad, nz) order, so every substep's k_pre reads it contiguously
   {
     const size_t nq = (size_t)ncrms * nzm * d.nx_rad;
-    if (ctx->qrad_cap < nq) {
-      if (ctx->qrad_t) HIPCHECK(hipFree(ctx->qrad_t));
-      ctx->qrad_t = nullptr;
-      ctx->qrad_cap = 0;
-      if (hipMalloc(&ctx->qrad_t, nq * sizeof(double)) != hipSuccess) return CRM_ERR_NOMEM;
-      ctx->qrad_cap = nq;
+    if (!sc) {
+      if (ctx->qrad_cap < nq) {
+        if (ctx->qrad_t) HIPCHECK(hipFree(ctx->qrad_t));
+        ctx->qrad_t = nullptr;
+        ctx->qrad_cap = 0;
+        if (hipMalloc(&ctx->qrad_t, nq * sizeof(double)) != hipSuccess) return CRM_ERR_NOMEM;
+        ctx->qrad_cap = nq;
+      }
+      qrad_buf = ctx->qrad_t;
     }
-    k_qrad_order<<<dim3((unsigned)((nq + 255) / 256)), b256, 0, s>>>(rad->qrad, ctx->qrad_t, ncrms, d.ld, nzm,
+    k_qrad_order<<<dim3((unsigned)((nq + 255) / 256)), b256, 0, s>>>(rad->qrad, qrad_buf, ncrms, d.ld, nzm,
                                                                      d.nx_rad);
   }
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipMemsetAsync(ctx->gmax + GMAX_NPREC, 0, 3 * sizeof(unsigned long long), s));
+  k_zero_u64<<<dim3(1), dim3(64), 0, s>>>(gmax + GMAX_NPREC, 3);
 
   if (!(dt_gl >= dm->dt) || !(dt_gl / dm->dt < 1.0e6)) return CRM_ERR_ARG;
   int nstop = (int)(dt_gl / dm->dt);
@@ -602,7 +724,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   // k_decide stops the replay at the first substep that needs more subcycles (every later
   // kernel returns at entry), and the loop below resumes that substep from the host's record
   // of its bookkeeping before it (slot rotation, buffer swaps, Adams-Bashforth history).
-  if (ctx->use_graphs == CRM_GRAPHS_CALL && !acc && !net_inputs && ctx->prof_mask == 0 && phase_stop == 0 &&
+  if (ctx->use_graphs == CRM_GRAPHS_CALL && !sc && !acc && !net_inputs && ctx->prof_mask == 0 && phase_stop == 0 &&
       nstop <= 2048) {
     struct Snap {
       int na, nb, nc;
@@ -610,8 +732,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       double dt3[3];
     };
     std::vector<Snap> snap((size_t)nstop + 1);
-    const double* qrad = ctx->qrad_t;
-    d.stop = ctx->gmax + GMAX_STOP;
+    const double* qrad = qrad_buf;
+    d.stop = gmax + GMAX_STOP;
     DevT<R> dk;  // the key: the call's starting arguments
     std::memcpy(&dk, &d, sizeof(d));
     std::string key = "call";
@@ -633,15 +755,15 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         abcoefs(n, dt3, na, nb, nc, d.at, d.bt, d.ct);
         d.dt3na = dt3[na];
         if (launch) {
-          k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_FLAGS, d.stop);
+          k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_FLAGS, d.stop);
           NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.g(CRM_K_KURANT), L.b(CRM_K_KURANT), 0, s>>>(d)));
-          k_decide<<<1, 64, 0, s>>>(ctx->gmax, n, ctx->debug_stop);
+          k_decide<<<1, 64, 0, s>>>(gmax, n, ctx->debug_stop);
           k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d);
           NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad)));
           NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
           NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na)));
           NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
-          k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL, d.stop);
+          k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_PCFL, d.stop);
           NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d)));
           NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d)));
         }
@@ -661,7 +783,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         }
       }
     };
-    HIPCHECK(hipMemsetAsync(ctx->gmax + GMAX_STOP, 0, sizeof(unsigned long long), s));
+    HIPCHECK(hipMemsetAsync(gmax + GMAX_STOP, 0, sizeof(unsigned long long), s));
     hipGraphExec_t exec = ctx->graph_get(key);
     if (!exec) {
       std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
@@ -678,7 +800,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     HIPCHECK(hipGraphLaunch(exec, s));
     ctx->graph_launches += nstop;
     unsigned long long stopped_at = 0;
-    HIPCHECK(hipMemcpyAsync(&stopped_at, ctx->gmax + GMAX_STOP, sizeof(stopped_at), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(&stopped_at, gmax + GMAX_STOP, sizeof(stopped_at), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     if (stopped_at == 0) {
       nstep = nstop;
@@ -699,7 +821,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     if (nstep >= nstop) {
       if (acc && !ceased) {
         unsigned long long flag = 0;
-        HIPCHECK(hipMemcpyAsync(&flag, ctx->gmax + 3, sizeof(flag), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(&flag, gmax + 3, sizeof(flag), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
         cease(flag, nstep);
       }
@@ -708,9 +830,9 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     nstep++;
     d.nstep = nstep;
     unsigned long long bits[GMAX_PCFL];
-    HIPCHECK(hipMemsetAsync(ctx->gmax + GMAX_FLAGS, 0, (GMAX_PCFL - GMAX_FLAGS) * sizeof(unsigned long long), s));
+    k_zero_u64<<<dim3(2), dim3(256), 0, s>>>(gmax + GMAX_FLAGS, GMAX_PCFL - GMAX_FLAGS);
     TIMED(CRM_K_KURANT, NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.g(CRM_K_KURANT), L.b(CRM_K_KURANT), 0, s>>>(d))));
-    HIPCHECK(hipMemcpyAsync(bits, ctx->gmax, sizeof(bits), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(bits, gmax, sizeof(bits), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     cease(bits[3], nstep - 1);
     // non-negative doubles order as their bit patterns
@@ -728,22 +850,22 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     const bool last = (nstep == nstop);
     // Steady substeps (one subcycle, constant Adams-Bashforth coefficients from the third
     // substep on) replay a captured HIP graph of the whole substep, statistics included.
-    if (ctx->use_graphs != CRM_GRAPHS_OFF && ncycle == 1 && nstep >= 3 && !acc && !net_inputs && ctx->prof_mask == 0 &&
-        phase_stop == 0) {
+    if (ctx->use_graphs != CRM_GRAPHS_OFF && !sc && ncycle == 1 && nstep >= 3 && !acc && !net_inputs &&
+        ctx->prof_mask == 0 && phase_stop == 0) {
       d.icycle = 1;
       d.dtn = dt;
       dt3[na] = d.dtn;
       d.dtfactor = 1.0;
       abcoefs(nstep, dt3, na, nb, nc, d.at, d.bt, d.ct);
       d.dt3na = dt3[na];
-      const double* qrad = ctx->qrad_t;
+      const double* qrad = qrad_buf;
       auto body = [&]() {
         k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d);
         NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad)));
         NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
         NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na)));
         NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
-        k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL, d.stop);
+        k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_PCFL, d.stop);
         NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d)));
         NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d)));
         std::swap(d.u, d.dudt[nc]);
@@ -800,7 +922,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       if (icyc == 1)
         TIMED(CRM_K_PRECIP_INIT,
               (k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d)));
-      const double* qrad = ctx->qrad_t;
+      const double* qrad = qrad_buf;
       TIMED(CRM_K_PRE, NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad))));
       if (last && phase_stop == 7) goto stopped;
       TIMED(CRM_K_SGS, NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d))));
@@ -809,7 +931,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       if (last && phase_stop == 11) goto stopped;
       TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
       if (last && phase_stop == 13) goto stopped;
-      k_clear_u64<<<1, 64, 0, s>>>(ctx->gmax + GMAX_PCFL, d.stop);
+      k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_PCFL, d.stop);
       TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d))));
       TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d))));
       if (last && phase_stop == 15) goto stopped;
@@ -843,7 +965,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   TIMED(CRM_K_FINAL, NX_DISPATCH(nx, (k_final<NXV, R><<<L.g(CRM_K_FINAL), L.b(CRM_K_FINAL), 0, s>>>(d, *in, *st, *rad, *out, dt_gl, nstop))));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(ctx->ev1, s));
-  HIPCHECK(hipMemcpyAsync(tail, ctx->gmax + GMAX_NPREC, sizeof(tail), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(tail, gmax + GMAX_NPREC, sizeof(tail), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
   err_flag = tail[1];
   ctx->stats.nprec_max = (int)tail[0];
@@ -1028,6 +1150,16 @@ int Engine<R>::run(const crm_dims* dims, const crm_column_info* col, double dt_g
   (void)hipSetDevice(device);
   if (device_views) return run_core(this, dims, col, dt_gl, in, state, rad, out, phase_stop);
   return run_host(this, dims, col, dt_gl, in, state, rad, out, phase_stop);
+}
+
+template <typename R>
+int Engine<R>::run_noalloc(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+                           crm_state_view* state, crm_rad_view* rad, crm_output_view* out, char* scratch,
+                           size_t scratch_bytes) {
+  if (!dims || !in || !state || !rad || !out || !scratch) return CRM_ERR_ARG;
+  (void)hipSetDevice(device);
+  const CallScratch sc{scratch, scratch_bytes};
+  return run_core(this, dims, col, dt_gl, in, state, rad, out, 0, &sc);
 }
 
 template <typename R>

@@ -30,6 +30,11 @@ struct CrmEngine {
   virtual int autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
                        crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) = 0;
   virtual hipStream_t get_stream() const = 0;
+  // One crm() call on device views whose every device buffer is carved from the caller's
+  // region [scratch, scratch + scratch_bytes): plain launches, no graphs, no events.
+  virtual int run_noalloc(const crm_dims* dims, const crm_column_info* col, double dt_gl,
+                          const crm_input_view* in, crm_state_view* state, crm_rad_view* rad,
+                          crm_output_view* out, char* scratch, size_t scratch_bytes) = 0;
 };
 
 // Serialises stream captures with the library's null-stream work (buffer set-up, operator

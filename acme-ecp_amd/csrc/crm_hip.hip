@@ -1,6 +1,7 @@
 // crm_hip.hip -- the C ABI of include/crm_hip.h and include/crm_hip_debug.h, and the FP64
 // engine (crm_driver.hpp instantiated with FP64 bulk fields).  The mixed-precision engine is
 // compiled separately (crm_hip_mixed.hip); a context owns one engine of its precision.
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -54,6 +55,33 @@ struct crm_hip_ctx {
   CrmEngine* eng = nullptr;
 };
 
+// ---- device memory of crm_hip_run_device_noalloc ------------------------------------------
+// One fixed array per GPU holds every device buffer of a call of the entry point that
+// allocates nothing.  In host code the array's name is only a stand-in: its device address on
+// each GPU is looked up once per device by crm_hip_init (hipGetSymbolAddress with that device
+// current) and kept in noalloc_base[device]; every later use goes through that address.
+static __device__ __align__(256) char scratch[CRM_HIP_NOALLOC_SCRATCH_BYTES];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::atomic<char*> noalloc_base[NOALLOC_MAX_DEVICES];
+static std::atomic<int> noalloc_state[NOALLOC_MAX_DEVICES];  // 0 not looked up, 1 ok, -1 failed
+
+// Called with `device` current.
+static void noalloc_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lock(crm_device_mutex());
+  if (noalloc_state[device].load() != 0) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    fprintf(stderr, "crm_hip: device %d: address of the no-allocation scratch array: %s\n", device,
+            hipGetErrorString(e));
+    noalloc_state[device].store(-1);
+    return;
+  }
+  noalloc_base[device].store((char*)ptr);
+  noalloc_state[device].store(1);
+}
+
 std::mutex& crm_device_mutex() {
   static std::mutex m;
   return m;
@@ -83,6 +111,7 @@ int crm_hip_init(crm_hip_ctx** out, const crm_hip_config* cfg) {
     delete ctx;
     return rc;
   }
+  noalloc_setup(ctx->device);  // the engine has made ctx->device current
   *out = ctx;
   return CRM_OK;
 }
@@ -97,6 +126,15 @@ int crm_hip_run_device(crm_hip_ctx* ctx, const crm_dims* dims, const crm_column_
                        const crm_input_view* in, crm_state_view* state, crm_rad_view* rad, crm_output_view* out) {
   if (!ctx) return CRM_ERR_ARG;
   return ctx->eng->run(dims, col, dt_gl, in, state, rad, out, true, 0);
+}
+
+int crm_hip_run_device_noalloc(crm_hip_ctx* ctx, const crm_dims* dims, const crm_column_info* col, double dt_gl,
+                               const crm_input_view* in, crm_state_view* state, crm_rad_view* rad,
+                               crm_output_view* out) {
+  if (!ctx || !ctx->eng || ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) return CRM_ERR_ARG;
+  if (noalloc_state[ctx->device].load() != 1) return CRM_ERR_DEVICE;
+  return ctx->eng->run_noalloc(dims, col, dt_gl, in, state, rad, out, noalloc_base[ctx->device].load(),
+                               CRM_HIP_NOALLOC_SCRATCH_BYTES);
 }
 
 int crm_hip_accel_init(crm_hip_ctx* ctx, int use_crm_accel, double crm_accel_factor, int crm_accel_uv) {
@@ -219,6 +257,7 @@ const char* crm_hip_errstr(int code) {
     case CRM_ERR_NOMEM: return "device allocation failed";
     case CRM_ERR_UNSUPPORTED: return "size class or option not provided by this build";
     case CRM_ERR_ACCEL: return "bad crm_accel_factor and nstop pair: (1+crm_accel_factor) does not divide nstop";
+    case CRM_ERR_SCRATCH: return "batch too large for the fixed device memory of crm_hip_run_device_noalloc";
     default: return "unknown error";
   }
 }

@@ -34,6 +34,13 @@ __global__ void k_clear_u64(unsigned long long* p, const unsigned long long* sto
   if (threadIdx.x < KURANT_SLOTS) p[threadIdx.x * KURANT_STRIDE] = 0ull;
 }
 
+// Zeroes n 8-byte words (a kernel rather than a memset, so that the call may sit in a graph a
+// caller captures); any grid, grid-stride.
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, size_t n) {
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x)
+    p[j] = 0ull;
+}
+
 // The subcycle decision of kurant.F90:53-75 on the device, for a captured call: one subcycle
 // when cfl / 0.7 <= 1; anything else (more subcycles, the reference's abort, a non-finite
 // value) stops the captured call at this substep for the host to take over; so does

@@ -12,7 +12,7 @@ module crm_hip_bindings
 
   integer(c_int), parameter, public :: CRM_OK = 0, CRM_ERR_ARG = 1, CRM_ERR_CFL = 2, &
        CRM_ERR_NXRAD = 3, CRM_ERR_DAMPING = 4, CRM_ERR_NAN = 5, CRM_ERR_DEVICE = 6, &
-       CRM_ERR_NOMEM = 7, CRM_ERR_UNSUPPORTED = 8, CRM_ERR_ACCEL = 9
+       CRM_ERR_NOMEM = 7, CRM_ERR_UNSUPPORTED = 8, CRM_ERR_ACCEL = 9, CRM_ERR_SCRATCH = 10
   integer(c_int), parameter, public :: CRM_PREC_FP64 = 0, CRM_PREC_MIXED = 1
 
   type, bind(C), public :: crm_dims
@@ -72,7 +72,8 @@ module crm_hip_bindings
     integer(c_int) :: accel_ceased
   end type
 
-  public :: crm_hip_init, crm_hip_run, crm_hip_run_device, crm_hip_accel_init, crm_hip_get_stats, &
+  public :: crm_hip_init, crm_hip_run, crm_hip_run_device, crm_hip_run_device_noalloc, &
+            crm_hip_accel_init, crm_hip_get_stats, &
             crm_hip_finalize, crm_hip_errstr, crm_hip_message
   public :: crm_hip_gcm_input, crm_hip_gcm_flux_bypass, crm_hip_gcm_tend, crm_hip_check_energy_chng
 
@@ -99,6 +100,22 @@ module crm_hip_bindings
 
     integer(c_int) function crm_hip_run_device(ctx, dims, col, dt_gl, inp, state, rad, out) &
         bind(C, name="crm_hip_run_device")
+      import :: c_int, c_ptr, c_double, crm_dims, crm_column_info, crm_input_view, &
+                crm_state_view, crm_rad_view, crm_output_view
+      type(c_ptr), value :: ctx
+      type(crm_dims), intent(in) :: dims
+      type(crm_column_info), intent(in) :: col
+      real(c_double), value :: dt_gl
+      type(crm_input_view), intent(in) :: inp
+      type(crm_state_view), intent(inout) :: state
+      type(crm_rad_view), intent(inout) :: rad
+      type(crm_output_view), intent(inout) :: out
+    end function
+
+    ! crm_hip_run_device for callers that cannot allocate device memory: every device buffer of
+    ! the call comes from the library's fixed per-GPU array (no counterpart in the reference)
+    integer(c_int) function crm_hip_run_device_noalloc(ctx, dims, col, dt_gl, inp, state, rad, out) &
+        bind(C, name="crm_hip_run_device_noalloc")
       import :: c_int, c_ptr, c_double, crm_dims, crm_column_info, crm_input_view, &
                 crm_state_view, crm_rad_view, crm_output_view
       type(c_ptr), value :: ctx

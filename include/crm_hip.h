@@ -45,7 +45,8 @@ enum {
   CRM_ERR_DEVICE = 6,     /* HIP runtime error                                              */
   CRM_ERR_NOMEM = 7,      /* device allocation failed                                       */
   CRM_ERR_UNSUPPORTED = 8, /* configuration not provided by this build                      */
-  CRM_ERR_ACCEL = 9       /* 1 + crm_accel_factor does not divide nstop (accelerate_crm.F90:78-106) */
+  CRM_ERR_ACCEL = 9,      /* 1 + crm_accel_factor does not divide nstop (accelerate_crm.F90:78-106) */
+  CRM_ERR_SCRATCH = 10    /* batch needs more than CRM_HIP_NOALLOC_SCRATCH_BYTES (noalloc entry) */
 };
 
 /* Arithmetic of the bulk stencils. The pressure solve (DFT + Thomas sweep), the hydrostatic
@@ -159,6 +160,25 @@ int crm_hip_run(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *c
 int crm_hip_run_device(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col,
                        double dt_gl, const crm_input_view *in, crm_state_view *state,
                        crm_rad_view *rad, crm_output_view *out);
+
+/* No counterpart in the reference: crm_hip_run_device for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as crm_hip_run_device, and it calls no hipMalloc, hipFree or other
+ * allocating routine: every device buffer of the call (fields, level profiles, transform
+ * operators, the call's radiative heating, the batch-global maxima) is carved from one fixed
+ * array of CRM_HIP_NOALLOC_SCRATCH_BYTES per GPU that the library holds until the process ends.
+ * It runs plain launches on the context's stream (no graphs, no kernel timing) and rejects an
+ * attached microphysics network or sample recording (CRM_ERR_UNSUPPORTED). A batch whose
+ * buffers do not fit returns CRM_ERR_SCRATCH before anything is launched: about 0.85 MB per
+ * 64 x 58 instance in FP64 (about 300 instances), 0.2 MB per 32 x 28 instance; the mixed
+ * precision form needs about half. Every call on one GPU shares that GPU's array, and each
+ * call clears what it needs: calls that overlap on one GPU (from several contexts or threads)
+ * race and must be serialised by the caller. The first call of a size class builds its
+ * transform operators on the host (host memory only). */
+#define CRM_HIP_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int crm_hip_run_device_noalloc(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col,
+                               double dt_gl, const crm_input_view *in, crm_state_view *state,
+                               crm_rad_view *rad, crm_output_view *out);
 
 /* Mean-state acceleration settings of a context, the replacement of crm_accel_init
  * (accelerate_crm.F90:30-38,45-75, which reads them from the physics namelist): with

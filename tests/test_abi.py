@@ -49,11 +49,11 @@ def test_product_path_fails_loudly_without_library(monkeypatch, tmp_path):
 
 def test_status_strings():
     seen = set()
-    for code in range(9):
+    for code in range(11):
         s = crm_hip.errstr(code)
         assert s and s != "unknown error"
         seen.add(s)
-    assert len(seen) == 9
+    assert len(seen) == 11
     assert crm_hip.errstr(99) == "unknown error"
     assert "16" in crm_hip.errstr(crm_hip.CRM_ERR_CFL)
 
