@@ -27,8 +27,9 @@ PRECISION = {"fp64": 0, "mixed": 1}
 
 # Every function include/crm_hip.h declares.
 ABI_SYMBOLS = ["crm_hip_init", "crm_hip_run", "crm_hip_run_device", "crm_hip_run_device_noalloc",
-               "crm_hip_accel_init",
+               "crm_hip_accel_init", "crm_hip_set_coriolis",
                "crm_hip_run_many", "crm_hip_set_schedule", "crm_hip_get_schedule",
+               "crm_hip_set_variants", "crm_hip_get_variants",
                "crm_hip_autotune", "crm_hip_set_graphs", "crm_hip_get_stats", "crm_hip_finalize",
                "crm_hip_errstr"]
 # include/crm_hip_debug.h
@@ -92,9 +93,12 @@ def load():
     lib.crm_hip_debug_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
     lib.crm_hip_get_stats.argtypes = [P, C.POINTER(CrmHipStats)]
     lib.crm_hip_accel_init.argtypes = [P, C.c_int, C.c_double, C.c_int]
+    lib.crm_hip_set_coriolis.argtypes = [P, C.c_int]
     IA = C.c_int * len(KERNELS)
     lib.crm_hip_set_schedule.argtypes = [P, IA]
     lib.crm_hip_get_schedule.argtypes = [P, IA]
+    lib.crm_hip_set_variants.argtypes = [P, IA]
+    lib.crm_hip_get_variants.argtypes = [P, IA]
     lib.crm_hip_autotune.argtypes = views + [IA]
     lib.crm_hip_set_graphs.argtypes = [P, C.c_int]
     lib.crm_hip_debug_stop_call_at.argtypes = [P, C.c_int]
@@ -198,6 +202,12 @@ class CrmHip:
         if rc:
             raise CrmError(rc, "crm_hip_accel_init")
 
+    def set_coriolis(self, on):
+        """The reference's docoriolis switch (params.F90:107) for later calls (3D CRMs)."""
+        rc = self.lib.crm_hip_set_coriolis(self.ctx, int(bool(on)))
+        if rc:
+            raise CrmError(rc, "crm_hip_set_coriolis")
+
     def schedule(self):
         """{kernel class: threads per workgroup} of this context."""
         a = (C.c_int * len(KERNELS))()
@@ -211,6 +221,20 @@ class CrmHip:
         rc = self.lib.crm_hip_set_schedule(self.ctx, a)
         if rc:
             raise CrmError(rc, "crm_hip_set_schedule")
+
+    def variants(self):
+        """{kernel class: launch variant} of this context."""
+        a = (C.c_int * len(KERNELS))()
+        self.lib.crm_hip_get_variants(self.ctx, a)
+        return dict(zip(KERNELS, list(a)))
+
+    def set_variants(self, variants):
+        cur = self.variants()
+        cur.update(variants)
+        a = (C.c_int * len(KERNELS))(*[cur[k] for k in KERNELS])
+        rc = self.lib.crm_hip_set_variants(self.ctx, a)
+        if rc:
+            raise CrmError(rc, "crm_hip_set_variants")
 
     def autotune(self, dcase, dt_gl):
         """Tune the workgroup size of every kernel class on a DeviceCase (advanced in place by

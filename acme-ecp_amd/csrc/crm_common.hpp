@@ -113,6 +113,7 @@ enum Prof {
   P_V0,                                                    // (3D only)
   P_TTEND, P_QTEND, P_UTEND,                               // large-scale tendencies
   P_VTEND,                                                 // (3D only)
+  P_UG0, P_VG0,                                            // large-scale wind (Coriolis, 3D)
   P_UWLE, P_UWSB, P_MKWLE1, P_MKWLE2, P_MKWSB1, P_MKWSB2,  // last-substep fluxes
   P_VWLE, P_VWSB,                                          // (3D only)
   P_MKADV1, P_MKADV2, P_MKDIFF1, P_MKDIFF2, P_QPSRC, P_QPEVP,
@@ -125,7 +126,7 @@ enum Prof {
 // Per-instance scalars, each [ncrms].
 enum Inst {
   I_DZ = 0, I_NDAMP, I_KMIN, I_KMAX, I_Z0, I_COLPREC, I_COLPRECS, I_CLTOT, I_CLHGH, I_CLMED,
-  I_CLLOW, NINST
+  I_CLLOW, I_FCOR, I_FCORZ, NINST
 };
 
 // Terminal-velocity coefficients of precipitation (microphysics.F90:499-504).
@@ -147,6 +148,7 @@ struct DevT {
   int ny;                     // 1 for 2D; 8 for the 3D 8 x 8 CRM (column index 8 j + i)
   double dx, dt, dtn, dt3na, at, bt, ct, dtfactor;
   int nstep, icycle;
+  int docoriolis;             // Coriolis terms (params.F90:107; 3D only here)
   // prognostic and diagnostic fields [c][k][i]
   R *u, *w, *t, *qt, *qp, *qn, *tabs, *qv, *qcl, *qci, *qpl, *qpi;
   R *tk, *tkh, *tke;

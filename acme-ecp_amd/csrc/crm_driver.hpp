@@ -44,6 +44,7 @@ struct HostTransform {
 
 enum { NSCAL = 17 };  // u t qt qp qn tabs qv qcl qci qpl qpi tk tkh tke dudt x3
 enum { NSCAL3 = 4 };  // 3D only: v dvdt x3
+constexpr int MAX_VARIANTS = 4;  // launch variants of a kernel class (variants_of)
 
 }  // namespace
 
@@ -52,6 +53,7 @@ struct Engine : CrmEngine {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_ready = nullptr;  // end of a call's set-up, for a joint capture
   // device state sized for (cap_ncrms, cap_nx, cap_nzm, cap_nxr, cap_y3); cap_nx counts the
   // columns of an instance (nx * ny) and cap_nxr its radiation columns (nx_rad * ny_rad)
   int cap_ncrms = 0, cap_nx = 0, cap_nzm = 0, cap_nxr = 0;
@@ -61,6 +63,9 @@ struct Engine : CrmEngine {
   unsigned long long* gmax = nullptr;
   int* gcol_dev = nullptr;
   int gcol_cap = 0;
+  double* lat_dev = nullptr;  // latitudes of the call's instances (Coriolis)
+  int lat_cap = 0;
+  int coriolis = 0;           // the reference's docoriolis switch (params.F90:107)
   std::map<std::pair<int, double>, Transform> transforms;
   // host copies of the operators, uploaded into the caller's memory by calls that allocate
   // nothing (crm_hip_run_device_noalloc)
@@ -85,6 +90,8 @@ struct Engine : CrmEngine {
   // launch schedule: threads per workgroup of each segment-kernel class (a multiple of 64 up
   // to 256; instances per workgroup = block / columns), set by the autotuner
   int block[CRM_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256};
+  // launch variant of each kernel class (variants_of), set by the autotuner
+  int variant[CRM_K_COUNT] = {};
   // HIP graphs of the steady substep (one subcycle, constant Adams-Bashforth coefficients),
   // keyed by the kernel arguments: the buffer rotation of the Adams-Bashforth scheme repeats
   // every 12 substeps, so a call reuses at most 12 graphs, and later calls reuse them too
@@ -143,6 +150,9 @@ struct Engine : CrmEngine {
   int run(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
           crm_state_view* state, crm_rad_view* rad, crm_output_view* out, bool device_views,
           int phase_stop) override;
+  int run_joint(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+                crm_state_view* state, crm_rad_view* rad, crm_output_view* out, JointCapture* jc,
+                int slot) override;
   int debug_get(const char* name, double* buf) override;
   void get_stats(crm_hip_stats* s) const override { *s = stats; }
   int profile_kernels(unsigned mask) override;
@@ -164,11 +174,25 @@ struct Engine : CrmEngine {
   void get_schedule(int* blk) const override {
     for (int k = 0; k < CRM_K_COUNT; k++) blk[k] = block[k];
   }
+  int set_variants(const int* v) override {
+    for (int k = 0; k < CRM_K_COUNT; k++)
+      if (v[k] < 0 || v[k] >= MAX_VARIANTS) return CRM_ERR_ARG;
+    for (int k = 0; k < CRM_K_COUNT; k++) variant[k] = v[k];
+    drop_graphs();
+    return CRM_OK;
+  }
+  void get_variants(int* v) const override {
+    for (int k = 0; k < CRM_K_COUNT; k++) v[k] = variant[k];
+  }
   void set_graphs(int mode) override {
     use_graphs = mode;
     drop_graphs();
   }
   void set_debug_stop(int nstep) override { debug_stop = nstep; }
+  void set_coriolis(int on) override {
+    coriolis = on ? 1 : 0;
+    drop_graphs();
+  }
   long long graph_count() const override { return graph_launches; }
   hipStream_t get_stream() const override { return stream; }
   int set_mlp(const crm_mlp* m) override {
@@ -401,7 +425,7 @@ static int get_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform& o
 // pool_sizes lays it out for exactly ncrms instances), then the transform operators, the
 // column ids, the call's radiative heating in the sweep layout and the batch-global maxima.
 struct ScratchPlan {
-  size_t pool, ft, gt, eign, gcol, qrad, gmax, total;
+  size_t pool, ft, gt, eign, gcol, lat, qrad, gmax, total;
 };
 template <typename R>
 static ScratchPlan scratch_plan(int ncrms, int nxa, int nya, int nzm, int nxr) {
@@ -418,6 +442,8 @@ static ScratchPlan scratch_plan(int ncrms, int nxa, int nya, int nzm, int nxr) {
   o += al256(sizeof(double) * n);
   p.gcol = o;
   o += al256(sizeof(int) * (size_t)ncrms);
+  p.lat = o;
+  o += al256(sizeof(double) * (size_t)ncrms);
   p.qrad = o;
   o += al256(sizeof(double) * (size_t)ncrms * nzm * nxr);
   p.gmax = o;
@@ -445,13 +471,43 @@ static int upload_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform
 }
 
 // ---- kernel dispatch over the compile-time column count ------------------------------------
+// Launch variants ("tile shapes") the autotuner chooses among, per kernel class: the pressure
+// solve of 2D 64-column CRMs has four (0: one instance per 4-wave workgroup, two workgroups per
+// CU, the gradient pass reading its inputs again; 1: one instance on 8 waves with every level's
+// inputs of the gradient pass kept in registers, one workgroup per CU; 2 and 3: variant 0 with
+// two or four levels of each pass unrolled together), MPDATA two (the default or the other
+// number of levels per trip).  Results are bitwise the same in every variant of a class.
+static int variants_of(int kid, int nx, bool y3) {
+  if (kid == CRM_K_PRESS) return (nx == 64 && !y3) ? 4 : 1;
+  if (kid == CRM_K_MPDATA) return 2;
+  return 1;
+}
+
 template <int NX, typename R, typename G>
-static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, const DevT<R>& d, int na, int nb, int nc) {
-  if constexpr (NX == 64 && !G::Y3) {  // matrix-core transforms; one instance per 8-wave block
-    if (nzm == 28)
-      k_press64r<28, R><<<dim3((unsigned)d.ncrms), dim3(512), press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
-    else
-      k_press64r<58, R><<<dim3((unsigned)d.ncrms), dim3(512), press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, const DevT<R>& d, int na, int nb, int nc,
+                         int variant) {
+  if constexpr (NX == 64 && !G::Y3) {  // matrix-core transforms
+    if (variant == 0) {  // one instance per 4-wave block
+      if (nzm == 28)
+        k_press64<28, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+      else
+        k_press64<58, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+    } else if (variant == 2) {  // the same, two levels per trip
+      if (nzm == 28)
+        k_press64<28, R, G, 2><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+      else
+        k_press64<58, R, G, 2><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+    } else if (variant == 3) {  // the same, four levels per trip
+      if (nzm == 28)
+        k_press64<28, R, G, 4><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+      else
+        k_press64<58, R, G, 4><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+    } else {  // one instance per 8-wave block, inputs of the gradient pass in registers
+      if (nzm == 28)
+        k_press64r<28, R><<<dim3((unsigned)d.ncrms), dim3(512), press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+      else
+        k_press64r<58, R><<<dim3((unsigned)d.ncrms), dim3(512), press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+    }
   } else if constexpr (NX == 64) {  // 3D 8 x 8: one instance per 4-wave block
     if (nzm == 28)
       k_press64<28, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
@@ -468,6 +524,15 @@ static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, cons
     else
       k_press<NX, 58, R><<<g, b, sh, s>>>(d, na, nb, nc);
   }
+}
+
+template <int NX, typename R, typename G>
+static void launch_mpdata(dim3 g, dim3 b, hipStream_t s, const DevT<R>& d, int variant) {
+  constexpr bool PAIR = (NX == 64 && !G::Y3);
+  if (variant == 1)
+    k_mpdata<NX, R, G, !PAIR><<<g, b, 0, s>>>(d);
+  else
+    k_mpdata<NX, R, G, PAIR><<<g, b, 0, s>>>(d);
 }
 
 struct Launcher {
@@ -561,7 +626,21 @@ static int validate(const crm_dims* dm) {
 template <typename R>
 static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* col, double dt_gl,
                     const crm_input_view* in, crm_state_view* st, crm_rad_view* rad, crm_output_view* out,
-                    int phase_stop, const CallScratch* sc = nullptr) {
+                    int phase_stop, const CallScratch* sc = nullptr, JointCapture* jc = nullptr,
+                    int slot = -1) {
+  // as a class of a joint capture: arrive exactly once, without a sequence on every path that
+  // does not replay a whole-call graph (so the last class to arrive is never left waiting)
+  struct Arrival {
+    JointCapture* jc;
+    int slot;
+    bool done = false;
+    int alone() {
+      if (!jc || done) return CRM_OK;
+      done = true;
+      return jc->arrive(slot, false, nullptr, nullptr, std::string(), nullptr);
+    }
+    ~Arrival() { (void)alone(); }
+  } arrival{jc, slot};
   int rc = validate(dm);
   if (rc) return rc;
   const bool y3 = dm->ny > 1;
@@ -574,7 +653,11 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   DevT<R> dscr{};  // the kernel arguments of a call in caller memory (sc)
   unsigned long long* gmax = ctx->gmax;
   int* gcol_dev = nullptr;
+  double* lat_buf = nullptr;
   double* qrad_buf = nullptr;
+  // Coriolis terms (coriolis.F90) need the instances' latitudes; in 2D they would also need the
+  // v the 2D form does not carry
+  if (ctx->coriolis && (dm->ny == 1 || !col || !col->lat)) return dm->ny == 1 ? CRM_ERR_UNSUPPORTED : CRM_ERR_ARG;
   // a call in caller memory runs plain launches without events: no graph or event is created
   struct MaskRestore {
     unsigned& m;
@@ -592,6 +675,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     tf.Gt = (double*)(b + pl.gt);
     tf.eign = (double*)(b + pl.eign);
     gcol_dev = (int*)(b + pl.gcol);
+    lat_buf = (double*)(b + pl.lat);
     qrad_buf = (double*)(b + pl.qrad);
     gmax = (unsigned long long*)(b + pl.gmax);
     // the region keeps the last call's contents: the pool and the maxima start from zero, as
@@ -620,13 +704,27 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   d.Gt = tf.Gt;
   d.eign = tf.eign;
   d.gmax = gmax;
+  d.docoriolis = ctx->coriolis;
   host_constants(d);
   Launcher L{nx, nzm, ncrms, s, ctx->block};
   const dim3 b256(256);
   ctx->last_ok = false;
   ctx->stats = crm_hip_stats{};
 
-  TIMED(CRM_K_SETUP, (k_setup<R><<<L.grid_inst(64), dim3(64), 0, s>>>(d, *in)));
+  if (ctx->coriolis) {
+    if (!sc) {
+      if (ctx->lat_cap < ncrms) {
+        if (ctx->lat_dev) HIPCHECK(hipFree(ctx->lat_dev));
+        ctx->lat_dev = nullptr;
+        ctx->lat_cap = 0;
+        if (hipMalloc(&ctx->lat_dev, sizeof(double) * ncrms) != hipSuccess) return CRM_ERR_NOMEM;
+        ctx->lat_cap = ncrms;
+      }
+      lat_buf = ctx->lat_dev;
+    }
+    HIPCHECK(hipMemcpyAsync(lat_buf, col->lat, sizeof(double) * ncrms, hipMemcpyHostToDevice, s));
+  }
+  TIMED(CRM_K_SETUP, (k_setup<R><<<L.grid_inst(64), dim3(64), 0, s>>>(d, *in, ctx->coriolis ? lat_buf : nullptr)));
   TIMED(CRM_K_INIT, NX_DISPATCH(nx, (k_init<NXV, R><<<L.g(CRM_K_INIT), L.b(CRM_K_INIT), 0, s>>>(d, *in, *st, *out, dt_gl))));
   if (col && col->igstep <= 1) {
     if (!sc) {
@@ -743,6 +841,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     key.append((const char*)dt3, sizeof(dt3));
     key.append((const char*)&qrad, sizeof(qrad));
     key.append((const char*)ctx->block, sizeof(ctx->block));
+    key.append((const char*)ctx->variant, sizeof(ctx->variant));
     // the call's sequence; with launch false only the host bookkeeping is replayed
     auto sequence = [&](bool launch) {
       for (int n = 1; n <= nstop; n++) {
@@ -762,9 +861,9 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
           NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad)));
           NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
           NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na)));
-          NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
+          NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc, ctx->variant[CRM_K_PRESS])));
           k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_PCFL, d.stop);
-          NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d)));
+          NX_DISPATCH(nx, (launch_mpdata<NXV, R, GEO>(L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), s, d, ctx->variant[CRM_K_MPDATA])));
           NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d)));
         }
         std::swap(d.u, d.dudt[nc]);
@@ -784,20 +883,27 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       }
     };
     HIPCHECK(hipMemsetAsync(gmax + GMAX_STOP, 0, sizeof(unsigned long long), s));
-    hipGraphExec_t exec = ctx->graph_get(key);
-    if (!exec) {
-      std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
-      hipGraph_t graph = nullptr;
-      HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      sequence(true);
-      HIPCHECK(hipStreamEndCapture(s, &graph));
-      HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
-      ctx->graph_put(key, exec);
+    if (jc) {  // one graph with the other classes of crm_hip_run_many
+      HIPCHECK(hipEventRecord(ctx->ev_ready, s));
+      arrival.done = true;
+      rc = jc->arrive(slot, true, s, ctx->ev_ready, key, [&](bool launch) { sequence(launch); });
+      if (rc) return rc;
     } else {
-      sequence(false);
+      hipGraphExec_t exec = ctx->graph_get(key);
+      if (!exec) {
+        std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
+        hipGraph_t graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        sequence(true);
+        HIPCHECK(hipStreamEndCapture(s, &graph));
+        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        ctx->graph_put(key, exec);
+      } else {
+        sequence(false);
+      }
+      HIPCHECK(hipGraphLaunch(exec, s));
     }
-    HIPCHECK(hipGraphLaunch(exec, s));
     ctx->graph_launches += nstop;
     unsigned long long stopped_at = 0;
     HIPCHECK(hipMemcpyAsync(&stopped_at, gmax + GMAX_STOP, sizeof(stopped_at), hipMemcpyDeviceToHost, s));
@@ -817,6 +923,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
     if (nstep >= 1) ctx->stats.ncycle_max = std::max(ctx->stats.ncycle_max, 1);
     d.stop = nullptr;
   }
+  rc = arrival.alone();
+  if (rc) return rc;
   while (true) {
     if (nstep >= nstop) {
       if (acc && !ceased) {
@@ -864,9 +972,9 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad)));
         NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
         NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na)));
-        NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc)));
+        NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc, ctx->variant[CRM_K_PRESS])));
         k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_PCFL, d.stop);
-        NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d)));
+        NX_DISPATCH(nx, (launch_mpdata<NXV, R, GEO>(L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), s, d, ctx->variant[CRM_K_MPDATA])));
         NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d)));
         std::swap(d.u, d.dudt[nc]);
         std::swap(d.w, d.dwdt[nc]);
@@ -888,6 +996,8 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       key.append((const char*)ints, sizeof(ints));
       key.append((const char*)&qrad, sizeof(qrad));
       key.append((const char*)ctx->block, sizeof(ctx->block));
+      key.append((const char*)ctx->variant, sizeof(ctx->variant));
+    key.append((const char*)ctx->variant, sizeof(ctx->variant));
       hipGraphExec_t exec = ctx->graph_get(key);
       if (!exec) {
         std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
@@ -929,10 +1039,10 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
       if (last && phase_stop == 9) goto stopped;
       TIMED(CRM_K_MOM, NX_DISPATCH(nx, (k_mom<NXV, R, GEO><<<L.g(CRM_K_MOM), L.b(CRM_K_MOM), 0, s>>>(d, na))));
       if (last && phase_stop == 11) goto stopped;
-      TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc))));
+      TIMED(CRM_K_PRESS, NX_DISPATCH(nx, (launch_press<NXV, R, GEO>(nzm, gpress, b256, press_lds, s, d, na, nb, nc, ctx->variant[CRM_K_PRESS]))));
       if (last && phase_stop == 13) goto stopped;
       k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_PCFL, d.stop);
-      TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (k_mpdata<NXV, R, GEO><<<L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), 0, s>>>(d))));
+      TIMED(CRM_K_MPDATA, NX_DISPATCH(nx, (launch_mpdata<NXV, R, GEO>(L.g(CRM_K_MPDATA), L.b(CRM_K_MPDATA), s, d, ctx->variant[CRM_K_MPDATA]))));
       TIMED(CRM_K_PRECIP, NX_DISPATCH(nx, (k_precip<NXV, R><<<L.g(CRM_K_PRECIP), L.b(CRM_K_PRECIP), 0, s>>>(d))));
       if (last && phase_stop == 15) goto stopped;
       std::swap(d.u, d.dudt[nc]);
@@ -1136,8 +1246,10 @@ Engine<R>::~Engine() {
   if (stage) (void)hipFree(stage);
   if (gmax) (void)hipFree(gmax);
   if (gcol_dev) (void)hipFree(gcol_dev);
+  if (lat_dev) (void)hipFree(lat_dev);
   for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   if (ev0) (void)hipEventDestroy(ev0);
+  if (ev_ready) (void)hipEventDestroy(ev_ready);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -1150,6 +1262,22 @@ int Engine<R>::run(const crm_dims* dims, const crm_column_info* col, double dt_g
   (void)hipSetDevice(device);
   if (device_views) return run_core(this, dims, col, dt_gl, in, state, rad, out, phase_stop);
   return run_host(this, dims, col, dt_gl, in, state, rad, out, phase_stop);
+}
+
+template <typename R>
+int Engine<R>::run_joint(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
+                         crm_state_view* state, crm_rad_view* rad, crm_output_view* out, JointCapture* jc,
+                         int slot) {
+  if (!dims || !in || !state || !rad || !out) {
+    if (jc) (void)jc->arrive(slot, false, nullptr, nullptr, std::string(), nullptr);
+    return CRM_ERR_ARG;
+  }
+  (void)hipSetDevice(device);
+  if (!ev_ready && hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming) != hipSuccess) {
+    if (jc) (void)jc->arrive(slot, false, nullptr, nullptr, std::string(), nullptr);
+    return CRM_ERR_DEVICE;
+  }
+  return run_core(this, dims, col, dt_gl, in, state, rad, out, 0, nullptr, jc, slot);
 }
 
 template <typename R>
@@ -1266,25 +1394,46 @@ int Engine<R>::debug_get(const char* name, double* buf) {
 template <typename R>
 int Engine<R>::autotune(const crm_dims* dims, const crm_column_info* col, double dt_gl, const crm_input_view* in,
                         crm_state_view* state, crm_rad_view* rad, crm_output_view* out, int* best) {
+  // every workgroup size (64, 128, 256) with every launch variant; each trial call times
+  // every kernel class
   static const int cand[] = {64, 128, 256};
   const unsigned saved_mask = prof_mask;
-  double t[3][CRM_K_COUNT] = {};
-  for (int c = 0; c < 3; c++) {
-    for (int k = 0; k < CRM_K_COUNT; k++) block[k] = cand[c];
-    drop_graphs();
-    profile_kernels((1u << CRM_K_COUNT) - 1);
-    int rc = run(dims, col, dt_gl, in, state, rad, out, true, 0);
-    if (rc) {
-      profile_kernels(saved_mask);
-      return rc;
+  const bool y3 = dims && dims->ny > 1;
+  const int nxc = dims ? dims->nx * dims->ny : 0;
+  constexpr int NV = MAX_VARIANTS;
+  double t[3][NV][CRM_K_COUNT] = {};
+  for (int v = 0; v < NV; v++) {
+    for (int c = 0; c < 3; c++) {
+      bool any = false;  // a class whose variant v exists
+      for (int k = 0; k < CRM_K_COUNT; k++) {
+        block[k] = cand[c];
+        const int nv = variants_of(k, nxc, y3);
+        variant[k] = v < nv ? v : 0;
+        any |= v < nv;
+      }
+      if (!any) continue;
+      drop_graphs();
+      profile_kernels((1u << CRM_K_COUNT) - 1);
+      int rc = run(dims, col, dt_gl, in, state, rad, out, true, 0);
+      if (rc) {
+        profile_kernels(saved_mask);
+        for (int k = 0; k < CRM_K_COUNT; k++) variant[k] = 0;
+        return rc;
+      }
+      for (int k = 0; k < CRM_K_COUNT; k++)
+        if (v < variants_of(k, nxc, y3)) t[c][v][k] = kt_n[k] ? kt_ms[k] / (double)kt_n[k] : 0.0;
     }
-    for (int k = 0; k < CRM_K_COUNT; k++) t[c][k] = kt_n[k] ? kt_ms[k] / (double)kt_n[k] : 0.0;
   }
   for (int k = 0; k < CRM_K_COUNT; k++) {
-    int bc = 2;  // 256 unless a smaller workgroup measured faster
-    for (int c = 0; c < 3; c++)
-      if (t[c][k] > 0.0 && t[c][k] < t[bc][k]) bc = c;
+    int bc = 2, bv = 0;  // 256 threads and the default variant unless another measured faster
+    for (int v = 0; v < variants_of(k, nxc, y3); v++)
+      for (int c = 0; c < 3; c++)
+        if (t[c][v][k] > 0.0 && t[c][v][k] < t[bc][bv][k]) {
+          bc = c;
+          bv = v;
+        }
     block[k] = cand[bc];
+    variant[k] = bv;
     if (best) best[k] = block[k];
   }
   drop_graphs();

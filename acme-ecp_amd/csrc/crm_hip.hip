@@ -2,6 +2,8 @@
 // engine (crm_driver.hpp instantiated with FP64 bulk fields).  The mixed-precision engine is
 // compiled separately (crm_hip_mixed.hip); a context owns one engine of its precision.
 #include <atomic>
+#include <list>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -144,6 +146,102 @@ int crm_hip_accel_init(crm_hip_ctx* ctx, int use_crm_accel, double crm_accel_fac
 
 static_assert(CRM_HIP_NKERNEL == CRM_K_COUNT, "kernel class count");
 
+// ---- one graph for several classes (crm_hip_run_many) ---------------------------------------
+namespace {
+// Captured joint graphs, keyed by the classes' own whole-call keys in class order; least
+// recently used first out.  Replayed only under joint_mutex, so no evicted graph is running.
+std::mutex joint_mutex;
+std::map<std::string, hipGraphExec_t> joint_graphs;
+std::list<std::string> joint_lru;
+constexpr size_t JOINT_GRAPHS_MAX = 8;
+}  // namespace
+
+int JointCapture::arrive(int slot, bool joins, hipStream_t stream, hipEvent_t ready, std::string key,
+                         std::function<void(bool)> sequence) {
+  std::unique_lock<std::mutex> lk(m);
+  Part& p = parts[slot];
+  p.joins = joins;
+  p.stream = stream;
+  p.ready = ready;
+  p.key = std::move(key);
+  p.sequence = std::move(sequence);
+  arrived++;
+  if (arrived == (int)parts.size()) {
+    rc = launch_locked();
+    released = true;
+    cv.notify_all();
+    return rc;
+  }
+  if (!joins) return CRM_OK;  // runs on its own; nothing of it is in the joint graph
+  cv.wait(lk, [&] { return released; });
+  return rc;
+}
+
+int JointCapture::launch_locked() {
+  std::string key;
+  int njoin = 0;
+  for (size_t i = 0; i < parts.size(); i++) {
+    const Part& p = parts[i];
+    if (!p.joins) continue;
+    njoin++;
+    const size_t len = p.key.size();
+    key.append((const char*)&i, sizeof(i));
+    key.append((const char*)&len, sizeof(len));
+    key.append(p.key);
+  }
+  if (njoin == 0) return CRM_OK;
+  if (hipSetDevice(device) != hipSuccess) return CRM_ERR_DEVICE;
+  std::lock_guard<std::mutex> jl(joint_mutex);
+  hipGraphExec_t exec = nullptr;
+  auto it = joint_graphs.find(key);
+  if (it != joint_graphs.end()) {
+    exec = it->second;
+    joint_lru.remove(key);
+    joint_lru.push_front(key);
+    for (Part& p : parts)
+      if (p.joins) p.sequence(false);
+  } else {
+    // capture every class's sequence on its own stream, forked from and joined into master
+    std::lock_guard<std::mutex> capture_lock(crm_device_mutex());
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(master, hipStreamCaptureModeThreadLocal) != hipSuccess) return CRM_ERR_DEVICE;
+    bool ok = hipEventRecord(fork, master) == hipSuccess;
+    for (size_t i = 0; i < parts.size() && ok; i++) {
+      Part& p = parts[i];
+      if (!p.joins) continue;
+      ok = hipStreamWaitEvent(p.stream, fork, 0) == hipSuccess;
+      if (!ok) break;
+      p.sequence(true);
+      ok = hipEventRecord(join[i], p.stream) == hipSuccess && hipStreamWaitEvent(master, join[i], 0) == hipSuccess;
+    }
+    const hipError_t e = hipStreamEndCapture(master, &graph);
+    if (!ok || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return CRM_ERR_DEVICE;
+    }
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return CRM_ERR_DEVICE;
+    while (joint_graphs.size() >= JOINT_GRAPHS_MAX && !joint_lru.empty()) {
+      auto old = joint_graphs.find(joint_lru.back());
+      if (old != joint_graphs.end()) {
+        (void)hipGraphExecDestroy(old->second);
+        joint_graphs.erase(old);
+      }
+      joint_lru.pop_back();
+    }
+    joint_graphs[key] = exec;
+    joint_lru.push_front(key);
+  }
+  // the graph runs after every class's set-up, and every class continues after the graph
+  for (Part& p : parts)
+    if (p.joins && hipStreamWaitEvent(master, p.ready, 0) != hipSuccess) return CRM_ERR_DEVICE;
+  if (hipGraphLaunch(exec, master) != hipSuccess || hipEventRecord(done, master) != hipSuccess) return CRM_ERR_DEVICE;
+  for (Part& p : parts)
+    if (p.joins && hipStreamWaitEvent(p.stream, done, 0) != hipSuccess) return CRM_ERR_DEVICE;
+  return CRM_OK;
+}
+
 int crm_hip_run_many(int n, crm_hip_ctx* const* ctxs, const crm_class_desc* classes, double dt_gl, int* status) {
   if (n < 1 || !ctxs || !classes) return CRM_ERR_ARG;
   for (int i = 0; i < n; i++) {
@@ -152,20 +250,48 @@ int crm_hip_run_many(int n, crm_hip_ctx* const* ctxs, const crm_class_desc* clas
       if (ctxs[j] == ctxs[i]) return CRM_ERR_ARG;
   }
   std::vector<int> rcs(n, CRM_OK);
+  // The classes' whole-call graphs are captured together into one graph when every context is
+  // on one device; each class still runs its set-up, any resumed substeps and its outputs on
+  // its own stream and host thread.
+  bool one_device = true;
+  for (int i = 1; i < n; i++) one_device &= ctxs[i]->device == ctxs[0]->device;
+  JointCapture jc;
+  bool joint = one_device && hipSetDevice(ctxs[0]->device) == hipSuccess;
+  if (joint) {
+    jc.device = ctxs[0]->device;
+    jc.parts.resize(n);
+    jc.join.assign(n, nullptr);
+    joint = hipStreamCreate(&jc.master) == hipSuccess &&
+            hipEventCreateWithFlags(&jc.fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&jc.done, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < n && joint; i++) joint = hipEventCreateWithFlags(&jc.join[i], hipEventDisableTiming) == hipSuccess;
+  }
   auto one = [&](int i) {
     if (hipSetDevice(ctxs[i]->device) != hipSuccess) {
+      if (joint) (void)jc.arrive(i, false, nullptr, nullptr, std::string(), nullptr);
       rcs[i] = CRM_ERR_DEVICE;
       return;
     }
     crm_class_desc c = classes[i];
-    rcs[i] = ctxs[i]->eng->run(&c.dims, &c.col, dt_gl, &c.in, &c.state, &c.rad, &c.out, true, 0);
+    if (joint)
+      rcs[i] = ctxs[i]->eng->run_joint(&c.dims, &c.col, dt_gl, &c.in, &c.state, &c.rad, &c.out, &jc, i);
+    else
+      rcs[i] = ctxs[i]->eng->run(&c.dims, &c.col, dt_gl, &c.in, &c.state, &c.rad, &c.out, true, 0);
   };
-  // one host thread per class: each drives its own stream, so the classes' kernels overlap on
-  // the device while every class keeps its per-substep host decisions
+  // one host thread per class: each drives its own stream, so the classes' set-up and any
+  // host-driven substeps overlap on the device
   std::vector<std::thread> th;
   for (int i = 1; i < n; i++) th.emplace_back(one, i);
   one(0);
   for (auto& t : th) t.join();
+  if (jc.master) {
+    (void)hipStreamSynchronize(jc.master);
+    (void)hipStreamDestroy(jc.master);
+  }
+  for (hipEvent_t e : jc.join)
+    if (e) (void)hipEventDestroy(e);
+  if (jc.fork) (void)hipEventDestroy(jc.fork);
+  if (jc.done) (void)hipEventDestroy(jc.done);
   int first = CRM_OK;
   for (int i = 0; i < n; i++) {
     if (status) status[i] = rcs[i];
@@ -182,6 +308,23 @@ int crm_hip_set_schedule(crm_hip_ctx* ctx, const int block[CRM_HIP_NKERNEL]) {
 int crm_hip_get_schedule(const crm_hip_ctx* ctx, int block[CRM_HIP_NKERNEL]) {
   if (!ctx || !ctx->eng || !block) return CRM_ERR_ARG;
   ctx->eng->get_schedule(block);
+  return CRM_OK;
+}
+
+int crm_hip_set_coriolis(crm_hip_ctx* ctx, int docoriolis) {
+  if (!ctx || !ctx->eng) return CRM_ERR_ARG;
+  ctx->eng->set_coriolis(docoriolis);
+  return CRM_OK;
+}
+
+int crm_hip_set_variants(crm_hip_ctx* ctx, const int variant[CRM_HIP_NKERNEL]) {
+  if (!ctx || !ctx->eng || !variant) return CRM_ERR_ARG;
+  return ctx->eng->set_variants(variant);
+}
+
+int crm_hip_get_variants(const crm_hip_ctx* ctx, int variant[CRM_HIP_NKERNEL]) {
+  if (!ctx || !ctx->eng || !variant) return CRM_ERR_ARG;
+  ctx->eng->get_variants(variant);
   return CRM_OK;
 }
 

@@ -73,9 +73,19 @@ __device__ double z0_est(double z, double bflx, double wnd, double ustar, double
 }
 
 template <typename R>
-__global__ void k_setup(DevT<R> d, crm_input_view in) {
+__global__ void k_setup(DevT<R> d, crm_input_view in, const double* lat) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.ncrms) return;
+  if (lat) {
+    // crm_module.F90:287,321-322: latitude in degrees from radians (57.296_r8), then the
+    // Coriolis parameters with the REAL(4) pi and literals of the reference build
+    const double pi = (double)3.141592653589793f;
+    const double lat0 = lat[c] * 57.296;
+    const double fcor = 4 * pi / 86400. * sin(lat0 * pi / 180.);
+    const double om = 2 * pi / (3600. * 24.);
+    inst(d, I_FCOR, c) = fcor;
+    inst(d, I_FCORZ, c) = sqrt(4. * (om * om) - fcor * fcor);
+  }
   const int ld = d.ld, plev = d.plev, nzm = d.nzm;
   auto IN = [&](const double* p, int l) { return p[(size_t)c + (size_t)ld * l]; };
   double* z = prof(d, P_Z, c);
@@ -315,6 +325,8 @@ __global__ void CRM_LB k_init(DevT<R> d, crm_input_view in, crm_state_view st, c
                                   FAC_FUS_D * IN(in.qiil) - t00) * idt_gl;
         prof(d, P_QTEND, c)[k] = (IN(in.ql) + IN(in.qccl) + IN(in.qiil) - q0) * idt_gl;
         prof(d, P_UTEND, c)[k] = (uln - u0) * idt_gl;
+        prof(d, P_UG0, c)[k] = uln;  // crm_module.F90:576-577
+        if (y3) prof(d, P_VG0, c)[k] = fmin(umax, fmax(-umax, IN(in.vl)));
       }
     }
   }
@@ -1062,6 +1074,15 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
       dv = dv - (fy - s.roty(fy, -1));
       dv = dv - (fvv_n - fvv_k) * rhoi_k;
     }
+    if (d.docoriolis) {  // coriolis.F90:15-31, after advect_mom and sgs_mom (crm_module.F90:825)
+      const R fcor = inst(d, I_FCOR, c), fcorz = inst(d, I_FCORZ, c);
+      const R w_m = s.rot(w_k, -1);
+      const R v_av = R(0.25) * (v_k + v_jp + v_m + v_m_jp);
+      const R w_av = R(0.25) * (w_n + wn_m + w_k + w_m);
+      du = du + fcor * (v_av - (R)prof(d, P_VG0, c)[k]) - fcorz * w_av;
+      const R u_av = R(0.25) * (u_k + u_p + u_jm + u_p_jm);
+      dv = dv - R(0.5) * (fcor + fcor) * (u_av - (R)prof(d, P_UG0, c)[k]);
+    }
     if (live) {
       dudt[o] = du;
       dvdt[o] = dv;
@@ -1418,7 +1439,9 @@ __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PLD = 66;  // slab row stride in doubles: conflict-free A-fragment reads
 
-template <int NZM, typename R, typename G = Seg<64>>
+// UNR: levels of the right-hand side and gradient passes unrolled together, so that their
+// loads are in flight at once (a launch variant the autotuner may choose)
+template <int NZM, typename R, typename G = Seg<64>, int UNR = 1>
 __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   CRM_STOP_CHECK(d);
   constexpr int NX = 64, NK = 64;
@@ -1440,8 +1463,33 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   const R *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
   const R *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
   const int r16 = lane & 15, q4 = lane >> 4;
-  // 1. right-hand side (rows >= NZM of the slab are zero)
-  for (int k = wave; k < NK; k += 4) {
+  // 1. right-hand side (rows >= NZM of the slab are zero).  The loads of UNR levels are
+  // issued before any of them is used (2D), so that they are in flight together.
+  static_assert(UNR == 1 || !G::Y3, "levels per trip: 2D only");
+  for (int k0 = wave; k0 < NK; k0 += 4 * UNR) {
+    double ld_[UNR][12];
+#pragma unroll
+    for (int q = 0; q < UNR; q++) {
+      const int k = k0 + 4 * q;
+      if (!G::Y3 && k < NZM) {
+        const size_t o = ((size_t)c * NZM + k) * NX + i, ow = ((size_t)c * nz + k) * NX + i, own = ow + NX;
+        ld_[q][0] = d.u[o];
+        ld_[q][1] = ua[o];
+        ld_[q][2] = ub[o];
+        ld_[q][3] = uc[o];
+        ld_[q][4] = d.w[ow];
+        ld_[q][5] = wa[ow];
+        ld_[q][6] = wb[ow];
+        ld_[q][7] = wc[ow];
+        ld_[q][8] = d.w[own];
+        ld_[q][9] = wa[own];
+        ld_[q][10] = wb[own];
+        ld_[q][11] = wc[own];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < UNR; q++) {
+    const int k = k0 + 4 * q;
     double pv = 0.0;
     if (k < NZM) {
       size_t o = ((size_t)c * NZM + k) * NX + i;
@@ -1450,22 +1498,24 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
       double rdz = 1. / (adz[k] * dz);
       double rup = rhow[k + 1] / rho[k] * rdz;
       double rdn = rhow[k] / rho[k] * rdz;
-      double uu = d.u[o], uap = ua[o], ubp = ub[o], ucp = uc[o];
       if constexpr (G::Y3) {  // press_rhs.F90:45-69
+        double uu = d.u[o], uap = ua[o], ubp = ub[o], ucp = uc[o];
         double vv = d.v[o], vap = d.dvdt[na][o], vbp = d.dvdt[nb][o], vcp = d.dvdt[nc][o];
         pv = (rdx * (s.rot(uu, 1) - uu) + rdx * (s.roty(vv, 1) - vv) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
              (rdx * (s.rot(uap, 1) - uap) + rdx * (s.roty(vap, 1) - vap) + (wa[own] * rup - wa[ow] * rdn)) +
              btat * (rdx * (s.rot(ubp, 1) - ubp) + rdx * (s.roty(vbp, 1) - vbp) + (wb[own] * rup - wb[ow] * rdn)) +
              ctat * (rdx * (s.rot(ucp, 1) - ucp) + rdx * (s.roty(vcp, 1) - vcp) + (wc[own] * rup - wc[ow] * rdn));
       } else {
-        pv = (rdx * (s.rot(uu, 1) - uu) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
-             (rdx * (s.rot(uap, 1) - uap) + (wa[own] * rup - wa[ow] * rdn)) +
-             btat * (rdx * (s.rot(ubp, 1) - ubp) + (wb[own] * rup - wb[ow] * rdn)) +
-             ctat * (rdx * (s.rot(ucp, 1) - ucp) + (wc[own] * rup - wc[ow] * rdn));
+        const double uu = ld_[q][0], uap = ld_[q][1], ubp = ld_[q][2], ucp = ld_[q][3];
+        pv = (rdx * (s.rot(uu, 1) - uu) + (ld_[q][8] * rup - ld_[q][4] * rdn)) * dta +
+             (rdx * (s.rot(uap, 1) - uap) + (ld_[q][9] * rup - ld_[q][5] * rdn)) +
+             btat * (rdx * (s.rot(ubp, 1) - ubp) + (ld_[q][10] * rup - ld_[q][6] * rdn)) +
+             ctat * (rdx * (s.rot(ucp, 1) - ucp) + (ld_[q][11] * rup - ld_[q][7] * rdn));
       }
       pv = pv * rho[k];
     }
-    P[k * PLD + i] = pv;
+    if (k < NK) P[k * PLD + i] = pv;
+    }
   }
   // one 64x64x64 transform of the slab in place: P <- P * M (M row-major 64x64)
   auto transform = [&](const double* M) {
@@ -1547,18 +1597,40 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   R* wan = d.dwdt[na];
   R* unc = d.dudt[nc];
   R* wnc = d.dwdt[nc];
-  for (int k = wave; k < NZM; k += 4) {
+  for (int k0 = wave; k0 < NZM; k0 += 4 * UNR) {
+    double lg_[UNR][8];
+#pragma unroll
+    for (int q = 0; q < UNR; q++) {
+      const int k = k0 + 4 * q;
+      if (k < NZM) {
+        const size_t o = ((size_t)c * NZM + k) * NX + i, ow = ((size_t)c * nz + k) * NX + i;
+        lg_[q][0] = ua[o];
+        lg_[q][1] = wa[ow];
+        lg_[q][2] = d.u[o];
+        lg_[q][3] = d.w[ow];
+        lg_[q][4] = ub[o];
+        lg_[q][5] = uc[o];
+        lg_[q][6] = wb[ow];
+        lg_[q][7] = wc[ow];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < UNR; q++) {
+    const int k = k0 + 4 * q;
+    if (k >= NZM) break;
     size_t o = ((size_t)c * NZM + k) * NX + i;
     size_t ow = ((size_t)c * nz + k) * NX + i;
     double pk = P[k * PLD + i];
     double pim = s.rot(pk, -1);
     double pkb = P[(k > 0 ? k - 1 : 0) * PLD + i];
     double rdz = 1. / (dz * adzw[k]);
-    double du_a = ua[o] - (pk - pim) * rdx;
-    double dw_a = wa[ow] - (pk - pkb) * rdz;
-    double uu = d.u[o], ww = d.w[ow];
-    double un = uu + dt3 * (d.at * du_a + d.bt * ub[o] + d.ct * uc[o]);
-    double wn = ww + dt3 * (d.at * dw_a + d.bt * wb[ow] + d.ct * wc[ow]);
+    double du_a = lg_[q][0] - (pk - pim) * rdx;
+    double dw_a = lg_[q][1] - (pk - pkb) * rdz;
+    double uu = lg_[q][2], ww = lg_[q][3];
+    // the b and c terms of the Adams-Bashforth step are summed first, as k_press64r keeps
+    // that sum in registers: both kernels give the same bits
+    double un = uu + dt3 * (d.at * du_a + (d.bt * lg_[q][4] + d.ct * lg_[q][5]));
+    double wn = ww + dt3 * (d.at * dw_a + (d.bt * lg_[q][6] + d.ct * lg_[q][7]));
     if constexpr (G::Y3) {  // press_grad.F90:29, adams.F90:29,32
       double pjm = s.roty(pk, -1);
       double dv_a = d.dvdt[na][o] - (pk - pjm) * rdx;
@@ -1574,6 +1646,7 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
     wnc[ow] = wn;
     d.u[o] = 0.5 * (uu + un) * (rho[k] * dtdx);
     d.w[ow] = 0.5 * (ww + wn) * (rhow[k] * dtdz);
+    }
   }
   if (wave == 0) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
 }
@@ -1799,7 +1872,9 @@ __device__ __forceinline__ T term_vel(const TermVel& tv, T q, T rho, T tabs) {
 // iteration.  Statistics: fadv (advect_scalar.F90:62-72) and the vertical flux of
 // advect_scalar2D.F90:113-118,204-206.  With want_cfl, also the precipitation Courant number
 // of precip_fall (microphysics.F90:596-611) on the advected field.
-template <int NX, typename R, typename G>
+// PAIR: the steady state advances two levels per trip (the launch variant the autotuner may
+// choose; the default for the 2D 64-column segment)
+template <int NX, typename R, typename G, bool PAIR>
 __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f, double* fadv,
                            double* flux, bool want_cfl) {
   constexpr bool Y3 = G::Y3;  // 3D: advect_scalar3D.F90:8-349 (dy == dx)
@@ -2042,9 +2117,9 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
   for (; K < 5 && K <= nzm + 4; K++) step(K, std::false_type{});
   // two levels per trip: fewer of the prefetched values are copied along the ring before
   // their use (a copy waits for its load); with the unconditional prefetch, measured 8%
-  // faster at 64 x 58.  Only for the 2D 64-column segment: the other shapes already hold
-  // more per lane and spilled more with it (the uneven classes measured slower)
-  if constexpr (NX == 64 && !Y3) {
+  // faster at 64 x 58.  The other shapes hold more per lane and spilled more with it (the
+  // uneven classes measured slower), so there it is the alternative variant
+  if constexpr (PAIR) {
     for (; K + 1 < nzm; K += 2) {
       step(K, std::true_type{});
       step(K + 1, std::true_type{});
@@ -2059,15 +2134,15 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
 }
 
 // advect_all_scalars.F90:7-94: t, then the two microphysics fields.
-template <int NX, typename R, typename G = SegFor<NX>>
+template <int NX, typename R, typename G = SegFor<NX>, bool PAIR = (NX == 64 && !G::Y3)>
 __global__ void CRM_LB k_mpdata(DevT<R> d) {
   CRM_STOP_CHECK(d);
   G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
-  mpdata_one<NX, R, G>(d, s, c, live, d.t, nullptr, nullptr, false);
-  mpdata_one<NX, R, G>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
-  mpdata_one<NX, R, G>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
+  mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.t, nullptr, nullptr, false);
+  mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
+  mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
 }
 
 // ------------------------------------------------------------------------------------------

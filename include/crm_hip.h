@@ -188,6 +188,15 @@ int crm_hip_run_device_noalloc(crm_hip_ctx *ctx, const crm_dims *dims, const crm
  * 872-876). Default: off. */
 int crm_hip_accel_init(crm_hip_ctx *ctx, int use_crm_accel, double crm_accel_factor, int crm_accel_uv);
 
+/* The reference's Coriolis switch docoriolis (crm/params.F90:107, .false. there and never set by
+ * setparm) for later calls of a context: the f-plane Coriolis terms of coriolis.F90:7-53 with
+ * fcor = 4 pi/86400 sin(lat) and fcorz from the instance's latitude (crm_module.F90:287,321-322)
+ * and the large-scale wind ug0, vg0 (crm_module.F90:576-577), added to the momentum tendencies
+ * after advection and SGS diffusion (crm_module.F90:825). 3D CRMs only: in 2D the reference's
+ * Coriolis term would make the v the 2D form does not carry evolve, and a call then returns
+ * CRM_ERR_UNSUPPORTED. Default: off. */
+int crm_hip_set_coriolis(crm_hip_ctx *ctx, int docoriolis);
+
 /* ---- Batches of uneven size classes, launch schedule, graphs -------------------------------
  * These have no counterpart in the reference, whose crm() runs one compile-time size class per
  * build (cam/crmdims.F90:13-22); they serve callers that advance several classes at once. */
@@ -217,10 +226,21 @@ int crm_hip_run_many(int n, crm_hip_ctx *const *ctxs, const crm_class_desc *clas
 int crm_hip_set_schedule(crm_hip_ctx *ctx, const int block[CRM_HIP_NKERNEL]);
 int crm_hip_get_schedule(const crm_hip_ctx *ctx, int block[CRM_HIP_NKERNEL]);
 
-/* Autotuner: runs the call once per candidate workgroup size (64, 128, 256) with every kernel
- * class timed on the context's stream, keeps the fastest size per class for later calls of
- * this context and writes it to block_out (may be NULL). The views are device pointers and are
- * advanced by every trial call: pass scratch copies. */
+/* Launch variant ("tile shape") of each kernel class of a context, 0 the default; results are
+ * bitwise the same in every variant. CRM_K_PRESS of 2D 64-column CRMs: 0 one instance per
+ * 4-wave workgroup (two workgroups per CU), 1 one instance on 8 waves with the gradient pass's
+ * inputs kept in registers from the right-hand side pass (one workgroup per CU), 2 and 3 as 0
+ * with two or four levels of each pass unrolled together. CRM_K_MPDATA: 0 the default number
+ * of levels per trip of the sweep's steady state (two for 2D 64-column CRMs, else one), 1 the
+ * other. Other classes have variant 0 only. */
+int crm_hip_set_variants(crm_hip_ctx *ctx, const int variant[CRM_HIP_NKERNEL]);
+int crm_hip_get_variants(const crm_hip_ctx *ctx, int variant[CRM_HIP_NKERNEL]);
+
+/* Autotuner: runs the call once per candidate workgroup size (64, 128, 256) and launch variant,
+ * with every kernel class timed on the context's stream, keeps the fastest (size, variant) of
+ * each class for later calls of this context and writes the sizes to block_out (may be NULL;
+ * crm_hip_get_variants gives the variants). The views are device pointers and are advanced by
+ * every trial call: pass scratch copies. */
 int crm_hip_autotune(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col, double dt_gl,
                      const crm_input_view *in, crm_state_view *state, crm_rad_view *rad,
                      crm_output_view *out, int block_out[CRM_HIP_NKERNEL]);

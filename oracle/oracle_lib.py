@@ -68,6 +68,14 @@ def reference(case):
     return _Runner(ref_so_path(case), "ref_crm", "ref_get")
 
 
+def reference_set_coriolis(case, on):
+    """The reference's docoriolis switch (params.F90:107) in the build for case's size class
+    (process-wide, like the module variable it sets)."""
+    lib = C.CDLL(ref_so_path(case))
+    lib.ref_set_coriolis.argtypes = [C.c_int]
+    lib.ref_set_coriolis(int(bool(on)))
+
+
 def oracle():
     """C restatement of the CRM."""
     return _Runner(ORACLE_SO, "oracle_crm", "oracle_get")

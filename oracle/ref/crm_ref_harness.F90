@@ -97,6 +97,13 @@ contains
     call free_reference_state()
   end subroutine
 
+  ! The reference's Coriolis switch (params.F90:107, docoriolis, off by default) for later calls.
+  subroutine ref_set_coriolis(flag) bind(C, name="ref_set_coriolis")
+    use params, only: docoriolis
+    integer(c_int), value :: flag
+    docoriolis = flag /= 0
+  end subroutine
+
   ! Returns 0 on success, CRM_ERR_* otherwise.  phase_stop > 0 returns right after that phase
   ! of the last substep (module state left allocated for ref_get); 0 runs the whole call.
   function ref_crm(d, col, dt_gl, inp, st, rad, out, phase_stop) result(status) &

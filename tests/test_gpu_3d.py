@@ -154,3 +154,45 @@ def test_mixed_precision_3d_one_substep():
         assert max(errs.values()) < 1e-5, _worst(errs)
     finally:
         h.close()
+
+
+@pytest.mark.parametrize("nsub", [1, 5])
+def test_coriolis_against_reference(nsub):
+    """The reference's Coriolis switch (params.F90:107) on in both: f-plane terms from each
+    instance's latitude (coriolis.F90:7-53, crm_module.F90:321-322) against the reference
+    routines with docoriolis set, at the 2D bars."""
+    case = _case(4, seed=14)
+    case.lat[:] = [0.5, -0.8, 1.2, 0.0]
+    want, got = case.copy(), case.copy()
+    ref = _reference(want)
+    oracle_lib.reference_set_coriolis(want, True)
+    try:
+        assert ref.run(want, nsub * SHAPE["dt"]) == 0
+    finally:
+        oracle_lib.reference_set_coriolis(want, False)
+    h = CrmHip(0)
+    try:
+        h.set_coriolis(True)
+        h.run(got, nsub * SHAPE["dt"])
+    finally:
+        h.close()
+    errs = compare_case(got, want, dt_gl=nsub * SHAPE["dt"])
+    worst = sorted(errs.items(), key=lambda kv: -kv[1])[:4]
+    print(worst)
+    assert max(errs.values()) <= (TOL_1 if nsub == 1 else TOL_N), worst
+    # the terms act: the same call without them differs
+    plain = case.copy()
+    assert oracle_lib.reference(plain).run(plain, nsub * SHAPE["dt"]) == 0
+    assert relerr(got.state["v_wind"], plain.state["v_wind"]) > 1e-8
+
+
+def test_coriolis_needs_3d():
+    """In 2D the reference's Coriolis term would evolve the v the 2D form does not carry."""
+    from crm_hip import CRM_ERR_UNSUPPORTED
+    from crm_inputs import make_case
+    h = CrmHip(0)
+    try:
+        h.set_coriolis(True)
+        assert h.run(make_case(2, 32, 28, 4000.0, 20.0), 20.0, check=False) == CRM_ERR_UNSUPPORTED
+    finally:
+        h.close()

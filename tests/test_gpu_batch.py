@@ -90,6 +90,50 @@ def test_autotune_schedule():
         ref.close()
 
 
+@pytest.mark.parametrize("cls", [CLASSES[2], CLASSES[1], CLASSES[3]], ids=lambda c: f"{c[0]}x{c[1]}x{c[2]}x{c[3]}")
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_launch_variants_do_not_change_results(cls, prec):
+    """Every launch variant of the pressure solve (4-wave reading its inputs twice, one, two or
+    four levels per trip; 8-wave with them in registers) and of MPDATA (one or two levels per
+    trip) gives the default variant's bits."""
+    case = _case(cls, seed=87)
+    ref = case.copy()
+    h = CrmHip(0, prec)
+    try:
+        h.run(ref, 4 * cls[5])
+        nv = 4 if (cls[1] == 64 and cls[2] == 1) else 1
+        for press in range(nv):
+            for mp in range(2):
+                h.set_variants({"press": press, "mpdata": mp})
+                assert h.variants()["press"] == press and h.variants()["mpdata"] == mp
+                got = case.copy()
+                h.run(got, 4 * cls[5])
+                _same(ref, got)
+    finally:
+        h.close()
+
+
+def test_autotune_chooses_variants():
+    """At 64 columns the tuner measures every pressure and MPDATA variant and keeps one of
+    them; results stay bitwise those of the default."""
+    cls = CLASSES[2]
+    case = _case(cls, seed=88)
+    h, ref = CrmHip(0), CrmHip(0)
+    try:
+        h.autotune(DeviceCase(case.copy()), 3 * cls[5])
+        v = h.variants()
+        print(v)
+        assert v["press"] in (0, 1, 2, 3) and v["mpdata"] in (0, 1)
+        assert all(x == 0 for k, x in v.items() if k not in ("press", "mpdata"))
+        a, b = case.copy(), case.copy()
+        h.run(a, 100.0)
+        ref.run(b, 100.0)
+        _same(a, b)
+    finally:
+        h.close()
+        ref.close()
+
+
 def test_run_many_matches_separate_calls():
     """Four uneven classes advanced concurrently, each equal to its own separate call."""
     cases = [_case(c, seed=84 + j) for j, c in enumerate(CLASSES)]
@@ -109,6 +153,52 @@ def test_run_many_matches_separate_calls():
         assert status == [0] * len(CLASSES)
         for dc, a in zip(dcs, alone):
             _same(dc.download(dc.case), a)
+    finally:
+        for h in hips:
+            h.close()
+
+
+def test_run_many_one_graph():
+    """crm_hip_run_many captures the classes' whole calls into one graph (cached, so the second
+    and third calls replay it): quiet soundings run every class's graph to the end, a class
+    stopped mid-call by the test hook resumes on its own, a class with mean-state acceleration
+    runs outside the graph, and a mixed-precision class joins; every class stays bitwise equal
+    to its own separate call, call after call."""
+    cls = [CLASSES[0], CLASSES[1], CLASSES[2], CLASSES[3], CLASSES[1]]
+    precs = ["fp64", "fp64", "mixed", "fp64", "fp64"]
+    cases = [make_case(c[0], c[1], c[3], c[4], c[5], ny=c[2], randomize=True, seed_base=200 + j)
+             for j, c in enumerate(cls)]
+
+    def configure(h, j):
+        if j == 1:
+            h.debug_stop_call_at(3)
+        if j == 4:
+            h.accel_init(True, 1.0)
+
+    alone = []
+    for j, c in enumerate(cases):
+        h = CrmHip(0, precs[j])
+        try:
+            configure(h, j)
+            a = c.copy()
+            outs = []
+            for _ in range(3):
+                h.run(a, 200.0)
+                outs.append(a.copy())
+            alone.append(outs)
+        finally:
+            h.close()
+    hips = [CrmHip(0, p) for p in precs]
+    try:
+        for j, h in enumerate(hips):
+            configure(h, j)
+        dcs = [DeviceCase(c.copy()) for c in cases]
+        for step in range(3):
+            assert run_many(hips, dcs, 200.0) == [0] * len(cls)
+            for dc, a in zip(dcs, alone):
+                _same(dc.download(dc.case.copy()), a[step])
+        assert hips[0].graph_launches() == 3 * 10  # quiet: every substep from the graph
+        assert hips[4].graph_launches() == 0       # acceleration: plain launches
     finally:
         for h in hips:
             h.close()
