@@ -248,7 +248,8 @@ class CrmHip:
 
     def set_graphs(self, mode):
         """0 (or False): plain launches; 1: graphs per steady substep; 2 (or True, the
-        default): one graph per call with the subcycle decision on the device."""
+        default): one graph per call with the subcycle decision on the device; 3: as 2, and
+        run_many over contexts that all have mode 3 captures the classes into one graph."""
         if isinstance(mode, bool):
             mode = 2 if mode else 0
         rc = self.lib.crm_hip_set_graphs(self.ctx, int(mode))

@@ -145,7 +145,10 @@ struct Gam {
 template <typename R>
 struct DevT {
   int ncrms, nzm, nz, plev, ld, nx_rad;
-  int ny;                     // 1 for 2D; 8 for the 3D 8 x 8 CRM (column index 8 j + i)
+  int ny;                     // 1 for 2D; 8 or 4 for the 3D 8 x 8 or 4 x 4 CRM (column nx j + i)
+  // radiation column groups (crm_module.F90:979-1015): row length nx, radiation columns
+  // nx_rad_x along x, and group sizes nx / nx_rad_x and ny / ny_rad (powers of two)
+  int rad_row, rad_nxx, rad_gx, rad_gy;
   double dx, dt, dtn, dt3na, at, bt, ct, dtfactor;
   int nstep, icycle;
   int docoriolis;             // Coriolis terms (params.F90:107; 3D only here)
@@ -186,6 +189,18 @@ struct DevT {
 // words, one 128-byte line apart, after the four flag words of gmax: instance c updates slot
 // c % KURANT_SLOTS, so thousands of instances finishing together do not queue on one address;
 // the reader takes the maximum of the slots (the host for kurant, k_precip for the other).
+// Radiation column of storage column i: i_rad = (i-1)/(nx/crm_nx_rad)+1 and j_rad likewise,
+// stored i_rad + nx_rad*j_rad (crm_module.F90:979-1015, crm_rad_module.F90:17-34).
+template <typename R>
+__device__ __forceinline__ int rad_col(const DevT<R>& d, int i) {
+  return (i % d.rad_row) / d.rad_gx + d.rad_nxx * ((i / d.rad_row) / d.rad_gy);
+}
+// the one lane of each radiation group that owns its sums
+template <typename R>
+__device__ __forceinline__ bool rad_lead(const DevT<R>& d, int i) {
+  return (i % d.rad_row) % d.rad_gx == 0 && (i / d.rad_row) % d.rad_gy == 0;
+}
+
 constexpr int GMAX_FLAGS = 4, KURANT_SLOTS = 32, KURANT_STRIDE = 16;
 constexpr int GMAX_STOP = 0;  // flag word: the substep that stopped a captured call
 constexpr int GMAX_NPREC = 1;  // the call's largest nprec (microphysics.F90:602-662)
@@ -231,6 +246,9 @@ __device__ __forceinline__ double& inst(const DevT<R>& d, int p, int c) {
 #ifndef CRM_SEG_BPERMUTE
 constexpr int DPP_QP_XOR1 = 0xB1, DPP_QP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
 constexpr int DPP_WAVE_ROL1 = 0x134, DPP_WAVE_ROR1 = 0x13C, DPP_ROW_ROR1 = 0x121, DPP_ROW_ROR15 = 0x12F;
+// quad_perm [1,2,3,0] and [3,0,1,2]: lane j of each quad reads lane (j+1)%4, (j+3)%4;
+// row_ror:12 and row_ror:4: lane j of each row of 16 reads lane (j+4)%16, (j+12)%16
+constexpr int DPP_QP_ROT1 = 0x39, DPP_QP_ROT3 = 0x93, DPP_ROW_ROR12 = 0x12C, DPP_ROW_ROR4 = 0x124;
 
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
@@ -307,6 +325,8 @@ struct Seg {
     if (d > 0) {
       if constexpr (NX == 64) {
         return lane_mv<DPP_WAVE_ROL1>(v);
+      } else if constexpr (NX == 4) {
+        return lane_mv<DPP_QP_ROT1>(v);
       } else if constexpr (NX == 32) {
         T r = lane_mv<DPP_WAVE_ROL1>(v);      // lane 31 (63) got lane 32 (0)
         T x = lane_x32(r, lane);              // lane 31 (63) now holds what lane 63 (31) got
@@ -321,6 +341,8 @@ struct Seg {
     } else {
       if constexpr (NX == 64) {
         return lane_mv<DPP_WAVE_ROR1>(v);
+      } else if constexpr (NX == 4) {
+        return lane_mv<DPP_QP_ROT3>(v);
       } else if constexpr (NX == 32) {
         T r = lane_mv<DPP_WAVE_ROR1>(v);
         T x = lane_x32(r, lane);
@@ -355,6 +377,16 @@ struct Seg {
   template <typename T>
   __device__ __forceinline__ T gsum(T v, int g) const {
     return reduce(v, g, [](T a, T b) { return a + b; });
+  }
+  // sum over aligned blocks of gx x gy columns of rows of `row` lanes (3D radiation groups;
+  // powers of two): the x groups first, then the rows of a block pairwise, so every lane of a
+  // block ends with the bitwise same value
+  template <typename T>
+  __device__ __forceinline__ T gsum2(T v, int gx, int row, int gy) const {
+    if (gx == row) return gsum(v, row * gy);  // whole rows: one contiguous group
+    v = gsum(v, gx);
+    for (int m = row; m < row * gy; m <<= 1) v = v + (T)__shfl_xor(v, m, 64);
+    return v;
   }
   template <typename T>
   __device__ __forceinline__ T max(T v) const {
@@ -412,6 +444,23 @@ struct Seg {
 #endif
     return i == 0;
   }
+  // N segment sums at once: FP64 values through multi_sum (an FP64 add takes no DPP operand,
+  // so each butterfly step of a lone sum costs two moves and an add), FP32 values one
+  // reduction each (DPP adds); sum j is then held where sum_owner<T>(j) is true
+  template <typename T, int N>
+  __device__ __forceinline__ void sums(T (&v)[N]) const {
+    if constexpr (std::is_same<T, double>::value) {
+      multi_sum(v);
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; j++) v[j] = sum(v[j]);
+    }
+  }
+  template <typename T>
+  __device__ __forceinline__ bool sum_owner(int j) const {
+    if constexpr (std::is_same<T, double>::value) return owns(j);
+    return i == 0;
+  }
 #ifndef CRM_SEG_BPERMUTE
   // (a, b) -> (a with its upper half replaced by b's lower half, b with its lower half
   // replaced by a's upper half); swap16 does the same for odd/even 16-lane rows
@@ -430,25 +479,34 @@ struct Seg {
 #endif
 };
 
-// A 3D CRM of 8 x 8 columns on one wave: lane = 8 j + i.  i here is the column index within
-// the instance (0..63, the storage index), as for the 2D segments; x neighbours come from
-// rotations inside each aligned group of 8 lanes, y neighbours from a rotation by 8 lanes,
-// and reductions run over all 64 columns (nx*ny).
+// A 3D CRM of NXY x NXY columns (8 x 8: one instance per wave; 4 x 4: four instances per
+// wave): lane = NXY*NXY*instance + NXY*j + i.  i here is the column index within the instance
+// (the storage index), as for the 2D segments; x neighbours come from rotations inside each
+// aligned group of NXY lanes, y neighbours from a rotation by NXY lanes inside the instance,
+// and reductions run over all NXY*NXY columns (nx*ny).
+template <int NXY>
 struct Seg3 {
   static constexpr bool Y3 = true;
-  Seg<64> all;
-  Seg<8> row;
+  static constexpr int NN = NXY * NXY;
+  static_assert(NXY == 8 || NXY == 4, "3D CRMs of 8 x 8 or 4 x 4 columns");
+  Seg<NN> all;
+  Seg<NXY> row;
   int lane, i;
   __device__ __forceinline__ Seg3() {
     lane = threadIdx.x & 63;
-    i = lane;
+    i = lane & (NN - 1);
   }
   // value held by the column at x offset d (+1 or -1), periodic in x
   template <typename T>
   __device__ __forceinline__ T rot(T v, int d) const { return row.rot(v, d); }
   // value held by the column at y offset d (+1 or -1), periodic in y
   template <typename T>
-  __device__ __forceinline__ T roty(T v, int d) const { return __shfl(v, (lane + 8 * d) & 63, 64); }
+  __device__ __forceinline__ T roty(T v, int d) const {
+#ifndef CRM_SEG_BPERMUTE
+    if constexpr (NXY == 4) return d > 0 ? lane_mv<DPP_ROW_ROR12>(v) : lane_mv<DPP_ROW_ROR4>(v);
+#endif
+    return __shfl(v, (lane & ~(NN - 1)) | ((lane + NXY * d) & (NN - 1)), 64);
+  }
   template <typename T>
   __device__ __forceinline__ T sum(T v) const { return all.sum(v); }
   template <typename T>
@@ -456,6 +514,10 @@ struct Seg3 {
   __device__ __forceinline__ int imax(int v) const { return all.imax(v); }
   __device__ __forceinline__ int imin(int v) const { return all.imin(v); }
   __device__ __forceinline__ bool any(bool p) const { return __any(p); }
+  template <typename T, int N>
+  __device__ __forceinline__ void sums(T (&v)[N]) const { all.sums(v); }
+  template <typename T>
+  __device__ __forceinline__ bool sum_owner(int j) const { return all.template sum_owner<T>(j); }
 };
 
 // A 2D CRM of 128 columns on the two waves of a 128-thread workgroup, one instance per
@@ -511,6 +573,11 @@ struct Seg128 {
   __device__ __forceinline__ T gsum(T v, int g) const {
     return g <= 64 ? w.gsum(v, g) : sum(v);
   }
+  // a 2D CRM: the group is gx contiguous columns of one row (gy == 1)
+  template <typename T>
+  __device__ __forceinline__ T gsum2(T v, int gx, int, int) const {
+    return gsum(v, gx);
+  }
   template <typename T>
   __device__ __forceinline__ T max(T v) const {
     return combine(w.max(v), [](T a, T b) { return fmax(a, b); });
@@ -538,6 +605,17 @@ struct Seg128 {
     for (int j = 0; j < N; j++) v[j] = p[j] + p[SLOTS + j];
   }
   __device__ __forceinline__ bool owns(int) const { return i == 0; }
+  template <typename T, int N>
+  __device__ __forceinline__ void sums(T (&v)[N]) const {
+    if constexpr (std::is_same<T, double>::value) {
+      multi_sum(v);
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; j++) v[j] = sum(v[j]);
+    }
+  }
+  template <typename T>
+  __device__ __forceinline__ bool sum_owner(int) const { return i == 0; }
 };
 
 // The lane geometry of a 2D CRM of NX columns.

@@ -44,6 +44,7 @@ struct HostTransform {
 
 enum { NSCAL = 17 };  // u t qt qp qn tabs qv qcl qci qpl qpi tk tkh tke dudt x3
 enum { NSCAL3 = 4 };  // 3D only: v dvdt x3
+enum { NPOOL = 9 };  // blocks of the field pool (pool_sizes)
 constexpr int MAX_VARIANTS = 4;  // launch variants of a kernel class (variants_of)
 
 }  // namespace
@@ -188,6 +189,7 @@ struct Engine : CrmEngine {
     use_graphs = mode;
     drop_graphs();
   }
+  int get_graphs() const override { return use_graphs; }
   void set_debug_stop(int nstep) override { debug_stop = nstep; }
   void set_coriolis(int on) override {
     coriolis = on ? 1 : 0;
@@ -277,7 +279,8 @@ static void prof_harvest(Engine<R>* ctx) {
   } while (0)
 
 static bool supported_nx(int nx) { return nx == 8 || nx == 16 || nx == 32 || nx == 64 || nx == 128; }
-static bool supported_nz(int nz) { return nz == 28 || nz == 58; }
+// level counts: any from 8 to 64 (the pressure solve holds one level per lane of a wave)
+static bool supported_nz(int nz) { return nz >= 8 && nz <= 64; }
 
 // REAL(4) Gamma of the reference (gammafff.c:13 via precip_init.F90:24-45)
 static double gammafff(double x_in) {
@@ -312,7 +315,7 @@ static size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Byte sizes of the blocks of the field pool for ncap instances (each block 256-byte aligned).
 template <typename R>
-static void pool_sizes(int ncap, int nx, int nzm, int nxr, bool y3, size_t sizes[9]) {
+static void pool_sizes(int ncap, int nx, int nzm, int nxr, bool y3, size_t sizes[NPOOL]) {
   const int nz = nzm + 1;
   const size_t scal = (size_t)ncap * nzm * nx, wl = (size_t)ncap * nz * nx;
   constexpr bool mirror = !std::is_same<R, double>::value;
@@ -329,7 +332,7 @@ static void pool_sizes(int ncap, int nx, int nzm, int nxr, bool y3, size_t sizes
 
 template <typename R>
 static size_t pool_bytes_for(int ncap, int nx, int nzm, int nxr, bool y3) {
-  size_t sizes[9], bytes = 0;
+  size_t sizes[NPOOL], bytes = 0;
   pool_sizes<R>(ncap, nx, nzm, nxr, y3, sizes);
   for (size_t b : sizes) bytes += al256(b);
   return bytes;
@@ -338,7 +341,7 @@ static size_t pool_bytes_for(int ncap, int nx, int nzm, int nxr, bool y3) {
 // Point the field pointers of d into the pool at p (laid out by pool_sizes).
 template <typename R>
 static void carve_pool(DevT<R>& d, char* p, int ncap, int nx, int nzm, int nxr, bool y3) {
-  size_t sizes[9];
+  size_t sizes[NPOOL];
   pool_sizes<R>(ncap, nx, nzm, nxr, y3, sizes);
   constexpr bool mirror = !std::is_same<R, double>::value;
   const int nz = nzm + 1;
@@ -486,43 +489,55 @@ static int variants_of(int kid, int nx, bool y3) {
 template <int NX, typename R, typename G>
 static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, const DevT<R>& d, int na, int nb, int nc,
                          int variant) {
+  // 28 and 58 levels (the reference's SP configurations) have compile-time level loops; any
+  // other level count up to 64 runs the same kernels with the count read at run time (NZM 0)
+  const bool z28 = nzm == 28, z58 = nzm == 58;
+  const dim3 gi((unsigned)d.ncrms);
   if constexpr (NX == 64 && !G::Y3) {  // matrix-core transforms
-    if (variant == 0) {  // one instance per 4-wave block
-      if (nzm == 28)
-        k_press64<28, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+    if (!z28 && !z58) {
+      k_press64<0, R, G><<<gi, b, press64_lds_bytes<0>(), s>>>(d, na, nb, nc);
+    } else if (variant == 0) {  // one instance per 4-wave block
+      if (z28)
+        k_press64<28, R, G><<<gi, b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
       else
-        k_press64<58, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+        k_press64<58, R, G><<<gi, b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
     } else if (variant == 2) {  // the same, two levels per trip
-      if (nzm == 28)
-        k_press64<28, R, G, 2><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+      if (z28)
+        k_press64<28, R, G, 2><<<gi, b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
       else
-        k_press64<58, R, G, 2><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+        k_press64<58, R, G, 2><<<gi, b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
     } else if (variant == 3) {  // the same, four levels per trip
-      if (nzm == 28)
-        k_press64<28, R, G, 4><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+      if (z28)
+        k_press64<28, R, G, 4><<<gi, b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
       else
-        k_press64<58, R, G, 4><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+        k_press64<58, R, G, 4><<<gi, b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
     } else {  // one instance per 8-wave block, inputs of the gradient pass in registers
-      if (nzm == 28)
-        k_press64r<28, R><<<dim3((unsigned)d.ncrms), dim3(512), press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+      if (z28)
+        k_press64r<28, R><<<gi, dim3(512), press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
       else
-        k_press64r<58, R><<<dim3((unsigned)d.ncrms), dim3(512), press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+        k_press64r<58, R><<<gi, dim3(512), press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
     }
   } else if constexpr (NX == 64) {  // 3D 8 x 8: one instance per 4-wave block
-    if (nzm == 28)
-      k_press64<28, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+    if (z28)
+      k_press64<28, R, G><<<gi, b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
+    else if (z58)
+      k_press64<58, R, G><<<gi, b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
     else
-      k_press64<58, R, G><<<dim3((unsigned)d.ncrms), b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+      k_press64<0, R, G><<<gi, b, press64_lds_bytes<0>(), s>>>(d, na, nb, nc);
   } else if constexpr (NX == 128) {  // one instance per block of 4 waves
-    if (nzm == 28)
-      k_press128<28, R><<<dim3((unsigned)d.ncrms), b, (size_t)28 * 128 * sizeof(double), s>>>(d, na, nb, nc);
-    else
-      k_press128<58, R><<<dim3((unsigned)d.ncrms), b, (size_t)58 * 128 * sizeof(double), s>>>(d, na, nb, nc);
+    if (z28)
+      k_press128<28, R><<<gi, b, (size_t)28 * 128 * sizeof(double), s>>>(d, na, nb, nc);
+    else if (z58)
+      k_press128<58, R><<<gi, b, (size_t)58 * 128 * sizeof(double), s>>>(d, na, nb, nc);
+    else  // the Thomas factors in LDS after the slab
+      k_press128<0, R><<<gi, b, (size_t)2 * nzm * 128 * sizeof(double), s>>>(d, na, nb, nc);
   } else {
-    if (nzm == 28)
-      k_press<NX, 28, R><<<g, b, sh, s>>>(d, na, nb, nc);
-    else
-      k_press<NX, 58, R><<<g, b, sh, s>>>(d, na, nb, nc);
+    if (z28)
+      k_press<NX, 28, R, G><<<g, b, sh, s>>>(d, na, nb, nc);
+    else if (z58)
+      k_press<NX, 58, R, G><<<g, b, sh, s>>>(d, na, nb, nc);
+    else  // the Thomas factors in LDS after the slab
+      k_press<NX, 0, R, G><<<g, b, 2 * sh, s>>>(d, na, nb, nc);
   }
 }
 
@@ -549,12 +564,18 @@ struct Launcher {
 };
 
 // NXV: columns per instance; GEO: the lane geometry (a 2D row of NXV columns, or the 8 x 8
-// columns of a 3D CRM on one wave).
+// or 4 x 4 columns of a 3D CRM; nx here counts nx*ny).
 #define NX_DISPATCH(nx, CALL)       \
   if (y3) {                         \
-    constexpr int NXV = 64;         \
-    using GEO = Seg3;               \
-    CALL;                           \
+    if ((nx) == 16) {               \
+      constexpr int NXV = 16;       \
+      using GEO = Seg3<4>;          \
+      CALL;                         \
+    } else {                        \
+      constexpr int NXV = 64;       \
+      using GEO = Seg3<8>;          \
+      CALL;                         \
+    }                               \
   } else                            \
     switch (nx) {                   \
       case 8: {                     \
@@ -605,11 +626,11 @@ static int validate(const crm_dims* dm) {
   if (!dm || dm->ncrms < 1 || dm->ld < dm->ncrms || dm->plev < dm->nz + 1) return CRM_ERR_ARG;
   if (!supported_nz(dm->nz)) return CRM_ERR_UNSUPPORTED;
   if (dm->ny != 1) {
-    // 3D: 8 x 8 columns (dy == dx); radiation on every column or on the domain mean
-    if (dm->nx != 8 || dm->ny != 8) return CRM_ERR_UNSUPPORTED;
+    // 3D: 8 x 8 or 4 x 4 columns (dy == dx); radiation groups of any power-of-two shape
+    if (!((dm->nx == 8 && dm->ny == 8) || (dm->nx == 4 && dm->ny == 4))) return CRM_ERR_UNSUPPORTED;
     if (dm->nx_rad < 1 || dm->ny_rad < 1 || dm->nx % dm->nx_rad || dm->ny % dm->ny_rad) return CRM_ERR_NXRAD;
-    if (!((dm->nx_rad == dm->nx && dm->ny_rad == dm->ny) || (dm->nx_rad == 1 && dm->ny_rad == 1)))
-      return CRM_ERR_UNSUPPORTED;
+    const int gx = dm->nx / dm->nx_rad, gy = dm->ny / dm->ny_rad;
+    if ((gx & (gx - 1)) || (gy & (gy - 1))) return CRM_ERR_UNSUPPORTED;
     if (60. < 2 * dm->dt) return CRM_ERR_DAMPING;
     return CRM_OK;
   }
@@ -698,6 +719,10 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   d.ld = dm->ld;
   d.nx_rad = nxr;
   d.ny = dm->ny;
+  d.rad_row = dm->nx;
+  d.rad_nxx = dm->nx_rad;
+  d.rad_gx = dm->nx / dm->nx_rad;
+  d.rad_gy = dm->ny / dm->ny_rad;
   d.dx = dm->dx;
   d.dt = dm->dt;
   d.Ft = tf.Ft;
@@ -822,7 +847,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
   // k_decide stops the replay at the first substep that needs more subcycles (every later
   // kernel returns at entry), and the loop below resumes that substep from the host's record
   // of its bookkeeping before it (slot rotation, buffer swaps, Adams-Bashforth history).
-  if (ctx->use_graphs == CRM_GRAPHS_CALL && !sc && !acc && !net_inputs && ctx->prof_mask == 0 && phase_stop == 0 &&
+  if (ctx->use_graphs >= CRM_GRAPHS_CALL && !sc && !acc && !net_inputs && ctx->prof_mask == 0 && phase_stop == 0 &&
       nstop <= 2048) {
     struct Snap {
       int na, nb, nc;

@@ -68,6 +68,7 @@ struct CrmEngine {
   virtual int set_variants(const int* variant) = 0;
   virtual void get_variants(int* variant) const = 0;
   virtual void set_graphs(int mode) = 0;
+  virtual int get_graphs() const = 0;
   virtual void set_debug_stop(int nstep) = 0;
   virtual void set_coriolis(int on) = 0;
   virtual long long graph_count() const = 0;

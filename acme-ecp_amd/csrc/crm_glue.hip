@@ -168,6 +168,57 @@ int finish(const crm_hip_ctx* ctx) {
   return hipStreamSynchronize(crm_ctx_stream(ctx)) == hipSuccess ? CRM_OK : CRM_ERR_DEVICE;
 }
 
+// rrtmg/radiation.F90:1584-1636: one thread per (column, radiation column, level)
+__global__ void k_rad_columns(int ncol, int pcols, int pver, int crm_nz, int nrad, crm_gcm_state_view st,
+                              crm_gcm_cloud_view cl, crm_rad_view rad, crm_rad_column_view out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)ncol * nrad * pver;
+  if (t >= n) return;
+  const int i = (int)(t % ncol);
+  const size_t rk = t / ncol;
+  const int r = (int)(rk % nrad), k = (int)(rk / nrad);
+  const size_t g = (size_t)i + (size_t)pcols * k;                  // (pcols,pver)
+  const size_t o = (size_t)i + (size_t)pcols * (r + (size_t)nrad * k);  // (pcols,nrad,pver)
+  const int m = pver - 1 - k;  // CRM level on GCM level k
+  if (m >= crm_nz) {  // above the CRM: the GCM column's values
+    out.t[o] = st.t[g];
+    out.qv[o] = st.qv[g];
+    out.qc[o] = st.qcl[g];
+    out.qi[o] = st.qci[g];
+    out.cld[o] = cl.cld[g];
+    out.fice[o] = cl.fice[g];
+    out.cicewp[o] = cl.cicewp[g];
+    out.cliqwp[o] = cl.cliqwp[g];
+    return;
+  }
+  const size_t a = (size_t)i + (size_t)pcols * (r + (size_t)nrad * m);  // (pcols,nrad,crm_nz)
+  const double cld = rad.cld[a], qc = rad.qc[a], qi = rad.qi[a];
+  out.cld[o] = cld;
+  if (cld > 0) {
+    const double qtot = qc + qi;
+    out.fice[o] = qi / qtot;
+    out.cicewp[o] = qi * st.pdel[g] / GRAVIT / fmax(0.01, cld);
+    out.cliqwp[o] = qc * st.pdel[g] / GRAVIT / fmax(0.01, cld);
+  } else {
+    out.fice[o] = 0.0;
+    out.cicewp[o] = 0.0;
+    out.cliqwp[o] = 0.0;
+  }
+  out.qi[o] = qi;
+  out.qc[o] = qc;
+  out.qv[o] = fmax(1.e-9, rad.qv[a]);
+  out.t[o] = rad.temperature[a];
+}
+
+// radiation.F90:2393-2398
+__global__ void k_rad_qrad(int ncol, int pcols, int nrad, int crm_nz, const double* qrs, const double* qrl,
+                           double* qrad) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)ncol * nrad * crm_nz) return;
+  const size_t o = (size_t)(t % ncol) + (size_t)pcols * (t / ncol);
+  qrad[o] = qrs[o] + qrl[o];
+}
+
 bool dims_ok(int ncol, int pcols, int pver) { return ncol >= 0 && pcols >= ncol && pver >= 1; }
 
 }  // namespace
@@ -224,6 +275,32 @@ int crm_hip_check_energy_chng(crm_hip_ctx* ctx, int ncol, int pcols, int pver, c
   k_check_energy<<<grid(ncol), dim3(128), 0, crm_ctx_stream(ctx)>>>(ncol, pcols, pver, *state, ztodt, flx_vap,
                                                                     flx_cnd, flx_ice, flx_sen, te_cur, tw_cur,
                                                                     te_tnd, tw_tnd, te_rer, tw_rer);
+  return finish(ctx);
+}
+
+int crm_hip_rad_columns(crm_hip_ctx* ctx, int ncol, int pcols, int pver, int crm_nz, int nrad,
+                        const crm_gcm_state_view* state, const crm_gcm_cloud_view* cloud, const crm_rad_view* rad,
+                        const crm_rad_column_view* cols) {
+  if (!ctx || !state || !cloud || !rad || !cols || !dims_ok(ncol, pcols, pver) || crm_nz < 2 || crm_nz > pver ||
+      nrad < 1)
+    return CRM_ERR_ARG;
+  if (ncol == 0) return CRM_OK;
+  if (hipSetDevice(crm_ctx_device(ctx)) != hipSuccess) return CRM_ERR_DEVICE;
+  const size_t n = (size_t)ncol * nrad * pver;
+  k_rad_columns<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, crm_ctx_stream(ctx)>>>(
+      ncol, pcols, pver, crm_nz, nrad, *state, *cloud, *rad, *cols);
+  return finish(ctx);
+}
+
+int crm_hip_rad_qrad(crm_hip_ctx* ctx, int ncol, int pcols, int nrad, int crm_nz, const double* qrs_crm,
+                     const double* qrl_crm, const crm_rad_view* rad) {
+  if (!ctx || !qrs_crm || !qrl_crm || !rad || !rad->qrad || ncol < 0 || pcols < ncol || nrad < 1 || crm_nz < 1)
+    return CRM_ERR_ARG;
+  if (ncol == 0) return CRM_OK;
+  if (hipSetDevice(crm_ctx_device(ctx)) != hipSuccess) return CRM_ERR_DEVICE;
+  const size_t n = (size_t)ncol * nrad * crm_nz;
+  k_rad_qrad<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, crm_ctx_stream(ctx)>>>(
+      ncol, pcols, nrad, crm_nz, qrs_crm, qrl_crm, const_cast<double*>(rad->qrad));
   return finish(ctx);
 }
 

@@ -477,7 +477,7 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
   const bool y3 = d.ny > 1;
   const int n_damp = (int)inst(d, I_NDAMP, c);
   const int kmin = (int)inst(d, I_KMIN, c), kmax = (int)inst(d, I_KMAX, c);
-  const int nxr = d.nx_rad, ir = i / (NX / nxr);
+  const int nxr = d.nx_rad, ir = rad_col(d, i);
   R* dudt = d.dudt[na];
   R* dwdt = d.dwdt[na];
   auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
@@ -526,7 +526,9 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
   for (int k = 0; k < nzm; k++) {
     size_t o = ((size_t)c * nzm + k) * NX + i;
     const Lvl L = nxt;
-    if (k + 1 < nzm) load(k + 1, nxt);
+    // clamped at the top, where the record is not used again: an unconditional load needs no
+    // copy that waits for earlier loads
+    load(::min(k + 1, nzm - 1), nxt);
     R tabs = L.tabs, qv = L.qv, qcl = L.qcl, qci = L.qci, qpl = L.qpl, qpi = L.qpi;
     R u = L.u, w = L.w;
     R t = L.t, qt = L.qt;
@@ -694,19 +696,20 @@ __global__ void CRM_LB k_sgs(DevT<R> d) {
     int kc = k + 1, kb = k - 1;
     const R n_tabs = nx_tabs, n_qcl = nx_qcl, n_qci = nx_qci, n_qv = nx_qv, n_qpl = nx_qpl, n_qpi = nx_qpi;
     const R n_t = nx_t, tk_k = tk_cur;
-    if (kc + 1 < nzm) {
-      nx_tabs = F(d.tabs, kc + 1);
-      nx_qcl = F(d.qcl, kc + 1);
-      nx_qci = F(d.qci, kc + 1);
-      nx_qv = F(d.qv, kc + 1);
-      nx_qpl = F(d.qpl, kc + 1);
-      nx_qpi = F(d.qpi, kc + 1);
-      nx_t = F(d.t, kc + 1);
-    }
+    // level kc+1, clamped at the top: past it the values are not used (the loop ends), and a
+    // load that always issues needs no copy that waits for earlier loads
+    const int kn = ::min(kc + 1, nzm - 1);
+    nx_tabs = F(d.tabs, kn);
+    nx_qcl = F(d.qcl, kn);
+    nx_qci = F(d.qci, kn);
+    nx_qv = F(d.qv, kn);
+    nx_qpl = F(d.qpl, kn);
+    nx_qpi = F(d.qpi, kn);
+    nx_t = F(d.t, kn);
     tk_cur = F(d.tk, kc);
-    R up_next = (kc + 1 < nzm) ? F(d.u, kc + 1) : R(0.0);
+    R up_next = F(d.u, kn);
     R vp_next = R(0.0);
-    if constexpr (G::Y3) vp_next = (kc + 1 < nzm) ? F(d.v, kc + 1) : R(0.0);
+    if constexpr (G::Y3) vp_next = F(d.v, kn);
     R w2 = W(kc + 1);
     // shear production def2 at level k
     R def2;
@@ -850,11 +853,15 @@ __device__ __forceinline__ void mom2(const DevT<R>& d, int na) {
     bool has_n = kn < nzm;
     int kcu = has_n ? kn : k;
     const R du_k = du_pre, dw_k = dw_pre;
-    if (has_n) du_pre = dudt[((size_t)c * nzm + kn) * NX + i];
-    if (k >= 1 && has_n) dw_pre = dwdt[((size_t)c * nz + kn) * NX + i];
-    R u_nn = (kn + 1 < nzm) ? F(d.u, kn + 1) : R(0.0);
-    R w_nn = (kn + 1 < nz) ? W(kn + 1) : R(0.0);
-    R tk_nn = (kn + 1 < nzm) ? F(d.tk, kn + 1) : R(0.0);
+    // the next levels, clamped at the top: values read past it enter only terms of levels
+    // that have a level above (has_n), and a load that always issues needs no copy that waits
+    // for earlier loads
+    const int kc1 = ::min(kn, nzm - 1), kc2 = ::min(kn + 1, nzm - 1);
+    du_pre = dudt[((size_t)c * nzm + kc1) * NX + i];
+    dw_pre = dwdt[((size_t)c * nz + kn) * NX + i];
+    R u_nn = F(d.u, kc2);
+    R w_nn = W(::min(kn + 1, nz - 1));
+    R tk_nn = F(d.tk, kc2);
     R u_m = s.rot(u_k, -1), u_p = s.rot(u_k, 1);
     R ucu = has_n ? u_n : u_k, ucu_p = s.rot(ucu, 1);
     R wn_m = s.rot(w_n, -1), wn_p = s.rot(w_n, 1);
@@ -948,14 +955,13 @@ __device__ __forceinline__ void mom2(const DevT<R>& d, int na) {
   }
 }
 
-// 3D momentum (8 x 8 columns, one instance per wave): advect2_mom_xy.F90:22-58,
+// 3D momentum (NX = nx*ny columns of G's x-y geometry per instance): advect2_mom_xy.F90:22-58,
 // advect2_mom_z.F90:7-118 (3D branch :47-63), diffuse_mom3D.F90:7-207, dy == dx.  Each
 // tendency receives its contributions in the reference's order: x then y advection, vertical
 // advection, x then y diffusion, vertical diffusion.
-template <typename R>
+template <int NX, typename R, typename G>
 __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
-  constexpr int NX = 64;
-  Seg3 s;
+  G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   const int nzm = d.nzm, nz = d.nz, i = s.i;
@@ -990,15 +996,15 @@ __device__ __forceinline__ void mom3(const DevT<R>& d, int na) {
     const bool has_n = kn < nzm;
     const int kcu = has_n ? kn : k;
     const R du_k = du_pre, dv_k = dv_pre, dw_k = dw_pre;
-    if (has_n) {
-      du_pre = dudt[((size_t)c * nzm + kn) * NX + i];
-      dv_pre = dvdt[((size_t)c * nzm + kn) * NX + i];
-    }
-    if (k >= 1 && has_n) dw_pre = dwdt[((size_t)c * nz + kn) * NX + i];
-    R u_nn = (kn + 1 < nzm) ? F(d.u, kn + 1) : R(0.0);
-    R v_nn = (kn + 1 < nzm) ? F(d.v, kn + 1) : R(0.0);
-    R w_nn = (kn + 1 < nz) ? W(kn + 1) : R(0.0);
-    R tk_nn = (kn + 1 < nzm) ? F(d.tk, kn + 1) : R(0.0);
+    // the next levels, clamped at the top (as in mom2)
+    const int kc1 = ::min(kn, nzm - 1), kc2 = ::min(kn + 1, nzm - 1);
+    du_pre = dudt[((size_t)c * nzm + kc1) * NX + i];
+    dv_pre = dvdt[((size_t)c * nzm + kc1) * NX + i];
+    dw_pre = dwdt[((size_t)c * nz + kn) * NX + i];
+    R u_nn = F(d.u, kc2);
+    R v_nn = F(d.v, kc2);
+    R w_nn = W(::min(kn + 1, nz - 1));
+    R tk_nn = F(d.tk, kc2);
     // neighbours at level k
     R u_m = s.rot(u_k, -1), u_p = s.rot(u_k, 1), u_jm = s.roty(u_k, -1), u_jp = s.roty(u_k, 1);
     R u_p_jm = s.roty(u_p, -1);
@@ -1155,29 +1161,30 @@ template <int NX, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_mom(DevT<R> d, int na) {
   CRM_STOP_CHECK(d);
   if constexpr (G::Y3)
-    mom3<R>(d, na);
+    mom3<NX, R, G>(d, na);
   else
     mom2<NX, R>(d, na);
 }
 
 // ------------------------------------------------------------------------------------------
 // Pressure: press_rhs.F90:8-106, the fft991 x-transform as dense operators (see
-// fft991_operator.hpp), the vertical Thomas sweep of pressure.F90:160-237, the inverse
-// transform, press_grad.F90:8-76, then adams.F90:7-40.  4 waves share the 64/NX instances of
-// one wave; levels are dealt round-robin to the waves; the slab lives in LDS; the Thomas
-// sweep runs on wave 0 with its factor in registers (NZM compile-time).
+// fft991_operator.hpp; in 3D the x-y Kronecker operator, dy == dx), the vertical Thomas sweep
+// of pressure.F90:160-237, the inverse transform, press_grad.F90:8-76, then adams.F90:7-40.
+// 4 waves share the 64/NX instances of one wave; levels are dealt round-robin to the waves;
+// the slab lives in LDS; the Thomas sweep runs on wave 0 with its factor in registers (NZM
+// compile-time) or, for NZM == 0 (any level count, read from d.nzm), in LDS after the slab.
 // ------------------------------------------------------------------------------------------
-template <int NX, int NZM, typename R>
+template <int NX, int NZM, typename R, typename G = SegFor<NX>>
 __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
   CRM_STOP_CHECK(d);
-  extern __shared__ double slab[];  // [64/NX instances][NZM][NX] == [NZM][64] per lane column
+  extern __shared__ double slab[];  // [64/NX instances][nzm][NX] == [nzm][64] per lane column
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane & (NX - 1);
   const int cg = blockIdx.x * (64 / NX) + lane / NX;
   const bool live = cg < d.ncrms;
   const int c = live ? cg : d.ncrms - 1;
-  SegFor<NX> s;
-  const int nz = NZM + 1;
+  G s;
+  const int nzm = NZM > 0 ? NZM : d.nzm, nz = nzm + 1;
   const double dz = inst(d, I_DZ, c);
   const double* adz = prof(d, P_ADZ, c);
   const double* adzw = prof(d, P_ADZW, c);
@@ -1189,81 +1196,107 @@ __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
   const R *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
   const R *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
   // 1. right-hand side
-  for (int k = wave; k < NZM; k += 4) {
-    size_t o = ((size_t)c * NZM + k) * NX + i;
+  for (int k = wave; k < nzm; k += 4) {
+    size_t o = ((size_t)c * nzm + k) * NX + i;
     size_t ow = ((size_t)c * nz + k) * NX + i;
     size_t own = ow + NX;
     double rdz = 1. / (adz[k] * dz);
     double rup = rhow[k + 1] / rho[k] * rdz;
     double rdn = rhow[k] / rho[k] * rdz;
     double uu = d.u[o], uap = ua[o], ubp = ub[o], ucp = uc[o];
-    double pv = (rdx * (s.rot(uu, 1) - uu) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
-                (rdx * (s.rot(uap, 1) - uap) + (wa[own] * rup - wa[ow] * rdn)) +
-                btat * (rdx * (s.rot(ubp, 1) - ubp) + (wb[own] * rup - wb[ow] * rdn)) +
-                ctat * (rdx * (s.rot(ucp, 1) - ucp) + (wc[own] * rup - wc[ow] * rdn));
+    double pv;
+    if constexpr (G::Y3) {  // press_rhs.F90:45-69
+      double vv = d.v[o], vap = d.dvdt[na][o], vbp = d.dvdt[nb][o], vcp = d.dvdt[nc][o];
+      pv = (rdx * (s.rot(uu, 1) - uu) + rdx * (s.roty(vv, 1) - vv) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
+           (rdx * (s.rot(uap, 1) - uap) + rdx * (s.roty(vap, 1) - vap) + (wa[own] * rup - wa[ow] * rdn)) +
+           btat * (rdx * (s.rot(ubp, 1) - ubp) + rdx * (s.roty(vbp, 1) - vbp) + (wb[own] * rup - wb[ow] * rdn)) +
+           ctat * (rdx * (s.rot(ucp, 1) - ucp) + rdx * (s.roty(vcp, 1) - vcp) + (wc[own] * rup - wc[ow] * rdn));
+    } else {
+      pv = (rdx * (s.rot(uu, 1) - uu) + (d.w[own] * rup - d.w[ow] * rdn)) * dta +
+           (rdx * (s.rot(uap, 1) - uap) + (wa[own] * rup - wa[ow] * rdn)) +
+           btat * (rdx * (s.rot(ubp, 1) - ubp) + (wb[own] * rup - wb[ow] * rdn)) +
+           ctat * (rdx * (s.rot(ucp, 1) - ucp) + (wc[own] * rup - wc[ow] * rdn));
+    }
     P(k, i) = pv * rho[k];
   }
   __syncthreads();
   // 2. forward transform: slot m of level k (every lane reads the whole row before any
   // lane overwrites its entry, hence the barrier between the sum and the store)
-  for (int k0 = 0; k0 < NZM; k0 += 4) {
+  for (int k0 = 0; k0 < nzm; k0 += 4) {
     const int k = k0 + wave;
     double acc = 0.0;
-    if (k < NZM) {
+    if (k < nzm) {
 #pragma unroll 8
       for (int q = 0; q < NX; q++) acc += d.Ft[q * NX + i] * P(k, q);
     }
     __syncthreads();
-    if (k < NZM) P(k, i) = acc;
+    if (k < nzm) P(k, i) = acc;
   }
   __syncthreads();
   // 3. Thomas sweep per slot (wave 0)
   if (wave == 0) {
     const double eig = d.eign[i];
-    double alfa[NZM];
-    double dz2 = dz * dz;
+    constexpr int NA = NZM > 0 ? NZM : 1;
+    double alfa_r[NA];
+    double* alfa_l = slab + (size_t)nzm * 64;
+    auto alfa = [&](int k) -> double& {
+      if constexpr (NZM > 0)
+        return alfa_r[k];
+      else
+        return alfa_l[(size_t)k * 64 + lane];
+    };
     auto acoef = [&](int k) { return rhow[k] / (adz[k] * adzw[k] * dz * dz); };
     auto ccoef = [&](int k) { return rhow[k + 1] / (adz[k] * adzw[k + 1] * dz * dz); };
-    (void)dz2;
     double b;
     if (i == 0)
       b = 1.0 / (eig * rho[0] - acoef(0) - ccoef(0));
     else
       b = 1.0 / (eig * rho[0] - ccoef(0));
-    alfa[0] = -ccoef(0) * b;
+    alfa(0) = -ccoef(0) * b;
     double beta = P(0, i) * b;
     P(0, i) = beta;
-#pragma unroll
-    for (int k = 1; k < NZM - 1; k++) {
+    auto down = [&](int k) {
       double ak = acoef(k), ck = ccoef(k);
-      double e = 1.0 / (eig * rho[k] - ak - ck + ak * alfa[k - 1]);
-      alfa[k] = -ck * e;
+      double e = 1.0 / (eig * rho[k] - ak - ck + ak * alfa(k - 1));
+      alfa(k) = -ck * e;
       beta = (P(k, i) - ak * beta) * e;
       P(k, i) = beta;
+    };
+    // unrolled with a compile-time level count, so the factors stay in registers
+    if constexpr (NZM > 0) {
+#pragma unroll
+      for (int k = 1; k < NZM - 1; k++) down(k);
+    } else {
+      for (int k = 1; k < nzm - 1; k++) down(k);
     }
     {
-      int k = NZM - 1;
+      int k = nzm - 1;
       double ak = acoef(k);
-      double top = (P(k, i) - ak * beta) / (eig * rho[k] - ak + ak * alfa[k - 1]);
+      double top = (P(k, i) - ak * beta) / (eig * rho[k] - ak + ak * alfa(k - 1));
       P(k, i) = top;
-#pragma unroll
-      for (int kk = NZM - 2; kk >= 0; kk--) {
-        top = alfa[kk] * top + P(kk, i);
+      auto up = [&](int kk) {
+        top = alfa(kk) * top + P(kk, i);
         P(kk, i) = top;
+      };
+      if constexpr (NZM > 0) {
+#pragma unroll
+        for (int kk = NZM - 2; kk >= 0; kk--) up(kk);
+      } else {
+        for (int kk = nzm - 2; kk >= 0; kk--) up(kk);
       }
     }
   }
   __syncthreads();
   // 4. inverse transform (same read-all-then-store pattern)
-  for (int k0 = 0; k0 < NZM; k0 += 4) {
+  for (int k0 = 0; k0 < nzm; k0 += 4) {
     const int k = k0 + wave;
     double acc = 0.0;
-    if (k < NZM) {
+    if (k < nzm) {
 #pragma unroll 8
       for (int q = 0; q < NX; q++) acc += d.Gt[q * NX + i] * P(k, q);
     }
     __syncthreads();
-    if (k < NZM) P(k, i) = acc;
+    if (k < nzm) P(k, i) = acc;
   }
   __syncthreads();
   // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities
@@ -1272,8 +1305,8 @@ __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
   R* wan = d.dwdt[na];
   R* unc = d.dudt[nc];
   R* wnc = d.dwdt[nc];
-  for (int k = wave; k < NZM; k += 4) {
-    size_t o = ((size_t)c * NZM + k) * NX + i;
+  for (int k = wave; k < nzm; k += 4) {
+    size_t o = ((size_t)c * nzm + k) * NX + i;
     size_t ow = ((size_t)c * nz + k) * NX + i;
     double pk = P(k, i);
     double pim = s.rot(pk, -1);
@@ -1284,6 +1317,13 @@ __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
     double uu = d.u[o], ww = d.w[ow];
     double un = uu + dt3 * (d.at * du_a + d.bt * ub[o] + d.ct * uc[o]);
     double wn = ww + dt3 * (d.at * dw_a + d.bt * wb[ow] + d.ct * wc[ow]);
+    double dv_a = 0.0, vn = 0.0, vv = 0.0;
+    if constexpr (G::Y3) {  // press_grad.F90:29, adams.F90:29,32
+      double pjm = s.roty(pk, -1);
+      dv_a = d.dvdt[na][o] - (pk - pjm) * rdx;
+      vv = d.v[o];
+      vn = vv + dt3 * (d.at * dv_a + d.bt * d.dvdt[nb][o] + d.ct * d.dvdt[nc][o]);
+    }
     if (live) {
       uan[o] = du_a;
       wan[ow] = dw_a;
@@ -1291,9 +1331,14 @@ __global__ void CRM_LB k_press(DevT<R> d, int na, int nb, int nc) {
       wnc[ow] = wn;
       d.u[o] = 0.5 * (uu + un) * (rho[k] * dtdx);
       d.w[ow] = 0.5 * (ww + wn) * (rhow[k] * dtdz);
+      if constexpr (G::Y3) {
+        d.dvdt[na][o] = dv_a;
+        d.dvdt[nc][o] = vn;
+        d.v[o] = 0.5 * (vv + vn) * (rho[k] * dtdx);
+      }
     }
   }
-  if (wave == 0 && live) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
+  if (wave == 0 && live) wnc[((size_t)c * nz + nzm) * NX + i] = 0.0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1307,11 +1352,11 @@ template <int NZM, typename R>
 __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
   CRM_STOP_CHECK(d);
   constexpr int NX = 128;
-  extern __shared__ double slab[];  // [NZM][128]
+  extern __shared__ double slab[];  // [nzm][128]
   const int i = threadIdx.x & (NX - 1), grp = threadIdx.x >> 7;
   const int c = blockIdx.x;  // one block per instance
   const int ip = (i + 1) & (NX - 1), im = (i - 1) & (NX - 1);
-  const int nz = NZM + 1;
+  const int nzm = NZM > 0 ? NZM : d.nzm, nz = nzm + 1;
   const double dz = inst(d, I_DZ, c);
   const double* adz = prof(d, P_ADZ, c);
   const double* adzw = prof(d, P_ADZW, c);
@@ -1323,8 +1368,8 @@ __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
   const R *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
   const R *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
   // 1. right-hand side
-  for (int k = grp; k < NZM; k += 2) {
-    const size_t row = ((size_t)c * NZM + k) * NX, o = row + i, op = row + ip;
+  for (int k = grp; k < nzm; k += 2) {
+    const size_t row = ((size_t)c * nzm + k) * NX, o = row + i, op = row + ip;
     const size_t ow = ((size_t)c * nz + k) * NX + i, own = ow + NX;
     double rdz = 1. / (adz[k] * dz);
     double rup = rhow[k + 1] / rho[k] * rdz;
@@ -1338,21 +1383,29 @@ __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
   }
   __syncthreads();
   // 2. forward transform (every thread reads its whole row before any entry is overwritten)
-  for (int k0 = 0; k0 < NZM; k0 += 2) {
+  for (int k0 = 0; k0 < nzm; k0 += 2) {
     const int k = k0 + grp;
     double acc = 0.0;
-    if (k < NZM) {
+    if (k < nzm) {
 #pragma unroll 8
       for (int q = 0; q < NX; q++) acc += d.Ft[q * NX + i] * P(k, q);
     }
     __syncthreads();
-    if (k < NZM) P(k, i) = acc;
+    if (k < nzm) P(k, i) = acc;
   }
   __syncthreads();
   // 3. Thomas sweep per slot (group 0)
   if (grp == 0) {
     const double eig = d.eign[i];
-    double alfa[NZM];
+    constexpr int NA = NZM > 0 ? NZM : 1;
+    double alfa_r[NA];
+    double* alfa_l = slab + (size_t)nzm * NX;  // NZM == 0: the factors in LDS after the slab
+    auto alfa = [&](int k) -> double& {
+      if constexpr (NZM > 0)
+        return alfa_r[k];
+      else
+        return alfa_l[(size_t)k * NX + i];
+    };
     auto acoef = [&](int k) { return rhow[k] / (adz[k] * adzw[k] * dz * dz); };
     auto ccoef = [&](int k) { return rhow[k + 1] / (adz[k] * adzw[k + 1] * dz * dz); };
     double b;
@@ -1360,40 +1413,51 @@ __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
       b = 1.0 / (eig * rho[0] - acoef(0) - ccoef(0));
     else
       b = 1.0 / (eig * rho[0] - ccoef(0));
-    alfa[0] = -ccoef(0) * b;
+    alfa(0) = -ccoef(0) * b;
     double beta = P(0, i) * b;
     P(0, i) = beta;
-#pragma unroll
-    for (int k = 1; k < NZM - 1; k++) {
+    auto down = [&](int k) {
       double ak = acoef(k), ck = ccoef(k);
-      double e = 1.0 / (eig * rho[k] - ak - ck + ak * alfa[k - 1]);
-      alfa[k] = -ck * e;
+      double e = 1.0 / (eig * rho[k] - ak - ck + ak * alfa(k - 1));
+      alfa(k) = -ck * e;
       beta = (P(k, i) - ak * beta) * e;
       P(k, i) = beta;
+    };
+    // unrolled with a compile-time level count, so the factors stay in registers
+    if constexpr (NZM > 0) {
+#pragma unroll
+      for (int k = 1; k < NZM - 1; k++) down(k);
+    } else {
+      for (int k = 1; k < nzm - 1; k++) down(k);
     }
     {
-      int k = NZM - 1;
+      int k = nzm - 1;
       double ak = acoef(k);
-      double top = (P(k, i) - ak * beta) / (eig * rho[k] - ak + ak * alfa[k - 1]);
+      double top = (P(k, i) - ak * beta) / (eig * rho[k] - ak + ak * alfa(k - 1));
       P(k, i) = top;
-#pragma unroll
-      for (int kk = NZM - 2; kk >= 0; kk--) {
-        top = alfa[kk] * top + P(kk, i);
+      auto up = [&](int kk) {
+        top = alfa(kk) * top + P(kk, i);
         P(kk, i) = top;
+      };
+      if constexpr (NZM > 0) {
+#pragma unroll
+        for (int kk = NZM - 2; kk >= 0; kk--) up(kk);
+      } else {
+        for (int kk = nzm - 2; kk >= 0; kk--) up(kk);
       }
     }
   }
   __syncthreads();
   // 4. inverse transform
-  for (int k0 = 0; k0 < NZM; k0 += 2) {
+  for (int k0 = 0; k0 < nzm; k0 += 2) {
     const int k = k0 + grp;
     double acc = 0.0;
-    if (k < NZM) {
+    if (k < nzm) {
 #pragma unroll 8
       for (int q = 0; q < NX; q++) acc += d.Gt[q * NX + i] * P(k, q);
     }
     __syncthreads();
-    if (k < NZM) P(k, i) = acc;
+    if (k < nzm) P(k, i) = acc;
   }
   __syncthreads();
   // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities
@@ -1402,8 +1466,8 @@ __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
   R* wan = d.dwdt[na];
   R* unc = d.dudt[nc];
   R* wnc = d.dwdt[nc];
-  for (int k = grp; k < NZM; k += 2) {
-    size_t o = ((size_t)c * NZM + k) * NX + i;
+  for (int k = grp; k < nzm; k += 2) {
+    size_t o = ((size_t)c * nzm + k) * NX + i;
     size_t ow = ((size_t)c * nz + k) * NX + i;
     double pk = P(k, i);
     double pim = P(k, im);
@@ -1421,7 +1485,7 @@ __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
     d.u[o] = 0.5 * (uu + un) * (rho[k] * dtdx);
     d.w[ow] = 0.5 * (ww + wn) * (rhow[k] * dtdz);
   }
-  if (grp == 0) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
+  if (grp == 0) wnc[((size_t)c * nz + nzm) * NX + i] = 0.0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1446,13 +1510,13 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   CRM_STOP_CHECK(d);
   constexpr int NX = 64, NK = 64;
   extern __shared__ double lds[];
-  double* P = lds;              // [NK][PLD] slab: rhs, coefficients, pressure
-  double* ALF = lds + NK * PLD; // [NZM][64] Thomas factors
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = blockIdx.x;  // grid is exactly ncrms blocks
+  double* P = lds;              // [NK][PLD] slab: rhs, coefficients, pressure
+  double* ALF = lds + NK * PLD; // [nzm][64] Thomas factors
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane;
   G s;  // 3D (8 x 8 columns): the transforms are the x-y Kronecker operators, dy == dx
-  const int nz = NZM + 1;
+  const int nzm = NZM > 0 ? NZM : d.nzm, nz = nzm + 1;
   const double dz = inst(d, I_DZ, c);
   const double* adz = prof(d, P_ADZ, c);
   const double* adzw = prof(d, P_ADZW, c);
@@ -1463,7 +1527,7 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   const R *ua = d.dudt[na], *ub = d.dudt[nb], *uc = d.dudt[nc];
   const R *wa = d.dwdt[na], *wb = d.dwdt[nb], *wc = d.dwdt[nc];
   const int r16 = lane & 15, q4 = lane >> 4;
-  // 1. right-hand side (rows >= NZM of the slab are zero).  The loads of UNR levels are
+  // 1. right-hand side (rows >= nzm of the slab are zero).  The loads of UNR levels are
   // issued before any of them is used (2D), so that they are in flight together.
   static_assert(UNR == 1 || !G::Y3, "levels per trip: 2D only");
   for (int k0 = wave; k0 < NK; k0 += 4 * UNR) {
@@ -1471,8 +1535,8 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
 #pragma unroll
     for (int q = 0; q < UNR; q++) {
       const int k = k0 + 4 * q;
-      if (!G::Y3 && k < NZM) {
-        const size_t o = ((size_t)c * NZM + k) * NX + i, ow = ((size_t)c * nz + k) * NX + i, own = ow + NX;
+      if (!G::Y3 && k < nzm) {
+        const size_t o = ((size_t)c * nzm + k) * NX + i, ow = ((size_t)c * nz + k) * NX + i, own = ow + NX;
         ld_[q][0] = d.u[o];
         ld_[q][1] = ua[o];
         ld_[q][2] = ub[o];
@@ -1491,8 +1555,8 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
     for (int q = 0; q < UNR; q++) {
     const int k = k0 + 4 * q;
     double pv = 0.0;
-    if (k < NZM) {
-      size_t o = ((size_t)c * NZM + k) * NX + i;
+    if (k < nzm) {
+      size_t o = ((size_t)c * nzm + k) * NX + i;
       size_t ow = ((size_t)c * nz + k) * NX + i;
       size_t own = ow + NX;
       double rdz = 1. / (adz[k] * dz);
@@ -1545,12 +1609,12 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   transform(d.Ft);
   // 3. Thomas sweep per slot (wave 0, lane = slot), factors in LDS
   if (wave == 0) {
-    static_assert(NZM <= 64, "one level per lane");
+    static_assert(NZM <= 64, "one level per lane");  // NZM == 0: the host admits nzm <= 64
     const double eig = d.eign[i];
     // the level coefficients do not depend on the recurrence: lane k computes level k's,
     // and the sweep broadcasts them (the sweep then waits on one division per level, not
     // three plus the profile loads)
-    const int kl = i < NZM ? i : NZM - 1;
+    const int kl = i < nzm ? i : nzm - 1;
     const double ac_l = rhow[kl] / (adz[kl] * adzw[kl] * dz * dz);
     const double cc_l = rhow[kl + 1] / (adz[kl] * adzw[kl + 1] * dz * dz);
     const double rho_l = rho[kl];
@@ -1570,7 +1634,7 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
     ALF[i] = alfa;
     double beta = P[i] * b;
     P[i] = beta;
-    for (int k = 1; k < NZM - 1; k++) {
+    for (int k = 1; k < nzm - 1; k++) {
       double ak = acoef(k), ck = ccoef(k);
       double e = 1.0 / (eig * lane_of(rho_l, k) - ak - ck + ak * alfa);
       alfa = -ck * e;
@@ -1579,11 +1643,11 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
       P[k * PLD + i] = beta;
     }
     {
-      int k = NZM - 1;
+      int k = nzm - 1;
       double ak = acoef(k);
       double top = (P[k * PLD + i] - ak * beta) / (eig * rho[k] - ak + ak * alfa);
       P[k * PLD + i] = top;
-      for (int kk = NZM - 2; kk >= 0; kk--) {
+      for (int kk = nzm - 2; kk >= 0; kk--) {
         top = ALF[kk * 64 + i] * top + P[kk * PLD + i];
         P[kk * PLD + i] = top;
       }
@@ -1597,13 +1661,13 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
   R* wan = d.dwdt[na];
   R* unc = d.dudt[nc];
   R* wnc = d.dwdt[nc];
-  for (int k0 = wave; k0 < NZM; k0 += 4 * UNR) {
+  for (int k0 = wave; k0 < nzm; k0 += 4 * UNR) {
     double lg_[UNR][8];
 #pragma unroll
     for (int q = 0; q < UNR; q++) {
       const int k = k0 + 4 * q;
-      if (k < NZM) {
-        const size_t o = ((size_t)c * NZM + k) * NX + i, ow = ((size_t)c * nz + k) * NX + i;
+      if (k < nzm) {
+        const size_t o = ((size_t)c * nzm + k) * NX + i, ow = ((size_t)c * nz + k) * NX + i;
         lg_[q][0] = ua[o];
         lg_[q][1] = wa[ow];
         lg_[q][2] = d.u[o];
@@ -1617,8 +1681,8 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
 #pragma unroll
     for (int q = 0; q < UNR; q++) {
     const int k = k0 + 4 * q;
-    if (k >= NZM) break;
-    size_t o = ((size_t)c * NZM + k) * NX + i;
+    if (k >= nzm) break;
+    size_t o = ((size_t)c * nzm + k) * NX + i;
     size_t ow = ((size_t)c * nz + k) * NX + i;
     double pk = P[k * PLD + i];
     double pim = s.rot(pk, -1);
@@ -1648,7 +1712,7 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
     d.w[ow] = 0.5 * (ww + wn) * (rhow[k] * dtdz);
     }
   }
-  if (wave == 0) wnc[((size_t)c * nz + NZM) * NX + i] = 0.0;
+  if (wave == 0) wnc[((size_t)c * nz + nzm) * NX + i] = 0.0;
 }
 
 // k_press64 for 2D CRMs of 64 columns on 8 waves (one instance per 512-thread workgroup):
@@ -1819,8 +1883,8 @@ __global__ void __launch_bounds__(512) k_press64r(DevT<R> d, int na, int nb, int
 }
 
 template <int NZM>
-constexpr size_t press64_lds_bytes() {
-  return (size_t)(64 * PLD + NZM * 64) * sizeof(double);
+constexpr size_t press64_lds_bytes() {  // NZM == 0: room for the most levels, 64
+  return (size_t)(64 * PLD + (NZM > 0 ? NZM : 64) * 64) * sizeof(double);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1866,6 +1930,28 @@ __device__ __forceinline__ T term_vel(const TermVel& tv, T q, T rho, T tabs) {
   return v;
 }
 
+// An upper bound, to within a relative 1e-5, of term_vel's FP64 value, with the three powers in
+// FP32: the regime weights are term_vel's own (FP64), and each power's argument is raised to at
+// least 1e-30 first.  x**c is increasing in x for c > 0, so that can only raise the velocity, and
+// it keeps FP32's underflow of a tiny argument from turning a small velocity into zero.  The
+// FP32 powers (hardware base-2 logarithm and exponential) are within ~1e-6 relative here.
+__device__ __forceinline__ float term_vel_bound(const TermVel& tv, double q, double rho, double tabs) {
+  float v = 0.f;
+  if (q > QP_THRESHOLD) {
+    const double omp = fmax(0.0, fmin(1.0, (tabs - TPRMIN) * A_PR()));
+    const double omg = fmax(0.0, fmin(1.0, (tabs - TGRMIN) * A_GR()));
+    const double qrr = omp * q, qsg = q - qrr, qgg = omg * qsg, qss = qsg - qgg;
+    auto pw = [](double x, double c) { return fpow<float>((float)fmax(x, 1e-30), (float)c); };
+    float pr = 0.f, pg = 0.f, ps = 0.f;
+    if (omp > 0.) pr = pw(rho * qrr, tv.crain);
+    if (omp < 1. && omg > 0.) pg = pw(rho * qgg, tv.cgrau);
+    if (omp < 1. && omg < 1.) ps = pw(rho * qss, tv.csnow);
+    const float fomp = (float)omp, fomg = (float)omg;
+    const float vsg = fomg * (float)tv.vgrau * pg + (1.f - fomg) * (float)tv.vsnow * ps;
+    v = fomp * (float)tv.vrain * pr + (1.f - fomp) * vsg;
+  }
+  return v;
+}
 
 // One scalar of MPDATA, software-pipelined over levels.  At iteration K stage s works on
 // level K-s; ring variables are named by how far behind the front they are at the top of the
@@ -2065,8 +2151,21 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
       }
       if (want_cfl) {
         R wmax = dz * adz[L5 >= 1 ? L5 - 1 : 0] / (R)d.dtn;
-        R wp = pratio[L5] * term_vel(d.tv, fn, rho[L5], tq_b);
-        pcfl = fmax(pcfl, wp * (R(1.) / wmax));
+        if constexpr (std::is_same<R, double>::value) {
+          // Only a Courant number above 0.9 changes nprec = max(1, ceiling(prec_cfl/0.9)), so
+          // the exact FP64 value is needed only where the FP32 bound, with a 1e-3 margin, could
+          // exceed 0.9 (or is not a number); elsewhere the bound, below 0.9, stands in for it
+          // and nprec is the same
+          R cfl = pratio[L5] * (R)term_vel_bound(d.tv, fn, rho[L5], tq_b) * (R(1.) / wmax);
+          if (!(cfl * R(1.001) <= R4(0.9))) {
+            R wp = pratio[L5] * term_vel(d.tv, fn, rho[L5], tq_b);
+            cfl = wp * (R(1.) / wmax);
+          }
+          pcfl = fmax(pcfl, cfl);
+        } else {
+          R wp = pratio[L5] * term_vel(d.tv, fn, rho[L5], tq_b);
+          pcfl = fmax(pcfl, wp * (R(1.) / wmax));
+        }
       }
     }
     f_5 = f_4;
@@ -2338,10 +2437,14 @@ __global__ void CRM_LB k_diffuse(DevT<R> d) {
   }
   R tkh_k = Q(d.tkh, 0), tkh_n = Q(d.tkh, 1);
   for (int k = 0; k < nzm; k++) {
+    // level k+2, clamped at the top: a value read past it is never used (level k+1's values
+    // enter only inner levels, k+1 <= nzm-2), and a load that always issues needs no copy
+    // that would wait for earlier loads
+    const int k2 = ::min(k + 2, nzm - 1);
     R f_nn[NF];
 #pragma unroll
-    for (int q = 0; q < NF; q++) f_nn[q] = (k + 2 < nzm) ? Q(fs[q], k + 2) : R(0.0);
-    const R tkh_nn = (k + 2 < nzm) ? Q(d.tkh, k + 2) : R(0.0);
+    for (int q = 0; q < NF; q++) f_nn[q] = Q(fs[q], k2);
+    const R tkh_nn = Q(d.tkh, k2);
     const R rdx5 = R(0.5) * rdx2 * grdf[k];
     const R tkx = rdx5 * (tkh_k + s.rot(tkh_k, 1));
     R tky = R(0.0);
@@ -2354,6 +2457,9 @@ __global__ void CRM_LB k_diffuse(DevT<R> d) {
       tkz = rdz5 * (tkh_k + tkh_n);
     }
     const R rhoi = irhoadz[k];
+    // FP64: the level sums of qt and qp (vertical SGS flux, change), [2 (q - 1)] and
+    // [2 (q - 1) + 1], in one batched reduction
+    R red[2 * (NF - 1)];
 #pragma unroll
     for (int q = 0; q < NF; q++) {
       const R fk = f_k[q];
@@ -2364,25 +2470,39 @@ __global__ void CRM_LB k_diffuse(DevT<R> d) {
         dfh = dfh - (flxy - s.roty(flxy, -1));
       }
       R flxv;
-      if (inner) {
+      if (inner)
         flxv = -tkz * (f_n[q] - fk) * rhoi_z;
-        if (fluxes[q]) {
-          const R sv = s.sum(flxv);
-          if (live && i == 0) fluxes[q][k + 1] = sv;
-        }
-      } else {
+      else
         flxv = R(0.0) * (R(1.) / dz) * (R(1.) / adzw[nzm]) * rhow[nzm];
-      }
       const R dfdt = dtn * (dfh - (flxv - flx_prev[q]) * rhoi);
       const R fnew = fk + dfdt;
       if (live) fs[q][((size_t)c * nzm + k) * NX + i] = fnew;
-      if (fdiffs[q]) {
-        const R sd = s.sum(fnew - fk);
-        if (live && i == 0) fdiffs[q][k] = sd;
+      if (q > 0) {
+        if constexpr (std::is_same<R, double>::value) {  // reduced together below
+          red[2 * (q - 1)] = flxv;
+          red[2 * (q - 1) + 1] = fnew - fk;
+        } else {  // FP32: one DPP reduction each, in place (measured faster than batched)
+          if (inner) {
+            const R sv = s.sum(flxv);
+            if (live && i == 0) fluxes[q][k + 1] = sv;
+          }
+          const R sd = s.sum(fnew - fk);
+          if (live && i == 0) fdiffs[q][k] = sd;
+        }
       }
       flx_prev[q] = flxv;
       f_k[q] = f_n[q];
       f_n[q] = f_nn[q];
+    }
+    if constexpr (std::is_same<R, double>::value) {
+      s.sums(red);
+      if (live) {
+#pragma unroll
+        for (int q = 1; q < NF; q++) {
+          if (inner && s.template sum_owner<R>(2 * (q - 1))) fluxes[q][k + 1] = red[2 * (q - 1)];
+          if (s.template sum_owner<R>(2 * (q - 1) + 1)) fdiffs[q][k] = red[2 * (q - 1) + 1];
+        }
+      }
     }
     tkh_k = tkh_n;
     tkh_n = tkh_nn;
@@ -2475,23 +2595,44 @@ __global__ void k_mlp_inputs(DevT<R> d, float* X) {
 
 // diagnose (diagnose.F90:8-177) at level k: tabs from t and the condensate split, the level
 // means, the cloud-ice level range of ice_fall; writes the microphysics fields and tabs.
+// With src_evp (from micro_proc), the level sums of the precipitation source and evaporation
+// join the same batched reduction and are written to P_QPSRC and P_QPEVP.
 template <int NX, typename R, typename S_>
 __device__ __forceinline__ void diagnose_level(const DevT<R>& d, const S_& s, int c, bool live, int k, size_t o,
                                                bool y3, R t, R q, R qp, R qn, R qv, R qcl, R qci, R qpl, R qpi,
-                                               R u, R v, int& kmin, int& kmax) {
+                                               R u, R v, int& kmin, int& kmax, const R* src_evp = nullptr) {
   const R factor = R(1.) / (R)NX;
   const cprof<R> gamaz = profc(d, P_GAMAZ, c);
   R tabs = t - gamaz[k] + FAC_COND * (qcl + qpl) + FAC_SUB * (qci + qpi);
-  R u0 = s.sum(u) * factor;
-  R v0 = y3 ? s.sum(v) * factor : R(0.0);
-  R t0 = s.sum(t) * factor;
-  R tabs0 = s.sum(tabs) * factor;
-  R q0 = s.sum(qv + qcl + qci) * factor;
-  R qn0 = s.sum(qcl + qci) * factor;
-  R qp0 = s.sum(qpl + qpi) * factor;
+  // FP64: the level sums in one batched reduction, q0 and qn0 at 0 and 4, which one lane holds
+  // together (P_QV0 is their difference); FP32: one DPP reduction each (measured faster)
+  R sv[9] = {qv + qcl + qci, u, t, tabs, qcl + qci, qpl + qpi, y3 ? v : R(0.0),
+             src_evp ? src_evp[0] : R(0.0), src_evp ? src_evp[1] : R(0.0)};
+  if constexpr (std::is_same<R, double>::value) {
+    s.sums(sv);
+  } else {
+    sv[1] = s.sum(u);
+    if (y3) sv[6] = s.sum(v);
+    sv[2] = s.sum(t);
+    sv[3] = s.sum(tabs);
+    sv[0] = s.sum(sv[0]);
+    sv[4] = s.sum(sv[4]);
+    sv[5] = s.sum(sv[5]);
+    if (src_evp) {
+      sv[7] = s.sum(sv[7]);
+      sv[8] = s.sum(sv[8]);
+    }
+  }
+  const R q0 = sv[0] * factor, u0 = sv[1] * factor, t0 = sv[2] * factor, tabs0 = sv[3] * factor;
+  const R qn0 = sv[4] * factor, qp0 = sv[5] * factor, v0 = y3 ? sv[6] * factor : R(0.0);
   if (qcl + qci > R(0.) && tabs < R4(273.15)) {
     kmin = ::min(kmin, k);
     kmax = ::max(kmax, k);
+  }
+  auto own = [&](int j) { return s.template sum_owner<R>(j); };
+  if (live && src_evp) {
+    if (own(7)) prof(d, P_QPSRC, c)[k] = sv[7];
+    if (own(8)) prof(d, P_QPEVP, c)[k] = sv[8];
   }
   if (live) {
     d.qt[o] = q;
@@ -2503,16 +2644,18 @@ __device__ __forceinline__ void diagnose_level(const DevT<R>& d, const S_& s, in
     d.qci[o] = qci;
     d.qpl[o] = qpl;
     d.qpi[o] = qpi;
-    if (s.i == 0) {
-      prof(d, P_U0, c)[k] = u0;
-      prof(d, P_T0, c)[k] = t0;
-      prof(d, P_TABS0, c)[k] = tabs0;
+    if (own(0)) {
       prof(d, P_Q0, c)[k] = q0;
-      prof(d, P_QN0, c)[k] = qn0;
-      prof(d, P_QP0, c)[k] = qp0;
       prof(d, P_QV0, c)[k] = q0 - qn0;
-      if (y3) prof(d, P_V0, c)[k] = v0;
-      if constexpr (!std::is_same<R, double>::value) {  // FP32 copies read by the bulk kernels
+    }
+    if (own(1)) prof(d, P_U0, c)[k] = u0;
+    if (own(2)) prof(d, P_T0, c)[k] = t0;
+    if (own(3)) prof(d, P_TABS0, c)[k] = tabs0;
+    if (own(4)) prof(d, P_QN0, c)[k] = qn0;
+    if (own(5)) prof(d, P_QP0, c)[k] = qp0;
+    if (y3 && own(6)) prof(d, P_V0, c)[k] = v0;
+    if constexpr (!std::is_same<R, double>::value) {  // FP32 copies read by the bulk kernels
+      if (s.i == 0) {
         if (y3) profr(d, P_V0, c)[k] = v0;
         profr(d, P_U0, c)[k] = u0;
         profr(d, P_T0, c)[k] = t0;
@@ -2578,12 +2721,14 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
     R qp = qp_pre;
     const R q_in = q, qp_in = qp;  // the entry state, for recorded training pairs
     const R u_k = u_pre, v_k = v_pre;
-    if (k + 1 < nzm) {
-      t_pre = d.t[o + NX];
-      qt_pre = d.qt[o + NX];
-      qp_pre = d.qp[o + NX];
-      u_pre = d.u[o + NX];
-      if (y3) v_pre = d.v[o + NX];
+    {  // clamped at the top, where the values are not used again (an unconditional load needs
+       // no copy that waits for earlier loads)
+      const size_t on = ((size_t)c * nzm + ::min(k + 1, nzm - 1)) * NX + i;
+      t_pre = d.t[on];
+      qt_pre = d.qt[on];
+      qp_pre = d.qp[on];
+      u_pre = d.u[on];
+      if (y3) v_pre = d.v[on];
     }
     // the network's rates where it applies (inside its training envelope): condensate and
     // precipitation stepped by the rates, clipped at zero, total water q + qp kept, the
@@ -2743,10 +2888,15 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
       q = q + (dq - qp);
     }
     }  // plain microphysics
-    R ssrc = s.sum(src), sevp = s.sum(evp);
-    if (live && i == 0) {
-      prof(d, P_QPSRC, c)[k] = ssrc;
-      prof(d, P_QPEVP, c)[k] = sevp;
+    // FP64: the level sums of src and evp are taken with diagnose's batched reduction; here
+    // in FP32, or before the means of the mean-state acceleration
+    constexpr bool batched = std::is_same<R, double>::value;
+    if (accel || !batched) {
+      R ssrc = s.sum(src), sevp = s.sum(evp);
+      if (live && i == 0) {
+        prof(d, P_QPSRC, c)[k] = ssrc;
+        prof(d, P_QPEVP, c)[k] = sevp;
+      }
     }
     // micro_diagnose, then diagnose (or, with mean-state acceleration pending, the means it
     // needs; k_accel then applies it and diagnoses)
@@ -2792,7 +2942,9 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
         d.qpi[o] = qpi;
       }
     } else {
-      diagnose_level<NX, R>(d, s, c, live, k, o, y3, t, q, qp, qn, qv, qcl, qci, qpl, qpi, u_k, v_k, kmin, kmax);
+      const R se[2] = {src, evp};
+      diagnose_level<NX, R>(d, s, c, live, k, o, y3, t, q, qp, qn, qv, qcl, qci, qpl, qpi, u_k, v_k, kmin, kmax,
+                            batched ? se : nullptr);
     }
   }
   if (accel) return;
@@ -2887,7 +3039,7 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
   const double* adz = prof(d, P_ADZ, c);
   const double* pres = prof(d, P_PRES, c);
   const double thr = R4(0.001), wmin = 2.;
-  const int nxr = d.nx_rad, g = NX / nxr;
+  const int nxr = d.nx_rad, g = NX / nxr, ir = rad_col(d, i);
   auto Q = [&](const R* p, int k) -> double { return p[((size_t)c * nzm + k) * NX + i]; };
   auto Wv = [&](int k) -> double { return d.w[((size_t)c * nz + k) * NX + i]; };
   // the fields and profiles of one level; the sweep runs down from the top and reads level
@@ -2916,11 +3068,11 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
   load(nzm - 1, cur);
   // radiation sums: exactly one lane owns each (instance, level, radiation column), so they
   // are plain read-modify-writes; the reads are issued at the top of the level
-  const bool rad_owner = live && (i & (g - 1)) == 0;
+  const bool rad_owner = live && rad_lead(d, i);
   const size_t rad_blk = (size_t)d.ncrms * nzm * nxr;
   for (int k = nzm - 1; k >= 0; k--) {
-    if (k >= 1) load(k - 1, nxt);
-    const size_t rad_base = (size_t)c * nzm * nxr + (size_t)k * nxr + i / g;
+    load(::max(k - 1, 0), nxt);  // at k = 0 level 0 again, unused (unconditional, as above)
+    const size_t rad_base = (size_t)c * nzm * nxr + (size_t)k * nxr + ir;
     // the sums are R: in the mixed engine FP32, which halves the bytes of this kernel's
     // largest traffic (each addition rounds once, from the FP64 sum of the two terms)
     R ra[5] = {0, 0, 0, 0, 0};
@@ -2984,12 +3136,12 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
     s.multi_sum(sv);
     // radiation-column sums (crm_module.F90:979-1015)
     double r_t = tabs, r_qv = fmax(0.0, cur.qv), r_qc = qcl, r_qi = qci, r_cld = (qcl + qci > 0) ? 1.0 : 0.0;
-    if (g > 1) {
-      r_t = s.gsum(r_t, g);
-      r_qv = s.gsum(r_qv, g);
-      r_qc = s.gsum(r_qc, g);
-      r_qi = s.gsum(r_qi, g);
-      r_cld = s.gsum(r_cld, g);
+    if (g > 1) {  // the group's columns: contiguous lanes in 2D, a block of rows in 3D
+      r_t = s.gsum2(r_t, d.rad_gx, d.rad_row, d.rad_gy);
+      r_qv = s.gsum2(r_qv, d.rad_gx, d.rad_row, d.rad_gy);
+      r_qc = s.gsum2(r_qc, d.rad_gx, d.rad_row, d.rad_gy);
+      r_qi = s.gsum2(r_qi, d.rad_gx, d.rad_row, d.rad_gy);
+      r_cld = s.gsum2(r_cld, d.rad_gx, d.rad_row, d.rad_gy);
     }
     if (live) {
       // the reference's index mix for the cloud top (crm_module.F90:918-925)

@@ -14,6 +14,8 @@ module crm_hip_bindings
        CRM_ERR_NXRAD = 3, CRM_ERR_DAMPING = 4, CRM_ERR_NAN = 5, CRM_ERR_DEVICE = 6, &
        CRM_ERR_NOMEM = 7, CRM_ERR_UNSUPPORTED = 8, CRM_ERR_ACCEL = 9, CRM_ERR_SCRATCH = 10
   integer(c_int), parameter, public :: CRM_PREC_FP64 = 0, CRM_PREC_MIXED = 1
+  integer(c_int), parameter, public :: CRM_GRAPHS_OFF = 0, CRM_GRAPHS_SUBSTEP = 1, &
+                                       CRM_GRAPHS_CALL = 2, CRM_GRAPHS_JOINT = 3
 
   type, bind(C), public :: crm_dims
     integer(c_int) :: ncrms, ld, plev, nx, ny, nz, nx_rad, ny_rad
@@ -135,6 +137,19 @@ module crm_hip_bindings
       type(c_ptr), value :: ctx
       integer(c_int), value :: use_crm_accel, crm_accel_uv
       real(c_double), value :: crm_accel_factor
+    end function
+
+    ! the reference's docoriolis switch (params.F90:107); 3D CRMs only
+    integer(c_int) function crm_hip_set_coriolis(ctx, docoriolis) bind(C, name="crm_hip_set_coriolis")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: docoriolis
+    end function
+
+    integer(c_int) function crm_hip_set_graphs(ctx, mode) bind(C, name="crm_hip_set_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
     end function
 
     integer(c_int) function crm_hip_get_stats(ctx, stats) bind(C, name="crm_hip_get_stats")

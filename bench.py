@@ -217,6 +217,7 @@ def measure(prec, case, args, torch):
 UNEVEN = [(2048, 16, 1, 28, 4000.0, 20.0), (1024, 32, 1, 28, 4000.0, 20.0),
           (1024, 64, 1, 58, 1000.0, 5.0), (256, 128, 1, 58, 1000.0, 5.0),
           (128, 8, 8, 28, 4000.0, 20.0)]
+UNEVEN_NAMES = [f"{n}x{nx}x{ny}x{nz}" for n, nx, ny, nz, dx, dt in UNEVEN]
 
 
 def measure_uneven(prec, args, torch):
@@ -224,15 +225,17 @@ def measure_uneven(prec, args, torch):
     dt_gl = WORKLOAD["dt_gl"]
     cases = [make_case(n, nx, nz, dx, dt, ny=ny, randomize=True) for n, nx, ny, nz, dx, dt in UNEVEN]
     hips = [CrmHip(0, prec) for _ in UNEVEN]
-    scheds = []
+    scheds, variants = [], []
     for h, c, cls in zip(hips, cases, UNEVEN):
         scratch = DeviceCase(c.copy(), "cuda:0")
-        # tuning calls of 3 substeps on a scratch copy
+        # tuning calls of 3 substeps on a scratch copy: workgroup size and launch variant
         sched = h.autotune(scratch, 3 * cls[5])
         # classes with a fixed launch shape (one instance per workgroup for the pressure solve,
         # one thread per instance or level for the set-up kernels) are not reported
         scheds.append({k: v for k, v in sched.items()
                        if k not in ("setup", "setperturb", "precip_init", "press")})
+        v = h.variants()
+        variants.append({"press": v["press"], "mpdata": v["mpdata"]})
         del scratch
     dcs = [DeviceCase(c, "cuda:0") for c in cases]
     torch.cuda.synchronize()
@@ -247,14 +250,47 @@ def measure_uneven(prec, args, torch):
     sec = time.perf_counter() - t0
     colsteps = sum(n * nx * ny * int(dt_gl / dt) for n, nx, ny, nz, dx, dt in UNEVEN)
     graphs = sum(h.graph_launches() for h in hips)
+    # each class alone on the device (its own whole-call graph): the critical path of the batch
+    alone = []
+    for j, (h, dc) in enumerate(zip(hips, dcs)):
+        h.run_device(dc, dt_gl)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.alone_steps):
+            h.run_device(dc, dt_gl)
+        torch.cuda.synchronize()
+        alone.append(1e3 * (time.perf_counter() - t1) / args.alone_steps)
+        log(f"[uneven {prec}] {UNEVEN_NAMES[j]} alone: {alone[-1]:.1f} ms per GCM step")
+    # the classes' whole calls captured into one graph (CRM_GRAPHS_JOINT) instead of one graph
+    # per class on its own stream
+    for h in hips:
+        h.set_graphs(3)
+    run_many(hips, dcs, dt_gl)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.alone_steps):
+        run_many(hips, dcs, dt_gl)
+    torch.cuda.synchronize()
+    joint_ms = 1e3 * (time.perf_counter() - t1) / args.alone_steps
+    log(f"[uneven {prec}] classes in one joint graph: {joint_ms:.1f} ms per GCM step")
     for h in hips:
         h.close()
+    levels = sum(n * nx * ny * nz * int(dt_gl / dt) for n, nx, ny, nz, dx, dt in UNEVEN)
     return {"value": colsteps * args.steps / sec, "unit": "column-steps/s",
             "ms_per_step": 1e3 * sec / args.steps,
+            "column_level_steps_per_s": levels * args.steps / sec,
             "classes": [f"{n}x{nx}x{ny}x{nz} dx {dx / 1000:g} km dt {dt:g} s" for n, nx, ny, nz, dx, dt in UNEVEN],
-            "schedule_threads_per_workgroup": scheds, "graph_substeps": graphs,
-            "note": "one 1200 s GCM step of every class per step, classes concurrent on their own "
-                    "streams"}
+            "schedule_threads_per_workgroup": scheds, "schedule_variants": variants,
+            "graph_substeps": graphs,
+            "class_alone_ms_per_step": alone,
+            "sum_alone_ms_per_step": sum(alone),
+            "critical_path_class": UNEVEN_NAMES[max(range(len(alone)), key=lambda j: alone[j])],
+            "joint_graph_ms_per_step": joint_ms,
+            "note": "one 1200 s GCM step of every class per step (crm_hip_run_many): each "
+                    "class replays its own whole-call HIP graph on its own stream, "
+                    "concurrently; class_alone_ms_per_step is each class run by itself; "
+                    "joint_graph_ms_per_step is the classes' calls captured into one graph "
+                    "(CRM_GRAPHS_JOINT)"}
 
 
 def _clone(dc):
@@ -398,6 +434,8 @@ def main():
                          "at most 16)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="skip the mixed-precision measurement")
+    ap.add_argument("--alone-steps", type=int, default=3,
+                    help="GCM steps per class for the uneven batch's per-class and joint-graph times")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch measurement")
     ap.add_argument("--no-learned", action="store_true",

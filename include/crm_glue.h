@@ -2,8 +2,9 @@
  * physics state, the surface-flux bypass, the tendencies and radiation conversions after the
  * call, and the column energy and water check.  Replaces the host loops of crm_physics_tend
  * (components/cam/src/physics/cam/crm_physics.F90:1050-1330), crm_surface_flux_bypass_tend
- * (crm_physics.F90:1652-1699) and check_energy_chng (cam/check_energy.F90:358-561), so a
- * physics state resident in HBM reaches the CRM and returns without host round trips.
+ * (crm_physics.F90:1652-1699), check_energy_chng (cam/check_energy.F90:358-561) and the CRM
+ * side of the radiation (rrtmg/radiation.F90:1584-1636, 2393-2398), so a physics state
+ * resident in HBM reaches the CRM and returns without host round trips.
  *
  * Every array is a device pointer in the Fortran layout of the physics types: (pcols, pver)
  * element (i, k) at p[i + pcols*k], 0-based, level 0 the model top; (pcols, pver+1)
@@ -70,6 +71,36 @@ int crm_hip_gcm_tend(crm_hip_ctx *ctx, int ncol, int pcols, int pver, int crm_nz
                      const double *pdel, crm_output_view *out, const crm_rad_view *rad, double *qrs,
                      double *qrl, double *ptend_s, double *ptend_qv, double *ptend_qcl, double *ptend_qci,
                      double *sp_rad_flux);
+
+/* The GCM column's cloud fields that the radiation reads (pcols,pver): cloud fraction, ice
+ * fraction, in-cloud ice and liquid water paths [kg/m2]. */
+typedef struct crm_gcm_cloud_view {
+  const double *cld, *fice, *cicewp, *cliqwp;
+} crm_gcm_cloud_view;
+
+/* The radiation's inputs in each CRM radiation column: (pcols,nrad,pver) each, element
+ * (i, r, k) at p[i + pcols*(r + nrad*k)], r = ii + nx_rad*jj (0-based). */
+typedef struct crm_rad_column_view {
+  double *t, *qv, *qc, *qi, *cld, *fice, *cicewp, *cliqwp;
+} crm_rad_column_view;
+
+/* Before the radiation (rrtmg/radiation.F90:1584-1636, sam1mom): its inputs in every CRM
+ * radiation column.  Each radiation column starts from the physics state (t, qv, qcl, qci) and
+ * the GCM cloud fields; on the CRM levels (GCM level pver-1-m of CRM level m) they are taken
+ * from that column's crm_rad averages: cld; where cld > 0 the ice fraction qi/(qc+qi) and the
+ * in-cloud water paths q pdel/gravit/max(0.01, cld) of ice and liquid, else 0; qi, qc;
+ * vapour max(1e-9, qv); t.  With sam1mom the effective radii are the GCM column's in every
+ * radiation column (radiation.F90:1675-1686), so they are not repeated here.  The radiation
+ * scheme itself is the caller's. */
+int crm_hip_rad_columns(crm_hip_ctx *ctx, int ncol, int pcols, int pver, int crm_nz, int nrad,
+                        const crm_gcm_state_view *state, const crm_gcm_cloud_view *cloud,
+                        const crm_rad_view *rad, const crm_rad_column_view *cols);
+
+/* After the radiation (radiation.F90:2393-2398): crm_rad qrad (pcols,nx_rad,ny_rad,crm_nz) is
+ * the sum qrs_crm + qrl_crm of the shortwave and longwave heating the radiation computed in
+ * each radiation column (same layout), written through rad->qrad. */
+int crm_hip_rad_qrad(crm_hip_ctx *ctx, int ncol, int pcols, int nrad, int crm_nz, const double *qrs_crm,
+                     const double *qrl_crm, const crm_rad_view *rad);
 
 /* check_energy_chng (check_energy.F90:358-561) without rain and snow constituents (absent with
  * sam1mom): the column integrals of total energy te and total water tw of the state, the

@@ -249,9 +249,13 @@ int crm_hip_autotune(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_in
  * replay a captured graph of the whole substep, the subcycle count decided on the host before
  * each.  CRM_GRAPHS_CALL (the default): a call replays one captured graph of all its substeps
  * with one subcycle each; the subcycle count is decided on the device, and a substep that needs
- * more stops the replay there and continues as with CRM_GRAPHS_SUBSTEP.  CRM_GRAPHS_OFF: plain
- * launches.  Results are bitwise the same in every mode. */
-enum { CRM_GRAPHS_OFF = 0, CRM_GRAPHS_SUBSTEP = 1, CRM_GRAPHS_CALL = 2 };
+ * more stops the replay there and continues as with CRM_GRAPHS_SUBSTEP; crm_hip_run_many runs
+ * each class's graph on its own stream, concurrently.  CRM_GRAPHS_JOINT: as CRM_GRAPHS_CALL, and
+ * when every context of a crm_hip_run_many call has this mode, the classes' whole-call graphs
+ * are captured together into one graph (class streams forked from and joined into one capture)
+ * and launched once.  CRM_GRAPHS_OFF: plain launches.  Results are bitwise the same in every
+ * mode. */
+enum { CRM_GRAPHS_OFF = 0, CRM_GRAPHS_SUBSTEP = 1, CRM_GRAPHS_CALL = 2, CRM_GRAPHS_JOINT = 3 };
 int crm_hip_set_graphs(crm_hip_ctx *ctx, int mode);
 
 int crm_hip_get_stats(const crm_hip_ctx *ctx, crm_hip_stats *stats);

@@ -83,6 +83,59 @@ def gcm_tend(ncol, crm_nz, pdel, out, qrad, qrs, qrl, ptend):
     return f
 
 
+def incloud_path(q, pdel, cld):
+    """radiation.F90:1601-1604: in-cloud water path [kg/m2] of a level, the cover floored at 1%."""
+    return q * pdel / GRAVIT / max(0.01, cld)
+
+
+def rad_columns(ncol, crm_nz, state, cloud, rad):
+    """The radiation's inputs in every CRM radiation column (rrtmg/radiation.F90:1584-1636,
+    sam1mom): for radiation column (ii, jj), r = ii + nx_rad jj, the physics state and the GCM
+    cloud fields with the CRM levels (GCM level k = pver-1-m of CRM level m) replaced from the
+    crm_rad averages.  state: dict t, qv, qcl, qci, pdel (pcols, pver); cloud: dict cld, fice,
+    cicewp, cliqwp (pcols, pver); rad: dict temperature, qv, qc, qi, cld (pcols, nx_rad,
+    ny_rad, crm_nz).  Returns dict t, qv, qc, qi, cld, fice, cicewp, cliqwp of shape
+    (pcols, nrad, pver) (columns past ncol zero)."""
+    pcols, pver = state["t"].shape
+    nxr, nyr = rad["cld"].shape[1:3]
+    nrad = nxr * nyr
+    src = {"t": state["t"], "qv": state["qv"], "qc": state["qcl"], "qi": state["qci"],
+           "cld": cloud["cld"], "fice": cloud["fice"], "cicewp": cloud["cicewp"], "cliqwp": cloud["cliqwp"]}
+    out = {f: np.zeros((pcols, nrad, pver), order="F") for f in src}
+    for jj in range(nyr):
+        for ii in range(nxr):
+            r = ii + nxr * jj
+            for f in src:
+                out[f][:ncol, r, :] = src[f][:ncol]
+            for m in range(crm_nz):
+                k = pver - 1 - m
+                for i in range(ncol):
+                    cld = rad["cld"][i, ii, jj, m]
+                    qc, qi = rad["qc"][i, ii, jj, m], rad["qi"][i, ii, jj, m]
+                    out["cld"][i, r, k] = cld                                             # :1595
+                    if cld > 0:                                                           # :1598-1604
+                        qtot = qc + qi
+                        out["fice"][i, r, k] = qi / qtot
+                        out["cicewp"][i, r, k] = incloud_path(qi, state["pdel"][i, k], cld)
+                        out["cliqwp"][i, r, k] = incloud_path(qc, state["pdel"][i, k], cld)
+                    else:                                                                 # :1605-1609
+                        out["fice"][i, r, k] = 0.0
+                        out["cicewp"][i, r, k] = 0.0
+                        out["cliqwp"][i, r, k] = 0.0
+                    out["qi"][i, r, k] = qi                                               # :1627-1630
+                    out["qc"][i, r, k] = qc
+                    out["qv"][i, r, k] = max(1.e-9, rad["qv"][i, ii, jj, m])
+                    out["t"][i, r, k] = rad["temperature"][i, ii, jj, m]
+    return out
+
+
+def rad_qrad(ncol, qrs_crm, qrl_crm, qrad):
+    """radiation.F90:2393-2398: the CRM's radiative heating, shortwave plus longwave, per
+    radiation column and CRM level (pcols, nx_rad, ny_rad, crm_nz), written into qrad."""
+    qrad[:ncol] = qrs_crm[:ncol] + qrl_crm[:ncol]
+    return qrad
+
+
 def energy_integrals(ncol, state):
     """Column integrals of check_energy.F90:440-484 (no rain or snow constituents)."""
     n = slice(0, ncol)

@@ -86,9 +86,8 @@ def test_cfl_violation_is_reported(hip):
     (dict(dt=40.0), CRM_ERR_DAMPING),         # damping.F90:39-42
     (dict(nx=48), CRM_ERR_UNSUPPORTED),       # column count without a compiled kernel
     (dict(nx=256), CRM_ERR_UNSUPPORTED),
-    (dict(nz=40), CRM_ERR_UNSUPPORTED),       # level count without a compiled kernel
-    (dict(nx=16, ny=16), CRM_ERR_UNSUPPORTED),  # 3D is compiled for 8 x 8 columns only
-    (dict(nx=8, ny=8, nx_rad=2, ny_rad=2), CRM_ERR_UNSUPPORTED),  # 3D radiation groups
+    (dict(nz=72), CRM_ERR_UNSUPPORTED),       # more levels than lanes of a wave
+    (dict(nx=16, ny=16), CRM_ERR_UNSUPPORTED),  # 3D is compiled for 8 x 8 and 4 x 4 columns
     (dict(nx=8, ny=8, nx_rad=3, ny_rad=8), CRM_ERR_NXRAD),
 ])
 def test_status_codes(hip, kw, code):

@@ -158,12 +158,14 @@ def test_run_many_matches_separate_calls():
             h.close()
 
 
-def test_run_many_one_graph():
-    """crm_hip_run_many captures the classes' whole calls into one graph (cached, so the second
-    and third calls replay it): quiet soundings run every class's graph to the end, a class
-    stopped mid-call by the test hook resumes on its own, a class with mean-state acceleration
-    runs outside the graph, and a mixed-precision class joins; every class stays bitwise equal
-    to its own separate call, call after call."""
+@pytest.mark.parametrize("mode", [2, 3], ids=["per_class_graphs", "joint_graph"])
+def test_run_many_one_graph(mode):
+    """crm_hip_run_many with CRM_GRAPHS_JOINT captures the classes' whole calls into one graph
+    (cached, so the second and third calls replay it); with CRM_GRAPHS_CALL each class replays
+    its own graph on its own stream.  Quiet soundings run every class's graph to the end, a
+    class stopped mid-call by the test hook resumes on its own, a class with mean-state
+    acceleration runs outside the graph, and a mixed-precision class joins; every class stays
+    bitwise equal to its own separate call, call after call."""
     cls = [CLASSES[0], CLASSES[1], CLASSES[2], CLASSES[3], CLASSES[1]]
     precs = ["fp64", "fp64", "mixed", "fp64", "fp64"]
     cases = [make_case(c[0], c[1], c[3], c[4], c[5], ny=c[2], randomize=True, seed_base=200 + j)
@@ -191,6 +193,7 @@ def test_run_many_one_graph():
     hips = [CrmHip(0, p) for p in precs]
     try:
         for j, h in enumerate(hips):
+            h.set_graphs(mode)
             configure(h, j)
         dcs = [DeviceCase(c.copy()) for c in cases]
         for step in range(3):

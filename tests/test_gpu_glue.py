@@ -145,3 +145,41 @@ def test_gcm_step_on_the_device(hip):
     got = dc.download(case.copy())
     errs = compare_case(got, want, dt_gl=60.0)
     assert max(errs.values()) <= 1e-10, sorted(errs.items(), key=lambda kv: -kv[1])[:4]
+
+
+def test_rad_columns_and_qrad(hip):
+    """The radiation's inputs in every CRM radiation column (radiation.F90:1584-1636) from the
+    crm_rad averages of a real crm() call (radiation on 8 column groups), and crm_qrad from the
+    per-column shortwave and longwave heating (:2393-2398), bitwise against the restatement;
+    columns past ncol untouched."""
+    import torch
+    case = _case()
+    hip.run(case, 200.0)  # crm_rad averages with clouds in some groups
+    st = glue_cases.physics_state(NCOL, PCOLS, PVER, seed=21)
+    rng = np.random.default_rng(4)
+    cloud = {f: np.asfortranarray(rng.uniform(0, 1, (PCOLS, PVER))) for f in crm_glue.CLOUD}
+    nrad = 8
+    want = G.rad_columns(NCOL, CRM_NZ, st, cloud, case.rad)
+    dc = DeviceCase(case, "cuda:0")
+    out = {f: torch.full((PVER, nrad, PCOLS), -7.0, dtype=torch.float64, device="cuda:0")
+           for f in crm_glue.RAD_COLUMN}
+    crm_glue.rad_columns(hip, NCOL, PCOLS, PVER, CRM_NZ, nrad, {k: _dev(v) for k, v in st.items()},
+                         {k: _dev(v) for k, v in cloud.items()}, dc, out)
+    for f in crm_glue.RAD_COLUMN:
+        got = out[f].cpu().numpy().T  # (pcols, nrad, pver)
+        assert np.array_equal(got[:NCOL], want[f][:NCOL]), f
+        assert np.all(got[NCOL:] == -7.0), f
+    assert (want["cld"][:NCOL, :, PVER - CRM_NZ:] > 0).any()  # some cloud in the averages
+    qrs = rng.normal(0, 1e-5, case.rad["qrad"].shape)
+    qrl = rng.normal(0, 1e-5, case.rad["qrad"].shape)
+    want_q = case.rad["qrad"].copy()
+    G.rad_qrad(NCOL, qrs, qrl, want_q)
+    crm_glue.rad_qrad(hip, NCOL, PCOLS, nrad, CRM_NZ, _dev4(qrs), _dev4(qrl), dc)
+    got_q = dc.download(case.copy()).rad["qrad"]
+    assert np.array_equal(got_q, want_q)
+
+
+def _dev4(a):
+    """A (pcols, nx_rad, ny_rad, nz) Fortran array as a device tensor in the same memory order."""
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).transpose(3, 2, 1, 0))).cuda()

@@ -3,6 +3,7 @@ fixes: closed forms of the column integrals, the closure of check_energy_chng fo
 advanced by exactly the fluxes it is told about (a dropped latent or sensible term fails it),
 rotation invariants of the input wind, and the Q dp round trip of qrad."""
 import numpy as np
+import pytest
 
 import glue_cases
 import glue_oracle as G
@@ -119,3 +120,69 @@ def test_qrad_round_trip_and_tendencies():
     assert np.array_equal(ptend["s"][:NCOL, top:], out["sltend"][:NCOL, top:])
     assert np.array_equal(ptend["s"][:NCOL, :top], (qrs + qrl)[:NCOL, :top])
     assert np.array_equal(ptend["qcl"][:NCOL], out["qcltend"][:NCOL])
+
+
+def _rad_column_case():
+    """pver 5, crm_nz 3, radiation columns 2 x 1: GCM values 100 + level (state) and 0.2 + 0.01
+    level (clouds); CRM level m of radiation column r holds distinct averages."""
+    pcols, pver, nz, ncol = 3, 5, 3, 2
+    lev = np.arange(pver, dtype=float)
+    state = {f: np.asfortranarray(np.tile(base + lev, (pcols, 1)))
+             for f, base in (("t", 100.0), ("qv", 1e-3), ("qcl", 2e-3), ("qci", 3e-3))}
+    state["pdel"] = np.asfortranarray(np.tile(1000.0 + 10 * lev, (pcols, 1)))
+    cloud = {f: np.asfortranarray(np.tile(0.2 + 0.01 * lev, (pcols, 1))) for f in ("cld", "fice", "cicewp", "cliqwp")}
+    rad = {f: np.zeros((pcols, 2, 1, nz), order="F") for f in ("temperature", "qv", "qc", "qi", "cld")}
+    rad["temperature"][:, 0, 0, :] = [250.0, 260.0, 270.0]
+    rad["temperature"][:, 1, 0, :] = [251.0, 261.0, 271.0]
+    rad["qv"][:, :, 0, :] = 5e-3
+    rad["qv"][:, 1, 0, 2] = -1e-12                      # a negative average: the 1e-9 floor
+    rad["cld"][:, 0, 0, :] = [0.5, 0.0, 0.001]          # cloudy, clear, a trace (the 1% floor)
+    rad["qc"][:, 0, 0, :] = [1e-4, 0.0, 1e-6]
+    rad["qi"][:, 0, 0, :] = [3e-4, 0.0, 0.0]
+    return ncol, nz, state, cloud, rad
+
+
+def test_rad_columns_closed_forms():
+    """radiation.F90:1584-1636: per radiation column, the CRM levels (GCM level pver-1-m of CRM
+    level m) take the crm_rad averages; the in-cloud water paths are q pdel / g over the cloud
+    fraction floored at 1%; the ice fraction is qi / (qc + qi); vapour is floored at 1e-9; the
+    levels above the CRM keep the GCM column's values."""
+    ncol, nz, state, cloud, rad = _rad_column_case()
+    out = G.rad_columns(ncol, nz, state, cloud, rad)
+    # levels above the CRM (k = 0, 1) untouched, in both radiation columns
+    for f, src in (("t", state["t"]), ("qc", state["qcl"]), ("cld", cloud["cld"])):
+        assert np.array_equal(out[f][:ncol, :, :2], np.repeat(src[:ncol, None, :2], 2, axis=1))
+    # level order: CRM level 0 is the bottom GCM level
+    assert out["t"][0, 0, 4] == 250.0 and out["t"][0, 0, 2] == 270.0 and out["t"][1, 1, 3] == 261.0
+    # the cloudy cell: 0.5 cover, 0.1 g/kg liquid, 0.3 g/kg ice, pdel 1040 Pa at k = 4
+    assert out["fice"][0, 0, 4] == pytest.approx(0.75, rel=1e-15)
+    assert out["cicewp"][0, 0, 4] == pytest.approx(3e-4 * 1040.0 / 9.80616 / 0.5, rel=1e-15)
+    assert out["cliqwp"][0, 0, 4] == pytest.approx(1e-4 * 1040.0 / 9.80616 / 0.5, rel=1e-15)
+    # clear: zero paths and ice fraction; a trace of cover counts as 1%
+    assert out["cicewp"][0, 0, 3] == 0.0 and out["fice"][0, 0, 3] == 0.0 and out["cld"][0, 0, 3] == 0.0
+    assert out["cliqwp"][0, 0, 2] == pytest.approx(1e-6 * 1020.0 / 9.80616 / 0.01, rel=1e-15)
+    # vapour floor
+    assert out["qv"][0, 1, 2] == 1e-9 and out["qv"][0, 0, 2] == 5e-3
+    # columns past ncol stay zero
+    assert not out["t"][ncol:].any()
+
+
+def test_rad_columns_planted_mistakes(monkeypatch):
+    """The checks notice a dropped cover floor and a flipped level order."""
+    ncol, nz, state, cloud, rad = _rad_column_case()
+    # the in-cloud path divided by the cover itself: the trace cell comes out 10x too large
+    monkeypatch.setattr(G, "incloud_path", lambda q, dp, cld: q * dp / G.GRAVIT / cld)
+    got = G.rad_columns(ncol, nz, state, cloud, rad)
+    assert got["cliqwp"][0, 0, 2] != pytest.approx(1e-6 * 1020.0 / 9.80616 / 0.01, rel=1e-3)
+    monkeypatch.undo()
+    # levels taken top-first: the CRM's bottom temperature lands at the wrong GCM level
+    flipped = {f: v[..., ::-1].copy() for f, v in rad.items()}
+    got = G.rad_columns(ncol, nz, state, cloud, flipped)
+    assert got["t"][0, 0, 4] != 250.0
+
+
+def test_rad_qrad_is_shortwave_plus_longwave():
+    q = np.zeros((3, 2, 1, 4))
+    qrs, qrl = np.full_like(q, 1.5), np.full_like(q, -0.25)
+    G.rad_qrad(2, qrs, qrl, q)
+    assert np.all(q[:2] == 1.25) and not q[2].any()

@@ -84,7 +84,17 @@ def test_128_column_class_bitexact():
     _assert_cases_equal(cr, co)
 
 
-@pytest.mark.parametrize("n", [8, 16, 32, 64, 128])
+def test_other_level_count_bitexact():
+    """A level count other than the SP configurations' 28 and 58 (crm_nz is a build option of
+    the reference, crmdims.F90:13-22): 40 levels, 2 km, 10 s, a few substeps and a GCM step."""
+    case = convective_case(2, nx=64, nz=40, dx=2000.0, dt=10.0, seed=2)
+    _, _, cr, co = _run_both(case, 100.0)
+    _assert_cases_equal(cr, co)
+    _, _, cr, co = _run_both(case, 1200.0)
+    _assert_cases_equal(cr, co)
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32, 64, 128])
 @pytest.mark.parametrize("isign", [-1, 1])
 def test_fft991_bitexact(n, isign):
     """The restated x-transform against the reference's fft991_crm (fft.F90:5-153)."""
