@@ -388,6 +388,32 @@ struct Seg {
     for (int m = row; m < row * gy; m <<= 1) v = v + (T)__shfl_xor(v, m, 64);
     return v;
   }
+  // gsum2 of N values at once: each butterfly step is taken (or skipped) for all of them
+  // together, one uniform branch per step instead of one per value; per value the additions
+  // are gsum2's, so the bits are the same
+  template <typename T, int N>
+  __device__ __forceinline__ void gsums2(T (&v)[N], int gx, int row, int gy) const {
+    const int g = gx == row ? row * gy : gx;
+#ifdef CRM_SEG_BPERMUTE
+#pragma unroll
+    for (int j = 0; j < N; j++) v[j] = gsum(v[j], g);
+#else
+    auto step = [&](auto mv) {
+#pragma unroll
+      for (int j = 0; j < N; j++) v[j] = v[j] + mv(v[j]);
+    };
+    if (g > 1) step([](T x) { return lane_mv<DPP_QP_XOR1>(x); });
+    if (g > 2) step([](T x) { return lane_mv<DPP_QP_XOR2>(x); });
+    if (g > 4) step([](T x) { return lane_mv<DPP_HALF_MIRROR>(x); });
+    if (g > 8) step([](T x) { return lane_mv<DPP_MIRROR>(x); });
+    if (g > 16) step([&](T x) { return lane_x16(x, lane); });
+    if (g > 32) step([&](T x) { return lane_x32(x, lane); });
+#endif
+    if (gx != row)
+      for (int m = row; m < row * gy; m <<= 1)
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = v[j] + (T)__shfl_xor(v[j], m, 64);
+  }
   template <typename T>
   __device__ __forceinline__ T max(T v) const {
     return reduce(v, NX, [](T a, T b) { return fmax(a, b); });
@@ -577,6 +603,11 @@ struct Seg128 {
   template <typename T>
   __device__ __forceinline__ T gsum2(T v, int gx, int, int) const {
     return gsum(v, gx);
+  }
+  template <typename T, int N>
+  __device__ __forceinline__ void gsums2(T (&v)[N], int gx, int, int) const {
+#pragma unroll
+    for (int j = 0; j < N; j++) v[j] = gsum(v[j], gx);
   }
   template <typename T>
   __device__ __forceinline__ T max(T v) const {

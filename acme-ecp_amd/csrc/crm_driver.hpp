@@ -478,11 +478,12 @@ static int upload_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform
 // solve of 2D 64-column CRMs has four (0: one instance per 4-wave workgroup, two workgroups per
 // CU, the gradient pass reading its inputs again; 1: one instance on 8 waves with every level's
 // inputs of the gradient pass kept in registers, one workgroup per CU; 2 and 3: variant 0 with
-// two or four levels of each pass unrolled together), MPDATA two (the default or the other
-// number of levels per trip).  Results are bitwise the same in every variant of a class.
+// two or four levels of each pass unrolled together), MPDATA four (0, 1: the default or the
+// other number of levels per trip; 2, 3: the same with registers bounded for three waves per
+// SIMD).  Results are bitwise the same in every variant of a class.
 static int variants_of(int kid, int nx, bool y3) {
   if (kid == CRM_K_PRESS) return (nx == 64 && !y3) ? 4 : 1;
-  if (kid == CRM_K_MPDATA) return 2;
+  if (kid == CRM_K_MPDATA) return 4;
   return 1;
 }
 
@@ -546,6 +547,10 @@ static void launch_mpdata(dim3 g, dim3 b, hipStream_t s, const DevT<R>& d, int v
   constexpr bool PAIR = (NX == 64 && !G::Y3);
   if (variant == 1)
     k_mpdata<NX, R, G, !PAIR><<<g, b, 0, s>>>(d);
+  else if (variant == 2)
+    k_mpdata_w3<NX, R, G, PAIR><<<g, b, 0, s>>>(d);
+  else if (variant == 3)
+    k_mpdata_w3<NX, R, G, !PAIR><<<g, b, 0, s>>>(d);
   else
     k_mpdata<NX, R, G, PAIR><<<g, b, 0, s>>>(d);
 }
@@ -1450,13 +1455,21 @@ int Engine<R>::autotune(const crm_dims* dims, const crm_column_info* col, double
     }
   }
   for (int k = 0; k < CRM_K_COUNT; k++) {
-    int bc = 2, bv = 0;  // 256 threads and the default variant unless another measured faster
+    // 256 threads and the default variant unless another measured faster by more than 3%
+    // (timing noise of a short trial call is a few percent)
+    int bc = 2, bv = 0;
+    double bt = t[2][0][k];
     for (int v = 0; v < variants_of(k, nxc, y3); v++)
       for (int c = 0; c < 3; c++)
-        if (t[c][v][k] > 0.0 && t[c][v][k] < t[bc][bv][k]) {
+        if (t[c][v][k] > 0.0 && t[c][v][k] < bt) {
           bc = c;
           bv = v;
+          bt = t[c][v][k];
         }
+    if (!(bt < 0.97 * t[2][0][k])) {
+      bc = 2;
+      bv = 0;
+    }
     block[k] = cand[bc];
     variant[k] = bv;
     if (best) best[k] = block[k];

@@ -2233,15 +2233,26 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
 }
 
 // advect_all_scalars.F90:7-94: t, then the two microphysics fields.
-template <int NX, typename R, typename G = SegFor<NX>, bool PAIR = (NX == 64 && !G::Y3)>
-__global__ void CRM_LB k_mpdata(DevT<R> d) {
-  CRM_STOP_CHECK(d);
+template <int NX, typename R, typename G, bool PAIR>
+__device__ __forceinline__ void mpdata_all(const DevT<R>& d) {
   G s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
   mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.t, nullptr, nullptr, false);
   mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
   mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
+}
+template <int NX, typename R, typename G = SegFor<NX>, bool PAIR = (NX == 64 && !G::Y3)>
+__global__ void CRM_LB k_mpdata(DevT<R> d) {
+  CRM_STOP_CHECK(d);
+  mpdata_all<NX, R, G, PAIR>(d);
+}
+// The same sweep with its registers bounded for three waves per SIMD (168 VGPRs; what does not
+// fit is spilled): more waves to hide latency at the cost of the spills (launch variants)
+template <int NX, typename R, typename G = SegFor<NX>, bool PAIR = (NX == 64 && !G::Y3)>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_mpdata_w3(DevT<R> d) {
+  CRM_STOP_CHECK(d);
+  mpdata_all<NX, R, G, PAIR>(d);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -3119,30 +3130,30 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       if (ws < -2 * wmin) mcudn = rw;
     }
     // mass fluxes through w level k (crm_module.F90:1017-1049)
+    // the level whose condensate decides: the one below for an updraft, above for a
+    // downdraft; one saturation evaluation serves both branches
     double mui = 0, mdi = 0;
-    if (wk > 0.) {
-      const Lvl& X = k >= 1 ? nxt : cur;
-      double qs = qsatw(X.tabs, X.pres);
-      if (X.qcl + X.qci > fmin(R4(1.e-5), 0.01 * qs)) mui = cur.rhow * wk;
-    } else if (wk < 0.) {
-      const Lvl& X = k + 1 < nzm ? up : cur;
-      double qs = qsatw(X.tabs, X.pres);
-      if (X.qcl + X.qci > fmin(R4(1.e-5), 0.01 * qs))
+    if (wk > 0. || wk < 0.) {
+      const bool upd = wk > 0.;
+      const Lvl& Xu = k >= 1 ? nxt : cur;
+      const Lvl& Xd = k + 1 < nzm ? up : cur;
+      const double xt = upd ? Xu.tabs : Xd.tabs, xp = upd ? Xu.pres : Xd.pres;
+      const double xn = upd ? Xu.qcl + Xu.qci : Xd.qcl + Xd.qci;
+      const double qs = qsatw(xt, xp);
+      const bool cloudy = xn > fmin(R4(1.e-5), 0.01 * qs);
+      if (upd) {
+        if (cloudy) mui = cur.rhow * wk;
+      } else if (cloudy || Xd.qpl + Xd.qpi > R4(1.0e-4)) {
         mdi = cur.rhow * wk;
-      else if (X.qpl + X.qpi > R4(1.0e-4))
-        mdi = cur.rhow * wk;
+      }
     }
     double sv[10] = {top_hit, cld, mcup, mcdn, mcuup, mcudn, qcl, qci, mui, mdi};
     s.multi_sum(sv);
     // radiation-column sums (crm_module.F90:979-1015)
-    double r_t = tabs, r_qv = fmax(0.0, cur.qv), r_qc = qcl, r_qi = qci, r_cld = (qcl + qci > 0) ? 1.0 : 0.0;
-    if (g > 1) {  // the group's columns: contiguous lanes in 2D, a block of rows in 3D
-      r_t = s.gsum2(r_t, d.rad_gx, d.rad_row, d.rad_gy);
-      r_qv = s.gsum2(r_qv, d.rad_gx, d.rad_row, d.rad_gy);
-      r_qc = s.gsum2(r_qc, d.rad_gx, d.rad_row, d.rad_gy);
-      r_qi = s.gsum2(r_qi, d.rad_gx, d.rad_row, d.rad_gy);
-      r_cld = s.gsum2(r_cld, d.rad_gx, d.rad_row, d.rad_gy);
-    }
+    double rs[5] = {tabs, fmax(0.0, cur.qv), qcl, qci, (qcl + qci > 0) ? 1.0 : 0.0};
+    // the group's columns: contiguous lanes in 2D, a block of rows in 3D
+    if (g > 1) s.gsums2(rs, d.rad_gx, d.rad_row, d.rad_gy);
+    const double r_t = rs[0], r_qv = rs[1], r_qc = rs[2], r_qi = rs[3], r_cld = rs[4];
     if (live) {
       // the reference's index mix for the cloud top (crm_module.F90:918-925)
       if (s.owns(0)) accum(&prof(d, P_CLDTOP, c)[nzm - 1 - k], sv[0]);

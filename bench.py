@@ -66,8 +66,8 @@ def measured_traffic(kernel):
     if not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
-        parts = line.split()
-        if len(parts) == 3 and not line.startswith("#") and parts[0] == kernel:
+        parts = line.split("#")[0].split()
+        if len(parts) == 3 and parts[0] == kernel:
             return float(parts[1]) + float(parts[2])
     return None
 
@@ -156,6 +156,15 @@ def measure(prec, case, args, torch):
     dcase = DeviceCase(case, "cuda:0")
     torch.cuda.synchronize()
     hip = CrmHip(0, prec)
+    # the workgroup sizes and launch variants the autotuner picks for this class, from trial
+    # calls of 12 substeps on a scratch copy of the state
+    scratch = DeviceCase(case.copy(), "cuda:0")
+    sched = hip.autotune(scratch, 12 * w["dt"])
+    del scratch
+    tuned = {"threads_per_workgroup": {k: v for k, v in sched.items()
+                                       if k not in ("setup", "setperturb", "precip_init", "press")},
+             "variants": {k: v for k, v in hip.variants().items() if k in ("press", "mpdata")}}
+    log(f"[{prec}] autotuned: {tuned}")
     # warmup on the path a caller gets (the default: one captured HIP graph per call, no
     # events); the first call also sizes the device pool and captures the graph
     for n in range(max(1, args.warmup)):
@@ -199,7 +208,7 @@ def measure(prec, case, args, torch):
             b = KERNEL_WORDS[k] * word * case.ncrms * w["nx"] * w["nz"]
             e["hbm_frac_algorithmic"] = b / (ms / n * 1e-3) / 1e9 / HBM_PEAK_GBS
         per_kernel[k] = e
-    return {"value": value, "ms_per_step": 1e3 * sec / args.steps, "max_ncycle": max(ncyc),
+    return {"value": value, "ms_per_step": 1e3 * sec / args.steps, "max_ncycle": max(ncyc), "schedule": tuned,
             "max_nprec": max(nprec), "graph_substeps_timed": graph_substeps,
             "dcase": dcase, "kernels": per_kernel,
             "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
@@ -493,7 +502,7 @@ def main():
         mixed = {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
                  "speedup_vs_fp64": r32["value"] / value, "roofline": r32["roofline"],
                  "error_vs_fp64": state_error(r64["dcase"], r32["dcase"], torch),
-                 "kernels": r32["kernels"],
+                 "kernels": r32["kernels"], "autotuned": r32["schedule"],
                  "note": "FP32 bulk fields, advection, SGS and microphysics; FP64 pressure solve, "
                          "reference profiles, statistics and outputs; error after the same "
                          f"{max(1, args.warmup) + args.steps + args.profile_steps} GCM steps "
@@ -523,11 +532,12 @@ def main():
         "config": {"workload": f"{ncrms} 2D CRMs of 64x58 (SP1V1: radiation on {w['nx_rad']} "
                                f"column groups), dx 1 km, dt 5 s, one 1200 s GCM step "
                                f"({nstop} substeps) per call, inputs resident in HBM, default "
-                               f"path (one HIP graph per call)",
+                               f"path (one HIP graph per call), autotuned launch shapes",
                    "ncrms": ncrms, "nx": w["nx"], "nz": w["nz"], "nx_rad": w["nx_rad"],
                    "substeps": nstop, "precision": "fp64", "max_ncycle": r64["max_ncycle"],
                    "max_nprec": r64["max_nprec"],
-                   "graph_substeps_timed": r64["graph_substeps_timed"]},
+                   "graph_substeps_timed": r64["graph_substeps_timed"],
+                   "autotuned": r64["schedule"]},
         "roofline": r64["roofline"],
         "whole_step_hbm": {"achieved": whole, "frac": whole / HBM_PEAK_GBS, "unit": "GB/s",
                            "bytes_per_column_substep": STATE_WORDS_PER_LEVEL * 8 * w["nz"]},

@@ -232,13 +232,15 @@ int crm_hip_get_schedule(const crm_hip_ctx *ctx, int block[CRM_HIP_NKERNEL]);
  * inputs kept in registers from the right-hand side pass (one workgroup per CU), 2 and 3 as 0
  * with two or four levels of each pass unrolled together. CRM_K_MPDATA: 0 the default number
  * of levels per trip of the sweep's steady state (two for 2D 64-column CRMs, else one), 1 the
- * other. Other classes have variant 0 only. */
+ * other, 2 and 3 as 0 and 1 with the registers bounded for three waves per SIMD. Other classes
+ * have variant 0 only. */
 int crm_hip_set_variants(crm_hip_ctx *ctx, const int variant[CRM_HIP_NKERNEL]);
 int crm_hip_get_variants(const crm_hip_ctx *ctx, int variant[CRM_HIP_NKERNEL]);
 
 /* Autotuner: runs the call once per candidate workgroup size (64, 128, 256) and launch variant,
  * with every kernel class timed on the context's stream, keeps the fastest (size, variant) of
- * each class for later calls of this context and writes the sizes to block_out (may be NULL;
+ * each class (the default 256 threads and variant 0 unless another is faster by more than 3%)
+ * for later calls of this context and writes the sizes to block_out (may be NULL;
  * crm_hip_get_variants gives the variants). The views are device pointers and are advanced by
  * every trial call: pass scratch copies. */
 int crm_hip_autotune(crm_hip_ctx *ctx, const crm_dims *dims, const crm_column_info *col, double dt_gl,
