@@ -481,13 +481,13 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
   R* dudt = d.dudt[na];
   R* dwdt = d.dwdt[na];
   auto F = [&](const R* p, int k) { return p[((size_t)c * nzm + k) * NX + i]; };
-  // cloud-ice flux fz(k) (ice_fall.F90:94-128); zero outside [kmin-1, kmax]
-  auto qci_at = [&](int k) { return F(d.qci, k); };
-  auto icefz = [&](int k) -> R {
+  // cloud-ice flux fz(k) (ice_fall.F90:94-128) from qci at levels max(0, k-1), k and
+  // min(nzm-1, k+1); zero outside [kmin-1, kmax]
+  auto icefz = [&](int k, R qci_b, R qci_k, R qci_c) -> R {
     if (k < ::max(0, kmin - 1) || k > kmax || k > nzm - 1) return R(0.0);
     int kc = ::min(nzm - 1, k + 1), kb = ::max(0, k - 1);
     R coef = dtn / (R(0.5) * (adz[kb] + adz[k]) * dz);
-    R qiu = rho[kc] * qci_at(kc), qic = rho[k] * qci_at(k), qid = rho[kb] * qci_at(kb);
+    R qiu = rho[kc] * qci_c, qic = rho[k] * qci_k, qid = rho[kb] * qci_b;
     R vt = fmin(R4(0.4), R4(8.66) * fpow<R>(fmax(R(0.0), qic) + R4(1.e-10), R4(0.24)));
     R phi;
     if (fabs(qic - qid) < R4(1.0e-25)) {
@@ -498,12 +498,19 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
     }
     return -vt * (qic - R(0.5) * (R(1.) - coef * vt) * phi * (qic - qid));
   };
-  R fz_k = icefz(0);
+  // qci of levels k, k+1, k+2 at level k (clamped at the top, where the clamped level is the
+  // one the flux uses): the flux icefz(k+1) needs all three, so they are read at least one
+  // level ahead, and level k+3 is read at the top of level k.  Vector loads complete in issue
+  // order, so a read of this level issued after the prefetch of the next would wait for it.
+  R qi0 = F(d.qci, 0), qi1 = F(d.qci, ::min(1, nzm - 1)), qi2 = F(d.qci, ::min(2, nzm - 1));
+  // the surface precipitation the ice flux adds to at level 0, read before the sweep's loads
+  const double psfc0 = d.precsfc[(size_t)c * NX + i], pssfc0 = d.precssfc[(size_t)c * NX + i];
+  R fz_k = icefz(0, qi0, qi0, qi1);
   // level k-1 thermodynamics for the buoyancy average
   R pb_tabs = 0, pb_qv = 0, pb_qcl = 0, pb_qci = 0, pb_qpl = 0, pb_qpi = 0;
   // the fields of one level, read one level ahead of their use
   struct Lvl {
-    R tabs, qv, qcl, qci, qpl, qpi, u, w, t, qt, v;
+    R tabs, qv, qcl, qpl, qpi, u, w, t, qt, v;
     double qr;
   };
   auto load = [&](int k, Lvl& L) {
@@ -511,7 +518,6 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
     L.tabs = d.tabs[o];
     L.qv = d.qv[o];
     L.qcl = d.qcl[o];
-    L.qci = d.qci[o];
     L.qpl = d.qpl[o];
     L.qpi = d.qpi[o];
     L.u = d.u[o];
@@ -529,7 +535,8 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
     // clamped at the top, where the record is not used again: an unconditional load needs no
     // copy that waits for earlier loads
     load(::min(k + 1, nzm - 1), nxt);
-    R tabs = L.tabs, qv = L.qv, qcl = L.qcl, qci = L.qci, qpl = L.qpl, qpi = L.qpi;
+    const R qi3 = F(d.qci, ::min(k + 3, nzm - 1));
+    R tabs = L.tabs, qv = L.qv, qcl = L.qcl, qci = qi0, qpl = L.qpl, qpi = L.qpi;
     R u = L.u, w = L.w;
     R t = L.t, qt = L.qt;
     // buoyancy on w level k from scalar levels k-1 and k
@@ -581,7 +588,7 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
       qt = qt - dtn * (qv - qv0[k]) * tk;
     }
     // cloud-ice sedimentation
-    R fz_kp = icefz(k + 1);
+    R fz_kp = icefz(k + 1, qi0, qi1, qi2);
     if (k >= ::max(0, kmin - 2) && k <= kmax) {
       R coef = dtn / (dz * adz[k] * rho[k]);
       R dqi = coef * (fz_k - fz_kp);
@@ -590,10 +597,13 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
     }
     if (k == 0 && live) {
       R dqi = -(dtn / dz) * fz_k;
-      d.precsfc[(size_t)c * NX + i] += dqi;
-      d.precssfc[(size_t)c * NX + i] += dqi;
+      d.precsfc[(size_t)c * NX + i] = psfc0 + dqi;
+      d.precssfc[(size_t)c * NX + i] = pssfc0 + dqi;
     }
     fz_k = fz_kp;
+    qi0 = qi1;
+    qi1 = qi2;
+    qi2 = qi3;
     if (live) {
       d.t[o] = t;
       d.qt[o] = qt;
@@ -1999,6 +2009,10 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
   R mx1_a = 0, mn1_a = 0;
   R mx_a = 0, mx_b = 0, mn_a = 0, mn_b = 0;
   R uu3_a = 0, ww3_a = 0;
+  // the donor-cell part of the vertical flux statistic from stage 1, kept until stage 4 adds
+  // the antidiffusive part three levels later (reading it back from memory would wait for the
+  // prefetch issued before it: vector loads complete in issue order)
+  R s1_a = 0, s1_b = 0, s1_c = 0;
   // One pipeline step; ST: steady state (5 <= K < nzm), where every stage is active and every
   // boundary case is the interior one, so the guards and level selects fold away.
   auto step = [&](const int K, auto steady) {
@@ -2038,7 +2052,7 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
     }
     // stage 1: donor-cell update and first bounds at level K-1
     const int L1 = K - 1;
-    R f1_new = 0, mx0_new = 0, mn0_new = 0;
+    R f1_new = 0, mx0_new = 0, mn0_new = 0, s1_new = 0;
     if (ST || (L1 >= 0 && L1 < nzm)) {
       R fkb = (ST || L1 >= 1) ? f_2 : f_1, fkc = (ST || L1 + 1 < nzm) ? f_0 : f_1;
       R fm = s.rot(f_1, -1), fp = s.rot(f_1, 1);
@@ -2052,8 +2066,7 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
         mx0_new = fmax(fmax(fmax(fmax(fm, fp), fkb), fkc), f_1);
         mn0_new = fmin(fmin(fmin(fmin(fm, fp), fkb), fkc), f_1);
       }
-      R s1 = s.sum(ww1_1);
-      if (flux && live && i == 0) flux[L1] = s1;
+      if (flux) s1_new = s.sum(ww1_1);
     }
     // stage 2: antidiffusive velocities and second bounds at level K-2
     const int L2 = K - 2;
@@ -2131,8 +2144,10 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
         R mxjm = s.roty(mx_a, -1), mnjm = s.roty(mn_a, -1);
         vv3_new = ppos(vv2_b) * fmin(fmin(R(1.0), mx_a), mnjm) - pneg(vv2_b) * fmin(fmin(R(1.0), mxjm), mn_a);
       }
-      R s3 = s.sum(ww3_new);
-      if (flux && live && i == 0) flux[L4] = flux[L4] + s3;
+      if (flux) {
+        R s3 = s.sum(ww3_new);
+        if (live && i == 0) flux[L4] = (double)s1_c + s3;
+      }
     }
     // stage 5: corrected update at level K-5
     const int L5 = K - 5;
@@ -2207,6 +2222,9 @@ __device__ void mpdata_one(const DevT<R>& d, const G& s, int c, bool live, R* f,
     mn_a = mn_new;
     uu3_a = uu3_new;
     ww3_a = ww3_new;
+    s1_c = s1_b;
+    s1_b = s1_a;
+    s1_a = s1_new;
   };
   int K = 0;
   // Peeled into fill, steady state and drain.  FP64 issues every instruction it keeps
@@ -2320,9 +2338,13 @@ __global__ void CRM_LB k_precip(DevT<R> d) {
       const R cq = nq1, ca = na1;
       nq1 = nq2;
       na1 = na2;
-      if (K + 2 < nzm) {
-        nq2 = Q(d.qp, K + 2);
-        na2 = Q(d.tabs, K + 2);
+      {  // unconditional, clamped at the top level: past it the loaded values are not used
+         // (stage a is idle there), and a load that always issues writes the ring register
+         // directly (a conditional one is preceded by a copy that waits for earlier loads).
+         // Level nzm-1 is written only at step nzm+2, after the last of these reads.
+        const int kn = ::min(K + 2, nzm - 1);
+        nq2 = Q(d.qp, kn);
+        na2 = Q(d.tabs, kn);
       }
       const R tt_new = (ST || (K >= 1 && K - 1 < nzm)) ? Q(d.t, K - 1) : R(0.0);
       // a: level K, upwind flux
@@ -2725,7 +2747,9 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
   R v_pre = y3 ? d.v[o0] : R(0.0);
   for (int k = 0; k < nzm; k++) {
     size_t o = ((size_t)c * nzm + k) * NX + i;
-    auto CO = [&](int q) { return d.coef[((size_t)q * d.ncrms + c) * d.nz + k]; };
+    // the precip_init coefficients are constant during this kernel: scalar loads, which do
+    // not wait for the vector prefetch of the next level (a vector load here would)
+    auto CO = [&](int q) { return ((cprof<R>)d.coef)[((size_t)q * d.ncrms + c) * d.nz + k]; };
     const R pres = presp[k];
     R t = t_pre;
     R q = fmax(R(0.0), qt_pre);
@@ -3053,11 +3077,20 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
   const int nxr = d.nx_rad, g = NX / nxr, ir = rad_col(d, i);
   auto Q = [&](const R* p, int k) -> double { return p[((size_t)c * nzm + k) * NX + i]; };
   auto Wv = [&](int k) -> double { return d.w[((size_t)c * nz + k) * NX + i]; };
-  // the fields and profiles of one level; the sweep runs down from the top and reads level
-  // k-1 while it works on level k (the profiles too: vector loads complete in issue order, so
-  // a profile read issued after the prefetch would wait for it)
+  // radiation sums: exactly one lane owns each (instance, level, radiation column), so they
+  // are plain read-modify-writes
+  const bool rad_owner = live && rad_lead(d, i);
+  const size_t rad_blk = (size_t)d.ncrms * nzm * nxr;
+  auto rad_at = [&](int k) { return (size_t)c * nzm * nxr + (size_t)k * nxr + ir; };
+  // the fields, profiles and radiation sums of one level; the sweep runs down from the top and
+  // reads level k-1 while it works on level k.  Everything the level needs from memory is in
+  // this read: vector loads complete in issue order, so a read issued after the prefetch (a
+  // profile, or the radiation sums the level adds to) would wait for it.  The sums are R: in
+  // the mixed engine FP32, which halves the bytes of this kernel's largest traffic (each
+  // addition rounds once, from the FP64 sum of the two terms)
   struct Lvl {
     double qcl, qci, qpl, qpi, tabs, qv, w, rho, rhow, adz, pres;
+    R ra[5];
   };
   auto load = [&](int k, Lvl& L) {
     L.qcl = Q(d.qcl, k);
@@ -3071,24 +3104,21 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
     L.rhow = rhow[k];
     L.adz = adz[k];
     L.pres = pres[k];
+    for (int q = 0; q < 5; q++) L.ra[q] = R(0.0);
+    if (rad_owner)
+      for (int q = 0; q < 5; q++) L.ra[q] = d.rad[q * rad_blk + rad_at(k)];
   };
   double cwp = 0, cwph = 0, cwpm = 0, cwpl = 0, clt = 0, cmt = 0, cht = 0, ctt = 0;
   bool flag_top = true;
   double w_up = Wv(nzm);  // lid
   Lvl up{}, cur, nxt{};
   load(nzm - 1, cur);
-  // radiation sums: exactly one lane owns each (instance, level, radiation column), so they
-  // are plain read-modify-writes; the reads are issued at the top of the level
-  const bool rad_owner = live && rad_lead(d, i);
-  const size_t rad_blk = (size_t)d.ncrms * nzm * nxr;
   for (int k = nzm - 1; k >= 0; k--) {
-    load(::max(k - 1, 0), nxt);  // at k = 0 level 0 again, unused (unconditional, as above)
-    const size_t rad_base = (size_t)c * nzm * nxr + (size_t)k * nxr + ir;
-    // the sums are R: in the mixed engine FP32, which halves the bytes of this kernel's
-    // largest traffic (each addition rounds once, from the FP64 sum of the two terms)
-    R ra[5] = {0, 0, 0, 0, 0};
-    if (rad_owner)
-      for (int q = 0; q < 5; q++) ra[q] = d.rad[q * rad_blk + rad_base];
+    // at k = 0 level 0 again, unused (unconditional, as above); its radiation sums are read
+    // before this level's are written, and never written from that read
+    load(::max(k - 1, 0), nxt);
+    const size_t rad_base = rad_at(k);
+    const R* ra = cur.ra;
     const double qcl = cur.qcl, qci = cur.qci, qpl = cur.qpl, qpi = cur.qpi, tabs = cur.tabs;
     const double wk = cur.w;
     // column water paths from the top (the reference's level nz-k)
