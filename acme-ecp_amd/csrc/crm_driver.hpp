@@ -45,7 +45,7 @@ struct HostTransform {
 enum { NSCAL = 17 };  // u t qt qp qn tabs qv qcl qci qpl qpi tk tkh tke dudt x3
 enum { NSCAL3 = 4 };  // 3D only: v dvdt x3
 enum { NPOOL = 9 };  // blocks of the field pool (pool_sizes)
-constexpr int MAX_VARIANTS = 4;  // launch variants of a kernel class (variants_of)
+constexpr int MAX_VARIANTS = 5;  // launch variants of a kernel class (variants_of)
 
 }  // namespace
 
@@ -475,14 +475,15 @@ static int upload_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform
 
 // ---- kernel dispatch over the compile-time column count ------------------------------------
 // Launch variants ("tile shapes") the autotuner chooses among, per kernel class: the pressure
-// solve of 2D 64-column CRMs has four (0: one instance per 4-wave workgroup, two workgroups per
+// solve of 2D 64-column CRMs has five (0: one instance per 4-wave workgroup, two workgroups per
 // CU, the gradient pass reading its inputs again; 1: one instance on 8 waves with every level's
 // inputs of the gradient pass kept in registers, one workgroup per CU; 2 and 3: variant 0 with
-// two or four levels of each pass unrolled together), MPDATA four (0, 1: the default or the
+// two or four levels of each pass unrolled together; 4: variant 0 with the Thomas factors in
+// registers, so that the workgroup needs only the slab of LDS), MPDATA four (0, 1: the default or the
 // other number of levels per trip; 2, 3: the same with registers bounded for three waves per
 // SIMD).  Results are bitwise the same in every variant of a class.
 static int variants_of(int kid, int nx, bool y3) {
-  if (kid == CRM_K_PRESS) return (nx == 64 && !y3) ? 4 : 1;
+  if (kid == CRM_K_PRESS) return (nx == 64 && !y3) ? 5 : 1;
   if (kid == CRM_K_MPDATA) return 4;
   return 1;
 }
@@ -512,6 +513,11 @@ static void launch_press(int nzm, dim3 g, dim3 b, size_t sh, hipStream_t s, cons
         k_press64<28, R, G, 4><<<gi, b, press64_lds_bytes<28>(), s>>>(d, na, nb, nc);
       else
         k_press64<58, R, G, 4><<<gi, b, press64_lds_bytes<58>(), s>>>(d, na, nb, nc);
+    } else if (variant == 4) {  // variant 0 with the Thomas factors in registers
+      if (z28)
+        k_press64<28, R, G, 1, true><<<gi, b, press64_slab_bytes(), s>>>(d, na, nb, nc);
+      else
+        k_press64<58, R, G, 1, true><<<gi, b, press64_slab_bytes(), s>>>(d, na, nb, nc);
     } else {  // one instance per 8-wave block, inputs of the gradient pass in registers
       if (z28)
         k_press64r<28, R><<<gi, dim3(512), press64_lds_bytes<28>(), s>>>(d, na, nb, nc);

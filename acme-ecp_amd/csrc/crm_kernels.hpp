@@ -1514,15 +1514,19 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PLD = 66;  // slab row stride in doubles: conflict-free A-fragment reads
 
 // UNR: levels of the right-hand side and gradient passes unrolled together, so that their
-// loads are in flight at once (a launch variant the autotuner may choose)
-template <int NZM, typename R, typename G = Seg<64>, int UNR = 1>
-__global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
+// loads are in flight at once (a launch variant the autotuner may choose).  RALF: the Thomas
+// factors in registers of the sweeping wave instead of LDS (NZM > 0 only), so that the block
+// needs only the slab and three blocks fit a CU instead of two (another launch variant).
+template <int NZM, typename R, typename G = Seg<64>, int UNR = 1, bool RALF = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RALF ? 3 : CRM_WAVES_PER_EU)))
+k_press64(DevT<R> d, int na, int nb, int nc) {
   CRM_STOP_CHECK(d);
+  static_assert(!RALF || NZM > 0, "factors in registers: compile-time level count");
   constexpr int NX = 64, NK = 64;
   extern __shared__ double lds[];
   const int c = blockIdx.x;  // grid is exactly ncrms blocks
   double* P = lds;              // [NK][PLD] slab: rhs, coefficients, pressure
-  double* ALF = lds + NK * PLD; // [nzm][64] Thomas factors
+  double* ALF = lds + NK * PLD; // [nzm][64] Thomas factors (not allocated with RALF)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane;
   G s;  // 3D (8 x 8 columns): the transforms are the x-y Kronecker operators, dy == dx
@@ -1640,26 +1644,59 @@ __global__ void CRM_LB k_press64(DevT<R> d, int na, int nb, int nc) {
       b = 1.0 / (eig * rho[0] - acoef(0) - ccoef(0));
     else
       b = 1.0 / (eig * rho[0] - ccoef(0));
+    // the factors: LDS, or (RALF) an array the unrolled sweeps index with constants, which
+    // the compiler keeps in registers
+    double alf_r[RALF ? (NZM > 0 ? NZM : 1) : 1];
+    auto put_alf = [&](int k, double v) {
+      if constexpr (RALF)
+        alf_r[k] = v;
+      else
+        ALF[k * 64 + i] = v;
+    };
+    auto get_alf = [&](int k) {
+      if constexpr (RALF)
+        return alf_r[k];
+      else
+        return ALF[k * 64 + i];
+    };
     double alfa = -ccoef(0) * b;
-    ALF[i] = alfa;
+    put_alf(0, alfa);
     double beta = P[i] * b;
     P[i] = beta;
-    for (int k = 1; k < nzm - 1; k++) {
+    auto fwd = [&](int k) {
       double ak = acoef(k), ck = ccoef(k);
       double e = 1.0 / (eig * lane_of(rho_l, k) - ak - ck + ak * alfa);
       alfa = -ck * e;
-      ALF[k * 64 + i] = alfa;
+      put_alf(k, alfa);
       beta = (P[k * PLD + i] - ak * beta) * e;
       P[k * PLD + i] = beta;
+    };
+    if constexpr (RALF) {
+#pragma unroll
+      for (int k = 1; k < NZM - 1; k++) {
+        fwd(k);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else {
+      for (int k = 1; k < nzm - 1; k++) fwd(k);
     }
     {
       int k = nzm - 1;
       double ak = acoef(k);
       double top = (P[k * PLD + i] - ak * beta) / (eig * rho[k] - ak + ak * alfa);
       P[k * PLD + i] = top;
-      for (int kk = nzm - 2; kk >= 0; kk--) {
-        top = ALF[kk * 64 + i] * top + P[kk * PLD + i];
+      auto bwd = [&](int kk) {
+        top = get_alf(kk) * top + P[kk * PLD + i];
         P[kk * PLD + i] = top;
+      };
+      if constexpr (RALF) {
+#pragma unroll
+        for (int kk = NZM - 2; kk >= 0; kk--) {
+          bwd(kk);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      } else {
+        for (int kk = nzm - 2; kk >= 0; kk--) bwd(kk);
       }
     }
   }
@@ -1895,6 +1932,9 @@ __global__ void __launch_bounds__(512) k_press64r(DevT<R> d, int na, int nb, int
 template <int NZM>
 constexpr size_t press64_lds_bytes() {  // NZM == 0: room for the most levels, 64
   return (size_t)(64 * PLD + (NZM > 0 ? NZM : 64) * 64) * sizeof(double);
+}
+constexpr size_t press64_slab_bytes() {  // the slab alone (Thomas factors in registers)
+  return (size_t)64 * PLD * sizeof(double);
 }
 
 // ------------------------------------------------------------------------------------------

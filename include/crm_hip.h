@@ -230,7 +230,8 @@ int crm_hip_get_schedule(const crm_hip_ctx *ctx, int block[CRM_HIP_NKERNEL]);
  * bitwise the same in every variant. CRM_K_PRESS of 2D 64-column CRMs: 0 one instance per
  * 4-wave workgroup (two workgroups per CU), 1 one instance on 8 waves with the gradient pass's
  * inputs kept in registers from the right-hand side pass (one workgroup per CU), 2 and 3 as 0
- * with two or four levels of each pass unrolled together. CRM_K_MPDATA: 0 the default number
+ * with two or four levels of each pass unrolled together, 4 as 0 with the Thomas factors in
+ * registers (the workgroup's LDS is the slab alone: three workgroups per CU). CRM_K_MPDATA: 0 the default number
  * of levels per trip of the sweep's steady state (two for 2D 64-column CRMs, else one), 1 the
  * other, 2 and 3 as 0 and 1 with the registers bounded for three waves per SIMD. Other classes
  * have variant 0 only. */

@@ -90,19 +90,20 @@ def test_autotune_schedule():
         ref.close()
 
 
-@pytest.mark.parametrize("cls", [CLASSES[2], CLASSES[1], CLASSES[3]], ids=lambda c: f"{c[0]}x{c[1]}x{c[2]}x{c[3]}")
+@pytest.mark.parametrize("cls", [CLASSES[2], (8, 64, 1, 28, 4000.0, 20.0), CLASSES[1], CLASSES[3]],
+                         ids=lambda c: f"{c[0]}x{c[1]}x{c[2]}x{c[3]}")
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_launch_variants_do_not_change_results(cls, prec):
     """Every launch variant of the pressure solve (4-wave reading its inputs twice, one, two or
-    four levels per trip; 8-wave with them in registers) and of MPDATA (one or two levels per
-    trip, registers bounded for two or three waves per SIMD) gives the default variant's
-    bits."""
+    four levels per trip, Thomas factors in LDS or in registers; 8-wave with the inputs in
+    registers) and of MPDATA (one or two levels per trip, registers bounded for two or three
+    waves per SIMD) gives the default variant's bits."""
     case = _case(cls, seed=87)
     ref = case.copy()
     h = CrmHip(0, prec)
     try:
         h.run(ref, 4 * cls[5])
-        nv = 4 if (cls[1] == 64 and cls[2] == 1) else 1
+        nv = 5 if (cls[1] == 64 and cls[2] == 1) else 1
         for press in range(nv):
             for mp in range(4):
                 h.set_variants({"press": press, "mpdata": mp})
@@ -124,7 +125,7 @@ def test_autotune_chooses_variants():
         h.autotune(DeviceCase(case.copy()), 3 * cls[5])
         v = h.variants()
         print(v)
-        assert v["press"] in (0, 1, 2, 3) and v["mpdata"] in (0, 1, 2, 3)
+        assert v["press"] in (0, 1, 2, 3, 4) and v["mpdata"] in (0, 1, 2, 3)
         assert all(x == 0 for k, x in v.items() if k not in ("press", "mpdata"))
         a, b = case.copy(), case.copy()
         h.run(a, 100.0)
