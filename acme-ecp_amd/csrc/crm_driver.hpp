@@ -45,7 +45,7 @@ struct HostTransform {
 enum { NSCAL = 17 };  // u t qt qp qn tabs qv qcl qci qpl qpi tk tkh tke dudt x3
 enum { NSCAL3 = 4 };  // 3D only: v dvdt x3
 enum { NPOOL = 9 };  // blocks of the field pool (pool_sizes)
-constexpr int MAX_VARIANTS = 5;  // launch variants of a kernel class (variants_of)
+constexpr int MAX_VARIANTS = 6;  // launch variants of a kernel class (variants_of)
 
 }  // namespace
 
@@ -479,12 +479,14 @@ static int upload_transform(Engine<R>* ctx, int nx, int ny, double dx, Transform
 // CU, the gradient pass reading its inputs again; 1: one instance on 8 waves with every level's
 // inputs of the gradient pass kept in registers, one workgroup per CU; 2 and 3: variant 0 with
 // two or four levels of each pass unrolled together; 4: variant 0 with the Thomas factors in
-// registers, so that the workgroup needs only the slab of LDS), MPDATA four (0, 1: the default or the
-// other number of levels per trip; 2, 3: the same with registers bounded for three waves per
-// SIMD).  Results are bitwise the same in every variant of a class.
+// registers, so that the workgroup needs only the slab of LDS), MPDATA six (0, 1: the default
+// or the other number of levels per trip; 2, 3: the same with registers bounded for three
+// waves per SIMD; 4, 5: the default sweep with each scalar on its own waves, registers
+// unbounded or bounded for three waves per SIMD).  Results are bitwise the same in every
+// variant of a class.
 static int variants_of(int kid, int nx, bool y3) {
   if (kid == CRM_K_PRESS) return (nx == 64 && !y3) ? 5 : 1;
-  if (kid == CRM_K_MPDATA) return 4;
+  if (kid == CRM_K_MPDATA) return 6;
   return 1;
 }
 
@@ -557,6 +559,10 @@ static void launch_mpdata(dim3 g, dim3 b, hipStream_t s, const DevT<R>& d, int v
     k_mpdata_w3<NX, R, G, PAIR><<<g, b, 0, s>>>(d);
   else if (variant == 3)
     k_mpdata_w3<NX, R, G, !PAIR><<<g, b, 0, s>>>(d);
+  else if (variant == 4)
+    k_mpdata_split<NX, R, G, PAIR, false><<<dim3(g.x * 3), b, 0, s>>>(d, (int)g.x);
+  else if (variant == 5)
+    k_mpdata_split<NX, R, G, PAIR, true><<<dim3(g.x * 3), b, 0, s>>>(d, (int)g.x);
   else
     k_mpdata<NX, R, G, PAIR><<<g, b, 0, s>>>(d);
 }

@@ -2312,6 +2312,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   CRM_STOP_CHECK(d);
   mpdata_all<NX, R, G, PAIR>(d);
 }
+// The three scalars on separate waves (launch variants): the grid is three copies of the
+// one-scalar grid of nb blocks, qp (the longest sweep, with its Courant number) first, then
+// qt, then t.  The scalars are independent (each writes only its own field and statistics),
+// so the results are those of the sequential sweep; the GPU holds three times as many sweeps,
+// and a SIMD whose registers admit fewer waves than it has instances no longer ends with one
+// wave running its three sweeps alone.
+template <int NX, typename R, typename G = SegFor<NX>, bool PAIR = (NX == 64 && !G::Y3), bool W3 = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W3 ? 3 : CRM_WAVES_PER_EU)))
+k_mpdata_split(DevT<R> d, int nb) {
+  CRM_STOP_CHECK(d);
+  G s;
+  const int scalar = (int)blockIdx.x / nb, bl = (int)blockIdx.x % nb;
+  int g = (bl * (int)blockDim.x + (int)threadIdx.x) / NX;
+  if constexpr (NX >= 64) g = __builtin_amdgcn_readfirstlane(g);
+  const int c = g < d.ncrms ? g : d.ncrms - 1;
+  const bool live = g < d.ncrms;
+  if (scalar == 0)
+    mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.qp, prof(d, P_MKADV2, c), prof(d, P_MKWLE2, c), true);
+  else if (scalar == 1)
+    mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.qt, prof(d, P_MKADV1, c), prof(d, P_MKWLE1, c), false);
+  else
+    mpdata_one<NX, R, G, PAIR>(d, s, c, live, d.t, nullptr, nullptr, false);
+}
 
 // ------------------------------------------------------------------------------------------
 // Precipitation sedimentation: precip_fall (microphysics.F90:529-833) with the batch-global

@@ -233,8 +233,9 @@ int crm_hip_get_schedule(const crm_hip_ctx *ctx, int block[CRM_HIP_NKERNEL]);
  * with two or four levels of each pass unrolled together, 4 as 0 with the Thomas factors in
  * registers (the workgroup's LDS is the slab alone: three workgroups per CU). CRM_K_MPDATA: 0 the default number
  * of levels per trip of the sweep's steady state (two for 2D 64-column CRMs, else one), 1 the
- * other, 2 and 3 as 0 and 1 with the registers bounded for three waves per SIMD. Other classes
- * have variant 0 only. */
+ * other, 2 and 3 as 0 and 1 with the registers bounded for three waves per SIMD, 4 and 5 as 0
+ * and 2 with each of the three scalars advected on its own waves. Other classes have variant 0
+ * only. */
 int crm_hip_set_variants(crm_hip_ctx *ctx, const int variant[CRM_HIP_NKERNEL]);
 int crm_hip_get_variants(const crm_hip_ctx *ctx, int variant[CRM_HIP_NKERNEL]);
 

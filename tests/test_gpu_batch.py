@@ -97,7 +97,8 @@ def test_launch_variants_do_not_change_results(cls, prec):
     """Every launch variant of the pressure solve (4-wave reading its inputs twice, one, two or
     four levels per trip, Thomas factors in LDS or in registers; 8-wave with the inputs in
     registers) and of MPDATA (one or two levels per trip, registers bounded for two or three
-    waves per SIMD) gives the default variant's bits."""
+    waves per SIMD, the scalars in one sweep or on separate waves) gives the default variant's
+    bits."""
     case = _case(cls, seed=87)
     ref = case.copy()
     h = CrmHip(0, prec)
@@ -105,7 +106,7 @@ def test_launch_variants_do_not_change_results(cls, prec):
         h.run(ref, 4 * cls[5])
         nv = 5 if (cls[1] == 64 and cls[2] == 1) else 1
         for press in range(nv):
-            for mp in range(4):
+            for mp in range(6):
                 h.set_variants({"press": press, "mpdata": mp})
                 assert h.variants()["press"] == press and h.variants()["mpdata"] == mp
                 got = case.copy()
@@ -125,7 +126,7 @@ def test_autotune_chooses_variants():
         h.autotune(DeviceCase(case.copy()), 3 * cls[5])
         v = h.variants()
         print(v)
-        assert v["press"] in (0, 1, 2, 3, 4) and v["mpdata"] in (0, 1, 2, 3)
+        assert v["press"] in (0, 1, 2, 3, 4) and v["mpdata"] in (0, 1, 2, 3, 4, 5)
         assert all(x == 0 for k, x in v.items() if k not in ("press", "mpdata"))
         a, b = case.copy(), case.copy()
         h.run(a, 100.0)
