@@ -3174,8 +3174,31 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
   double cwp = 0, cwph = 0, cwpm = 0, cwpl = 0, clt = 0, cmt = 0, cht = 0, ctt = 0;
   bool flag_top = true;
   double w_up = Wv(nzm);  // lid
-  Lvl up{}, cur, nxt{};
+  Lvl up2{}, up{}, cur, nxt{};
   load(nzm - 1, cur);
+  // mass flux through a w level with vertical velocity wj (crm_module.F90:1017-1049): the level
+  // whose condensate decides is Xu (the level below, or the surface level) for an updraft and
+  // the level above (Xa, or Xb, the top level, when a is false) for a downdraft; one saturation
+  // evaluation serves both branches
+  // (the levels are passed as values, selected field by field: a reference chosen at run time
+  // between two records would put them in memory)
+  auto mass_flux = [&](double wj, double rhowj, const Lvl& Xu, const Lvl& Xa, const Lvl& Xb, bool a,
+                       double& mui, double& mdi) {
+    if (wj > 0. || wj < 0.) {
+      const bool upd = wj > 0.;
+      const double dt = a ? Xa.tabs : Xb.tabs, dp = a ? Xa.pres : Xb.pres;
+      const double dn = a ? Xa.qcl + Xa.qci : Xb.qcl + Xb.qci, dq = a ? Xa.qpl + Xa.qpi : Xb.qpl + Xb.qpi;
+      const double xt = upd ? Xu.tabs : dt, xp = upd ? Xu.pres : dp;
+      const double xn = upd ? Xu.qcl + Xu.qci : dn;
+      const double qs = qsatw(xt, xp);
+      const bool cloudy = xn > fmin(R4(1.e-5), 0.01 * qs);
+      // selects rather than stores under branches (a store to one of two locals chosen at run
+      // time would put them in memory)
+      const double f = rhowj * wj;
+      mui = (upd && cloudy) ? f : 0.0;
+      mdi = (!upd && (cloudy || dq > R4(1.0e-4))) ? f : 0.0;
+    }
+  };
   for (int k = nzm - 1; k >= 0; k--) {
     // at k = 0 level 0 again, unused (unconditional, as above); its radiation sums are read
     // before this level's are written, and never written from that read
@@ -3222,24 +3245,10 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       if (ws > 2 * wmin) mcuup = rw;
       if (ws < -2 * wmin) mcudn = rw;
     }
-    // mass fluxes through w level k (crm_module.F90:1017-1049)
-    // the level whose condensate decides: the one below for an updraft, above for a
-    // downdraft; one saturation evaluation serves both branches
+    // mass fluxes through w level k+1 (w level nzm, the lid, has none): its levels are all read
+    // by now, so the sweep does not wait here for the read of level k-1 it just issued
     double mui = 0, mdi = 0;
-    if (wk > 0. || wk < 0.) {
-      const bool upd = wk > 0.;
-      const Lvl& Xu = k >= 1 ? nxt : cur;
-      const Lvl& Xd = k + 1 < nzm ? up : cur;
-      const double xt = upd ? Xu.tabs : Xd.tabs, xp = upd ? Xu.pres : Xd.pres;
-      const double xn = upd ? Xu.qcl + Xu.qci : Xd.qcl + Xd.qci;
-      const double qs = qsatw(xt, xp);
-      const bool cloudy = xn > fmin(R4(1.e-5), 0.01 * qs);
-      if (upd) {
-        if (cloudy) mui = cur.rhow * wk;
-      } else if (cloudy || Xd.qpl + Xd.qpi > R4(1.0e-4)) {
-        mdi = cur.rhow * wk;
-      }
-    }
+    if (k + 1 < nzm) mass_flux(w_up, up.rhow, cur, up2, up, k + 2 < nzm, mui, mdi);
     double sv[10] = {top_hit, cld, mcup, mcdn, mcuup, mcudn, qcl, qci, mui, mdi};
     s.multi_sum(sv);
     // radiation-column sums (crm_module.F90:979-1015)
@@ -3257,8 +3266,8 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       if (s.owns(5)) accum(&prof(d, P_MCUDN, c)[k], sv[5]);
       if (s.owns(6)) accum(&prof(d, P_GLIQ, c)[k], sv[6]);
       if (s.owns(7)) accum(&prof(d, P_GICE, c)[k], sv[7]);
-      if (s.owns(8)) accum(&prof(d, P_MUI, c)[k], sv[8]);
-      if (s.owns(9)) accum(&prof(d, P_MDI, c)[k], sv[9]);
+      if (k + 1 < nzm && s.owns(8)) accum(&prof(d, P_MUI, c)[k + 1], sv[8]);
+      if (k + 1 < nzm && s.owns(9)) accum(&prof(d, P_MDI, c)[k + 1], sv[9]);
       if (rad_owner) {
         d.rad[0 * rad_blk + rad_base] = (R)((double)ra[0] + r_t);
         d.rad[1 * rad_blk + rad_base] = (R)((double)ra[1] + r_qv);
@@ -3268,8 +3277,19 @@ __global__ void CRM_LB k_stats(DevT<R> d) {
       }
     }
     w_up = wk;
+    up2 = up;
     up = cur;
     cur = nxt;
+  }
+  {  // w level 0, with level 0 (now up) below and level 1 (up2) above
+    double mui = 0, mdi = 0;
+    mass_flux(w_up, up.rhow, up, up2, up, nzm > 1, mui, mdi);
+    double sv[10] = {0, 0, 0, 0, 0, 0, 0, 0, mui, mdi};
+    s.multi_sum(sv);
+    if (live) {
+      if (s.owns(8)) accum(&prof(d, P_MUI, c)[0], sv[8]);
+      if (s.owns(9)) accum(&prof(d, P_MDI, c)[0], sv[9]);
+    }
   }
   double cv[4] = {cwp > thr ? ctt : 0.0, cwph > thr ? cht : 0.0, cwpm > thr ? cmt : 0.0, cwpl > thr ? clt : 0.0};
   s.multi_sum(cv);
