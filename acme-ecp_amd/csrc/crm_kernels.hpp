@@ -529,9 +529,21 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
   };
   Lvl nxt;
   load(0, nxt);
+  // the results of a level are stored at the top of the next one, before its reads: vector
+  // memory operations complete in issue order, so the wait for the reads at the end of a
+  // level then never includes stores issued after them
+  R st_t = 0, st_qt = 0, st_du = 0, st_dv = 0, st_dw = 0;
+  auto store_level = [&](int k) {
+    const size_t o = ((size_t)c * nzm + k) * NX + i;
+    d.t[o] = st_t;
+    d.qt[o] = st_qt;
+    dudt[o] = st_du;
+    if (y3) d.dvdt[na][o] = st_dv;
+    dwdt[((size_t)c * nz + k) * NX + i] = st_dw;
+  };
   for (int k = 0; k < nzm; k++) {
-    size_t o = ((size_t)c * nzm + k) * NX + i;
     const Lvl L = nxt;
+    if (k > 0 && live) store_level(k - 1);
     // clamped at the top, where the record is not used again: an unconditional load needs no
     // copy that waits for earlier loads
     load(::min(k + 1, nzm - 1), nxt);
@@ -604,15 +616,16 @@ __global__ void CRM_LB k_pre(DevT<R> d, int na, const double* qrad) {
     qi0 = qi1;
     qi1 = qi2;
     qi2 = qi3;
-    if (live) {
-      d.t[o] = t;
-      d.qt[o] = qt;
-      dudt[o] = du;
-      if (y3) d.dvdt[na][o] = dv;
-      dwdt[((size_t)c * nz + k) * NX + i] = dw;
-    }
+    st_t = t;
+    st_qt = qt;
+    st_du = du;
+    st_dv = dv;
+    st_dw = dw;
   }
-  if (live) dwdt[((size_t)c * nz + nzm) * NX + i] = R(0.0);
+  if (live) {
+    store_level(nzm - 1);
+    dwdt[((size_t)c * nz + nzm) * NX + i] = R(0.0);
+  }
 }
 
 // ------------------------------------------------------------------------------------------
