@@ -896,9 +896,16 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         abcoefs(n, dt3, na, nb, nc, d.at, d.bt, d.ct);
         d.dt3na = dt3[na];
         if (launch) {
-          k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_FLAGS, d.stop);
-          NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.g(CRM_K_KURANT), L.b(CRM_K_KURANT), 0, s>>>(d)));
+          // the kurant maximum of substep n: from k_kurant at the first substep, after it from
+          // the previous substep's k_micro (the same state, the same maximum); the slots are
+          // cleared once k_decide has read them, for this substep's k_micro (the last
+          // substep's maximum is not read)
+          if (n == 1) {
+            k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_FLAGS, d.stop);
+            NX_DISPATCH(nx, (k_kurant<NXV, R><<<L.g(CRM_K_KURANT), L.b(CRM_K_KURANT), 0, s>>>(d)));
+          }
           k_decide<<<1, 64, 0, s>>>(gmax, n, ctx->debug_stop);
+          k_clear_u64<<<1, 64, 0, s>>>(gmax + GMAX_FLAGS, d.stop);
           k_precip_init<R><<<dim3((unsigned)(((size_t)ncrms * nzm + 127) / 128)), dim3(128), 0, s>>>(d);
           NX_DISPATCH(nx, (k_pre<NXV, R><<<L.g(CRM_K_PRE), L.b(CRM_K_PRE), 0, s>>>(d, na, qrad)));
           NX_DISPATCH(nx, (k_sgs<NXV, R, GEO><<<L.g(CRM_K_SGS), L.b(CRM_K_SGS), 0, s>>>(d)));
@@ -913,7 +920,7 @@ static int run_core(Engine<R>* ctx, const crm_dims* dm, const crm_column_info* c
         if (y3) std::swap(d.v, d.dvdt[nc]);
         if (launch) {
           NX_DISPATCH(nx, (k_diffuse<NXV, R, GEO><<<L.g(CRM_K_DIFFUSE), L.b(CRM_K_DIFFUSE), 0, s>>>(d)));
-          NX_DISPATCH(nx, (k_micro<NXV, R><<<L.g(CRM_K_MICRO), L.b(CRM_K_MICRO), 0, s>>>(d, 0)));
+          NX_DISPATCH(nx, (k_micro<NXV, R, true><<<L.g(CRM_K_MICRO), L.b(CRM_K_MICRO), 0, s>>>(d, 0)));
         }
         int nn = na;
         na = nc;

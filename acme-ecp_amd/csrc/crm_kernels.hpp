@@ -2798,13 +2798,17 @@ __device__ __forceinline__ void accel_means(const DevT<R>& d, const S_& s, int c
   }
 }
 
-template <int NX, typename R>
+// KUR: also gather the kurant maximum (k_kurant's, kurant.F90:9-83 and sgs.F90:329-372) of
+// the state the kernel leaves, for the next substep of a captured call: u, v, w and tkh are
+// final by then (nothing after it in the substep changes them), and k_micro already reads u
+// and v every level, so only w and tkh are added to its prefetch.
+template <int NX, typename R, bool KUR = false>
 __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
   CRM_STOP_CHECK(d);
   SegFor<NX> s;
   int c;
   bool live = seg_instance<NX>(d.ncrms, c);
-  const int nzm = d.nzm, i = s.i;
+  const int nzm = d.nzm, nz = d.nz, i = s.i;
   const bool y3 = d.ny > 1;
   const R dtn = d.dtn, factor = R(1.) / (R)NX;
   const cprof<R> gamaz = profc(d, P_GAMAZ, c);
@@ -2821,6 +2825,19 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
   const size_t o0 = ((size_t)c * nzm) * NX + i;
   R t_pre = d.t[o0], qt_pre = d.qt[o0], qp_pre = d.qp[o0], u_pre = d.u[o0];
   R v_pre = y3 ? d.v[o0] : R(0.0);
+  // KUR: w of levels k and k+1 (the second read one level ahead) and tkh of level k
+  auto Wk = [&](int k) { return d.w[((size_t)c * nz + k) * NX + i]; };
+  R w_a = 0, w_b = 0, tkh_pre = 0;
+  double cfl = 0.0;
+  const double d_dz = inst(d, I_DZ, c);
+  const cprof<double> kur_adzw = profcd(d, P_ADZW, c);
+  const cprof<double> kur_grdf = profcd(d, P_GRDFX, c);
+  const double kur_dxf = sqrt((1. / d.dx) * (1. / d.dx) + (y3 ? (1. / d.dx) * (1. / d.dx) : 0.0));
+  if constexpr (KUR) {
+    w_a = Wk(0);
+    w_b = Wk(1);
+    tkh_pre = d.tkh[o0];
+  }
   for (int k = 0; k < nzm; k++) {
     size_t o = ((size_t)c * nzm + k) * NX + i;
     // the precip_init coefficients are constant during this kernel: scalar loads, which do
@@ -2832,6 +2849,7 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
     R qp = qp_pre;
     const R q_in = q, qp_in = qp;  // the entry state, for recorded training pairs
     const R u_k = u_pre, v_k = v_pre;
+    const R w_k = w_a, w_k1 = w_b, tkh_k = tkh_pre;
     {  // clamped at the top, where the values are not used again (an unconditional load needs
        // no copy that waits for earlier loads)
       const size_t on = ((size_t)c * nzm + ::min(k + 1, nzm - 1)) * NX + i;
@@ -2840,6 +2858,11 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
       qp_pre = d.qp[on];
       u_pre = d.u[on];
       if (y3) v_pre = d.v[on];
+      if constexpr (KUR) {
+        w_a = w_b;
+        w_b = Wk(::min(k + 2, nzm));
+        tkh_pre = d.tkh[on];
+      }
     }
     // the network's rates where it applies (inside its training envelope): condensate and
     // precipitation stepped by the rates, clipped at zero, total water q + qp kept, the
@@ -3057,6 +3080,19 @@ __global__ void CRM_LB k_micro(DevT<R> d, int accel) {
       diagnose_level<NX, R>(d, s, c, live, k, o, y3, t, q, qp, qn, qv, qcl, qci, qpl, qpi, u_k, v_k, kmin, kmax,
                             batched ? se : nullptr);
     }
+    if constexpr (KUR) {  // k_kurant's level terms, from the same values (the lid's w counts as zero)
+      const double uu = u_k, vv = y3 ? (double)v_k : 0.0, tkh = tkh_k;
+      const double wm = fabs((double)w_k), wm_up = k + 1 < nzm ? fabs((double)w_k1) : 0.0;
+      const double uhm = sqrt(uu * uu + vv * vv);
+      const double dzz = d_dz * kur_adzw[k];
+      const double sgs = fmax(fmax(0.5 * tkh * 1.0 * d.dt / (dzz * dzz), 0.5 * tkh * kur_grdf[k] * d.dt / (d.dx * d.dx)), 0.0);
+      cfl = fmax(cfl, sgs);
+      cfl = fmax(cfl, fmax(uhm * d.dt * kur_dxf, fmax(wm, wm_up) * d.dt / (d_dz * kur_adzw[k])));
+    }
+  }
+  if constexpr (KUR) {
+    cfl = s.max(cfl);
+    if (live && i == 0) atomic_max_nonneg(&d.gmax[GMAX_FLAGS + (c % KURANT_SLOTS) * KURANT_STRIDE], cfl);
   }
   if (accel) return;
   kmin = s.imin(kmin);
