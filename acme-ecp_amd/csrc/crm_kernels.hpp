@@ -1404,19 +1404,34 @@ __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
                 ctat * (rdx * ((double)uc[op] - ucp) + (wc[own] * rup - wc[ow] * rdn));
     P(k, i) = pv * rho[k];
   }
-  __syncthreads();
-  // 2. forward transform (every thread reads its whole row before any entry is overwritten)
-  for (int k0 = 0; k0 < nzm; k0 += 2) {
-    const int k = k0 + grp;
-    double acc = 0.0;
-    if (k < nzm) {
-#pragma unroll 8
-      for (int q = 0; q < NX; q++) acc += d.Ft[q * NX + i] * P(k, q);
+  // one 128-point transform of every level of the slab in place, P(k, i) <- sum_q M[q][i]
+  // P(k, q): each operator entry is read once and applied to all of the group's levels (an
+  // accumulator per level in registers, each summed in q order as a per-level loop would);
+  // every level is read before any is overwritten
+  constexpr int NJ = (NZM > 0 ? NZM : 64) / 2 + 1;  // levels per group, an upper bound
+  auto transform = [&](const double* __restrict__ M) {
+    double acc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) acc[j] = 0.0;
+    __syncthreads();
+    for (int q = 0; q < NX; q++) {
+      const double m = M[q * NX + i];
+#pragma unroll
+      for (int j = 0; j < NJ; j++) {
+        const int k = grp + 2 * j;
+        if (k < nzm) acc[j] += m * P(k, q);
+      }
     }
     __syncthreads();
-    if (k < nzm) P(k, i) = acc;
-  }
-  __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const int k = grp + 2 * j;
+      if (k < nzm) P(k, i) = acc[j];
+    }
+    __syncthreads();
+  };
+  // 2. forward transform
+  transform(d.Ft);
   // 3. Thomas sweep per slot (group 0)
   if (grp == 0) {
     const double eig = d.eign[i];
@@ -1470,19 +1485,8 @@ __global__ void CRM_LB k_press128(DevT<R> d, int na, int nb, int nc) {
       }
     }
   }
-  __syncthreads();
-  // 4. inverse transform
-  for (int k0 = 0; k0 < nzm; k0 += 2) {
-    const int k = k0 + grp;
-    double acc = 0.0;
-    if (k < nzm) {
-#pragma unroll 8
-      for (int q = 0; q < NX; q++) acc += d.Gt[q * NX + i] * P(k, q);
-    }
-    __syncthreads();
-    if (k < nzm) P(k, i) = acc;
-  }
-  __syncthreads();
+  // 4. inverse transform (its leading barrier also ends the Thomas sweep)
+  transform(d.Gt);
   // 5. pressure gradient, then Adams-Bashforth and the half-step Courant velocities
   const double dtdx = d.dtn / d.dx, dtdz = d.dtn / dz, dt3 = d.dt3na;
   R* uan = d.dudt[na];
