@@ -356,3 +356,69 @@ def run_many(hips, dcases, dt_gl, check=True):
     if rc and check:
         raise CrmError(rc, "crm_hip_run_many")
     return list(status)
+
+
+def _variant_count(kind, dims):
+    """Launch variants of a kernel class for a CRM shape (variants_of, crm_driver.hpp): the
+    pressure solve has five at 64 columns in 2D and one elsewhere; MPDATA has six."""
+    if kind == "press":
+        return 5 if dims.nx == 64 and dims.ny == 1 else 1
+    if kind == "mpdata":
+        return 6
+    return 1
+
+
+def tune_many(hips, dcases, dt_gl, kinds=("press",), margin=0.02):
+    """Choose launch variants for size classes advanced together by run_many, timing the whole
+    batch instead of each class alone (CrmHip.autotune).  Run concurrently, the classes share
+    the CUs, and a variant that wins alone with a larger workgroup (the eight-wave pressure
+    workgroup, which needs eight free wave slots on one CU at once) can lose in the batch.
+
+    Coordinate descent over the classes in order and, per class, over each kernel class in
+    kinds: one trial per candidate variant, each a batch call timed after a call that captures
+    the changed graphs, every call from the same saved state.  A candidate replaces the current
+    variant only when the batch is more than margin faster with it.  The trials advance the
+    DeviceCases, which are then restored to the state they had on entry.  The default searches
+    the pressure variants only: MPDATA's variants measured within 1% of each other in a batch.
+    Returns the chosen {kernel class: variant} of each context and the number of trials."""
+    import time
+
+    import torch
+    saved = [{g: {k: t.clone() for k, t in d.items()} for g, d in dc.arrays.items() if g != "inp"}
+             for dc in dcases]
+
+    def restore():
+        for dc, s in zip(dcases, saved):
+            for g, d in s.items():
+                for k, t in d.items():
+                    dc.arrays[g][k].copy_(t)
+        torch.cuda.synchronize()
+
+    def trial():
+        restore()
+        run_many(hips, dcases, dt_gl)
+        restore()
+        t0 = time.perf_counter()
+        run_many(hips, dcases, dt_gl)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    trials = 1
+    best = trial()
+    for h, dc in zip(hips, dcases):
+        dims = dc.case.dims()
+        for kind in kinds:
+            cur = h.variants()[kind]
+            for v in range(_variant_count(kind, dims)):
+                if v == cur:
+                    continue
+                keep = h.variants()
+                h.set_variants({kind: v})
+                t = trial()
+                trials += 1
+                if t < best * (1.0 - margin):
+                    best, cur = t, v
+                else:
+                    h.set_variants(keep)
+    restore()
+    return [h.variants() for h in hips], trials

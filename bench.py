@@ -237,7 +237,7 @@ UNEVEN_NAMES = [f"{n}x{nx}x{ny}x{nz}" for n, nx, ny, nz, dx, dt in UNEVEN]
 
 
 def measure_uneven(prec, args, torch):
-    from crm_hip import CrmHip, DeviceCase, run_many
+    from crm_hip import CrmHip, DeviceCase, run_many, tune_many
     dt_gl = WORKLOAD["dt_gl"]
     cases = [make_case(n, nx, nz, dx, dt, ny=ny, randomize=True) for n, nx, ny, nz, dx, dt in UNEVEN]
     hips = [CrmHip(0, prec) for _ in UNEVEN]
@@ -254,6 +254,12 @@ def measure_uneven(prec, args, torch):
         variants.append({"press": v["press"], "mpdata": v["mpdata"]})
         del scratch
     dcs = [DeviceCase(c, "cuda:0") for c in cases]
+    # the pressure variants again, timing the whole batch from this state (restored after)
+    t_tune = time.perf_counter()
+    chosen, trials = tune_many(hips, dcs, dt_gl)
+    tune_s = time.perf_counter() - t_tune
+    batch_variants = [{"press": v["press"], "mpdata": v["mpdata"]} for v in chosen]
+    log(f"[uneven {prec}] batch tuning: {trials} trials in {tune_s:.1f} s, variants {batch_variants}")
     torch.cuda.synchronize()
     for _ in range(args.warmup):
         run_many(hips, dcs, dt_gl)
@@ -296,7 +302,8 @@ def measure_uneven(prec, args, torch):
             "ms_per_step": 1e3 * sec / args.steps,
             "column_level_steps_per_s": levels * args.steps / sec,
             "classes": [f"{n}x{nx}x{ny}x{nz} dx {dx / 1000:g} km dt {dt:g} s" for n, nx, ny, nz, dx, dt in UNEVEN],
-            "schedule_threads_per_workgroup": scheds, "schedule_variants": variants,
+            "schedule_threads_per_workgroup": scheds, "class_tuned_variants": variants,
+            "schedule_variants": batch_variants, "batch_tuning_trials": trials,
             "graph_substeps": graphs,
             "class_alone_ms_per_step": alone,
             "sum_alone_ms_per_step": sum(alone),
@@ -305,7 +312,10 @@ def measure_uneven(prec, args, torch):
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "note": "one 1200 s GCM step of every class per step (crm_hip_run_many): each "
                     "class replays its own whole-call HIP graph on its own stream, "
-                    "concurrently; class_alone_ms_per_step is each class run by itself; "
+                    "concurrently; class_tuned_variants are the launch variants each class's "
+                    "own tuning picked, schedule_variants those kept after timing the whole "
+                    "batch with each pressure variant (tune_many); "
+                    "class_alone_ms_per_step is each class run by itself; "
                     "joint_graph_ms_per_step is the classes' calls captured into one graph "
                     "(CRM_GRAPHS_JOINT)"}
 

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle_lib
-from crm_hip import KERNELS, CrmHip, DeviceCase, run_many
+from crm_hip import KERNELS, CrmHip, DeviceCase, run_many, tune_many
 from crm_inputs import make_case
 from parity import compare_case, convective_case
 
@@ -154,6 +154,38 @@ def test_run_many_matches_separate_calls():
         dcs = [DeviceCase(c.copy()) for c in cases]
         status = run_many(hips, dcs, 200.0)
         assert status == [0] * len(CLASSES)
+        for dc, a in zip(dcs, alone):
+            _same(dc.download(dc.case), a)
+    finally:
+        for h in hips:
+            h.close()
+
+
+def test_tune_many_restores_state_and_keeps_results():
+    """The batch tuner tries every pressure and MPDATA variant each class has, leaves the
+    DeviceCases in the state they had on entry, and the batch it tuned stays bitwise equal to
+    separate calls with the default variants."""
+    cls = [CLASSES[2], CLASSES[4]]  # 64 columns (5 pressure variants) and 128 columns (1)
+    cases = [_case(c, seed=90 + j) for j, c in enumerate(cls)]
+    alone = []
+    for c in cases:
+        h = CrmHip(0)
+        try:
+            a = c.copy()
+            h.run(a, 200.0)
+            alone.append(a)
+        finally:
+            h.close()
+    hips = [CrmHip(0) for _ in cls]
+    try:
+        dcs = [DeviceCase(c.copy()) for c in cases]
+        chosen, trials = tune_many(hips, dcs, 200.0, kinds=("press", "mpdata"))
+        assert trials == 1 + 4 + 5 + 5
+        assert chosen[0]["press"] in range(5) and chosen[1]["press"] == 0
+        assert all(v["mpdata"] in range(6) for v in chosen)
+        for dc, c in zip(dcs, cases):
+            _same(c, dc.download(c.copy()))
+        assert run_many(hips, dcs, 200.0) == [0, 0]
         for dc, a in zip(dcs, alone):
             _same(dc.download(dc.case), a)
     finally:
