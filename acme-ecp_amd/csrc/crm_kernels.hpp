@@ -1689,9 +1689,24 @@ k_press64(DevT<R> d, int na, int nb, int nc) {
       P[k * PLD + i] = beta;
     };
     if constexpr (RALF) {
+      // the same operations as fwd, with the terms that do not depend on the recurrence (the
+      // coefficient broadcasts and the slab read) fetched one level ahead, so that they
+      // overlap the previous level's division instead of waiting behind it
+      double ak_n = acoef(1), ck_n = ccoef(1), rk_n = lane_of(rho_l, 1), pk_n = P[PLD + i];
 #pragma unroll
       for (int k = 1; k < NZM - 1; k++) {
-        fwd(k);
+        const double ak = ak_n, ck = ck_n, rk = rk_n, pk = pk_n;
+        if (k + 1 < NZM - 1) {
+          ak_n = acoef(k + 1);
+          ck_n = ccoef(k + 1);
+          rk_n = lane_of(rho_l, k + 1);
+          pk_n = P[(k + 1) * PLD + i];
+        }
+        double e = 1.0 / (eig * rk - ak - ck + ak * alfa);
+        alfa = -ck * e;
+        put_alf(k, alfa);
+        beta = (pk - ak * beta) * e;
+        P[k * PLD + i] = beta;
         __builtin_amdgcn_sched_barrier(0);
       }
     } else {
@@ -1707,9 +1722,14 @@ k_press64(DevT<R> d, int na, int nb, int nc) {
         P[kk * PLD + i] = top;
       };
       if constexpr (RALF) {
+        // bwd with the slab read one level ahead
+        double pn = P[(NZM - 2) * PLD + i];
 #pragma unroll
         for (int kk = NZM - 2; kk >= 0; kk--) {
-          bwd(kk);
+          const double pk = pn;
+          if (kk > 0) pn = P[(kk - 1) * PLD + i];
+          top = get_alf(kk) * top + pk;
+          P[kk * PLD + i] = top;
           __builtin_amdgcn_sched_barrier(0);
         }
       } else {
